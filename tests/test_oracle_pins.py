@@ -1,0 +1,556 @@
+"""Pins for the CPU oracle (CPU-only, no GPU needed).
+
+The reference (FreeFlow) ships no tests and no golden vectors, so each oracle
+function is pinned here by mathematics that does not retype its formula:
+Hermite projections that determine a lattice reconstruction completely, the
+closed form of the two-rate moment collision, IEEE conversions checked against
+numpy/torch, brute-force geometry in float64, lattice symmetries and the
+analytic decay of a shear wave.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+L = ol.lib()
+
+EX2 = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1], float)
+EY2 = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1], float)
+EX3 = np.array([0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1], float)
+EY3 = np.array([0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1, 1, -1], float)
+EZ3 = np.array([0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1, 1, -1, -1, 1, 1, -1, 1, -1], float)
+CS2 = 1.0 / 3.0
+
+
+def _h1(c, k):
+    """1D Hermite polynomials on D1Q3: 1, c, c^2 - 1/3."""
+    return [np.ones_like(c), c, c * c - CS2][k]
+
+
+def rand_moments2(rng, n=1):
+    rho = 1 + rng.uniform(-0.05, 0.05, n)
+    u = rng.uniform(-0.08, 0.08, (n, 2))
+    pi = np.stack([u[:, 0] ** 2, u[:, 1] ** 2, u[:, 0] * u[:, 1]], 1) + rng.uniform(-2e-3, 2e-3, (n, 3))
+    return np.column_stack([rho, u, pi])  # [rho, ux, uy, pxx, pyy, pxy]
+
+
+def rand_moments3(rng, n=1):
+    rho = 1 + rng.uniform(-0.05, 0.05, n)
+    u = rng.uniform(-0.08, 0.08, (n, 3))
+    pd = u ** 2 + rng.uniform(-2e-3, 2e-3, (n, 3))
+    po = np.stack([u[:, 0] * u[:, 1], u[:, 1] * u[:, 2], u[:, 0] * u[:, 2]], 1) + rng.uniform(-2e-3, 2e-3, (n, 3))
+    return np.column_stack([rho, u, pd, po])  # [rho,ux,uy,uz,pxx,pyy,pzz,pxy,pyz,pxz]
+
+
+# ---------------------------------------------------------------- reconstruction
+def expected_hermite2(m):
+    """Hermite coefficients a_{ab} of the 2D reconstruction, indexed by the 1D
+    orders (a along x, b along y).  Orders 0-2 are the moments themselves;
+    third order follows the recursive-regularised form a3 = u a2 + ... - 2 rho uuu
+    with the reference's 2D variant that omits rho on the Pi terms
+    (src/lbm/LbmUtilsFunc.cu:488-490); fourth order is absent."""
+    rho, ux, uy, pxx, pyy, pxy = m
+    a = np.zeros((3, 3))
+    a[0, 0] = rho
+    a[1, 0], a[0, 1] = rho * ux, rho * uy
+    a[2, 0], a[0, 2], a[1, 1] = rho * pxx, rho * pyy, rho * pxy
+    a[2, 1] = pxx * uy + 2 * pxy * ux - 2 * rho * ux * ux * uy
+    a[1, 2] = pyy * ux + 2 * pxy * uy - 2 * rho * uy * uy * ux
+    a[2, 2] = 0.0
+    return a
+
+
+def expected_hermite3(m):
+    rho, ux, uy, uz, pxx, pyy, pzz, pxy, pyz, pxz = m
+    u = [ux, uy, uz]
+    P = np.array([[pxx, pxy, pxz], [pxy, pyy, pyz], [pxz, pyz, pzz]])
+    a = np.zeros((3, 3, 3))
+    a[0, 0, 0] = rho
+    for d in range(3):
+        idx = [0, 0, 0]
+        idx[d] = 1
+        a[tuple(idx)] = rho * u[d]
+        idx[d] = 2
+        a[tuple(idx)] = rho * P[d, d]
+    a[1, 1, 0], a[0, 1, 1], a[1, 0, 1] = rho * pxy, rho * pyz, rho * pxz
+    # third order, full recursive form with rho (3D reference)
+    for (aa, b) in itertools.permutations(range(3), 2):
+        idx = [0, 0, 0]
+        idx[aa] = 2
+        idx[b] = 1
+        a[tuple(idx)] = rho * (P[aa, aa] * u[b] + 2 * P[aa, b] * u[aa] - 2 * u[aa] * u[aa] * u[b])
+    a[1, 1, 1] = rho * (pxy * uz + pyz * ux + pxz * uy - 2 * ux * uy * uz)
+    return a
+
+
+def test_recon2d_hermite_projections_pin_all_nine():
+    # 9 tensor-product Hermite polynomials span D2Q9: matching all 9 projections
+    # determines the 9 populations uniquely.
+    rng = np.random.default_rng(1)
+    for m in rand_moments2(rng, 50):
+        f = np.empty(9)
+        L.orc_recon2d_all_d(np.ascontiguousarray(m), f)
+        a = expected_hermite2(m)
+        for i, j in itertools.product(range(3), range(3)):
+            proj = np.sum(f * _h1(EX2, i) * _h1(EY2, j))
+            assert abs(proj - a[i, j]) < 1e-13, (i, j, proj, a[i, j])
+
+
+def test_recon3d_hermite_projections_pin_all_27():
+    rng = np.random.default_rng(2)
+    for m in rand_moments3(rng, 40):
+        f = np.empty(27)
+        L.orc_recon3d_all_d(np.ascontiguousarray(m), f)
+        a = expected_hermite3(m)
+        for i, j, k in itertools.product(range(3), range(3), range(3)):
+            proj = np.sum(f * _h1(EX3, i) * _h1(EY3, j) * _h1(EZ3, k))
+            assert abs(proj - a[i, j, k]) < 1e-13, ((i, j, k), proj, a[i, j, k])
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_recon_single_index_matches_all(dim):
+    rng = np.random.default_rng(3)
+    q = 9 if dim == 2 else 27
+    gen = rand_moments2 if dim == 2 else rand_moments3
+    for m in gen(rng, 30):
+        md = np.ascontiguousarray(m)
+        mf = md.astype(np.float32)
+        fa = np.empty(q)
+        ff = np.empty(q, np.float32)
+        (L.orc_recon2d_all_d if dim == 2 else L.orc_recon3d_all_d)(md, fa)
+        (L.orc_recon2d_all_f if dim == 2 else L.orc_recon3d_all_f)(mf, ff)
+        at_d = L.orc_recon2d_at_d if dim == 2 else L.orc_recon3d_at_d
+        at_f = L.orc_recon2d_at_f if dim == 2 else L.orc_recon3d_at_f
+        for i in range(q):
+            assert abs(at_d(md, i) - fa[i]) < 1e-15
+            assert abs(at_f(mf, i) - ff[i]) <= 2e-7 * max(1.0, abs(ff[i]))
+        # float path within float rounding of the double path
+        assert np.max(np.abs(ff - fa)) < 1e-6
+
+
+# ---------------------------------------------------------------- collision
+def collide_closed_form(rho, u, Praw, omega):
+    """Bulk (trace) relaxed at rate 1, deviator at omega, towards the raw
+    equilibrium rho/3 I + rho u u (moment-space two-rate collision)."""
+    D = len(u)
+    eq = rho / 3.0 * np.eye(D) + rho * np.outer(u, u)
+    dev = lambda A: A - np.trace(A) / D * np.eye(D)
+    return eq + (1 - omega) * dev(Praw - eq)
+
+
+def test_collide2d_closed_form():
+    rng = np.random.default_rng(4)
+    for _ in range(50):
+        rho = 1 + rng.uniform(-0.05, 0.05)
+        u = rng.uniform(-0.1, 0.1, 2)
+        omega = rng.uniform(0.5, 1.99)
+        P = rho / 3 * np.eye(2) + rho * np.outer(u, u) + rng.uniform(-3e-3, 3e-3, (2, 2))
+        P = 0.5 * (P + P.T)
+        pi = np.array([P[0, 0], P[1, 1], P[0, 1]])
+        L.orc_collide2d_d(rho, u[0], u[1], omega, pi)
+        E = collide_closed_form(rho, u, P, omega)
+        assert np.allclose(pi, [E[0, 0], E[1, 1], E[0, 1]], atol=1e-14, rtol=0)
+
+
+def test_collide3d_closed_form():
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        rho = 1 + rng.uniform(-0.05, 0.05)
+        u = rng.uniform(-0.1, 0.1, 3)
+        omega = rng.uniform(0.5, 1.99)
+        P = rho / 3 * np.eye(3) + rho * np.outer(u, u) + rng.uniform(-3e-3, 3e-3, (3, 3))
+        P = 0.5 * (P + P.T)
+        pi = np.array([P[0, 0], P[1, 1], P[2, 2], P[0, 1], P[1, 2], P[0, 2]])
+        L.orc_collide3d_d(rho, u[0], u[1], u[2], omega, pi)
+        E = collide_closed_form(rho, u, P, omega)
+        assert np.allclose(pi, [E[0, 0], E[1, 1], E[2, 2], E[0, 1], E[1, 2], E[0, 2]], atol=1e-14, rtol=0)
+
+
+# ---------------------------------------------------------------- IEEE conversions
+def test_f2h_matches_numpy_float16():
+    rng = np.random.default_rng(6)
+    xs = np.concatenate([
+        rng.uniform(-1, 1, 200000), rng.uniform(-1e-4, 1e-4, 50000), rng.uniform(-7e4, 7e4, 20000),
+        np.float32([0.0, -0.0, 65504, 65520, 65519.99, 6.1035156e-05, 5.96e-08, 2.98e-08, 2.99e-8, 1e-9, np.inf, -np.inf]),
+    ]).astype(np.float32)
+    # halfway cases: exact midpoints between adjacent halves
+    h = np.arange(0, 0x7BFF, 7, dtype=np.uint16)
+    lo = h.view(np.float16).astype(np.float32)
+    hi = (h + 1).view(np.float16).astype(np.float32)
+    xs = np.concatenate([xs, (lo.astype(np.float64) + hi) / 2]).astype(np.float32)
+    ref = xs.astype(np.float16).view(np.uint16)
+    got = np.array([L.orc_f2h(float(x)) for x in xs], np.uint16)
+    assert np.array_equal(got, ref)
+    back = np.array([L.orc_h2f(int(v)) for v in ref[:5000]], np.float32)
+    assert np.array_equal(back, ref[:5000].view(np.float16).astype(np.float32))
+
+
+def test_bf16_matches_torch():
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(7)
+    xs = np.concatenate([rng.uniform(-2, 2, 50000), rng.standard_normal(20000) * 1e-3]).astype(np.float32)
+    ref = torch.from_numpy(xs).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    got = np.array([L.orc_f2bf(float(x)) for x in xs], np.uint16)
+    assert np.array_equal(got, ref)
+
+
+# ---------------------------------------------------------------- geometry
+def seg_intersect_f64(p1, p2, p3, p4):
+    """Parametric segment/segment intersection in float64: returns (t on the
+    link measured from p2, s on the segment) or None."""
+    d1 = p1 - p2
+    d2 = p4 - p3
+    den = d1[0] * d2[1] - d1[1] * d2[0]
+    if den == 0:
+        return None
+    r = p3 - p2
+    t = (r[0] * d2[1] - r[1] * d2[0]) / den
+    s = (r[0] * d1[1] - r[1] * d1[0]) / den
+    return t, s
+
+
+def test_line_intersect_vs_float64_bruteforce():
+    rng = np.random.default_rng(8)
+    agree = total = 0
+    for _ in range(20000):
+        p = rng.uniform(0, 1, 8).astype(np.float32)
+        hit = np.zeros(2, np.float32)
+        d = L.orc_line_intersect(p, hit)
+        pp = p.astype(np.float64).reshape(4, 2)
+        r = seg_intersect_f64(pp[0], pp[1], pp[2], pp[3])
+        inside = r is not None and 0 < r[0] < 1 and 0 < r[1] < 1
+        margin = r is not None and min(abs(r[0]), abs(1 - r[0]), abs(r[1]), abs(1 - r[1])) > 1e-4
+        if r is None or not margin:
+            continue
+        total += 1
+        if inside:
+            # depth = 1 - t where t is the fraction from p2 towards p1
+            assert d > 0 and abs(d - (1 - r[0])) < 1e-5
+            xo = r[0] * pp[0] + (1 - r[0]) * pp[1]
+            assert np.allclose(hit, xo, atol=1e-5)
+        else:
+            assert d == -1.0
+        agree += 1
+    assert total > 10000 and agree == total
+
+
+def test_line_intersect_vertical_segment_quirk():
+    # Only the x-extent of the segment is tested (SolidGeometryProxy_Device.cuh:117),
+    # so an exactly vertical segment never reports a hit.
+    p = np.float32([0.2, 0.5, 0.8, 0.5, 0.5, 0.0, 0.5, 1.0])
+    assert L.orc_line_intersect(p, np.zeros(2, np.float32)) == -1.0
+    p = np.float32([0.2, 0.5, 0.8, 0.5, 0.49, 0.0, 0.51, 1.0])
+    assert L.orc_line_intersect(p, np.zeros(2, np.float32)) > 0
+
+
+def moller_trumbore(o, d, v0, v1, v2):
+    e1, e2 = v1 - v0, v2 - v0
+    h = np.cross(d, e2)
+    a = np.dot(e1, h)
+    if abs(a) < 1e-12:
+        return None
+    f = 1 / a
+    s = o - v0
+    u = f * np.dot(s, h)
+    q = np.cross(s, e1)
+    v = f * np.dot(d, q)
+    t = f * np.dot(e2, q)
+    return t, u, v
+
+
+def test_triangle_intersect_vs_moller_trumbore():
+    rng = np.random.default_rng(9)
+    checked = 0
+    for _ in range(20000):
+        tri = rng.uniform(0, 1, (3, 3)).astype(np.float32)
+        p1 = rng.uniform(0, 1, 3).astype(np.float32)
+        p2 = (p1 + rng.uniform(-0.5, 0.5, 3)).astype(np.float32)
+        out = np.zeros(7, np.float32)
+        r = L.orc_intersect3(p1, p2, np.ascontiguousarray(tri.ravel()), out)
+        mt = moller_trumbore(p1.astype(float), (p2 - p1).astype(float), *tri.astype(float))
+        if mt is None:
+            continue
+        t, u, v = mt
+        margin = min(abs(t), abs(1 - t), abs(u), abs(v), abs(1 - u - v))
+        if margin < 1e-3:
+            continue
+        # reference treats the segment as closed (t in [0,1]) with 1e-6 tolerances
+        expect = 0 <= t <= 1 and u >= 0 and v >= 0 and u + v <= 1
+        assert (r == 0) == expect
+        if expect:
+            assert abs(out[0] - t) < 1e-4
+            assert np.allclose(out[1:4], [1 - u - v, u, v], atol=1e-4)
+        checked += 1
+    assert checked > 5000
+
+
+def ellipse_polygon(n, cx, cy, a, b, phase=0.0):
+    th = phase + 2 * np.pi * np.arange(n) / n
+    v = np.stack([cx + a * np.cos(th), cy + b * np.sin(th)], 1).astype(np.float32)
+    e = np.stack([np.arange(n), (np.arange(n) + 1) % n], 1).astype(np.uint32)
+    return v, e
+
+
+def test_hit_table_equals_bruteforce_min_over_all_segments():
+    # The bbox enumeration of FsiCoupling2D.cu:44-55 must find every hit: the
+    # key of each link equals the min over *all* segments of depthIndexCompact.
+    nx, ny, dx = 40, 30, 0.01
+    v, e = ellipse_polygon(37, 0.2, 0.15, 0.1, 0.07, phase=0.1)
+    dom = ol.OracleDomain(2, nx, ny, dx=dx)
+    dom.set_boundary(v, np.zeros_like(v), e)
+    table = dom.hit_table()
+    p = np.zeros(8, np.float32)
+    hit = np.zeros(2, np.float32)
+    nhit = 0
+    for j in range(ny):
+        for i in range(nx):
+            for l in range(1, 9):
+                best = 0xFFFFFFFF
+                p[0], p[1] = np.float32(i) * np.float32(dx), np.float32(j) * np.float32(dx)
+                p[2] = np.float32(np.float32(i) - np.float32(EX2[l])) * np.float32(dx)
+                p[3] = np.float32(np.float32(j) - np.float32(EY2[l])) * np.float32(dx)
+                for s in range(len(e)):
+                    p[4:6] = v[e[s, 0]]
+                    p[6:8] = v[e[s, 1]]
+                    dd = L.orc_line_intersect(p, hit)
+                    if 0 < dd < 1:
+                        best = min(best, L.orc_depth_index_compact(dd, s))
+                assert table[j * nx + i, l] == best
+                nhit += best != 0xFFFFFFFF
+    assert nhit > 100
+
+
+def test_depth_index_compact_layout():
+    assert L.orc_depth_index_compact(0.5, 7) == (16384 << 16) | 7
+    assert L.orc_depth_index_compact(0.25, 70000) == (8192 << 16) | (70000 & 0xFFFF)
+
+
+def test_fill_solid_vs_winding_number():
+    nx, ny, dx = 60, 40, 0.01
+    v, e = ellipse_polygon(50, 0.3, 0.2, 0.2, 0.12, phase=0.05)
+    flags = np.ones(nx * ny, np.uint8)
+    flags[:nx] = 5  # a wall row stays untouched
+    L.orc_fill_solid2d(nx, ny, dx, len(v), v.ravel().copy(), len(e), e.ravel().copy(), flags)
+    vv = v.astype(np.float64)
+    for j in range(1, ny):
+        for i in range(nx):
+            p = np.array([(i + 0.5) * dx, (j + 0.5) * dx])
+            ang = np.arctan2(vv[:, 1] - p[1], vv[:, 0] - p[0])
+            dang = np.diff(np.concatenate([ang, ang[:1]]))
+            dang = (dang + np.pi) % (2 * np.pi) - np.pi
+            wn = round(abs(dang.sum()) / (2 * np.pi))
+            # skip points within a hair of the polygon edge
+            dmin = np.min(np.hypot(vv[:, 0] - p[0], vv[:, 1] - p[1]))
+            if dmin < 2e-3:
+                continue
+            assert (flags[j * nx + i] == 4) == (wn == 1)
+    assert np.all(flags[:nx] == 5)
+
+
+# ---------------------------------------------------------------- full step
+def periodic_domain(dim, n, fp64, nu=0.02, stage=False):
+    if dim == 2:
+        return ol.OracleDomain(2, n, n, fp64=fp64, stage_fp16=stage, periodic=True, nu=nu)
+    return ol.OracleDomain(3, n, n, n, fp64=fp64, periodic=True, nu=nu)
+
+
+def eq_moments(u):
+    """Equilibrium moments [rho=1, u, Pi = u u] in stored order."""
+    n, D = u.shape
+    rho = np.ones((n, 1))
+    if D == 2:
+        pi = np.stack([u[:, 0] ** 2, u[:, 1] ** 2, u[:, 0] * u[:, 1]], 1)
+    else:
+        pi = np.stack([u[:, 0] ** 2, u[:, 1] ** 2, u[:, 2] ** 2, u[:, 0] * u[:, 1], u[:, 1] * u[:, 2], u[:, 0] * u[:, 2]], 1)
+    return np.column_stack([rho, u, pi])
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_uniform_equilibrium_is_fixed_point_clamped(dim):
+    n = 12 if dim == 2 else 6
+    dom = ol.OracleDomain(dim, n, n, n, fp64=True, stage_fp16=False, periodic=False)
+    u = np.tile([0.03, -0.02, 0.01][:dim], (dom.n, 1))
+    m = eq_moments(u)
+    dom.set_moments(m)
+    dom.step(5)
+    assert np.max(np.abs(dom.moments() - m)) < 1e-14
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_periodic_mass_conservation_and_translation(dim):
+    n = 16 if dim == 2 else 8
+    rng = np.random.default_rng(10 + dim)
+    dom = periodic_domain(dim, n, fp64=True)
+    u = rng.uniform(-0.03, 0.03, (dom.n, dim))
+    m = eq_moments(u)
+    m[:, 0] += rng.uniform(-1e-3, 1e-3, dom.n)
+    dom.set_moments(m)
+    dom.step(10)
+    r1 = dom.moments()
+    assert abs(r1[:, 0].sum() - m[:, 0].sum()) < 1e-11
+    # translating the initial field translates the result exactly (periodic)
+    shape = (n, n, 6) if dim == 2 else (n, n, n, 10)
+    sh = np.roll(m.reshape(shape), (3, 5) if dim == 2 else (2, 3, 5), axis=(0, 1) if dim == 2 else (0, 1, 2))
+    dom2 = periodic_domain(dim, n, fp64=True)
+    dom2.set_moments(sh.reshape(-1, m.shape[1]))
+    dom2.step(10)
+    r2 = np.roll(dom2.moments().reshape(shape), (-3, -5) if dim == 2 else (-2, -3, -5),
+                 axis=(0, 1) if dim == 2 else (0, 1, 2)).reshape(r1.shape)
+    assert np.array_equal(r1, r2)
+
+
+def test_rotation_symmetry_2d():
+    # rotate the initial field by 90 degrees (x,y) -> (-y, x): the evolved field
+    # must be the rotated evolved field (catches swapped/negated lattice entries).
+    n = 16
+    rng = np.random.default_rng(12)
+    dom = periodic_domain(2, n, fp64=True)
+    u = rng.uniform(-0.03, 0.03, (n * n, 2))
+    m = eq_moments(u)
+    m[:, 3:] += rng.uniform(-1e-4, 1e-4, (n * n, 3))
+    dom.set_moments(m)
+    dom.step(6)
+    r1 = dom.moments().reshape(n, n, 6)  # [y, x, k]
+
+    def rot(a):
+        # new(y', x') = old(y, x) with x' = n-1-y, y' = x ; vectors rotate too
+        b = np.empty_like(a)
+        for y in range(n):
+            for x in range(n):
+                rho, ux, uy, pxx, pyy, pxy = a[y, x]
+                b[x, n - 1 - y] = [rho, -uy, ux, pyy, pxx, -pxy]
+        return b
+
+    dom2 = periodic_domain(2, n, fp64=True)
+    dom2.set_moments(rot(m.reshape(n, n, 6)).reshape(-1, 6))
+    dom2.step(6)
+    r2 = dom2.moments().reshape(n, n, 6)
+    assert np.max(np.abs(r2 - rot(r1))) < 1e-13
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_shear_wave_decay_matches_viscosity(dim):
+    # u_x(y) = A sin(k y) decays as exp(-nu k^2 t) with nu = (1/omega - 1/2)/3.
+    # The wave varies along y only, so x (and z) can stay short; periodic pulls.
+    nx, ny, nz = 8, 64, 4
+    nu = 0.04
+    dom = ol.OracleDomain(dim, nx, ny, nz, fp64=True, stage_fp16=False, periodic=True, nu=nu)
+    A = 1e-4
+    k = 2 * np.pi / ny
+    ys = (np.arange(dom.n) // nx) % ny
+    u = np.zeros((dom.n, dim))
+    u[:, 0] = A * np.sin(k * ys)
+    dom.set_moments(eq_moments(u))
+    dom.step(60)  # let the non-equilibrium stress settle
+    a1 = np.sum(dom.moments()[:, 1] * np.sin(k * ys))
+    T = 400
+    dom.step(T)
+    a2 = np.sum(dom.moments()[:, 1] * np.sin(k * ys))
+    omega = 1 / (3 * nu + 0.5)
+    nu_eff = (1 / omega - 0.5) / 3
+    expect = np.exp(-nu_eff * k * k * T)
+    assert abs(a2 / a1 / expect - 1) < 2e-3, (a2 / a1, expect)
+
+
+def test_float_oracle_tracks_fp64_oracle():
+    n = 24
+    rng = np.random.default_rng(13)
+    u = rng.uniform(-0.03, 0.03, (n * n, 2))
+    m = eq_moments(u)
+    d32 = periodic_domain(2, n, fp64=False)
+    d64 = periodic_domain(2, n, fp64=True)
+    d32.set_moments(m)
+    d64.set_moments(m)
+    d32.step(50)
+    d64.step(50)
+    a, b = d32.moments(), d64.moments()
+    rel = np.linalg.norm(a[:, 1:3] - b[:, 1:3]) / np.linalg.norm(b[:, 1:3])
+    assert rel < 3e-5  # FP32 rounding only; a formula slip would be O(1e-2)
+
+
+def test_init_reference_is_equilibrium_closed_form():
+    nx, ny = 10, 6
+    dom = ol.OracleDomain(2, nx, ny, fp64=False)
+    flags = np.ones(nx * ny, np.uint8)
+    flags[0::nx] = 13  # InletLeft column
+    flags[nx - 1::nx] = 20  # OutletRight column
+    dom.set_flags(flags)
+    dom.init_reference([0.05, 0, 0, 0])
+    m = dom.moments()
+    inlet = flags == 13
+    assert np.all(m[:, 0] == 1.0)
+    assert np.allclose(m[inlet, 1], 0.05) and np.all(m[~inlet, 1] == 0)
+    assert np.allclose(m[inlet, 3], 0.05 ** 2, atol=1e-7)  # Pi_xx = u^2 (FP32: 1 ulp of 1/3)
+    assert np.allclose(m[:, 4], 0, atol=1e-7) and np.allclose(m[:, 5], 0, atol=1e-7)
+    assert np.array_equal(dom.moments(1), m)  # both ping-pong buffers
+
+
+def test_outlet_rule_small_case():
+    # OutletRight copies u_x from the fluid neighbour only when it flows out,
+    # and always sets Pi_xx = u_x^2 (FsiCoupling2D.cu:304-314).
+    nx, ny = 4, 1
+    dom = ol.OracleDomain(2, nx, ny, fp64=True, stage_fp16=False)
+    flags = np.array([5, 1, 1, 20], np.uint8)
+    dom.set_flags(flags)
+    m = np.zeros((4, 6))
+    m[:, 0] = 1
+    m[:, 1] = [0.0, 0.02, 0.02, 0.5]
+    m[:, 3] = [0.0, 4e-4, 4e-4, 0.25]
+    dom.set_moments(m)
+    dom.step(1)
+    r = dom.moments()
+    assert r[3, 1] == r[2, 1] and r[2, 1] > 0
+    assert r[3, 3] == r[2, 1] ** 2
+    # inflow case: u_x stays, Pi_xx still overwritten
+    m[:, 1] = [0.0, -0.02, -0.02, -0.01]
+    m[:, 3] = [0.0, 4e-4, 4e-4, 1e-4]
+    dom2 = ol.OracleDomain(2, nx, ny, fp64=True, stage_fp16=False)
+    dom2.set_flags(flags)
+    dom2.set_moments(m)
+    dom2.step(1)
+    r = dom2.moments()
+    assert r[2, 1] < 0 and r[3, 1] == -0.01 and r[3, 3] == r[2, 1] ** 2
+
+
+def hermite_synthesis2(a):
+    """Populations from tensor-product Hermite coefficients on D2Q9:
+    f_i = w_i sum_ab a_ab H_a(cx) H_b(cy) / (N_a N_b), N = (1, 1/3, 2/9)."""
+    N = [1.0, CS2, 2.0 / 9.0]
+    w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+    f = np.zeros(9)
+    for i, j in itertools.product(range(3), range(3)):
+        f += a[i, j] * _h1(EX2, i) * _h1(EY2, j) / (N[i] * N[j])
+    return w * f
+
+
+def test_first_step_force_on_static_body_closed_form():
+    # From a uniform equilibrium flow, every moving-wall link of a static body
+    # carries F = -(f_inv(i)^eq + w_i rho) c_i cf (fout is the rest state
+    # because Pi* = Pi - u u = 0).  Summed over the links of the hit table it
+    # must equal the sum of the per-vertex forces.
+    nx = ny = 40
+    u0 = np.array([0.05, 0.01])
+    v, e = ellipse_polygon(64, 19.3, 20.1, 7.3, 5.1, phase=np.pi / 64)
+    dom = ol.OracleDomain(2, nx, ny, fp64=True, stage_fp16=False, periodic=True, dx=1.0, dt=1.0, cf=1.0)
+    m = eq_moments(np.tile(u0, (nx * ny, 1)))
+    dom.set_moments(m)
+    dom.set_boundary(v, np.zeros_like(v), e)
+    table = dom.hit_table()
+    dom.step(1)
+    F = dom.vertex_forces().sum(0)
+    feq = hermite_synthesis2(expected_hermite2(m[0]))
+    w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+    inv = [0, 3, 4, 1, 2, 7, 8, 5, 6]
+    expect = np.zeros(2)
+    nlinks = 0
+    for cell in range(nx * ny):
+        for l in range(1, 9):
+            if table[cell, l] != 0xFFFFFFFF:
+                c = np.array([EX2[l], EY2[l]])
+                expect += -(feq[inv[l]] + w[l] * 1.0) * c
+                nlinks += 1
+    assert nlinks > 50
+    assert np.abs(expect).max() > 1e-3
+    assert np.allclose(F, expect, rtol=1e-6, atol=1e-9), (F, expect)
