@@ -1,0 +1,733 @@
+// capi.hip -- the extern "C" boundary: domain lifecycle, uploads, the step
+// driver and getters.  Host-side orchestration of the kernels in
+// kernels2d.hip, kernels3d.hip and links.hip.
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ff_internal.h"
+
+struct ff_domain : public ff::Domain {};
+
+namespace ff {
+
+static thread_local std::string g_error;
+
+void set_error(const std::string& msg) { g_error = msg; }
+int fail(int code, const std::string& msg) {
+  g_error = msg;
+  return code;
+}
+int hip_fail(hipError_t e, const char* what) {
+  g_error = std::string(what) + ": " + hipGetErrorString(e);
+  return e == hipErrorOutOfMemory ? FF_ERR_NOMEM : FF_ERR_HIP;
+}
+
+template <class T>
+static int dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+  return FF_OK;
+}
+template <class T>
+static void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+static void free_boundary(Domain& d) {
+  dfree(d.verts); dfree(d.vels); dfree(d.elems); dfree(d.elem_body); dfree(d.vforce);
+  dfree(d.body_elem_ptr); dfree(d.body_elem_idx); dfree(d.vert_inc_ptr); dfree(d.vert_inc_idx);
+  dfree(d.body_ref); dfree(d.elem_acc); dfree(d.wrench); dfree(d.wrench_hist);
+  dfree(d.ht_cell); dfree(d.ht_key); dfree(d.ht_f); dfree(d.ht_b); dfree(d.ht_hit); dfree(d.ht_list);
+  dfree(d.counters); dfree(d.rest_verts); dfree(d.s_coord); dfree(d.wave_params); dfree(d.vert_body);
+  d.nv = d.ne = d.n_bodies = 0;
+  d.has_boundary = false;
+  d.wave = false;
+  d.hist_cap = 0;
+}
+
+static int elem_stride(int dim) { return dim == 2 ? 7 : 15; }
+static int wrench_width(int dim) { return dim == 2 ? 3 : 6; }
+
+// Outlet list from the flags (mrApplyOutlet2DKernel / 3D branch table).
+static void build_outlets(Domain& d, std::vector<OutletCell>& out) {
+  out.clear();
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  for (long long c = 0; c < d.n; c++) {
+    const uint8_t f = d.h_flags[c];
+    if (f < kOutletLeft || f > kOutletBack) continue;
+    const int x = (int)(c % nx), y = (int)((c / nx) % ny), z = (int)(c / ((long long)nx * ny));
+    OutletCell o{};
+    o.cell = (int)c;
+    int nbx = x, nby = y, nbz = z;
+    if (d.dim == 2) {
+      if (f == kOutletUp) { nby = y - 1; o.comp = 2; o.pcomp = 4; o.sign = 1; }
+      else if (f == kOutletDown) { nby = y + 1; o.comp = 2; o.pcomp = 4; o.sign = -1; }
+      else if (f == kOutletLeft) { nbx = x + 1; o.comp = 1; o.pcomp = 3; o.sign = -1; }
+      else if (f == kOutletRight) { nbx = x - 1; o.comp = 1; o.pcomp = 3; o.sign = 1; }
+      else continue;  // Front/Back have no 2D branch
+    } else {
+      if (f == kOutletUp) { nbz = z - 1; o.comp = 3; o.pcomp = 6; o.sign = 1; }
+      else if (f == kOutletDown) { nbz = z + 1; o.comp = 3; o.pcomp = 6; o.sign = -1; }
+      else if (f == kOutletLeft) { nbx = x + 1; o.comp = 1; o.pcomp = 4; o.sign = -1; }
+      else if (f == kOutletRight) { nbx = x - 1; o.comp = 1; o.pcomp = 4; o.sign = 1; }
+      else if (f == kOutletFront) { nby = y + 1; o.comp = 2; o.pcomp = 5; o.sign = -1; }
+      else { nby = y - 1; o.comp = 2; o.pcomp = 5; o.sign = 1; }
+    }
+    // the reference reads out of the grid here; such cells are skipped
+    if (nbx < 0 || nbx >= nx || nby < 0 || nby >= ny || nbz < 0 || nbz >= nz) continue;
+    o.nb = (int)(((long long)nbz * ny + nby) * nx + nbx);
+    out.push_back(o);
+  }
+}
+
+static int upload_aos(Domain& d, const float* aos, int which) {
+  float* tmp = nullptr;
+  const size_t cnt = (size_t)d.n * d.nm;
+  int rc = dalloc(&tmp, cnt);
+  if (rc) return rc;
+  hipError_t e = hipMemcpy(tmp, aos, cnt * sizeof(float), hipMemcpyHostToDevice);
+  if (e != hipSuccess) { dfree(tmp); return hip_fail(e, "hipMemcpy"); }
+  for (int b = 0; b < 2; b++) {
+    if (which >= 0 && which != b) continue;
+    void* target = d.mom[b == 0 ? d.cur : (d.cur ^ 1)];
+    rc = launch_convert_in(d, tmp, target);
+    if (rc) { dfree(tmp); return rc; }
+  }
+  e = hipStreamSynchronize(d.stream);
+  dfree(tmp);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+  return FF_OK;
+}
+
+// Reference initial state computed on the host (initialize_state_kernel_2d/3d,
+// LbmInitializer2D.cu:32-95 / LbmInitializer3D.cu:33-113), same arithmetic.
+static void init_state_host(const Domain& d, const float* bc, std::vector<float>& aos) {
+  aos.assign((size_t)d.n * d.nm, 0.0f);
+  static const float ex2[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+  static const float ey2[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+  static const float w2[9] = {4.0f / 9.0f, 1.0f / 9.0f, 1.0f / 9.0f, 1.0f / 9.0f, 1.0f / 9.0f,
+                              1.0f / 36.0f, 1.0f / 36.0f, 1.0f / 36.0f, 1.0f / 36.0f};
+  static const float ex3[27] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1};
+  static const float ey3[27] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1, 1, -1};
+  static const float ez3[27] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1, 1, -1, -1, 1, 1, -1, 1, -1};
+  const float cs2 = 1.0f / 3.0f;
+  for (long long c = 0; c < d.n; c++) {
+    const uint8_t fl = d.h_flags[c];
+    float ux = 0.0f, uy = 0.0f, uz = 0.0f;
+    const float rho = 1.0f;
+    float* m = &aos[(size_t)c * d.nm];
+    if (d.dim == 2) {
+      if (fl == kInletLeft) ux = bc[0];
+      else if (fl == kInletDown) uy = bc[1];
+      else if (fl == kInletRight) ux = -bc[2];
+      else if (fl == kInletUp) uy = -bc[3];
+      const float u_sq = ux * ux + uy * uy;
+      float feq[9];
+      for (int k = 0; k < 9; k++) {
+        const float cu = ex2[k] * ux + ey2[k] * uy;
+        feq[k] = w2[k] * rho * (1.0f + 3.0f * cu + 4.5f * cu * cu - 1.5f * u_sq);
+      }
+      const float inv = 1 / rho;
+      const float pixx = feq[1] + feq[3] + feq[5] + feq[6] + feq[7] + feq[8];
+      const float piyy = feq[2] + feq[4] + feq[5] + feq[6] + feq[7] + feq[8];
+      const float pixy = feq[5] - feq[6] + feq[7] - feq[8];
+      m[0] = rho; m[1] = ux; m[2] = uy;
+      m[3] = (float)((double)(pixx * inv) - 1.0 * (double)cs2);
+      m[4] = (float)((double)(piyy * inv) - 1.0 * (double)cs2);
+      m[5] = pixy * inv;
+    } else {
+      if (fl == kInletLeft) ux = bc[0];
+      else if (fl == kInletDown) uz = bc[1];
+      else if (fl == kInletRight) ux = -bc[2];
+      else if (fl == kInletUp) uz = -bc[3];
+      else if (fl == kInletFront) uy = bc[4];
+      else if (fl == kInletBack) uy = -bc[5];
+      const float u_sq = ux * ux + uy * uy + uz * uz;
+      float f[27];
+      for (int k = 0; k < 27; k++) {
+        const float cu = ex3[k] * ux + ey3[k] * uy + ez3[k] * uz;
+        const float w = k == 0 ? 8.0f / 27.0f : (k < 7 ? 2.0f / 27.0f : (k < 19 ? 1.0f / 54.0f : 1.0f / 216.0f));
+        f[k] = w * rho * (1.0f + 3.0f * cu + 4.5f * cu * cu - 1.5f * u_sq);
+      }
+      const float pixx = f[1] + f[2] + f[7] + f[8] + f[9] + f[10] + f[13] + f[14] + f[15] + f[16] + f[19] + f[20] + f[21] + f[22] + f[23] + f[24] + f[25] + f[26];
+      const float piyy = f[3] + f[4] + f[7] + f[8] + f[11] + f[12] + f[13] + f[14] + f[17] + f[18] + f[19] + f[20] + f[21] + f[22] + f[23] + f[24] + f[25] + f[26];
+      const float pizz = f[5] + f[6] + f[9] + f[10] + f[11] + f[12] + f[15] + f[16] + f[17] + f[18] + f[19] + f[20] + f[21] + f[22] + f[23] + f[24] + f[25] + f[26];
+      const float pixy = (f[7] + f[8] + f[19] + f[20] + f[21] + f[22]) - (f[13] + f[14] + f[23] + f[24] + f[25] + f[26]);
+      const float piyz = (f[11] + f[12] + f[19] + f[20] + f[25] + f[26]) - (f[17] + f[18] + f[21] + f[22] + f[23] + f[24]);
+      const float pixz = (f[9] + f[10] + f[19] + f[20] + f[23] + f[24]) - (f[15] + f[16] + f[21] + f[22] + f[25] + f[26]);
+      const float inv = 1 / rho;
+      m[0] = rho; m[1] = ux; m[2] = uy; m[3] = uz;
+      m[4] = (float)((double)(pixx * inv) - 1.0 * (double)cs2);
+      m[5] = (float)((double)(piyy * inv) - 1.0 * (double)cs2);
+      m[6] = (float)((double)(pizz * inv) - 1.0 * (double)cs2);
+      m[7] = pixy * inv; m[8] = piyz * inv; m[9] = pixz * inv;
+    }
+  }
+}
+
+static unsigned next_pow2(unsigned long long v) {
+  unsigned p = 1024;
+  while (p < v && p < (1u << 30)) p <<= 1;
+  return p;
+}
+
+static int one_step(Domain& d, const float* verts, const float* vels, const uint32_t* elems, float* proxy_forces) {
+  void* in = d.mom[d.cur];
+  void* out = d.mom[d.cur ^ 1];
+  const bool links = d.has_boundary && d.ne > 0;
+  int rc;
+  if (links) {
+    if (d.wave && verts == d.verts) {
+      if ((rc = launch_wave(d))) return rc;
+    }
+    if ((rc = launch_links_reset(d))) return rc;
+    if ((rc = launch_links_build(d, verts, elems))) return rc;
+  }
+  rc = d.dim == 2 ? launch_bulk2d(d, in, out) : launch_bulk3d(d, in, out);
+  if (rc) return rc;
+  if (links) {
+    if ((rc = launch_boundary_cells(d, in, out, verts, vels, elems))) return rc;
+    if ((rc = launch_forces(d, verts, elems, proxy_forces))) return rc;
+  }
+  d.cur ^= 1;
+  if ((rc = launch_outlet(d, d.mom[d.cur]))) return rc;
+  d.step_index++;
+  return FF_OK;
+}
+
+static int check_overflow(Domain& d) {
+  if (!d.has_boundary) return FF_OK;
+  int c[2] = {0, 0};
+  FF_HIP(hipMemcpy(c, d.counters, sizeof(c), hipMemcpyDeviceToHost));
+  if (c[1]) {
+    int z = 0;
+    FF_HIP(hipMemcpy(d.counters + 1, &z, sizeof(int), hipMemcpyHostToDevice));
+    return fail(FF_ERR_CAPACITY, "boundary link table overflow (capacity " + std::to_string(d.ht_cap) + " cells)");
+  }
+  return FF_OK;
+}
+
+}  // namespace ff
+
+using namespace ff;
+
+#define FF_CHECK_DOMAIN(d) \
+  if (!(d)) return fail(FF_ERR_INVALID, "null domain")
+
+extern "C" {
+
+const char* ff_last_error(void) { return g_error.c_str(); }
+const char* ff_version(void) { return "freeflow_lbm gfx950 1.0"; }
+
+int ff_device_count(int* n) {
+  if (!n) return fail(FF_ERR_INVALID, "null output");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) { *n = 0; return hip_fail(e, "hipGetDeviceCount"); }
+  *n = c;
+  return FF_OK;
+}
+
+int ff_domain_create(const ff_domain_desc* desc, ff_domain** out) {
+  if (!desc || !out) return fail(FF_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (desc->dim != 2 && desc->dim != 3) return fail(FF_ERR_INVALID, "dim must be 2 or 3");
+  if (desc->nx <= 0 || desc->ny <= 0 || (desc->dim == 3 && desc->nz <= 0))
+    return fail(FF_ERR_INVALID, "grid sizes must be positive");
+  const long long nz = desc->dim == 3 ? desc->nz : 1;
+  const long long n = (long long)desc->nx * desc->ny * nz;
+  if (n >= (1ll << 31)) return fail(FF_ERR_INVALID, "grid larger than 2^31 cells");
+  if (desc->storage < 0 || desc->storage > 2) return fail(FF_ERR_INVALID, "unknown storage format");
+  if (!(desc->viscosity > 0.0)) return fail(FF_ERR_INVALID, "viscosity must be positive");
+  if (!(desc->dx > 0.0) || !(desc->dt > 0.0)) return fail(FF_ERR_INVALID, "dx and dt must be positive");
+  if (desc->options & ~(FF_OPT_STAGE_FP16 | FF_OPT_PERIODIC)) return fail(FF_ERR_INVALID, "unknown option bits");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(FF_ERR_NODEVICE, "no HIP device available");
+  if (desc->device < 0 || desc->device >= ndev) return fail(FF_ERR_INVALID, "device ordinal out of range");
+  FF_HIP(hipSetDevice(desc->device));
+  hipDeviceProp_t prop;
+  FF_HIP(hipGetDeviceProperties(&prop, desc->device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(FF_ERR_NODEVICE, std::string("library built for gfx950, device is ") + prop.gcnArchName);
+
+  ff_domain* dd = new ff_domain();
+  Domain& d = *dd;
+  d.device = desc->device;
+  d.dim = desc->dim;
+  d.nx = desc->nx;
+  d.ny = desc->ny;
+  d.nz = (int)nz;
+  d.n = n;
+  d.nm = d.dim == 2 ? 6 : 10;
+  d.q = d.dim == 2 ? 9 : 27;
+  d.storage = desc->storage;
+  d.options = desc->options;
+  if (d.dim == 3) d.options &= ~FF_OPT_STAGE_FP16;  // the 3D tile is FP32 in the reference
+  d.nu = desc->viscosity;
+  d.dx = desc->dx;
+  d.dt = desc->dt;
+  d.rho_f = desc->fluid_density;
+  // FsiCoupling2D.cu:368-371 and :236: float viscosity, omega from double arithmetic
+  d.nu_f = (float)d.nu;
+  d.omega = (float)(1.0 / ((double)d.nu_f * 3.0 + 0.5));
+  d.cl = (float)d.dx;
+  d.ct = (float)d.dt;
+  // SimulationParameters2D/3D::getCf (Config.hpp:96-100, 154-158): cm is float
+  const float cm = d.dim == 2 ? (float)(d.rho_f * d.dx * d.dx) : (float)(d.rho_f * d.dx * d.dx * d.dx);
+  d.cf = (float)((double)cm * d.dx / (d.dt * d.dt));
+  d.plane = plane_stride(n);
+  d.h_flags.assign((size_t)n, kFluid);
+  hipError_t e = hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete dd; return hip_fail(e, "hipStreamCreate"); }
+  const size_t bytes = d.plane * d.nm * storage_bytes(d.storage);
+  int rc = FF_OK;
+  for (int b = 0; b < 2 && !rc; b++) {
+    e = hipMalloc(&d.mom[b], bytes);
+    if (e != hipSuccess) rc = hip_fail(e, "hipMalloc(moments)");
+    else if ((e = hipMemset(d.mom[b], 0, bytes)) != hipSuccess) rc = hip_fail(e, "hipMemset");
+  }
+  if (!rc) rc = dalloc(&d.flags, (size_t)n);
+  if (!rc && (e = hipMemset(d.flags, kFluid, (size_t)n)) != hipSuccess) rc = hip_fail(e, "hipMemset(flags)");
+  if (!rc) rc = dalloc(&d.red, 1024);
+  if (!rc && (e = hipDeviceSynchronize()) != hipSuccess) rc = hip_fail(e, "hipDeviceSynchronize");
+  if (rc) { ff_domain_destroy(dd); return rc; }
+  *out = dd;
+  return FF_OK;
+}
+
+int ff_domain_destroy(ff_domain* dd) {
+  if (!dd) return FF_OK;
+  Domain& d = *dd;
+  (void)hipSetDevice(d.device);
+  if (d.stream) (void)hipStreamSynchronize(d.stream);
+  free_boundary(d);
+  for (int b = 0; b < 2; b++) if (d.mom[b]) (void)hipFree(d.mom[b]);
+  dfree(d.flags);
+  dfree(d.outlets);
+  dfree(d.red);
+  if (d.stream) (void)hipStreamDestroy(d.stream);
+  delete dd;
+  return FF_OK;
+}
+
+int ff_set_flags(ff_domain* dd, const uint8_t* flags) {
+  FF_CHECK_DOMAIN(dd);
+  if (!flags) return fail(FF_ERR_INVALID, "null flags");
+  Domain& d = *dd;
+  FF_HIP(hipSetDevice(d.device));
+  std::memcpy(d.h_flags.data(), flags, (size_t)d.n);
+  FF_HIP(hipStreamSynchronize(d.stream));
+  FF_HIP(hipMemcpy(d.flags, flags, (size_t)d.n, hipMemcpyHostToDevice));
+  std::vector<OutletCell> oc;
+  build_outlets(d, oc);
+  dfree(d.outlets);
+  d.n_outlets = (int)oc.size();
+  if (d.n_outlets) {
+    int rc = dalloc(&d.outlets, oc.size());
+    if (rc) return rc;
+    FF_HIP(hipMemcpy(d.outlets, oc.data(), oc.size() * sizeof(OutletCell), hipMemcpyHostToDevice));
+  }
+  return FF_OK;
+}
+
+int ff_get_flags(ff_domain* dd, uint8_t* flags) {
+  FF_CHECK_DOMAIN(dd);
+  if (!flags) return fail(FF_ERR_INVALID, "null flags");
+  Domain& d = *dd;
+  FF_HIP(hipSetDevice(d.device));
+  FF_HIP(hipStreamSynchronize(d.stream));
+  FF_HIP(hipMemcpy(flags, d.flags, (size_t)d.n, hipMemcpyDeviceToHost));
+  return FF_OK;
+}
+
+int ff_init_state(ff_domain* dd, const float* bc, int n_bc) {
+  FF_CHECK_DOMAIN(dd);
+  Domain& d = *dd;
+  FF_HIP(hipSetDevice(d.device));
+  float b[6] = {0, 0, 0, 0, 0, 0};
+  const int need = d.dim == 2 ? 4 : 6;
+  if (bc && n_bc == need) for (int k = 0; k < need; k++) b[k] = bc[k];
+  std::vector<float> aos;
+  init_state_host(d, b, aos);
+  d.cur = 0;
+  return upload_aos(d, aos.data(), -1);
+}
+
+int ff_set_moments(ff_domain* dd, const float* aos, int which) {
+  FF_CHECK_DOMAIN(dd);
+  if (!aos) return fail(FF_ERR_INVALID, "null moments");
+  if (which < -1 || which > 1) return fail(FF_ERR_INVALID, "which must be -1, 0 or 1");
+  FF_HIP(hipSetDevice(dd->device));
+  FF_HIP(hipStreamSynchronize(dd->stream));
+  return upload_aos(*dd, aos, which);
+}
+
+int ff_get_moments(ff_domain* dd, float* aos, int which) {
+  FF_CHECK_DOMAIN(dd);
+  if (!aos) return fail(FF_ERR_INVALID, "null output");
+  if (which < 0 || which > 1) return fail(FF_ERR_INVALID, "which must be 0 or 1");
+  Domain& d = *dd;
+  FF_HIP(hipSetDevice(d.device));
+  float* tmp = nullptr;
+  const size_t cnt = (size_t)d.n * d.nm;
+  int rc = dalloc(&tmp, cnt);
+  if (rc) return rc;
+  rc = launch_convert_out(d, d.mom[which == 0 ? d.cur : (d.cur ^ 1)], tmp);
+  if (!rc) {
+    hipError_t e = hipStreamSynchronize(d.stream);
+    if (e == hipSuccess) e = hipMemcpy(aos, tmp, cnt * sizeof(float), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = hip_fail(e, "download moments");
+  }
+  dfree(tmp);
+  if (!rc) rc = check_overflow(d);
+  return rc;
+}
+
+int ff_set_boundary(ff_domain* dd, int nv, const float* verts, const float* vels, int ne, const uint32_t* elems,
+                    const int32_t* elem_body, int n_bodies) {
+  FF_CHECK_DOMAIN(dd);
+  Domain& d = *dd;
+  const int D = d.dim;
+  if (nv < 0 || ne < 0) return fail(FF_ERR_INVALID, "negative sizes");
+  if (ne > 65536) return fail(FF_ERR_INVALID, "at most 65536 boundary elements (16-bit element field of the link key)");
+  if ((nv > 0 && (!verts || !vels)) || (ne > 0 && !elems)) return fail(FF_ERR_INVALID, "null boundary array");
+  if (n_bodies <= 0) n_bodies = 1;
+  for (long long k = 0; k < (long long)ne * D; k++)
+    if (elems[k] >= (uint32_t)nv) return fail(FF_ERR_INVALID, "element references a vertex out of range");
+  std::vector<int> eb(ne, 0);
+  if (elem_body)
+    for (int k = 0; k < ne; k++) {
+      if (elem_body[k] < 0 || elem_body[k] >= n_bodies) return fail(FF_ERR_INVALID, "elem_body out of range");
+      eb[k] = elem_body[k];
+    }
+  FF_HIP(hipSetDevice(d.device));
+  FF_HIP(hipStreamSynchronize(d.stream));
+  free_boundary(d);
+  d.nv = nv;
+  d.ne = ne;
+  d.n_bodies = n_bodies;
+  int rc = FF_OK;
+  const size_t vcount = (size_t)std::max(nv, 1) * D;
+  if ((rc = dalloc(&d.verts, vcount)) || (rc = dalloc(&d.vels, vcount)) ||
+      (rc = dalloc(&d.elems, (size_t)std::max(ne, 1) * D)) || (rc = dalloc(&d.elem_body, std::max(ne, 1))) ||
+      (rc = dalloc(&d.vforce, vcount)))
+    return rc;
+  if (nv) {
+    FF_HIP(hipMemcpy(d.verts, verts, (size_t)nv * D * sizeof(float), hipMemcpyHostToDevice));
+    FF_HIP(hipMemcpy(d.vels, vels, (size_t)nv * D * sizeof(float), hipMemcpyHostToDevice));
+  }
+  if (ne) {
+    FF_HIP(hipMemcpy(d.elems, elems, (size_t)ne * D * sizeof(uint32_t), hipMemcpyHostToDevice));
+    FF_HIP(hipMemcpy(d.elem_body, eb.data(), (size_t)ne * sizeof(int), hipMemcpyHostToDevice));
+  }
+  FF_HIP(hipMemset(d.vforce, 0, vcount * sizeof(float)));
+  // CSR body -> elements and vertex -> (element, end), in index order
+  std::vector<int> bptr(n_bodies + 1, 0), bidx(ne);
+  for (int k = 0; k < ne; k++) bptr[eb[k] + 1]++;
+  for (int b = 0; b < n_bodies; b++) bptr[b + 1] += bptr[b];
+  {
+    std::vector<int> fill(bptr.begin(), bptr.end() - 1);
+    for (int k = 0; k < ne; k++) bidx[fill[eb[k]]++] = k;
+  }
+  std::vector<int> vptr(nv + 1, 0), vinc((size_t)ne * D);
+  for (long long k = 0; k < (long long)ne * D; k++) vptr[elems[k] + 1]++;
+  for (int v = 0; v < nv; v++) vptr[v + 1] += vptr[v];
+  {
+    std::vector<int> fill(vptr.begin(), vptr.end() - 1);
+    for (int k = 0; k < ne; k++)
+      for (int j = 0; j < D; j++) vinc[fill[elems[(size_t)k * D + j]]++] = k * 4 + j;
+  }
+  // body reference: centroid of the body's vertices
+  std::vector<double> ref((size_t)n_bodies * D, 0.0);
+  {
+    std::vector<int> vb(nv, -1);
+    for (int k = 0; k < ne; k++)
+      for (int j = 0; j < D; j++) if (vb[elems[(size_t)k * D + j]] < 0) vb[elems[(size_t)k * D + j]] = eb[k];
+    std::vector<int> cnt(n_bodies, 0);
+    for (int v = 0; v < nv; v++)
+      if (vb[v] >= 0) {
+        cnt[vb[v]]++;
+        for (int c = 0; c < D; c++) ref[(size_t)vb[v] * D + c] += verts[(size_t)v * D + c];
+      }
+    for (int b = 0; b < n_bodies; b++)
+      if (cnt[b]) for (int c = 0; c < D; c++) ref[(size_t)b * D + c] /= cnt[b];
+    if ((rc = dalloc(&d.vert_body, std::max(nv, 1)))) return rc;
+    for (int v = 0; v < nv; v++) if (vb[v] < 0) vb[v] = 0;
+    if (nv) FF_HIP(hipMemcpy(d.vert_body, vb.data(), nv * sizeof(int), hipMemcpyHostToDevice));
+  }
+  if ((rc = dalloc(&d.body_elem_ptr, bptr.size())) || (rc = dalloc(&d.body_elem_idx, std::max(ne, 1))) ||
+      (rc = dalloc(&d.vert_inc_ptr, vptr.size())) || (rc = dalloc(&d.vert_inc_idx, std::max<size_t>(vinc.size(), 1))) ||
+      (rc = dalloc(&d.body_ref, ref.size())) || (rc = dalloc(&d.elem_acc, (size_t)std::max(ne, 1) * elem_stride(D))) ||
+      (rc = dalloc(&d.wrench, (size_t)n_bodies * wrench_width(D))))
+    return rc;
+  FF_HIP(hipMemcpy(d.body_elem_ptr, bptr.data(), bptr.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (ne) FF_HIP(hipMemcpy(d.body_elem_idx, bidx.data(), bidx.size() * sizeof(int), hipMemcpyHostToDevice));
+  FF_HIP(hipMemcpy(d.vert_inc_ptr, vptr.data(), vptr.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (!vinc.empty()) FF_HIP(hipMemcpy(d.vert_inc_idx, vinc.data(), vinc.size() * sizeof(int), hipMemcpyHostToDevice));
+  FF_HIP(hipMemcpy(d.body_ref, ref.data(), ref.size() * sizeof(double), hipMemcpyHostToDevice));
+  FF_HIP(hipMemset(d.wrench, 0, (size_t)n_bodies * wrench_width(D) * sizeof(double)));
+  // link table capacity: candidate cells of every element's box, with margin
+  unsigned long long est = 0;
+  for (int k = 0; k < ne; k++) {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int j = 0; j < D; j++)
+      for (int c = 0; c < D; c++) {
+        const double x = verts[(size_t)elems[(size_t)k * D + j] * D + c] / d.dx;
+        lo[c] = std::min(lo[c], x);
+        hi[c] = std::max(hi[c], x);
+      }
+    unsigned long long cells = 1;
+    for (int c = 0; c < D; c++) cells *= (unsigned long long)(hi[c] - lo[c] + 4.0);
+    est += cells;
+  }
+  d.ht_cap = (int)next_pow2(est * 2 + 1024);
+  const size_t K = (size_t)(d.q - 1);
+  if ((rc = dalloc(&d.ht_cell, d.ht_cap)) || (rc = dalloc(&d.ht_key, d.ht_cap * K)) ||
+      (rc = dalloc(&d.ht_f, d.ht_cap * K * D)) || (rc = dalloc(&d.ht_b, d.ht_cap * K * D)) ||
+      (rc = dalloc(&d.ht_hit, d.ht_cap * K * D)) || (rc = dalloc(&d.ht_list, d.ht_cap)) || (rc = dalloc(&d.counters, 2)))
+    return rc;
+  FF_HIP(hipMemset(d.ht_cell, 0xFF, (size_t)d.ht_cap * sizeof(int)));
+  FF_HIP(hipMemset(d.ht_key, 0xFF, (size_t)d.ht_cap * K * sizeof(uint32_t)));
+  FF_HIP(hipMemset(d.counters, 0, 2 * sizeof(int)));
+  // the memsets above run on the null stream; the domain stream is non-blocking
+  FF_HIP(hipDeviceSynchronize());
+  d.has_boundary = true;
+  return FF_OK;
+}
+
+int ff_update_boundary(ff_domain* dd, const float* verts, const float* vels) {
+  FF_CHECK_DOMAIN(dd);
+  Domain& d = *dd;
+  if (!d.has_boundary) return fail(FF_ERR_STATE, "no boundary set");
+  if (!verts || !vels) return fail(FF_ERR_INVALID, "null boundary array");
+  FF_HIP(hipSetDevice(d.device));
+  const size_t bytes = (size_t)d.nv * d.dim * sizeof(float);
+  FF_HIP(hipMemcpyAsync(d.verts, verts, bytes, hipMemcpyHostToDevice, d.stream));
+  FF_HIP(hipMemcpyAsync(d.vels, vels, bytes, hipMemcpyHostToDevice, d.stream));
+  FF_HIP(hipStreamSynchronize(d.stream));
+  d.wave = false;
+  return FF_OK;
+}
+
+int ff_set_body_ref(ff_domain* dd, const double* ref) {
+  FF_CHECK_DOMAIN(dd);
+  Domain& d = *dd;
+  if (!d.has_boundary) return fail(FF_ERR_STATE, "no boundary set");
+  if (!ref) return fail(FF_ERR_INVALID, "null reference points");
+  FF_HIP(hipSetDevice(d.device));
+  FF_HIP(hipStreamSynchronize(d.stream));
+  FF_HIP(hipMemcpy(d.body_ref, ref, (size_t)d.n_bodies * d.dim * sizeof(double), hipMemcpyHostToDevice));
+  return FF_OK;
+}
+
+int ff_set_wave(ff_domain* dd, const float* rest, const float* s, const double* params) {
+  FF_CHECK_DOMAIN(dd);
+  Domain& d = *dd;
+  if (!d.has_boundary) return fail(FF_ERR_STATE, "no boundary set");
+  if (!rest || !s || !params) return fail(FF_ERR_INVALID, "null wave array");
+  FF_HIP(hipSetDevice(d.device));
+  FF_HIP(hipStreamSynchronize(d.stream));
+  int rc;
+  if (!d.rest_verts && (rc = dalloc(&d.rest_verts, (size_t)std::max(d.nv, 1) * d.dim))) return rc;
+  if (!d.s_coord && (rc = dalloc(&d.s_coord, (size_t)std::max(d.nv, 1)))) return rc;
+  if (!d.wave_params && (rc = dalloc(&d.wave_params, (size_t)d.n_bodies * 4))) return rc;
+  FF_HIP(hipMemcpy(d.rest_verts, rest, (size_t)d.nv * d.dim * sizeof(float), hipMemcpyHostToDevice));
+  FF_HIP(hipMemcpy(d.s_coord, s, (size_t)d.nv * sizeof(float), hipMemcpyHostToDevice));
+  FF_HIP(hipMemcpy(d.wave_params, params, (size_t)d.n_bodies * 4 * sizeof(double), hipMemcpyHostToDevice));
+  d.wave = true;
+  return FF_OK;
+}
+
+int ff_step(ff_domain* dd, int nsteps) {
+  FF_CHECK_DOMAIN(dd);
+  if (nsteps < 0) return fail(FF_ERR_INVALID, "negative step count");
+  Domain& d = *dd;
+  FF_HIP(hipSetDevice(d.device));
+  for (int s = 0; s < nsteps; s++) {
+    int rc = one_step(d, d.verts, d.vels, d.elems, nullptr);
+    if (rc) return rc;
+  }
+  return FF_OK;
+}
+
+int ff_step_proxy(ff_domain* dd, const float* dv, const float* dvel, const uint32_t* de, float* dforces, int nv,
+                  int ne) {
+  FF_CHECK_DOMAIN(dd);
+  Domain& d = *dd;
+  if (!d.has_boundary) return fail(FF_ERR_STATE, "ff_set_boundary must describe the proxy topology first");
+  if (nv != d.nv || ne != d.ne) return fail(FF_ERR_INVALID, "proxy sizes differ from the boundary topology");
+  if (!dv || !dvel || !de) return fail(FF_ERR_INVALID, "null proxy pointer");
+  FF_HIP(hipSetDevice(d.device));
+  return one_step(d, dv, dvel, de, dforces);
+}
+
+int ff_get_vertex_forces(ff_domain* dd, float* out, int reset) {
+  FF_CHECK_DOMAIN(dd);
+  Domain& d = *dd;
+  if (!d.has_boundary) return fail(FF_ERR_STATE, "no boundary set");
+  if (!out) return fail(FF_ERR_INVALID, "null output");
+  FF_HIP(hipSetDevice(d.device));
+  FF_HIP(hipStreamSynchronize(d.stream));
+  const size_t bytes = (size_t)d.nv * d.dim * sizeof(float);
+  FF_HIP(hipMemcpy(out, d.vforce, bytes, hipMemcpyDeviceToHost));
+  if (reset) {
+    FF_HIP(hipMemsetAsync(d.vforce, 0, bytes, d.stream));
+    FF_HIP(hipStreamSynchronize(d.stream));
+  }
+  return check_overflow(d);
+}
+
+int ff_get_body_wrench(ff_domain* dd, double* out) {
+  FF_CHECK_DOMAIN(dd);
+  Domain& d = *dd;
+  if (!d.has_boundary) return fail(FF_ERR_STATE, "no boundary set");
+  if (!out) return fail(FF_ERR_INVALID, "null output");
+  FF_HIP(hipSetDevice(d.device));
+  FF_HIP(hipStreamSynchronize(d.stream));
+  FF_HIP(hipMemcpy(out, d.wrench, (size_t)d.n_bodies * wrench_width(d.dim) * sizeof(double), hipMemcpyDeviceToHost));
+  return check_overflow(d);
+}
+
+int ff_set_wrench_history(ff_domain* dd, int capacity) {
+  FF_CHECK_DOMAIN(dd);
+  Domain& d = *dd;
+  if (!d.has_boundary) return fail(FF_ERR_STATE, "no boundary set");
+  if (capacity < 0) return fail(FF_ERR_INVALID, "negative capacity");
+  FF_HIP(hipSetDevice(d.device));
+  FF_HIP(hipStreamSynchronize(d.stream));
+  dfree(d.wrench_hist);
+  d.hist_cap = 0;
+  d.hist_count = 0;
+  if (capacity == 0) return FF_OK;
+  int rc = dalloc(&d.wrench_hist, (size_t)capacity * d.n_bodies * wrench_width(d.dim));
+  if (rc) return rc;
+  d.hist_cap = capacity;
+  return FF_OK;
+}
+
+int ff_get_wrench_history(ff_domain* dd, double* out, int count, int* n_out) {
+  FF_CHECK_DOMAIN(dd);
+  Domain& d = *dd;
+  if (!out || !n_out || count < 0) return fail(FF_ERR_INVALID, "bad output");
+  FF_HIP(hipSetDevice(d.device));
+  FF_HIP(hipStreamSynchronize(d.stream));
+  const long long avail = std::min<long long>(d.hist_count, d.hist_cap);
+  const int n = (int)std::min<long long>(avail, count);
+  const size_t rec = (size_t)d.n_bodies * wrench_width(d.dim);
+  std::vector<double> ring((size_t)d.hist_cap * rec);
+  if (d.hist_cap) FF_HIP(hipMemcpy(ring.data(), d.wrench_hist, ring.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int k = 0; k < n; k++) {
+    const long long step = d.hist_count - n + k;
+    std::memcpy(out + (size_t)k * rec, ring.data() + (size_t)(step % d.hist_cap) * rec, rec * sizeof(double));
+  }
+  *n_out = n;
+  return FF_OK;
+}
+
+int ff_get_boundary(ff_domain* dd, float* verts, float* vels) {
+  FF_CHECK_DOMAIN(dd);
+  Domain& d = *dd;
+  if (!d.has_boundary) return fail(FF_ERR_STATE, "no boundary set");
+  FF_HIP(hipSetDevice(d.device));
+  FF_HIP(hipStreamSynchronize(d.stream));
+  const size_t bytes = (size_t)d.nv * d.dim * sizeof(float);
+  if (verts) FF_HIP(hipMemcpy(verts, d.verts, bytes, hipMemcpyDeviceToHost));
+  if (vels) FF_HIP(hipMemcpy(vels, d.vels, bytes, hipMemcpyDeviceToHost));
+  return FF_OK;
+}
+
+int ff_mass(ff_domain* dd, double* out) {
+  FF_CHECK_DOMAIN(dd);
+  if (!out) return fail(FF_ERR_INVALID, "null output");
+  Domain& d = *dd;
+  FF_HIP(hipSetDevice(d.device));
+  double* dev = nullptr;
+  int rc = dalloc(&dev, 1);
+  if (rc) return rc;
+  rc = launch_mass(d, dev);
+  if (!rc) {
+    hipError_t e = hipStreamSynchronize(d.stream);
+    if (e == hipSuccess) e = hipMemcpy(out, dev, sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = hip_fail(e, "mass");
+  }
+  dfree(dev);
+  return rc;
+}
+
+int ff_fill_solid(ff_domain* dd) {
+  FF_CHECK_DOMAIN(dd);
+  Domain& d = *dd;
+  if (d.dim != 2) return fail(FF_ERR_INVALID, "fill_solid is 2D only");
+  if (!d.has_boundary) return fail(FF_ERR_STATE, "no boundary set");
+  FF_HIP(hipSetDevice(d.device));
+  int rc = launch_fill_solid(d);
+  if (rc) return rc;
+  FF_HIP(hipStreamSynchronize(d.stream));
+  FF_HIP(hipMemcpy(d.h_flags.data(), d.flags, (size_t)d.n, hipMemcpyDeviceToHost));
+  return FF_OK;
+}
+
+int ff_sync(ff_domain* dd) {
+  FF_CHECK_DOMAIN(dd);
+  FF_HIP(hipSetDevice(dd->device));
+  FF_HIP(hipStreamSynchronize(dd->stream));
+  return check_overflow(*dd);
+}
+
+int ff_time_bulk_kernel(ff_domain* dd, int iters, double* avg_ms) {
+  FF_CHECK_DOMAIN(dd);
+  if (iters <= 0 || !avg_ms) return fail(FF_ERR_INVALID, "bad arguments");
+  Domain& d = *dd;
+  FF_HIP(hipSetDevice(d.device));
+  std::vector<hipEvent_t> ev(2 * iters);
+  for (auto& e : ev) FF_HIP(hipEventCreate(&e));
+  void* in = d.mom[d.cur];
+  void* out = d.mom[d.cur ^ 1];
+  int rc = FF_OK;
+  for (int k = 0; k < iters && !rc; k++) {
+    FF_HIP(hipEventRecord(ev[2 * k], d.stream));
+    rc = d.dim == 2 ? launch_bulk2d(d, in, out) : launch_bulk3d(d, in, out);
+    FF_HIP(hipEventRecord(ev[2 * k + 1], d.stream));
+  }
+  FF_HIP(hipStreamSynchronize(d.stream));
+  double tot = 0.0;
+  for (int k = 0; k < iters; k++) {
+    float ms = 0.f;
+    FF_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
+    tot += ms;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  *avg_ms = tot / iters;
+  return rc;
+}
+
+int ff_time_steps(ff_domain* dd, int nsteps, double* total_ms) {
+  FF_CHECK_DOMAIN(dd);
+  if (nsteps < 0 || !total_ms) return fail(FF_ERR_INVALID, "bad arguments");
+  Domain& d = *dd;
+  FF_HIP(hipSetDevice(d.device));
+  hipEvent_t a, b;
+  FF_HIP(hipEventCreate(&a));
+  FF_HIP(hipEventCreate(&b));
+  FF_HIP(hipEventRecord(a, d.stream));
+  int rc = FF_OK;
+  for (int s = 0; s < nsteps && !rc; s++) rc = one_step(d, d.verts, d.vels, d.elems, nullptr);
+  FF_HIP(hipEventRecord(b, d.stream));
+  FF_HIP(hipEventSynchronize(b));
+  float ms = 0.f;
+  FF_HIP(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *total_ms = ms;
+  if (rc) return rc;
+  return check_overflow(d);
+}
+
+}  // extern "C"

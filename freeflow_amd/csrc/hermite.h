@@ -1,0 +1,353 @@
+// hermite.h -- device-side HOME-LBM algebra (D2Q9 / D3Q27).
+//
+// The moment-encoded scheme of the reference keeps only rho, u and the
+// normalised second moment Pi per cell; populations are rebuilt on the fly
+// with a third-order Hermite expansion and relaxed in moment space.  The
+// arithmetic below is written in the same operation order as the reference
+// (src/lbm/LbmUtilsFunc.cu) and every translation unit that includes it is
+// compiled with -ffp-contract=off, so that the FP32 results match the
+// reference rounding, including its FP16 tile staging and the double
+// promotions of CalculateDistributionD2Q9ALL.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+#include <stdint.h>
+
+namespace ff {
+
+// Lattice direction ordering is fixed by the reference's tables
+// (LbmConstants.hpp:24-27, 48-50): it indexes the link keys, so it is kept.
+struct D2 {
+  static constexpr int Q = 9;
+  __host__ __device__ static constexpr int cx(int i) {
+    return i == 1 || i == 5 || i == 8 ? 1 : (i == 3 || i == 6 || i == 7 ? -1 : 0);
+  }
+  __host__ __device__ static constexpr int cy(int i) {
+    return i == 2 || i == 5 || i == 6 ? 1 : (i == 4 || i == 7 || i == 8 ? -1 : 0);
+  }
+  __host__ __device__ static constexpr int inv(int i) {
+    return i == 0 ? 0 : (i <= 4 ? ((i + 1) % 4) + 1 : ((i - 5 + 2) % 4) + 5);
+  }
+};
+
+struct D3 {
+  static constexpr int Q = 27;
+  // direction i = 0..26 in the reference order
+  __host__ __device__ static constexpr int cx(int i) {
+    constexpr int t[27] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1};
+    return t[i];
+  }
+  __host__ __device__ static constexpr int cy(int i) {
+    constexpr int t[27] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1, 1, -1};
+    return t[i];
+  }
+  __host__ __device__ static constexpr int cz(int i) {
+    constexpr int t[27] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1, 1, -1, -1, 1, 1, -1, 1, -1};
+    return t[i];
+  }
+  // opposite directions come in adjacent pairs (1,2), (3,4), ...
+  __host__ __device__ static constexpr int inv(int i) { return i == 0 ? 0 : (i & 1 ? i + 1 : i - 1); }
+};
+
+__device__ __forceinline__ bool is_wall2(uint8_t f) {  // FsiCoupling2D.cu:171-176
+  return f == 3 || f == 6 || f == 7 || f == 11 || f == 10 || f == 5;
+}
+__device__ __forceinline__ bool is_wall3(uint8_t f) {  // FsiCoupling3D.cu:209-216
+  return f == 3 || f == 5 || f == 7 || f == 11 || f == 10 || f == 8 || f == 9 || f == 6;
+}
+__device__ __forceinline__ bool is_updated(uint8_t f) { return f == 1 || f == 4; }  // Fluid, SolidDynamic
+
+// -------------------------------------------------------------------- 2D
+#define FF_W0_2 (4.0f / 9.0f)
+#define FF_W1_2 (1.0f / 9.0f)
+#define FF_W5_2 (1.0f / 36.0f)
+
+// All nine populations from (rho, u, Pi) -- CalculateDistributionD2Q9ALL.
+// Argument order (rho, ux, uy, pixx, pixy, piyy) as in the reference.
+__device__ __forceinline__ void recon2_all(float rho, float ux, float uy, float pixx, float pixy, float piyy,
+                                           float* f) {
+  const float m1x = 3.0f * rho * ux;
+  const float m1y = 3.0f * rho * uy;
+  const float m2xx = 3.0f * rho * pixx;
+  const float m2yy = 3.0f * rho * piyy;
+  const float m2xy = 9.0f * rho * pixy;
+  const float qxxy = 9.0f * (pixx * uy + 2.0f * pixy * ux - 2.0f * rho * ux * ux * uy);
+  const float qxyy = 9.0f * (piyy * ux + 2.0f * pixy * uy - 2.0f * rho * uy * uy * ux);
+  const float bulk = rho - 0.5f * (m2xx + m2yy);
+  f[0] = FF_W0_2 * bulk;
+  f[1] = (float)((double)FF_W1_2 * ((double)(bulk + m1x + 1.5f * m2xx) - 0.5 * (double)qxyy));
+  f[2] = (float)((double)FF_W1_2 * ((double)(bulk + m1y + 1.5f * m2yy) - 0.5 * (double)qxxy));
+  f[3] = (float)((double)FF_W1_2 * ((double)(bulk - m1x + 1.5f * m2xx) + 0.5 * (double)qxyy));
+  f[4] = (float)((double)FF_W1_2 * ((double)(bulk - m1y + 1.5f * m2yy) + 0.5 * (double)qxxy));
+  const float dbase = bulk + 1.5f * (m2xx + m2yy);
+  f[5] = FF_W5_2 * (dbase + (m1x + m1y) + m2xy + (qxxy + qxyy));
+  f[6] = FF_W5_2 * (dbase - m1x + m1y - m2xy + qxxy - qxyy);
+  f[7] = FF_W5_2 * (dbase - m1x - m1y + m2xy - qxxy - qxyy);
+  f[8] = FF_W5_2 * (dbase + m1x - m1y - m2xy - qxxy + qxyy);
+}
+
+// One population -- CalculateDistributionD2Q9AtIndex (all-float axis terms).
+__device__ __forceinline__ float recon2_at(float rho, float ux, float uy, float pixx, float pixy, float piyy, int i) {
+  const float m1x = 3.0f * rho * ux;
+  const float m1y = 3.0f * rho * uy;
+  const float m2xx = 3.0f * rho * pixx;
+  const float m2yy = 3.0f * rho * piyy;
+  const float m2xy = 9.0f * rho * pixy;
+  const float qxxy = 9.0f * (pixx * uy + 2.0f * pixy * ux - 2.0f * rho * ux * ux * uy);
+  const float qxyy = 9.0f * (piyy * ux + 2.0f * pixy * uy - 2.0f * rho * uy * uy * ux);
+  const float bulk = rho - 0.5f * (m2xx + m2yy);
+  const float dbase = bulk + 1.5f * (m2xx + m2yy);
+  float s, w;
+  switch (i) {
+    case 0: s = bulk; w = FF_W0_2; break;
+    case 1: s = bulk + m1x + 1.5f * m2xx - 0.5f * qxyy; w = FF_W1_2; break;
+    case 2: s = bulk + m1y + 1.5f * m2yy - 0.5f * qxxy; w = FF_W1_2; break;
+    case 3: s = bulk - m1x + 1.5f * m2xx + 0.5f * qxyy; w = FF_W1_2; break;
+    case 4: s = bulk - m1y + 1.5f * m2yy + 0.5f * qxxy; w = FF_W1_2; break;
+    case 5: s = dbase + (m1x + m1y) + m2xy + (qxxy + qxyy); w = FF_W5_2; break;
+    case 6: s = dbase - m1x + m1y - m2xy + qxxy - qxyy; w = FF_W5_2; break;
+    case 7: s = dbase - m1x - m1y + m2xy - qxxy - qxyy; w = FF_W5_2; break;
+    default: s = dbase + m1x - m1y - m2xy - qxxy + qxyy; w = FF_W5_2; break;
+  }
+  return w * s;
+}
+
+// Moments of the pulled populations, relaxation and normalisation -- the tail
+// of mrDeformableSolver2DKernel_Shrd (FsiCoupling2D.cu:226-252) with the
+// body force identically zero (:227-228).  out = [rho, ux, uy, Pxx, Pyy, Pxy].
+__device__ __forceinline__ void collide2(const float* p, float omega, float* out) {
+  const float rho = p[0] + p[1] + p[2] + p[3] + p[4] + p[5] + p[6] + p[7] + p[8];
+  const float inv = 1.0f / rho;
+  const float ux = (p[1] - p[3] + p[5] - p[6] - p[7] + p[8]) * inv;
+  const float uy = (p[2] - p[4] + p[5] + p[6] - p[7] - p[8]) * inv;
+  const float pxx = p[1] + p[3] + p[5] + p[6] + p[7] + p[8];
+  const float pyy = p[2] + p[4] + p[5] + p[6] + p[7] + p[8];
+  const float pxy = p[5] - p[6] + p[7] - p[8];
+  // LbmUtilsFuncGpu2D::Collision (LbmUtilsFunc.cu:512-543), F = 0
+  const float omw = 1.0f - omega;
+  const float pr = rho / 3.0f;
+  const float ru2 = rho * ux * ux;
+  const float rv2 = rho * uy * uy;
+  const float dev = (pxx - pyy) * 0.5f;
+  const float be = pr + 0.5f * (ru2 + rv2);
+  const float an = 0.5f * omega * (ru2 - rv2);
+  const float nxx = be + dev * omw + an;
+  const float nyy = be - dev * omw - an;
+  const float nxy = pxy * omw + (omega * rho * ux * uy);
+  out[0] = rho;
+  out[1] = ux;
+  out[2] = uy;
+  out[3] = nxx * inv - (1.0f / 3.0f);
+  out[4] = nyy * inv - (1.0f / 3.0f);
+  out[5] = nxy * inv;
+}
+
+// -------------------------------------------------------------------- 3D
+#define FF_W0_3 (8.0f / 27.0f)
+#define FF_W1_3 (2.0f / 27.0f)
+#define FF_W2_3 (1.0f / 54.0f)
+#define FF_W3_3 (1.0f / 216.0f)
+
+// CalculateDistributionD3Q27All; m = stored order
+// [rho, ux, uy, uz, pxx, pyy, pzz, pxy, pyz, pxz].
+__device__ __forceinline__ void recon3_all(const float* m, float* f) {
+  const float rho = m[0], ux = m[1], uy = m[2], uz = m[3];
+  const float pixx = m[4], piyy = m[5], pizz = m[6], pixy = m[7], piyz = m[8], pixz = m[9];
+  const float m1_x = 3.0f * rho * ux, m1_y = 3.0f * rho * uy, m1_z = 3.0f * rho * uz;
+  const float m2_xx = 3.0f * rho * pixx, m2_yy = 3.0f * rho * piyy, m2_zz = 3.0f * rho * pizz;
+  const float m2_xy = 9.0f * rho * pixy, m2_yz = 9.0f * rho * piyz, m2_xz = 9.0f * rho * pixz;
+#define FF_Q3(Paa, ub, Pab, ua) (9.0f * (rho * (Paa) * (ub) + 2.0f * rho * (Pab) * (ua) - 2.0f * rho * (ua) * (ua) * (ub)))
+  const float Q_xxy = FF_Q3(pixx, uy, pixy, ux);
+  const float Q_xyy = FF_Q3(piyy, ux, pixy, uy);
+  const float Q_xxz = FF_Q3(pixx, uz, pixz, ux);
+  const float Q_xzz = FF_Q3(pizz, ux, pixz, uz);
+  const float Q_yyz = FF_Q3(piyy, uz, piyz, uy);
+  const float Q_yzz = FF_Q3(pizz, uy, piyz, uz);
+#undef FF_Q3
+  const float Q_xyz = 27.0f * (rho * (pixy * uz + piyz * ux + pixz * uy - 2.0f * ux * uy * uz));
+  const float bulk = rho - 0.5f * (m2_xx + m2_yy + m2_zz);
+  f[0] = FF_W0_3 * bulk;
+  const float abx = bulk + 1.5f * m2_xx;
+  f[1] = FF_W1_3 * (abx + m1_x - 0.5f * (Q_xyy + Q_xzz));
+  f[2] = FF_W1_3 * (abx - m1_x + 0.5f * (Q_xyy + Q_xzz));
+  const float aby = bulk + 1.5f * m2_yy;
+  f[3] = FF_W1_3 * (aby + m1_y - 0.5f * (Q_xxy + Q_yzz));
+  f[4] = FF_W1_3 * (aby - m1_y + 0.5f * (Q_xxy + Q_yzz));
+  const float abz = bulk + 1.5f * m2_zz;
+  f[5] = FF_W1_3 * (abz + m1_z - 0.5f * (Q_xxz + Q_yyz));
+  f[6] = FF_W1_3 * (abz - m1_z + 0.5f * (Q_xxz + Q_yyz));
+  const float ebxy = rho + m2_xx + m2_yy - 0.5f * m2_zz;
+  const float eqxy = Q_xxy + Q_xyy;
+  const float eoxy = 0.5f * (Q_xzz + Q_yzz);
+  f[7] = FF_W2_3 * (ebxy + (m1_x + m1_y) + m2_xy + eqxy - eoxy);
+  f[8] = FF_W2_3 * (ebxy - (m1_x + m1_y) + m2_xy - eqxy + eoxy);
+  f[13] = FF_W2_3 * (ebxy + (m1_x - m1_y) - m2_xy - (Q_xxy - Q_xyy) - 0.5f * (Q_xzz - Q_yzz));
+  f[14] = FF_W2_3 * (ebxy - (m1_x - m1_y) - m2_xy + (Q_xxy - Q_xyy) + 0.5f * (Q_xzz - Q_yzz));
+  const float ebxz = rho + m2_xx + m2_zz - 0.5f * m2_yy;
+  const float eqxz = Q_xxz + Q_xzz;
+  const float eoxz = 0.5f * (Q_xyy + Q_yyz);
+  f[9] = FF_W2_3 * (ebxz + (m1_x + m1_z) + m2_xz + eqxz - eoxz);
+  f[10] = FF_W2_3 * (ebxz - (m1_x + m1_z) + m2_xz - eqxz + eoxz);
+  f[15] = FF_W2_3 * (ebxz + (m1_x - m1_z) - m2_xz - (Q_xxz - Q_xzz) - 0.5f * (Q_xyy - Q_yyz));
+  f[16] = FF_W2_3 * (ebxz - (m1_x - m1_z) - m2_xz + (Q_xxz - Q_xzz) + 0.5f * (Q_xyy - Q_yyz));
+  const float ebyz = rho + m2_yy + m2_zz - 0.5f * m2_xx;
+  const float eqyz = Q_yyz + Q_yzz;
+  const float eoyz = 0.5f * (Q_xxy + Q_xxz);
+  f[11] = FF_W2_3 * (ebyz + (m1_y + m1_z) + m2_yz + eqyz - eoyz);
+  f[12] = FF_W2_3 * (ebyz - (m1_y + m1_z) + m2_yz - eqyz + eoyz);
+  f[17] = FF_W2_3 * (ebyz + (m1_y - m1_z) - m2_yz - (Q_yyz - Q_yzz) - 0.5f * (Q_xxy - Q_xxz));
+  f[18] = FF_W2_3 * (ebyz - (m1_y - m1_z) - m2_yz + (Q_yyz - Q_yzz) + 0.5f * (Q_xxy - Q_xxz));
+  const float cM1 = m1_x + m1_y + m1_z;
+  const float cS = m2_xx + m2_yy + m2_zz;
+  const float cO = m2_xy + m2_yz + m2_xz;
+  const float cQ = Q_xxy + Q_xxz + Q_xyy + Q_yyz + Q_xzz + Q_yzz;
+  const float cc = rho + cS;
+  f[19] = FF_W3_3 * (cc + cM1 + cO + cQ + Q_xyz);
+  f[20] = FF_W3_3 * (cc - cM1 + cO - cQ - Q_xyz);
+  f[21] = FF_W3_3 * (cc + (m1_x + m1_y - m1_z) + (m2_xy - m2_yz - m2_xz) + (Q_xxy - Q_xxz + Q_xyy - Q_yyz + Q_xzz + Q_yzz) - Q_xyz);
+  f[22] = FF_W3_3 * (cc - (m1_x + m1_y - m1_z) + (m2_xy - m2_yz - m2_xz) - (Q_xxy - Q_xxz + Q_xyy - Q_yyz + Q_xzz + Q_yzz) + Q_xyz);
+  f[23] = FF_W3_3 * (cc + (m1_x - m1_y + m1_z) + (-m2_xy - m2_yz + m2_xz) + (-Q_xxy + Q_xxz + Q_xyy + Q_yyz + Q_xzz - Q_yzz) - Q_xyz);
+  f[24] = FF_W3_3 * (cc - (m1_x - m1_y + m1_z) + (-m2_xy - m2_yz + m2_xz) - (-Q_xxy + Q_xxz + Q_xyy + Q_yyz + Q_xzz - Q_yzz) + Q_xyz);
+  f[25] = FF_W3_3 * (cc + (-m1_x + m1_y + m1_z) + (-m2_xy + m2_yz - m2_xz) + (Q_xxy + Q_xxz - Q_xyy + Q_yyz - Q_xzz + Q_yzz) - Q_xyz);
+  f[26] = FF_W3_3 * (cc - (-m1_x + m1_y + m1_z) + (-m2_xy + m2_yz - m2_xz) - (Q_xxy + Q_xxz - Q_xyy + Q_yyz - Q_xzz + Q_yzz) + Q_xyz);
+}
+
+// CalculateDistributionD3Q27AtIndex (LbmUtilsFunc.cu:11-210).
+__device__ __forceinline__ float recon3_at(const float* m, int i) {
+  const float rho = m[0], ux = m[1], uy = m[2], uz = m[3];
+  const float pixx = m[4], piyy = m[5], pizz = m[6], pixy = m[7], piyz = m[8], pixz = m[9];
+  const float m1_x = 3.0f * rho * ux, m1_y = 3.0f * rho * uy, m1_z = 3.0f * rho * uz;
+  const float m2_xx = 3.0f * rho * pixx, m2_yy = 3.0f * rho * piyy, m2_zz = 3.0f * rho * pizz;
+  const float m2_xy = 9.0f * rho * pixy, m2_yz = 9.0f * rho * piyz, m2_xz = 9.0f * rho * pixz;
+#define FF_Q3(Paa, ub, Pab, ua) (9.0f * (rho * (Paa) * (ub) + 2.0f * rho * (Pab) * (ua) - 2.0f * rho * (ua) * (ua) * (ub)))
+  const float Q_xxy = FF_Q3(pixx, uy, pixy, ux);
+  const float Q_xyy = FF_Q3(piyy, ux, pixy, uy);
+  const float Q_xxz = FF_Q3(pixx, uz, pixz, ux);
+  const float Q_xzz = FF_Q3(pizz, ux, pixz, uz);
+  const float Q_yyz = FF_Q3(piyy, uz, piyz, uy);
+  const float Q_yzz = FF_Q3(pizz, uy, piyz, uz);
+#undef FF_Q3
+  const float Q_xyz = 27.0f * (rho * (pixy * uz + piyz * ux + pixz * uy - 2.0f * ux * uy * uz));
+  const float b1 = rho - 0.5f * (m2_xx + m2_yy + m2_zz);
+  const float b2 = rho + (m2_xx + m2_yy + m2_zz);
+  const float h = 0.5f, t = 1.5f;
+  float s, w;
+  switch (i) {
+    case 0: s = b1; w = FF_W0_3; break;
+    case 1: s = b1 + m1_x + t * m2_xx - h * (Q_xyy + Q_xzz); w = FF_W1_3; break;
+    case 2: s = b1 - m1_x + t * m2_xx + h * (Q_xyy + Q_xzz); w = FF_W1_3; break;
+    case 3: s = b1 + m1_y + t * m2_yy - h * (Q_xxy + Q_yzz); w = FF_W1_3; break;
+    case 4: s = b1 - m1_y + t * m2_yy + h * (Q_xxy + Q_yzz); w = FF_W1_3; break;
+    case 5: s = b1 + m1_z + t * m2_zz - h * (Q_xxz + Q_yyz); w = FF_W1_3; break;
+    case 6: s = b1 - m1_z + t * m2_zz + h * (Q_xxz + Q_yyz); w = FF_W1_3; break;
+    case 7: s = rho + (m1_x + m1_y) + (m2_xx + m2_yy - h * m2_zz) + m2_xy + (Q_xxy + Q_xyy) - h * (Q_xzz + Q_yzz); w = FF_W2_3; break;
+    case 8: s = rho - (m1_x + m1_y) + (m2_xx + m2_yy - h * m2_zz) + m2_xy - (Q_xxy + Q_xyy) + h * (Q_xzz + Q_yzz); w = FF_W2_3; break;
+    case 9: s = rho + (m1_x + m1_z) + (m2_xx + m2_zz - h * m2_yy) + m2_xz + (Q_xxz + Q_xzz) - h * (Q_xyy + Q_yyz); w = FF_W2_3; break;
+    case 10: s = rho - (m1_x + m1_z) + (m2_xx + m2_zz - h * m2_yy) + m2_xz - (Q_xxz + Q_xzz) + h * (Q_xyy + Q_yyz); w = FF_W2_3; break;
+    case 11: s = rho + (m1_y + m1_z) + (m2_yy + m2_zz - h * m2_xx) + m2_yz + (Q_yyz + Q_yzz) - h * (Q_xxy + Q_xxz); w = FF_W2_3; break;
+    case 12: s = rho - (m1_y + m1_z) + (m2_yy + m2_zz - h * m2_xx) + m2_yz - (Q_yyz + Q_yzz) + h * (Q_xxy + Q_xxz); w = FF_W2_3; break;
+    case 13: s = rho + (m1_x - m1_y) + (m2_xx + m2_yy - h * m2_zz) - m2_xy - (Q_xxy - Q_xyy) - h * (Q_xzz - Q_yzz); w = FF_W2_3; break;
+    case 14: s = rho - (m1_x - m1_y) + (m2_xx + m2_yy - h * m2_zz) - m2_xy + (Q_xxy - Q_xyy) + h * (Q_xzz - Q_yzz); w = FF_W2_3; break;
+    case 15: s = rho + (m1_x - m1_z) + (m2_xx + m2_zz - h * m2_yy) - m2_xz - (Q_xxz - Q_xzz) - h * (Q_xyy - Q_yyz); w = FF_W2_3; break;
+    case 16: s = rho - (m1_x - m1_z) + (m2_xx + m2_zz - h * m2_yy) - m2_xz + (Q_xxz - Q_xzz) + h * (Q_xyy - Q_yyz); w = FF_W2_3; break;
+    case 17: s = rho + (m1_y - m1_z) + (m2_yy + m2_zz - h * m2_xx) - m2_yz - (Q_yyz - Q_yzz) - h * (Q_xxy - Q_xxz); w = FF_W2_3; break;
+    case 18: s = rho - (m1_y - m1_z) + (m2_yy + m2_zz - h * m2_xx) - m2_yz + (Q_yyz - Q_yzz) + h * (Q_xxy - Q_xxz); w = FF_W2_3; break;
+    case 19: s = b2 + (m1_x + m1_y + m1_z) + (m2_xy + m2_yz + m2_xz) + (Q_xxy + Q_xxz + Q_xyy + Q_yyz + Q_xzz + Q_yzz) + Q_xyz; w = FF_W3_3; break;
+    case 20: s = b2 - (m1_x + m1_y + m1_z) + (m2_xy + m2_yz + m2_xz) - (Q_xxy + Q_xxz + Q_xyy + Q_yyz + Q_xzz + Q_yzz) - Q_xyz; w = FF_W3_3; break;
+    case 21: s = b2 + (m1_x + m1_y - m1_z) + (m2_xy - m2_yz - m2_xz) + (Q_xxy - Q_xxz + Q_xyy - Q_yyz + Q_xzz + Q_yzz) - Q_xyz; w = FF_W3_3; break;
+    case 22: s = b2 - (m1_x + m1_y - m1_z) + (m2_xy - m2_yz - m2_xz) - (Q_xxy - Q_xxz + Q_xyy - Q_yyz + Q_xzz + Q_yzz) + Q_xyz; w = FF_W3_3; break;
+    case 23: s = b2 + (m1_x - m1_y + m1_z) + (-m2_xy - m2_yz + m2_xz) + (-Q_xxy + Q_xxz + Q_xyy + Q_yyz + Q_xzz - Q_yzz) - Q_xyz; w = FF_W3_3; break;
+    case 24: s = b2 - (m1_x - m1_y + m1_z) + (-m2_xy - m2_yz + m2_xz) - (-Q_xxy + Q_xxz + Q_xyy + Q_yyz + Q_xzz - Q_yzz) + Q_xyz; w = FF_W3_3; break;
+    case 25: s = b2 + (-m1_x + m1_y + m1_z) + (-m2_xy + m2_yz - m2_xz) + (Q_xxy + Q_xxz - Q_xyy + Q_yyz - Q_xzz + Q_yzz) - Q_xyz; w = FF_W3_3; break;
+    default: s = b2 - (-m1_x + m1_y + m1_z) + (-m2_xy + m2_yz - m2_xz) - (Q_xxy + Q_xxz - Q_xyy + Q_yyz - Q_xzz + Q_yzz) + Q_xyz; w = FF_W3_3; break;
+  }
+  return w * s;
+}
+
+// Moments of the pulled populations + LbmUtilsFuncGpu3D::Collision
+// (LbmUtilsFunc.cu:337-414) + normalisation (FsiCoupling3D.cu:291-320), F = 0.
+__device__ __forceinline__ void collide3(const float* p, float omega, float* out) {
+  float rho = p[0];
+#pragma unroll
+  for (int i = 1; i < 27; i++) rho = rho + p[i];
+  const float inv = 1.0f / rho;
+  const float sxp = p[1] + p[7] + p[9] + p[13] + p[15] + p[19] + p[21] + p[23] + p[26];
+  const float sxm = p[2] + p[8] + p[10] + p[14] + p[16] + p[20] + p[22] + p[24] + p[25];
+  const float syp = p[3] + p[7] + p[11] + p[14] + p[17] + p[19] + p[21] + p[24] + p[25];
+  const float sym = p[4] + p[8] + p[12] + p[13] + p[18] + p[20] + p[22] + p[23] + p[26];
+  const float szp = p[5] + p[9] + p[11] + p[16] + p[18] + p[19] + p[22] + p[23] + p[25];
+  const float szm = p[6] + p[10] + p[12] + p[15] + p[17] + p[20] + p[21] + p[24] + p[26];
+  const float ux = (sxp - sxm) * inv;
+  const float uy = (syp - sym) * inv;
+  const float uz = (szp - szm) * inv;
+  float pixx = p[1] + p[2] + p[7] + p[8] + p[9] + p[10] + p[13] + p[14] + p[15] + p[16] + p[19] + p[20] + p[21] + p[22] + p[23] + p[24] + p[25] + p[26];
+  float piyy = p[3] + p[4] + p[7] + p[8] + p[11] + p[12] + p[13] + p[14] + p[17] + p[18] + p[19] + p[20] + p[21] + p[22] + p[23] + p[24] + p[25] + p[26];
+  float pizz = p[5] + p[6] + p[9] + p[10] + p[11] + p[12] + p[15] + p[16] + p[17] + p[18] + p[19] + p[20] + p[21] + p[22] + p[23] + p[24] + p[25] + p[26];
+  float pixy = (p[7] + p[8] + p[19] + p[20] + p[21] + p[22]) - (p[13] + p[14] + p[23] + p[24] + p[25] + p[26]);
+  float piyz = (p[11] + p[12] + p[19] + p[20] + p[25] + p[26]) - (p[17] + p[18] + p[21] + p[22] + p[23] + p[24]);
+  float pixz = (p[9] + p[10] + p[19] + p[20] + p[23] + p[24]) - (p[15] + p[16] + p[21] + p[22] + p[25] + p[26]);
+  const float third = 0.33333333f;
+  const float omw = 1.0f - omega;
+  const float r3 = rho * third;
+  const float rux = rho * ux * ux, ruy = rho * uy * uy, ruz = rho * uz * uz;
+  const float wr = omega * rho;
+  pixy = omw * pixy + wr * (ux * uy);
+  pixz = omw * pixz + wr * (ux * uz);
+  piyz = omw * piyz + wr * (uy * uz);
+  const float dxx = (2.0f * pixx - piyy - pizz) * third;
+  const float dyy = (2.0f * piyy - pixx - pizz) * third;
+  const float dzz = (2.0f * pizz - pixx - piyy) * third;
+  const float en = (rux + ruy + ruz) * third;
+  const float wt = omega * third;
+  pixx = r3 + en + omw * dxx + wt * (2.0f * rux - ruy - ruz);
+  piyy = r3 + en + omw * dyy + wt * (2.0f * ruy - rux - ruz);
+  pizz = r3 + en + omw * dzz + wt * (2.0f * ruz - rux - ruy);
+  out[0] = rho;
+  out[1] = ux;
+  out[2] = uy;
+  out[3] = uz;
+  out[4] = pixx * inv - (1.0f / 3.0f);
+  out[5] = piyy * inv - (1.0f / 3.0f);
+  out[6] = pizz * inv - (1.0f / 3.0f);
+  out[7] = pixy * inv;
+  out[8] = piyz * inv;
+  out[9] = pixz * inv;
+}
+
+// -------------------------------------------------------------------- storage
+// 16-bit storage: slot 0 holds rho-1, the velocity slots u, the rest Pi, each
+// scaled by a power of two (exact) before rounding.
+template <int DIM>
+__device__ __forceinline__ float slot_scale(int k) { return k == 0 ? 256.0f : (k <= DIM ? 16.0f : 256.0f); }
+template <int DIM>
+__device__ __forceinline__ float slot_inv_scale(int k) { return k == 0 ? (1.0f / 256.0f) : (k <= DIM ? (1.0f / 16.0f) : (1.0f / 256.0f)); }
+
+template <typename S> struct Store;
+template <> struct Store<float> {
+  template <int DIM> __device__ static __forceinline__ float load(const float* p, int k) { return *p; }
+  template <int DIM> __device__ static __forceinline__ void store(float* p, int k, float v) { *p = v; }
+};
+template <> struct Store<__half> {
+  template <int DIM> __device__ static __forceinline__ float load(const __half* p, int k) {
+    float v = __half2float(*p) * slot_inv_scale<DIM>(k);
+    return k == 0 ? v + 1.0f : v;
+  }
+  template <int DIM> __device__ static __forceinline__ void store(__half* p, int k, float v) {
+    float e = (k == 0 ? v - 1.0f : v) * slot_scale<DIM>(k);
+    *p = __float2half_rn(e);
+  }
+};
+template <> struct Store<__hip_bfloat16> {
+  template <int DIM> __device__ static __forceinline__ float load(const __hip_bfloat16* p, int k) {
+    float v = __bfloat162float(*p) * slot_inv_scale<DIM>(k);
+    return k == 0 ? v + 1.0f : v;
+  }
+  template <int DIM> __device__ static __forceinline__ void store(__hip_bfloat16* p, int k, float v) {
+    float e = (k == 0 ? v - 1.0f : v) * slot_scale<DIM>(k);
+    *p = __float2bfloat16(e);
+  }
+};
+
+}  // namespace ff

@@ -1,0 +1,183 @@
+/*
+ * freeflow_lbm.h -- C ABI of the MI355X (gfx950) HOME-LBM fluid step with
+ * moving-boundary coupling.
+ *
+ * This is the seam the reference's host code calls into.  In FreeFlow the
+ * fluid side is entered through
+ *   fsi::lbm::fsi_coupling::solveLbmAndFsiStep2D(LbmFlowField2D&,
+ *       const SolidGeometryProxy_Device<2>&, const SimulationParameters2D&,
+ *       cudaStream_t)                       (include/FSI_Simulator/core/FsiCoupling.hpp:21-25)
+ *   ... solveLbmAndFsiStep3D(...)          (FsiCoupling.hpp:34-38)
+ *   fillSolidFlags2D(...)                   (FsiCoupling.hpp:27-31)
+ * with the flow field owning two moment buffers, the node flags and a dense
+ * hit table (include/FSI_Simulator/lbm/LbmFlowField2D.hpp:51-59).  Here the
+ * flow field is an opaque ff_domain owned by the library; every function
+ * takes plain pointers and sizes, returns an int status (FF_OK = 0) and
+ * records a message retrievable with ff_last_error().
+ *
+ * Threading: calls on one domain must be serialised by the caller; different
+ * domains may be driven from different threads.  ff_step* calls are
+ * asynchronous on the domain's HIP stream; getters synchronise.
+ *
+ * Moments use the reference's per-cell order
+ *   2D: [rho, ux, uy, Pi_xx, Pi_yy, Pi_xy]
+ *   3D: [rho, ux, uy, uz, Pi_xx, Pi_yy, Pi_zz, Pi_xy, Pi_yz, Pi_xz]
+ * (src/core/FsiCoupling2D.cu:247-252, FsiCoupling3D.cu:310-320) and the
+ * reference's cell order y*nx+x / (z*ny+y)*nx+x; on the device they are kept
+ * as structure-of-arrays planes.
+ */
+#ifndef FREEFLOW_LBM_H
+#define FREEFLOW_LBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define FF_OK 0
+#define FF_ERR_INVALID 1  /* bad argument or size */
+#define FF_ERR_HIP 2      /* HIP runtime error (message has the HIP string) */
+#define FF_ERR_NOMEM 3    /* device allocation failed */
+#define FF_ERR_CAPACITY 4 /* a fixed-capacity device table overflowed */
+#define FF_ERR_STATE 5    /* call made in the wrong state (e.g. no boundary set) */
+#define FF_ERR_NODEVICE 6 /* no usable gfx950 device */
+
+/* moment storage formats */
+#define FF_STORAGE_F32 0
+#define FF_STORAGE_F16 1  /* binary16, rho-1 and Pi scaled by 2^8, u by 2^4 */
+#define FF_STORAGE_BF16 2 /* bfloat16, same encoding */
+
+/* option bits */
+#define FF_OPT_STAGE_FP16 0x1u /* 2D: round pulled populations to binary16 as the
+                                  reference's half LDS tile does
+                                  (src/core/FsiCoupling2D.cu:89,120-128) */
+#define FF_OPT_PERIODIC 0x2u   /* pull across the domain edges periodically instead of
+                                  the reference's clamped halo (FsiCoupling2D.cu:104-105) */
+
+/* Node flag values: the ordinals of fsi::lbm::LbmNodeFlag
+   (include/FSI_Simulator/lbm/LbmDataTypes.cuh:17-56). */
+#define FF_FLAG_INVALID 0
+#define FF_FLAG_FLUID 1
+#define FF_FLAG_SOLID 3
+#define FF_FLAG_SOLID_DYNAMIC 4
+#define FF_FLAG_WALL 5
+
+typedef struct ff_domain_desc {
+  int dim;               /* 2 (D2Q9) or 3 (D3Q27) */
+  int nx, ny, nz;        /* nz ignored for dim 2 */
+  double viscosity;      /* lattice viscosity: SimulationParameters*::fluid_viscosity */
+  double dx;             /* fluid_dx (cl) */
+  double dt;             /* dt (ct) */
+  double fluid_density;  /* fluid_density, used for cf = rho dx^D dx / dt^2
+                            (include/FSI_Simulator/utils/Config.hpp:96-100, 154-158) */
+  int storage;           /* FF_STORAGE_* */
+  unsigned options;      /* FF_OPT_* */
+  int device;            /* HIP device ordinal */
+} ff_domain_desc;
+
+typedef struct ff_domain ff_domain;
+
+/* Replaces LbmFlowField2D/3D construction (src/lbm/LbmFlowField2D.cpp:17-40,
+   LbmFlowField3D.cpp:18-42).  All cells start as Fluid with zero moments. */
+int ff_domain_create(const ff_domain_desc* desc, ff_domain** out);
+int ff_domain_destroy(ff_domain* d);
+
+/* Node flags, one LbmNodeFlag byte per cell (the flag upload of
+   LbmInitializer2D.cu:106-128). */
+int ff_set_flags(ff_domain* d, const uint8_t* flags);
+int ff_get_flags(ff_domain* d, uint8_t* flags);
+
+/* Equilibrium initial state from the flags and the inlet velocities
+   (initialize_state_kernel_2d/3d, LbmInitializer2D.cu:32-95,
+   LbmInitializer3D.cu:33-113): writes both ping-pong buffers.  n_bc is 4
+   (2D) or 6 (3D); any other count means zeros, as in the reference. */
+int ff_init_state(ff_domain* d, const float* bc_velocities, int n_bc);
+
+/* Host AoS moments in/out.  which: 0 = current, 1 = the other ping-pong
+   buffer, -1 = both (set only).  Outlet cells keep a history per buffer, as
+   in the reference, so a faithful restart sets both buffers. */
+int ff_set_moments(ff_domain* d, const float* aos, int which);
+int ff_get_moments(ff_domain* d, float* aos, int which);
+
+/* Boundary geometry: the data of SolidGeometryProxy_Device<2/3>
+   (include/FSI_Simulator/core/SolidGeometryProxy_Device.cuh:29-73):
+   nv vertices (dim floats each), their velocities, ne elements (segments:
+   2 vertex ids; triangles: 3).  elem_body (may be NULL) assigns each
+   element to one of n_bodies bodies for the per-body wrench.  At most
+   65536 elements (the reference's 16-bit element field of the hit key,
+   SolidGeometryProxy_Device.cuh:79-85).  Copies the arrays. */
+int ff_set_boundary(ff_domain* d, int nv, const float* verts, const float* vels, int ne, const uint32_t* elems,
+                    const int32_t* elem_body, int n_bodies);
+/* New vertex positions/velocities for the next step (same topology). */
+int ff_update_boundary(ff_domain* d, const float* verts, const float* vels);
+/* Reference point per body for the torque (default: the vertex centroid at
+   ff_set_boundary time). */
+int ff_set_body_ref(ff_domain* d, const double* ref);
+
+/* Prescribed travelling-wave kinematics evaluated on the device each step:
+   vertex k moves laterally (along y in 2D) by
+     A_b * s_k * sin(2 pi (s_k * L_b / lambda_b - f_b t) + phi_b)
+   from its rest position, where s_k in [0,1] is the body-axis coordinate
+   given per vertex and t = step * dt.  params holds 4 doubles per body:
+   {A (m), f (Hz), phi (rad), L/lambda}.  Replaces ff_update_boundary. */
+int ff_set_wave(ff_domain* d, const float* rest_verts, const float* s_coord, const double* params);
+
+/* One or more fluid steps with the library-held boundary: the work of
+   solveLbmAndFsiStep2D/3D (FsiCoupling2D.cu:358-432, FsiCoupling3D.cu:424-484):
+   link detection, stream + collide with moving-wall bounce-back and
+   momentum-exchange forces, ping-pong swap, outlet update. */
+int ff_step(ff_domain* d, int nsteps);
+
+/* Direct replacement of the proxy call: device pointers with the layout of
+   SolidGeometryProxy_Device (interleaved float vertices/velocities, uint32
+   elements).  The step's per-vertex forces are added to d_forces, as
+   addForceAt does (SolidGeometryProxy_Device.cuh:184-194, :401-421).  The
+   pointers must stay valid until the step has executed on the stream. */
+int ff_step_proxy(ff_domain* d, const float* d_vertices, const float* d_velocities, const uint32_t* d_elements,
+                  float* d_forces, int num_vertices, int num_elements);
+
+/* Per-vertex hydrodynamic forces accumulated over steps (nv*dim floats);
+   reset != 0 zeroes the accumulator after reading. */
+int ff_get_vertex_forces(ff_domain* d, float* out, int reset);
+/* Per-body wrench of the most recent step, FP64: 2D {Fx, Fy, Tz},
+   3D {Fx, Fy, Fz, Tx, Ty, Tz} per body, torque about the body reference. */
+int ff_get_body_wrench(ff_domain* d, double* out);
+/* Per-step wrench history of the last ff_step call (ring of up to
+   `capacity` steps, set by ff_set_wrench_history); writes min(count, steps)
+   records, most recent last. Returns the number written in *n_out. */
+int ff_set_wrench_history(ff_domain* d, int capacity);
+int ff_get_wrench_history(ff_domain* d, double* out, int count, int* n_out);
+/* Current vertex positions/velocities held by the library (after wave
+   kinematics this is the state used by the most recent step). */
+int ff_get_boundary(ff_domain* d, float* verts, float* vels);
+
+/* Global FP64 sum of rho over Fluid/SolidDynamic cells of the current buffer. */
+int ff_mass(ff_domain* d, double* out);
+
+/* fillSolidFlags2D (FsiCoupling2D.cu:318-354, 434-464): Fluid/SolidDynamic
+   cells become Fluid, then Fluid cells whose centre lies inside the boundary
+   polygon become SolidDynamic.  2D only. */
+int ff_fill_solid(ff_domain* d);
+
+int ff_sync(ff_domain* d);
+/* Average duration (ms) of the stream-collide kernel over `iters` launches on
+   the domain's stream, timed with HIP events around each launch; does not
+   advance the simulation state. */
+int ff_time_bulk_kernel(ff_domain* d, int iters, double* avg_ms);
+/* Wall time (ms) of `nsteps` ff_step-equivalent steps measured with HIP
+   events on the domain stream (state advances). */
+int ff_time_steps(ff_domain* d, int nsteps, double* total_ms);
+
+const char* ff_last_error(void);
+int ff_device_count(int* n);
+/* Library build identification string. */
+const char* ff_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FREEFLOW_LBM_H */

@@ -650,7 +650,8 @@ static inline bool is_wall3(uint8_t f) {  // FsiCoupling3D.cu:209-216
          f == WallLeft;
 }
 
-// Storage quantisation shared with the product's 16-bit storage modes:
+// Storage quantisation matching the product's 16-bit storage modes (applied
+// when a moment is stored: by set_moments, the step and the outlet update):
 // slot 0 holds rho-1, slots 1..D hold u, the rest hold Pi; each is scaled by a
 // power of two before rounding to the 16-bit format.
 static const double kScaleRho = 256.0, kScaleU = 16.0, kScalePi = 256.0;
@@ -798,12 +799,9 @@ static void step2d(Domain& d) {
       piyy = (R)((double)(piyy * invRho) - 1.0 * (double)cs2);
       pixy = pixy * invRho;
       double* o = &nxt[curind * 6];
-      o[0] = (double)rhoVar;
-      o[1] = (double)(ux + FX * invRho / R(2));
-      o[2] = (double)(uy + FY * invRho / R(2));
-      o[3] = (double)pixx;
-      o[4] = (double)piyy;
-      o[5] = (double)pixy;
+      const double ov[6] = {(double)rhoVar, (double)(ux + FX * invRho / R(2)), (double)(uy + FY * invRho / R(2)),
+                            (double)pixx, (double)piyy, (double)pixy};
+      for (int k = 0; k < 6; k++) o[k] = quantize(ov[k], k, 2, d.storage);
     }
   }
   // force accumulation, sequential (row, x, direction) order
@@ -837,12 +835,8 @@ static void step2d(Domain& d) {
       if (d.flags[nb] != Fluid) continue;
       R up = (R)mm[(size_t)nb * 6 + comp];
       if ((sign > 0 && up > 0) || (sign < 0 && up < 0)) mm[c * 6 + comp] = (double)up;
-      mm[c * 6 + pcomp] = (double)(up * up);
+      mm[c * 6 + pcomp] = quantize((double)(up * up), pcomp, 2, d.storage);
     }
-  if (d.storage) {
-    for (size_t c = 0; c < N; c++)
-      for (int k = 0; k < 6; k++) mm[c * 6 + k] = quantize(mm[c * 6 + k], k, 2, d.storage);
-  }
   d.steps_done++;
 }
 
@@ -961,16 +955,11 @@ static void step3d(Domain& d) {
         collide3d<R>(rhoVar, ux, uy, uz, FX, FY, FZ, Omega, pixx, piyy, pizz, pixy, piyz, pixz);
         R cs2 = std::is_same<R, float>::value ? (R)(1.0f / 3.0f) : W3<R>::cs2();
         double* o = &nxt[curind * 10];
-        o[0] = (double)rhoVar;
-        o[1] = (double)(ux + FX * invRho / R(2));
-        o[2] = (double)(uy + FY * invRho / R(2));
-        o[3] = (double)(uz + FZ * invRho / R(2));
-        o[4] = (double)(pixx * invRho - cs2);
-        o[5] = (double)(piyy * invRho - cs2);
-        o[6] = (double)(pizz * invRho - cs2);
-        o[7] = (double)(pixy * invRho);
-        o[8] = (double)(piyz * invRho);
-        o[9] = (double)(pixz * invRho);
+        const double ov[10] = {(double)rhoVar, (double)(ux + FX * invRho / R(2)), (double)(uy + FY * invRho / R(2)),
+                               (double)(uz + FZ * invRho / R(2)), (double)(pixx * invRho - cs2),
+                               (double)(piyy * invRho - cs2), (double)(pizz * invRho - cs2), (double)(pixy * invRho),
+                               (double)(piyz * invRho), (double)(pixz * invRho)};
+        for (int k = 0; k < 10; k++) o[k] = quantize(ov[k], k, 3, d.storage);
       }
   for (int z = 0; z < nz; z++)
     for (const LinkF& lf : links_per_plane[z]) {
@@ -1000,12 +989,8 @@ static void step3d(Domain& d) {
         if (d.flags[nb] != Fluid) continue;
         R up = (R)mm[(size_t)nb * 10 + comp];
         if ((sign > 0 && up > 0) || (sign < 0 && up < 0)) mm[c * 10 + comp] = (double)up;
-        mm[c * 10 + pcomp] = (double)(up * up);
+        mm[c * 10 + pcomp] = quantize((double)(up * up), pcomp, 3, d.storage);
       }
-  if (d.storage) {
-    for (size_t c = 0; c < N; c++)
-      for (int k = 0; k < 10; k++) mm[c * 10 + k] = quantize(mm[c * 10 + k], k, 3, d.storage);
-  }
   d.steps_done++;
 }
 
@@ -1123,8 +1108,10 @@ void orc_set_moments(void* h, const double* aos, int which) {
   for (int b = 0; b < 2; b++) {
     int target = b == 0 ? d->cur : (d->cur ^ 1);
     if (which >= 0 && which != b) continue;
-    for (size_t i = 0; i < d->n * d->nm; i++)
-      d->buf[target][i] = d->fp64 ? aos[i] : (double)(float)aos[i];
+    for (size_t i = 0; i < d->n * d->nm; i++) {
+      double v = d->fp64 ? aos[i] : (double)(float)aos[i];
+      d->buf[target][i] = quantize(v, (int)(i % d->nm), d->dim, d->storage);
+    }
   }
 }
 

@@ -1,0 +1,248 @@
+"""GPU parity of the D2Q9 path against the CPU oracle, through the C ABI.
+
+The 2D kernels are written in the reference's operation order and compiled
+without FMA contraction, so the bar is bit-exact moments (FP32 and the 16-bit
+storage modes, with and without the reference's FP16 tile staging).  Forces
+are summed in a different (deterministic) order than the oracle, so they are
+compared to a relative tolerance of 1e-5.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from freeflow_amd import capi, workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def pair(nx, ny, flags, m, stage=True, periodic=False, storage="f32", nu=0.005, dx=1.0, dt=1.0, rho=1.0):
+    g = capi.Domain(2, nx, ny, viscosity=nu, dx=dx, dt=dt, fluid_density=rho, storage=storage,
+                    stage_fp16=stage, periodic=periodic)
+    cf = capi.cf_of(2, rho, dx, dt)
+    o = ol.OracleDomain(2, nx, ny, fp64=False, stage_fp16=stage, periodic=periodic, nu=nu, dx=dx, dt=dt, cf=cf,
+                        storage={"f32": 0, "f16": 1, "bf16": 2}[storage])
+    g.set_flags(flags)
+    o.set_flags(flags)
+    mf = np.asarray(m, np.float32)
+    g.set_moments(mf)
+    o.set_moments(mf.astype(np.float64))
+    return g, o
+
+
+def assert_same(g, o, which=0):
+    a = g.moments(which).astype(np.float64)
+    b = o.moments(which)
+    if not np.array_equal(a, b):
+        bad = np.argwhere(a != b)
+        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+        raise AssertionError(f"{len(bad)} differing values, first {bad[:5].tolist()}, rel-L2 {rel:.3e}")
+
+
+@pytest.mark.parametrize("stage,periodic", [(True, False), (False, False), (True, True), (False, True)])
+def test_vortex_street_bitexact(stage, periodic):
+    nx, ny = 96, 48
+    note(f"case vortex_street nx={nx} ny={ny} stage={stage} periodic={periodic}")
+    flags = wl.vortex_street_flags(nx, ny)
+    m = wl.noisy_inflow_moments(flags, nx, ny, 0.05, 1e-3, seed=7)
+    g, o = pair(nx, ny, flags, m, stage=stage, periodic=periodic)
+    for _ in range(4):
+        g.step(25)
+        o.step(25)
+        assert_same(g, o, 0)
+    assert_same(g, o, 1)
+
+
+def test_reference_initializer_bitexact():
+    nx, ny = 64, 40
+    note("case init_state")
+    flags = wl.vortex_street_flags(nx, ny)
+    g = capi.Domain(2, nx, ny, viscosity=0.005)
+    o = ol.OracleDomain(2, nx, ny, fp64=False)
+    g.set_flags(flags)
+    o.set_flags(flags)
+    g.init_state([0.05, 0.0, 0.0, 0.0])
+    o.init_reference([0.05, 0.0, 0.0, 0.0])
+    assert_same(g, o, 0)
+    assert_same(g, o, 1)
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_16bit_storage_bitexact(storage):
+    nx, ny = 80, 40
+    note(f"case storage={storage}")
+    flags = wl.vortex_street_flags(nx, ny)
+    m = wl.noisy_inflow_moments(flags, nx, ny, 0.05, 1e-3, seed=8)
+    g, o = pair(nx, ny, flags, m, stage=False, storage=storage)
+    assert_same(g, o, 0)
+    g.step(60)
+    o.step(60)
+    assert_same(g, o, 0)
+    assert_same(g, o, 1)
+
+
+def test_config1_cylinder_links_bitexact():
+    # 400x200 vortex street, static 128-gon cylinder r=20 at (100,100), through
+    # the moving-wall link path with zero wall velocity (config 1).
+    c = wl.CONFIG1
+    nx, ny = c["nx"], c["ny"]
+    note(f"case config1 cylinder {nx}x{ny}")
+    flags = wl.vortex_street_flags(nx, ny)
+    m = wl.noisy_inflow_moments(flags, nx, ny, c["u_in"], 1e-3, seed=c["seed"])
+    g, o = pair(nx, ny, flags, m, nu=c["viscosity"])
+    v, e = wl.cylinder(c["n_poly"], c["cx"], c["cy"], c["r"])
+    g.set_boundary(v, np.zeros_like(v), e)
+    o.set_boundary(v, np.zeros_like(v), e)
+    g.step(100)
+    o.step(100)
+    assert_same(g, o, 0)
+    fg = g.vertex_forces().astype(np.float64)
+    fo = o.vertex_forces()
+    assert np.abs(fo).max() > 1e-3
+    assert np.linalg.norm(fg - fo) / np.linalg.norm(fo) < 1e-5
+    w = g.body_wrench()[0]
+    assert abs(w[0] - fg[:, 0].sum()) < 1e-4 * np.abs(fg[:, 0]).sum() + 1e-9
+
+
+def test_moving_swimmer_explicit_boundary():
+    # explicit per-step vertex positions/velocities (the reference proxy
+    # contract), an elliptical swimmer with a travelling wave, outlet box.
+    nx, ny, dx, dt = 240, 120, 0.005, 0.0025
+    note(f"case swimmer explicit {nx}x{ny}")
+    flags = wl.outlet_box_flags(nx, ny)
+    m = wl.equilibrium_moments(np.zeros((nx * ny, 2)))
+    g, o = pair(nx, ny, flags, m, nu=0.005, dx=dx, dt=dt, rho=10.0)
+    sw = wl.Swimmer2D(160, 0.6, 0.3, 0.2, 0.05, A=0.02, f=1.5, phi=0.3)
+    v, vel = sw.state(0.0)
+    g.set_boundary(v, vel, sw.elems)
+    o.set_boundary(v, vel, sw.elems)
+    for s in range(40):
+        v, vel = sw.state(s * dt)
+        v32, vel32 = v.astype(np.float32), vel.astype(np.float32)
+        g.update_boundary(v32, vel32)
+        o.update_boundary(v32, vel32)
+        g.step(1)
+        o.step(1)
+    assert_same(g, o, 0)
+    assert_same(g, o, 1)
+    fg = g.vertex_forces().astype(np.float64)
+    fo = o.vertex_forces()
+    assert np.linalg.norm(fg - fo) / np.linalg.norm(fo) < 1e-5
+
+
+def test_wave_kinematics_on_device():
+    nx, ny, dx, dt = 240, 120, 0.005, 0.0025
+    note("case wave kinematics")
+    flags = wl.outlet_box_flags(nx, ny)
+    g = capi.Domain(2, nx, ny, viscosity=0.005, dx=dx, dt=dt, fluid_density=10.0)
+    g.set_flags(flags)
+    g.set_moments(wl.equilibrium_moments(np.zeros((nx * ny, 2))))
+    sw = wl.Swimmer2D(160, 0.6, 0.3, 0.2, 0.05, A=0.02, f=1.5, phi=0.3)
+    v0, vel0 = sw.state(0.0)
+    g.set_boundary(v0, vel0, sw.elems)
+    g.set_wave(sw.rest, sw.s, sw.params[None, :])
+    g.step(7)  # kinematics of step k are evaluated at t = k dt before the step
+    v, vel = g.boundary()
+    ve, vele = sw.state(6 * dt)
+    assert np.abs(v - ve).max() < 1e-6
+    assert np.abs(vel - vele).max() < 1e-5
+
+
+def test_wave_mode_matches_oracle_fed_with_device_kinematics():
+    nx, ny, dx, dt = 240, 120, 0.005, 0.0025
+    note("case wave mode parity")
+    flags = wl.outlet_box_flags(nx, ny)
+    m = wl.equilibrium_moments(np.zeros((nx * ny, 2)))
+    g, o = pair(nx, ny, flags, m, nu=0.005, dx=dx, dt=dt, rho=10.0)
+    sw = wl.Swimmer2D(160, 0.6, 0.3, 0.2, 0.05, A=0.03, f=2.0, phi=1.0)
+    v0, vel0 = sw.state(0.0)
+    g.set_boundary(v0, vel0, sw.elems)
+    o.set_boundary(v0, vel0, sw.elems)
+    g.set_wave(sw.rest, sw.s, sw.params[None, :])
+    g.set_wrench_history(64)
+    for _ in range(30):
+        g.step(1)
+        v, vel = g.boundary()
+        o.update_boundary(v, vel)
+        o.step(1)
+    assert_same(g, o, 0)
+    hist = g.wrench_history(64)
+    assert hist.shape == (30, 1, 3)
+    assert np.array_equal(hist[-1], g.body_wrench())
+
+
+def test_fill_solid_matches_oracle():
+    nx, ny = 160, 80
+    note("case fill_solid")
+    flags = wl.vortex_street_flags(nx, ny)
+    g = capi.Domain(2, nx, ny, viscosity=0.005, dx=0.01)
+    g.set_flags(flags)
+    v, e = wl.polygon(90, 0.8, 0.4, 0.3, 0.15)
+    g.set_boundary(v, np.zeros_like(v), e)
+    g.fill_solid()
+    ref = flags.copy()
+    ol.lib().orc_fill_solid2d(nx, ny, 0.01, len(v), np.ascontiguousarray(v, np.float32).ravel(), len(e),
+                              np.ascontiguousarray(e, np.uint32).ravel(), ref)
+    assert np.array_equal(g.flags(), ref)
+    assert (ref == 4).sum() > 100
+
+
+def test_mass_fp64():
+    nx, ny = 128, 64
+    note("case mass")
+    flags = wl.vortex_street_flags(nx, ny)
+    m = wl.noisy_inflow_moments(flags, nx, ny, 0.05, 1e-3, seed=3)
+    m[:, 0] += np.random.default_rng(4).uniform(-1e-3, 1e-3, nx * ny)
+    g = capi.Domain(2, nx, ny, viscosity=0.005)
+    g.set_flags(flags)
+    g.set_moments(m)
+    upd = (flags == 1) | (flags == 4)
+    ref = np.sum(m[upd, 0].astype(np.float32).astype(np.float64))
+    assert abs(g.mass() - ref) < 1e-10 * ref
+
+
+def test_config2_full_size_parity_and_determinism():
+    # BASELINE config 2 at full size (1200x400, ~600-segment swimmer, wave
+    # kinematics on device): 20 steps against the oracle, then a repeat run
+    # must be bit-identical (moments and per-step wrench).
+    c = wl.CONFIG2
+    nx, ny = c["nx"], c["ny"]
+    note(f"case config2 full size {nx}x{ny}")
+    flags = wl.outlet_box_flags(nx, ny)
+    m = wl.equilibrium_moments(np.zeros((nx * ny, 2)))
+    sw = wl.config2_swimmer()
+    runs = []
+    for rep in range(2):
+        g = capi.Domain(**wl.config2_domain_args())
+        g.set_flags(flags)
+        g.set_moments(m)
+        v0, vel0 = sw.state(0.0)
+        g.set_boundary(v0, vel0, sw.elems)
+        g.set_wave(sw.rest, sw.s, sw.params[None, :])
+        g.set_wrench_history(32)
+        if rep == 0:
+            o = ol.OracleDomain(2, nx, ny, fp64=False, stage_fp16=True, nu=c["viscosity"], dx=c["dx"], dt=c["dt"],
+                                cf=capi.cf_of(2, c["rho"], c["dx"], c["dt"]))
+            o.set_flags(flags)
+            o.set_moments(m)
+            o.set_boundary(v0, vel0, sw.elems)
+            for _ in range(20):
+                g.step(1)
+                v, vel = g.boundary()
+                o.update_boundary(v, vel)
+                o.step(1)
+            assert_same(g, o, 0)
+            fo = o.vertex_forces()
+            fg = g.vertex_forces().astype(np.float64)
+            assert np.linalg.norm(fg - fo) / np.linalg.norm(fo) < 1e-5
+        else:
+            g.step(20)
+        runs.append((g.moments(), g.wrench_history(32)))
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][1], runs[1][1])
