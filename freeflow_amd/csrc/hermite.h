@@ -59,6 +59,15 @@ __device__ __forceinline__ bool is_wall3(uint8_t f) {  // FsiCoupling3D.cu:209-2
 }
 __device__ __forceinline__ bool is_updated(uint8_t f) { return f == 1 || f == 4; }  // Fluid, SolidDynamic
 
+// Materialise an FP32 value in a register.  Without it the backend may fuse the
+// preceding multiply into a float->half conversion (v_fma_mix*), rounding the
+// exact product once to FP16 where the reference rounds to FP32 first.
+__device__ __forceinline__ float opaque(float x) {
+  __asm__ volatile("" : "+v"(x));
+  return x;
+}
+__device__ __forceinline__ __half to_half_rn(float x) { return __float2half_rn(opaque(x)); }
+
 // -------------------------------------------------------------------- 2D
 #define FF_W0_2 (4.0f / 9.0f)
 #define FF_W1_2 (1.0f / 9.0f)
@@ -336,7 +345,7 @@ template <> struct Store<__half> {
   }
   template <int DIM> __device__ static __forceinline__ void store(__half* p, int k, float v) {
     float e = (k == 0 ? v - 1.0f : v) * slot_scale<DIM>(k);
-    *p = __float2half_rn(e);
+    *p = to_half_rn(e);
   }
 };
 template <> struct Store<__hip_bfloat16> {
@@ -346,7 +355,7 @@ template <> struct Store<__hip_bfloat16> {
   }
   template <int DIM> __device__ static __forceinline__ void store(__hip_bfloat16* p, int k, float v) {
     float e = (k == 0 ? v - 1.0f : v) * slot_scale<DIM>(k);
-    *p = __float2bfloat16(e);
+    *p = __float2bfloat16(opaque(e));
   }
 };
 

@@ -47,7 +47,7 @@ __global__ void __launch_bounds__(kTX2* kTY2)
     recon2_all(m[0], m[1], m[2], m[3], m[5], m[4], f);
 #pragma unroll
     for (int i = 0; i < 9; i++) {
-      if constexpr (STAGE) sp[i][k] = __float2half_rn(f[i]);
+      if constexpr (STAGE) sp[i][k] = to_half_rn(f[i]);
       else sp[i][k] = f[i];
     }
   }

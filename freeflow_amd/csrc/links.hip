@@ -264,7 +264,7 @@ __device__ __forceinline__ void staged_pops2(const S* in, size_t plane, size_t c
   recon2_all(m[0], m[1], m[2], m[3], m[5], m[4], f);
   if constexpr (STAGE) {
 #pragma unroll
-    for (int i = 0; i < 9; i++) f[i] = __half2float(__float2half_rn(f[i]));
+    for (int i = 0; i < 9; i++) f[i] = __half2float(to_half_rn(f[i]));
   }
 }
 

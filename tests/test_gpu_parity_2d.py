@@ -102,12 +102,20 @@ def test_config1_cylinder_links_bitexact():
     g.step(100)
     o.step(100)
     assert_same(g, o, 0)
-    fg = g.vertex_forces().astype(np.float64)
+    fg = g.vertex_forces(reset=True).astype(np.float64)
     fo = o.vertex_forces()
     assert np.abs(fo).max() > 1e-3
     assert np.linalg.norm(fg - fo) / np.linalg.norm(fo) < 1e-5
+    # one more step: the FP64 body wrench equals that step's vertex forces
+    # summed, and its torque equals sum (x_v - ref) x f_v (barycentric split)
+    g.step(1)
+    f1 = g.vertex_forces().astype(np.float64)
     w = g.body_wrench()[0]
-    assert abs(w[0] - fg[:, 0].sum()) < 1e-4 * np.abs(fg[:, 0]).sum() + 1e-9
+    scale = np.abs(f1).sum()
+    assert np.allclose(w[:2], f1.sum(0), rtol=0, atol=1e-5 * scale)
+    r = v.astype(np.float64) - v.astype(np.float64).mean(0)
+    tz = np.sum(r[:, 0] * f1[:, 1] - r[:, 1] * f1[:, 0])
+    assert abs(w[2] - tz) < 1e-4 * np.sum(np.abs(r).max() * np.abs(f1))
 
 
 def test_moving_swimmer_explicit_boundary():

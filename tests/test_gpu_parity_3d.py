@@ -125,12 +125,18 @@ def test_sphere_moving_wall_links(spin):
     g.step(20)
     o.step(20)
     compare(g, o)
-    fg = g.vertex_forces().astype(np.float64)
+    fg = g.vertex_forces(reset=True).astype(np.float64)
     fo = o.vertex_forces()
     assert np.abs(fo).max() > 0
     assert np.linalg.norm(fg - fo) / np.linalg.norm(fo) < 1e-5
+    g.step(1)
+    f1 = g.vertex_forces().astype(np.float64)
     w = g.body_wrench()[0]
-    assert np.allclose(w[:3], fg.sum(0), rtol=1e-4, atol=1e-6)
+    scale = np.abs(f1).sum()
+    assert np.allclose(w[:3], f1.sum(0), rtol=0, atol=1e-5 * scale)
+    r = v - v.mean(0)
+    T = np.cross(r, f1).sum(0)
+    assert np.allclose(w[3:], T, rtol=0, atol=1e-4 * np.abs(r).max() * scale)
 
 
 def test_fp16_storage_3d():
