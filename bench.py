@@ -134,7 +134,7 @@ def main():
     achieved = bpl * cells / (avg_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_bulk2d<float,stage_fp16>", "avg_kernel_us": round(avg_ms * 1e3, 3),
+            "kernel": "k_bulk2d_links<float,stage_fp16> (stream-collide tiles + link detection blocks)", "avg_kernel_us": round(avg_ms * 1e3, 3),
             "bytes_per_lup": bpl, "cells_per_launch": cells}
     cpu = None if args.no_cpu else cpu_baseline_config2(state, args.cpu_budget)
     out = {

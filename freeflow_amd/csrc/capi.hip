@@ -41,12 +41,14 @@ static void dfree(T*& p) {
 }
 
 static void free_boundary(Domain& d) {
-  dfree(d.verts); dfree(d.vels); dfree(d.elems); dfree(d.elem_body); dfree(d.vforce);
+  for (int b = 0; b < 2; b++) { dfree(d.vbuf[b]); dfree(d.ubuf[b]); }
+  dfree(d.elems); dfree(d.elem_body); dfree(d.vforce);
   dfree(d.body_elem_ptr); dfree(d.body_elem_idx); dfree(d.vert_inc_ptr); dfree(d.vert_inc_idx);
   dfree(d.body_ref); dfree(d.elem_acc); dfree(d.wrench); dfree(d.wrench_hist);
-  dfree(d.ht_cell); dfree(d.ht_key); dfree(d.ht_f); dfree(d.ht_b); dfree(d.ht_hit); dfree(d.ht_list);
-  dfree(d.counters); dfree(d.rest_verts); dfree(d.s_coord); dfree(d.wave_params); dfree(d.vert_body);
+  dfree(d.lt_cells); dfree(d.lt_keys); dfree(d.lk_f); dfree(d.lk_b); dfree(d.lk_h); dfree(d.lt_list);
+  dfree(d.st); dfree(d.rest_verts); dfree(d.s_coord); dfree(d.wave_params); dfree(d.vert_body);
   d.nv = d.ne = d.n_bodies = 0;
+  d.gen_host = 1;
   d.has_boundary = false;
   d.wave = false;
   d.hist_cap = 0;
@@ -178,38 +180,75 @@ static unsigned next_pow2(unsigned long long v) {
   return p;
 }
 
-static int one_step(Domain& d, const float* verts, const float* vels, const uint32_t* elems, float* proxy_forces) {
+LinkArgs link_args(Domain& d, const float* v0, const float* v1, const float* u0, const float* u1, const uint32_t* e) {
+  LinkArgs la;
+  la.v[0] = v0;
+  la.v[1] = v1;
+  la.u[0] = u0;
+  la.u[1] = u1;
+  la.e = e;
+  la.ne = d.ne;
+  la.nv = d.nv;
+  la.cells = d.lt_cells;
+  la.keys = d.lt_keys;
+  la.lf = d.lk_f;
+  la.lb = d.lk_b;
+  la.lh = d.lk_h;
+  la.list = d.lt_list;
+  la.cap = d.lt_cap;
+  la.st = d.st;
+  la.flags = d.flags;
+  la.nx = d.nx;
+  la.ny = d.ny;
+  la.nz = d.nz;
+  la.cl = d.cl;
+  la.ct = d.ct;
+  la.cf = d.cf;
+  la.omega = d.omega;
+  la.plane = d.plane;
+  return la;
+}
+
+// One step: [stream-collide (+ link detection)] -> [boundary cells] ->
+// [finalize: forces, outlet, next-step kinematics].  Without a boundary:
+// [stream-collide] -> [outlet].  `pv` != null selects caller-owned proxy arrays.
+static int one_step(Domain& d, const float* pv, const float* pu, const uint32_t* pe, float* proxy_forces) {
   void* in = d.mom[d.cur];
   void* out = d.mom[d.cur ^ 1];
-  const bool links = d.has_boundary && d.ne > 0;
   int rc;
-  if (links) {
-    if (d.wave && verts == d.verts) {
-      if ((rc = launch_wave(d))) return rc;
-    }
-    if ((rc = launch_links_reset(d))) return rc;
-    if ((rc = launch_links_build(d, verts, elems))) return rc;
+  if (!(d.has_boundary && d.ne > 0)) {
+    if ((rc = d.dim == 2 ? launch_bulk2d(d, in, out) : launch_bulk3d(d, in, out))) return rc;
+    d.cur ^= 1;
+    return launch_outlet(d, d.mom[d.cur]);
   }
-  rc = d.dim == 2 ? launch_bulk2d(d, in, out) : launch_bulk3d(d, in, out);
+  const LinkArgs la = pv ? link_args(d, pv, pv, pu, pu, pe) : link_args(d, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], d.elems);
+  if (d.dim == 2) {
+    if ((rc = launch_bulk2d_links(d, in, out, la))) return rc;
+  } else {
+    if ((rc = launch_links_build(d, la))) return rc;
+    if ((rc = launch_bulk3d(d, in, out))) return rc;
+  }
+  if ((rc = launch_boundary_cells(d, in, out, la))) return rc;
+  const bool wave = d.wave && !pv;
+  const bool saved = d.wave;
+  d.wave = wave;
+  rc = launch_finalize(d, la, out, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], proxy_forces);
+  d.wave = saved;
   if (rc) return rc;
-  if (links) {
-    if ((rc = launch_boundary_cells(d, in, out, verts, vels, elems))) return rc;
-    if ((rc = launch_forces(d, verts, elems, proxy_forces))) return rc;
-  }
   d.cur ^= 1;
-  if ((rc = launch_outlet(d, d.mom[d.cur]))) return rc;
-  d.step_index++;
+  d.gen_host++;
+  if (d.hist_cap > 0) d.hist_count++;
   return FF_OK;
 }
 
 static int check_overflow(Domain& d) {
   if (!d.has_boundary) return FF_OK;
-  int c[2] = {0, 0};
-  FF_HIP(hipMemcpy(c, d.counters, sizeof(c), hipMemcpyDeviceToHost));
-  if (c[1]) {
+  StepState st;
+  FF_HIP(hipMemcpy(&st, d.st, sizeof(st), hipMemcpyDeviceToHost));
+  if (st.overflow) {
     int z = 0;
-    FF_HIP(hipMemcpy(d.counters + 1, &z, sizeof(int), hipMemcpyHostToDevice));
-    return fail(FF_ERR_CAPACITY, "boundary link table overflow (capacity " + std::to_string(d.ht_cap) + " cells)");
+    FF_HIP(hipMemcpy(&d.st->overflow, &z, sizeof(int), hipMemcpyHostToDevice));
+    return fail(FF_ERR_CAPACITY, "boundary link table overflow (capacity " + std::to_string(d.lt_cap) + " cells)");
   }
   return FF_OK;
 }
@@ -415,13 +454,16 @@ int ff_set_boundary(ff_domain* dd, int nv, const float* verts, const float* vels
   d.n_bodies = n_bodies;
   int rc = FF_OK;
   const size_t vcount = (size_t)std::max(nv, 1) * D;
-  if ((rc = dalloc(&d.verts, vcount)) || (rc = dalloc(&d.vels, vcount)) ||
-      (rc = dalloc(&d.elems, (size_t)std::max(ne, 1) * D)) || (rc = dalloc(&d.elem_body, std::max(ne, 1))) ||
+  for (int b = 0; b < 2; b++)
+    if ((rc = dalloc(&d.vbuf[b], vcount)) || (rc = dalloc(&d.ubuf[b], vcount))) return rc;
+  if ((rc = dalloc(&d.elems, (size_t)std::max(ne, 1) * D)) || (rc = dalloc(&d.elem_body, std::max(ne, 1))) ||
       (rc = dalloc(&d.vforce, vcount)))
     return rc;
   if (nv) {
-    FF_HIP(hipMemcpy(d.verts, verts, (size_t)nv * D * sizeof(float), hipMemcpyHostToDevice));
-    FF_HIP(hipMemcpy(d.vels, vels, (size_t)nv * D * sizeof(float), hipMemcpyHostToDevice));
+    for (int b = 0; b < 2; b++) {
+      FF_HIP(hipMemcpy(d.vbuf[b], verts, (size_t)nv * D * sizeof(float), hipMemcpyHostToDevice));
+      FF_HIP(hipMemcpy(d.ubuf[b], vels, (size_t)nv * D * sizeof(float), hipMemcpyHostToDevice));
+    }
   }
   if (ne) {
     FF_HIP(hipMemcpy(d.elems, elems, (size_t)ne * D * sizeof(uint32_t), hipMemcpyHostToDevice));
@@ -487,15 +529,20 @@ int ff_set_boundary(ff_domain* dd, int nv, const float* verts, const float* vels
     for (int c = 0; c < D; c++) cells *= (unsigned long long)(hi[c] - lo[c] + 4.0);
     est += cells;
   }
-  d.ht_cap = (int)next_pow2(est * 2 + 1024);
+  d.lt_cap = (int)next_pow2(est * 2 + 1024);
   const size_t K = (size_t)(d.q - 1);
-  if ((rc = dalloc(&d.ht_cell, d.ht_cap)) || (rc = dalloc(&d.ht_key, d.ht_cap * K)) ||
-      (rc = dalloc(&d.ht_f, d.ht_cap * K * D)) || (rc = dalloc(&d.ht_b, d.ht_cap * K * D)) ||
-      (rc = dalloc(&d.ht_hit, d.ht_cap * K * D)) || (rc = dalloc(&d.ht_list, d.ht_cap)) || (rc = dalloc(&d.counters, 2)))
+  const size_t cap = (size_t)d.lt_cap;
+  if ((rc = dalloc(&d.lt_cells, cap)) || (rc = dalloc(&d.lt_keys, cap * K)) || (rc = dalloc(&d.lk_f, cap * K * D)) ||
+      (rc = dalloc(&d.lk_b, cap * K * D)) || (rc = dalloc(&d.lk_h, cap * K * D)) ||
+      (rc = dalloc(&d.lt_list, 2 * cap)) || (rc = dalloc(&d.st, 1)))
     return rc;
-  FF_HIP(hipMemset(d.ht_cell, 0xFF, (size_t)d.ht_cap * sizeof(int)));
-  FF_HIP(hipMemset(d.ht_key, 0xFF, (size_t)d.ht_cap * K * sizeof(uint32_t)));
-  FF_HIP(hipMemset(d.counters, 0, 2 * sizeof(int)));
+  // generation 0 marks every entry empty; steps run from generation 1
+  FF_HIP(hipMemset(d.lt_cells, 0, cap * sizeof(unsigned long long)));
+  FF_HIP(hipMemset(d.lt_keys, 0, cap * K * sizeof(unsigned long long)));
+  StepState st0{};
+  st0.gen = 1;
+  FF_HIP(hipMemcpy(d.st, &st0, sizeof(st0), hipMemcpyHostToDevice));
+  d.gen_host = 1;
   // the memsets above run on the null stream; the domain stream is non-blocking
   FF_HIP(hipDeviceSynchronize());
   d.has_boundary = true;
@@ -509,8 +556,10 @@ int ff_update_boundary(ff_domain* dd, const float* verts, const float* vels) {
   if (!verts || !vels) return fail(FF_ERR_INVALID, "null boundary array");
   FF_HIP(hipSetDevice(d.device));
   const size_t bytes = (size_t)d.nv * d.dim * sizeof(float);
-  FF_HIP(hipMemcpyAsync(d.verts, verts, bytes, hipMemcpyHostToDevice, d.stream));
-  FF_HIP(hipMemcpyAsync(d.vels, vels, bytes, hipMemcpyHostToDevice, d.stream));
+  for (int b = 0; b < 2; b++) {
+    FF_HIP(hipMemcpyAsync(d.vbuf[b], verts, bytes, hipMemcpyHostToDevice, d.stream));
+    FF_HIP(hipMemcpyAsync(d.ubuf[b], vels, bytes, hipMemcpyHostToDevice, d.stream));
+  }
   FF_HIP(hipStreamSynchronize(d.stream));
   d.wave = false;
   return FF_OK;
@@ -542,6 +591,10 @@ int ff_set_wave(ff_domain* dd, const float* rest, const float* s, const double* 
   FF_HIP(hipMemcpy(d.s_coord, s, (size_t)d.nv * sizeof(float), hipMemcpyHostToDevice));
   FF_HIP(hipMemcpy(d.wave_params, params, (size_t)d.n_bodies * 4 * sizeof(double), hipMemcpyHostToDevice));
   d.wave = true;
+  // kinematics of the next step to run; later steps are produced by the
+  // finalize kernel of the step before
+  if ((rc = launch_wave_now(d))) return rc;
+  FF_HIP(hipStreamSynchronize(d.stream));
   return FF_OK;
 }
 
@@ -551,7 +604,7 @@ int ff_step(ff_domain* dd, int nsteps) {
   Domain& d = *dd;
   FF_HIP(hipSetDevice(d.device));
   for (int s = 0; s < nsteps; s++) {
-    int rc = one_step(d, d.verts, d.vels, d.elems, nullptr);
+    int rc = one_step(d, nullptr, nullptr, nullptr, nullptr);
     if (rc) return rc;
   }
   return FF_OK;
@@ -624,8 +677,9 @@ int ff_get_wrench_history(ff_domain* dd, double* out, int count, int* n_out) {
   std::vector<double> ring((size_t)d.hist_cap * rec);
   if (d.hist_cap) FF_HIP(hipMemcpy(ring.data(), d.wrench_hist, ring.size() * sizeof(double), hipMemcpyDeviceToHost));
   for (int k = 0; k < n; k++) {
-    const long long step = d.hist_count - n + k;
-    std::memcpy(out + (size_t)k * rec, ring.data() + (size_t)(step % d.hist_cap) * rec, rec * sizeof(double));
+    // the step with generation g wrote slot (g - 1) % cap; the latest step is gen_host - 1
+    const long long g = (long long)d.gen_host - n + k;
+    std::memcpy(out + (size_t)k * rec, ring.data() + (size_t)((g - 1) % d.hist_cap) * rec, rec * sizeof(double));
   }
   *n_out = n;
   return FF_OK;
@@ -638,8 +692,10 @@ int ff_get_boundary(ff_domain* dd, float* verts, float* vels) {
   FF_HIP(hipSetDevice(d.device));
   FF_HIP(hipStreamSynchronize(d.stream));
   const size_t bytes = (size_t)d.nv * d.dim * sizeof(float);
-  if (verts) FF_HIP(hipMemcpy(verts, d.verts, bytes, hipMemcpyDeviceToHost));
-  if (vels) FF_HIP(hipMemcpy(vels, d.vels, bytes, hipMemcpyDeviceToHost));
+  // the buffer of the most recent step (both buffers agree before the first)
+  const int b = d.gen_host > 1 ? (int)((d.gen_host - 1) & 1) : (int)(d.gen_host & 1);
+  if (verts) FF_HIP(hipMemcpy(verts, d.vbuf[b], bytes, hipMemcpyDeviceToHost));
+  if (vels) FF_HIP(hipMemcpy(vels, d.ubuf[b], bytes, hipMemcpyDeviceToHost));
   return FF_OK;
 }
 
@@ -667,7 +723,7 @@ int ff_fill_solid(ff_domain* dd) {
   if (d.dim != 2) return fail(FF_ERR_INVALID, "fill_solid is 2D only");
   if (!d.has_boundary) return fail(FF_ERR_STATE, "no boundary set");
   FF_HIP(hipSetDevice(d.device));
-  int rc = launch_fill_solid(d);
+  int rc = launch_fill_solid(d, d.vbuf[d.gen_host & 1]);
   if (rc) return rc;
   FF_HIP(hipStreamSynchronize(d.stream));
   FF_HIP(hipMemcpy(d.h_flags.data(), d.flags, (size_t)d.n, hipMemcpyDeviceToHost));
@@ -690,10 +746,15 @@ int ff_time_bulk_kernel(ff_domain* dd, int iters, double* avg_ms) {
   for (auto& e : ev) FF_HIP(hipEventCreate(&e));
   void* in = d.mom[d.cur];
   void* out = d.mom[d.cur ^ 1];
+  // the stream-collide launch of a step (in 2D with a boundary it also runs
+  // the link detection, which is idempotent within a generation)
+  const bool links = d.has_boundary && d.ne > 0 && d.dim == 2;
+  const LinkArgs la = links ? link_args(d, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], d.elems) : LinkArgs{};
   int rc = FF_OK;
   for (int k = 0; k < iters && !rc; k++) {
     FF_HIP(hipEventRecord(ev[2 * k], d.stream));
-    rc = d.dim == 2 ? launch_bulk2d(d, in, out) : launch_bulk3d(d, in, out);
+    if (links) rc = launch_bulk2d_links(d, in, out, la);
+    else rc = d.dim == 2 ? launch_bulk2d(d, in, out) : launch_bulk3d(d, in, out);
     FF_HIP(hipEventRecord(ev[2 * k + 1], d.stream));
   }
   FF_HIP(hipStreamSynchronize(d.stream));
@@ -718,7 +779,7 @@ int ff_time_steps(ff_domain* dd, int nsteps, double* total_ms) {
   FF_HIP(hipEventCreate(&b));
   FF_HIP(hipEventRecord(a, d.stream));
   int rc = FF_OK;
-  for (int s = 0; s < nsteps && !rc; s++) rc = one_step(d, d.verts, d.vels, d.elems, nullptr);
+  for (int s = 0; s < nsteps && !rc; s++) rc = one_step(d, nullptr, nullptr, nullptr, nullptr);
   FF_HIP(hipEventRecord(b, d.stream));
   FF_HIP(hipEventSynchronize(b));
   float ms = 0.f;

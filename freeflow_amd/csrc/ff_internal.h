@@ -25,20 +25,50 @@ void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
 
-#define FF_HIP(call)                                  \
-  do {                                                \
-    hipError_t _e = (call);                           \
+#define FF_HIP(call)                                        \
+  do {                                                      \
+    hipError_t _e = (call);                                 \
     if (_e != hipSuccess) return ::ff::hip_fail(_e, #call); \
   } while (0)
 
 // One outlet cell: in-place update of the new buffer after a step.
 struct OutletCell {
-  int cell;   // outlet cell index
-  int nb;     // neighbour index (must be Fluid to act)
+  int cell;      // outlet cell index
+  int nb;        // neighbour index (must be Fluid to act)
   int8_t comp;   // velocity slot
   int8_t pcomp;  // Pi slot
   int8_t sign;   // +1: copy when u > 0, -1: copy when u < 0
   int8_t pad;
+};
+
+// Step state kept on the device so a step's launches need no host-side
+// arguments that change from step to step (they can be replayed as a graph).
+struct StepState {
+  unsigned gen;      // generation of the current step: steps since the boundary was set + 1
+  int count[2];      // link-table occupancy, by generation parity
+  int overflow;      // sticky: link table full
+  unsigned done;     // unused
+  int pad[3];
+};
+
+// Everything the link kernels read, passed by value.
+struct LinkArgs {
+  const float* v[2];   // vertex positions by generation parity
+  const float* u[2];   // vertex velocities by generation parity
+  const uint32_t* e;
+  int ne, nv;
+  unsigned long long* cells;  // cap
+  unsigned long long* keys;   // cap * (Q-1)
+  float* lf;                  // cap * (Q-1) * DIM  link force
+  float* lb;                  // cap * (Q-1) * DIM  barycentrics, lb[0] < 0: link not applied
+  float* lh;                  // cap * (Q-1) * DIM  hit point
+  int* list;                  // 2 * cap occupied slots, by generation parity
+  int cap;
+  StepState* st;
+  const uint8_t* flags;
+  int nx, ny, nz;
+  float cl, ct, cf, omega;
+  size_t plane;
 };
 
 // Library-held state of one fluid domain (the role of LbmFlowField2D/3D).
@@ -63,61 +93,61 @@ struct Domain {
   int n_outlets = 0;
   // boundary geometry (device, interleaved like the reference proxy)
   int nv = 0, ne = 0, n_bodies = 0;
-  float* verts = nullptr;
-  float* vels = nullptr;
+  float* vbuf[2] = {nullptr, nullptr};  // positions by generation parity
+  float* ubuf[2] = {nullptr, nullptr};  // velocities by generation parity
   uint32_t* elems = nullptr;
   int* elem_body = nullptr;
-  float* vforce = nullptr;       // accumulated per-vertex force
-  // CSR: body -> elements, vertex -> (element, end)
-  int* body_elem_ptr = nullptr;
+  float* vforce = nullptr;        // accumulated per-vertex force
+  int* body_elem_ptr = nullptr;   // CSR body -> elements
   int* body_elem_idx = nullptr;
-  int* vert_inc_ptr = nullptr;
-  int* vert_inc_idx = nullptr;   // element*4 + end (end < dim)
-  double* body_ref = nullptr;    // n_bodies * dim
-  double* elem_acc = nullptr;    // per element: dim (force) + dim*dim? see links.hip
-  double* wrench = nullptr;      // n_bodies * (2D:3, 3D:6), latest step
-  double* wrench_hist = nullptr; // ring
+  int* vert_inc_ptr = nullptr;    // CSR vertex -> (element << 2 | end)
+  int* vert_inc_idx = nullptr;
+  double* body_ref = nullptr;     // n_bodies * dim
+  double* elem_acc = nullptr;     // per element: dim*dim vertex shares, dim force, torque
+  double* wrench = nullptr;       // n_bodies * (2D:3, 3D:6), latest step
+  double* wrench_hist = nullptr;  // ring of hist_cap steps
   int hist_cap = 0;
-  long long hist_count = 0;      // steps recorded in the ring since last set
-  // link hash table (sparse replacement of the dense N*Q hit table)
-  int ht_cap = 0;                // power of two
-  int* ht_cell = nullptr;        // cap, -1 = empty
-  uint32_t* ht_key = nullptr;    // cap * (q-1), 0xFFFFFFFF = no hit
-  float* ht_f = nullptr;         // cap * (q-1) * dim  link force
-  float* ht_b = nullptr;         // cap * (q-1) * dim  barycentrics (b[0] < 0: link not applied)
-  float* ht_hit = nullptr;       // cap * (q-1) * dim  hit point
-  int* ht_list = nullptr;        // cap occupied slots
-  int* counters = nullptr;       // [0] occupied count, [1] overflow flag
+  long long hist_count = 0;       // steps recorded since the ring was set
+  // link table (sparse replacement of the dense N*Q hit table)
+  int lt_cap = 0;
+  unsigned long long* lt_cells = nullptr;
+  unsigned long long* lt_keys = nullptr;
+  float* lk_f = nullptr;
+  float* lk_b = nullptr;
+  float* lk_h = nullptr;
+  int* lt_list = nullptr;
+  StepState* st = nullptr;
+  unsigned gen_host = 1;          // mirror of st->gen
   // wave kinematics
   bool wave = false;
   float* rest_verts = nullptr;
   float* s_coord = nullptr;
   double* wave_params = nullptr;  // 4 per body
   int* vert_body = nullptr;
-  long long step_index = 0;       // steps taken since creation (wave time)
   // scratch
   double* red = nullptr;          // reduction scratch
   bool has_boundary = false;
 };
 
-// plane stride: moments planes aligned to 256 elements
+// plane stride: moment planes aligned to 256 elements
 inline size_t plane_stride(long long n) { return (size_t)((n + 255) / 256 * 256); }
 inline size_t storage_bytes(int storage) { return storage == FF_STORAGE_F32 ? 4 : 2; }
 
+LinkArgs link_args(Domain& d, const float* v0, const float* v1, const float* u0, const float* u1, const uint32_t* e);
+
 // launchers (defined in the .hip units)
 int launch_bulk2d(Domain& d, const void* in, void* out);
+int launch_bulk2d_links(Domain& d, const void* in, void* out, const LinkArgs& la);
 int launch_bulk3d(Domain& d, const void* in, void* out);
+int launch_links_build(Domain& d, const LinkArgs& la);
 int launch_outlet(Domain& d, void* buf);
-int launch_links_reset(Domain& d);
-int launch_links_build(Domain& d, const float* verts, const uint32_t* elems);
-int launch_boundary_cells(Domain& d, const void* in, void* out, const float* verts, const float* vels,
-                          const uint32_t* elems);
-int launch_forces(Domain& d, const float* verts, const uint32_t* elems, float* vforce_out);
-int launch_wave(Domain& d);
-int launch_init_state(Domain& d, const float* bc6);
+int launch_boundary_cells(Domain& d, const void* in, void* out, const LinkArgs& la);
+int launch_finalize(Domain& d, const LinkArgs& la, void* out, float* v_next0, float* v_next1, float* u_next0,
+                    float* u_next1, float* proxy_forces);
+int launch_wave_now(Domain& d);
 int launch_convert_in(Domain& d, const float* aos_dev, void* buf);
 int launch_convert_out(Domain& d, const void* buf, float* aos_dev);
 int launch_mass(Domain& d, double* out_dev);
-int launch_fill_solid(Domain& d);
+int launch_fill_solid(Domain& d, const float* verts);
 
 }  // namespace ff

@@ -1,0 +1,244 @@
+// geometry.h -- link/element predicates and the generation-tagged link table.
+//
+// Predicates restate SolidGeometryProxy_Device.cuh (line_intersect :89-124,
+// intersectAtIndex<2> :211-227, line_triangle_intersect :306-341,
+// getBoundingBox :198-209, :424-440, depthIndexCompact :79-85).  They take the
+// same discrete decisions as the reference only without FMA contraction and
+// with IEEE division/square root, which is how every unit including this file
+// is compiled.
+//
+// Link table: open addressing keyed by cell.  Entries carry the step's
+// generation in their upper 32 bits, so entries of earlier steps read as
+// empty and the table never needs clearing:
+//   cell word : (gen << 32) | cell
+//   key word  : (gen << 32) | ~depthIndexCompact   (atomicMax keeps the
+//               current generation and, within it, the smallest key)
+#pragma once
+
+#include <cfloat>
+#include <stdint.h>
+
+#include "hermite.h"
+
+namespace ff {
+
+__device__ __forceinline__ unsigned mix32(unsigned h) {
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+
+// Returns the slot of `cell` in this generation, inserting it if absent
+// (the first inserter appends the slot to `list`).  -1 on a full table.
+__device__ __forceinline__ int lt_insert(unsigned long long* cells, int cap, unsigned gen, int cell, int* list,
+                                         int* count, int* overflow) {
+  const unsigned long long want = ((unsigned long long)gen << 32) | (unsigned)cell;
+  unsigned h = mix32((unsigned)cell) & (unsigned)(cap - 1);
+  for (int probe = 0; probe < cap; probe++) {
+    unsigned long long cur = cells[h];
+    while ((unsigned)(cur >> 32) != gen) {  // stale or never used: try to claim
+      const unsigned long long prev = atomicCAS(&cells[h], cur, want);
+      if (prev == cur) {
+        list[atomicAdd(count, 1)] = (int)h;
+        return (int)h;
+      }
+      cur = prev;
+    }
+    if (cur == want) return (int)h;
+    h = (h + 1) & (unsigned)(cap - 1);
+  }
+  atomicOr(overflow, 1);
+  return -1;
+}
+
+__device__ __forceinline__ int lt_find(const unsigned long long* cells, int cap, unsigned gen, int cell) {
+  const unsigned long long want = ((unsigned long long)gen << 32) | (unsigned)cell;
+  unsigned h = mix32((unsigned)cell) & (unsigned)(cap - 1);
+  for (int probe = 0; probe < cap; probe++) {
+    const unsigned long long cur = cells[h];
+    if (cur == want) return (int)h;
+    if ((unsigned)(cur >> 32) != gen) return -1;
+    h = (h + 1) & (unsigned)(cap - 1);
+  }
+  return -1;
+}
+
+__device__ __forceinline__ void lt_offer_key(unsigned long long* keys, size_t idx, unsigned gen, uint32_t key) {
+  atomicMax(&keys[idx], ((unsigned long long)gen << 32) | (unsigned)(~key));
+}
+// 0xFFFFFFFF when the link has no key in this generation.
+__device__ __forceinline__ uint32_t lt_key(const unsigned long long* keys, size_t idx, unsigned gen) {
+  const unsigned long long v = keys[idx];
+  return (unsigned)(v >> 32) == gen ? ~(uint32_t)v : 0xFFFFFFFFu;
+}
+
+// depthIndexCompact (:79-85)
+__device__ __forceinline__ uint32_t depth_key(float depth, int elem) {
+  return (((uint32_t)(depth * 32768.0f)) << 16) | (uint32_t)(uint16_t)elem;
+}
+
+// line_intersect (:89-124): link p1->p2 against segment p3-p4; depth = 1 - t
+// in (0,1) or -1.  Only the segment's x-extent is tested (:117), so exactly
+// vertical segments never hit.
+__device__ __forceinline__ float seg_link(float x1, float y1, float x2, float y2, float x3, float y3, float x4,
+                                          float y4, float& xo, float& yo) {
+  const float a1 = (x1 - x2) * (y3 - y4) - (y1 - y2) * (x3 - x4);
+  const float b1 = (x4 - x2) * (y3 - y4) - (y4 - y2) * (x3 - x4);
+  if (fabsf(a1) <= 0.0f) return -1.0f;
+  const float t1 = b1 / a1;
+  if (t1 <= 0.0f || t1 >= 1.0f) return -1.0f;
+  xo = t1 * x1 + (1.0f - t1) * x2;
+  if ((x3 - xo) * (x4 - xo) < 0.0f) {
+    yo = t1 * y1 + (1.0f - t1) * y2;
+    return (float)(1.0 - (double)t1);
+  }
+  return -1.0f;
+}
+
+struct Hit {
+  float depth;
+  float b[3];
+  float hit[3];
+};
+
+// intersectAtIndex<2>: barycentrics from glm::distance (:220-221).
+__device__ __forceinline__ bool hit_elem2(const float* v, const uint32_t* e, int elem, float p1x, float p1y, float p2x,
+                                          float p2y, Hit& h) {
+  const uint32_t s = e[2 * elem], t = e[2 * elem + 1];
+  const float p3x = v[2 * s], p3y = v[2 * s + 1], p4x = v[2 * t], p4y = v[2 * t + 1];
+  float xo = 0.f, yo = 0.f;
+  const float d = seg_link(p1x, p1y, p2x, p2y, p3x, p3y, p4x, p4y, xo, yo);
+  if (!(d > 0.0f && d < 1.0f)) return false;
+  h.depth = d;
+  h.hit[0] = xo;
+  h.hit[1] = yo;
+  h.hit[2] = 0.0f;
+  const float ax = p4x - xo, ay = p4y - yo, bx = p4x - p3x, by = p4y - p3y;
+  h.b[0] = sqrtf(ax * ax + ay * ay) / sqrtf(bx * bx + by * by);
+  h.b[1] = 1.0f - h.b[0];
+  h.b[2] = 0.0f;
+  return true;
+}
+
+// line_triangle_intersect with the NumericUtils 1e-6 tolerances;
+// glm::normalize = v * (1 / sqrt(dot(v, v))).
+__device__ __forceinline__ bool hit_elem3(const float* v, const uint32_t* e, int elem, const float* p1, const float* p2,
+                                          Hit& h) {
+  float tv[3][3];
+#pragma unroll
+  for (int j = 0; j < 3; j++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) tv[j][k] = v[e[3 * elem + j] * 3 + k];
+  const float e1[3] = {tv[1][0] - tv[0][0], tv[1][1] - tv[0][1], tv[1][2] - tv[0][2]};
+  const float e2[3] = {tv[2][0] - tv[0][0], tv[2][1] - tv[0][1], tv[2][2] - tv[0][2]};
+  const float c[3] = {e1[1] * e2[2] - e2[1] * e1[2], e1[2] * e2[0] - e2[2] * e1[0], e1[0] * e2[1] - e2[0] * e1[1]};
+  const float cs = 1.0f / sqrtf(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+  const float n[3] = {c[0] * cs, c[1] * cs, c[2] * cs};
+  const float dd[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+  const float ds = 1.0f / sqrtf(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]);
+  const float d[3] = {dd[0] * ds, dd[1] * ds, dd[2] * ds};
+  if (fabsf(n[0] * d[0] + n[1] * d[1] + n[2] * d[2]) <= 1e-6f) return false;
+  const float a[3] = {tv[0][0] - p1[0], tv[0][1] - p1[1], tv[0][2] - p1[2]};
+  const float t = (n[0] * a[0] + n[1] * a[1] + n[2] * a[2]) / (n[0] * dd[0] + n[1] * dd[1] + n[2] * dd[2]);
+  if (t < 0.0f || t > 1.0f) return false;
+  const float ip[3] = {p1[0] + t * dd[0], p1[1] + t * dd[1], p1[2] + t * dd[2]};
+  const float w[3] = {ip[0] - tv[0][0], ip[1] - tv[0][1], ip[2] - tv[0][2]};
+  const float d00 = e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2];
+  const float d01 = e1[0] * e2[0] + e1[1] * e2[1] + e1[2] * e2[2];
+  const float d11 = e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2];
+  const float d20 = w[0] * e1[0] + w[1] * e1[1] + w[2] * e1[2];
+  const float d21 = w[0] * e2[0] + w[1] * e2[1] + w[2] * e2[2];
+  const float den = d00 * d11 - d01 * d01;
+  const float u0 = (d11 * d20 - d01 * d21) / den;
+  const float v0 = (d00 * d21 - d01 * d20) / den;
+  if (u0 >= 0.0f - 1e-6f && v0 >= 0.0f - 1e-6f && u0 + v0 <= 1.0f + 1e-6f) {
+    h.depth = t;
+    h.b[0] = 1.0f - u0 - v0;
+    h.b[1] = u0;
+    h.b[2] = v0;
+    h.hit[0] = ip[0];
+    h.hit[1] = ip[1];
+    h.hit[2] = ip[2];
+    return true;
+  }
+  return false;
+}
+
+// Candidate cells of an element: getBoundingBox (max starts at FLT_MIN) / dx,
+// widened by one cell, clipped to the grid.  lo/hi inclusive.
+template <int DIM>
+__device__ __forceinline__ void elem_window(const float* v, const uint32_t* e, int elem, float dx, const int* n, int* lo,
+                                            int* hi) {
+#pragma unroll
+  for (int a = 0; a < DIM; a++) {
+    float mn = FLT_MAX, mx = FLT_MIN;
+#pragma unroll
+    for (int j = 0; j < DIM; j++) {
+      const float c = v[DIM * e[DIM * elem + j] + a];
+      mn = fminf(mn, c);
+      mx = fmaxf(mx, c);
+    }
+    lo[a] = max((int)(mn / dx) - 1, 0);
+    hi[a] = min((int)(mx / dx) + 1, n[a] - 1);
+  }
+}
+
+}  // namespace ff
+
+#include "ff_internal.h"
+
+namespace ff {
+
+// Link detection for one block of (element, direction) pairs: 16 lanes per
+// pair walk the element's candidate window (the loops of
+// mrSolver2DComputeOccupancyFlagKernel_v3, FsiCoupling2D.cu:44-74, and its 3D
+// twin, FsiCoupling3D.cu:43-76) and offer depthIndexCompact keys for every
+// hit of an updated cell.  `block` indexes the pair groups.
+template <int DIM>
+__device__ __forceinline__ void build_links_block(const LinkArgs& la, long long block, unsigned gen) {
+  constexpr int K = DIM == 2 ? 8 : 26;
+  const int lane = threadIdx.x & 15;
+  const long long pair = block * (blockDim.x >> 4) + (threadIdx.x >> 4);
+  if (pair >= (long long)la.ne * K) return;
+  const int elem = (int)(pair / K);
+  const int l = (int)(pair % K) + 1;
+  const float* v = la.v[gen & 1];
+  const int n[3] = {la.nx, la.ny, la.nz};
+  int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  elem_window<DIM>(v, la.e, elem, la.cl, n, lo, hi);
+  const int wx = hi[0] - lo[0] + 1, wy = hi[1] - lo[1] + 1, wz = DIM == 3 ? hi[2] - lo[2] + 1 : 1;
+  if (wx <= 0 || wy <= 0 || wz <= 0) return;
+  const long long ncell = (long long)wx * wy * wz;
+  const int cx = DIM == 2 ? D2::cx(l) : D3::cx(l);
+  const int cy = DIM == 2 ? D2::cy(l) : D3::cy(l);
+  const int cz = DIM == 2 ? 0 : D3::cz(l);
+  int* list = la.list + (size_t)(gen & 1) * la.cap;
+  int* count = &la.st->count[gen & 1];
+  for (long long k = lane; k < ncell; k += 16) {
+    const int i = lo[0] + (int)(k % wx);
+    const int j = lo[1] + (int)((k / wx) % wy);
+    const int z = DIM == 3 ? lo[2] + (int)(k / ((long long)wx * wy)) : 0;
+    const int cell = (z * la.ny + j) * la.nx + i;
+    if (!is_updated(la.flags[cell])) continue;
+    Hit h;
+    bool hit;
+    if constexpr (DIM == 2) {
+      hit = hit_elem2(v, la.e, elem, (float)i * la.cl, (float)j * la.cl, ((float)i - (float)cx) * la.cl,
+                      ((float)j - (float)cy) * la.cl, h);
+    } else {
+      const float p1[3] = {(float)i * la.cl, (float)j * la.cl, (float)z * la.cl};
+      const float p2[3] = {((float)i - (float)cx) * la.cl, ((float)j - (float)cy) * la.cl,
+                           ((float)z - (float)cz) * la.cl};
+      hit = hit_elem3(v, la.e, elem, p1, p2, h);
+    }
+    if (!hit) continue;
+    const int slot = lt_insert(la.cells, la.cap, gen, cell, list, count, &la.st->overflow);
+    if (slot < 0) return;
+    lt_offer_key(la.keys, (size_t)slot * K + (l - 1), gen, depth_key(h.depth, elem));
+  }
+}
+
+}  // namespace ff

@@ -6,7 +6,8 @@
 // cell as if no moving-wall link existed; the few cells that own a
 // moving-wall link are recomputed afterwards by the boundary-cell kernel
 // (links.hip), so the bulk pass stays a pure stencil with one byte of flag
-// traffic per cell.
+// traffic per cell.  With a boundary present the same launch also runs the
+// step's link detection in extra workgroups.
 //
 // In 2D a wall-upstream population is overwritten by the ordinary pull in the
 // reference (FsiCoupling2D.cu:168-223: `is_blocked` stays false on the wall
@@ -15,20 +16,24 @@
 #include <type_traits>
 
 #include "ff_internal.h"
+#include "geometry.h"
 #include "hermite.h"
 
 namespace ff {
 
 constexpr int kTX2 = 32, kTY2 = 8;
 
+// One 32x8 tile: stage the reconstructed populations of the tile and its
+// one-cell halo in LDS (clamped or periodic halo), then pull, collide and
+// store every updated cell.  `bx, by` select the tile.
 template <typename S, bool STAGE, bool PERIODIC>
-__global__ void __launch_bounds__(kTX2* kTY2)
-    k_bulk2d(const S* __restrict__ in, S* __restrict__ out, const uint8_t* __restrict__ flags, int nx, int ny,
-             size_t plane, float omega) {
+__device__ __forceinline__ void bulk2d_tile(const S* __restrict__ in, S* __restrict__ out,
+                                            const uint8_t* __restrict__ flags, int nx, int ny, size_t plane,
+                                            float omega, int bx, int by) {
   constexpr int HX = kTX2 + 2, HY = kTY2 + 2, HN = HX * HY;
   using T = typename std::conditional<STAGE, __half, float>::type;
   __shared__ T sp[9][HN];
-  const int x0 = blockIdx.x * kTX2, y0 = blockIdx.y * kTY2;
+  const int x0 = bx * kTX2, y0 = by * kTY2;
   for (int k = threadIdx.x; k < HN; k += kTX2 * kTY2) {
     const int hx = k % HX, hy = k / HX;
     int gx = x0 + hx - 1, gy = y0 + hy - 1;
@@ -72,27 +77,72 @@ __global__ void __launch_bounds__(kTX2* kTY2)
   for (int q = 0; q < 6; q++) Store<S>::template store<2>(out + q * plane + cell, q, o[q]);
 }
 
+template <typename S, bool STAGE, bool PERIODIC>
+__global__ void __launch_bounds__(kTX2* kTY2)
+    k_bulk2d(const S* __restrict__ in, S* __restrict__ out, const uint8_t* __restrict__ flags, int nx, int ny,
+             size_t plane, float omega) {
+  bulk2d_tile<S, STAGE, PERIODIC>(in, out, flags, nx, ny, plane, omega, blockIdx.x, blockIdx.y);
+}
+
+// Link detection and stream-collide tiles in one launch: the first `nbuild`
+// blocks detect the step's moving-wall links (dispatched first: they carry the
+// longest dependent chains), the rest are tiles.  The two parts are
+// independent (boundary cells are recomputed afterwards).
+template <typename S, bool STAGE, bool PERIODIC>
+__global__ void __launch_bounds__(kTX2* kTY2)
+    k_bulk2d_links(const S* __restrict__ in, S* __restrict__ out, int tiles_x, int nbuild, LinkArgs la) {
+  if ((int)blockIdx.x < nbuild) {
+    build_links_block<2>(la, (long long)blockIdx.x, la.st->gen);
+  } else {
+    const int t = (int)blockIdx.x - nbuild;
+    bulk2d_tile<S, STAGE, PERIODIC>(in, out, la.flags, la.nx, la.ny, la.plane, la.omega, t % tiles_x, t / tiles_x);
+  }
+}
+
 template <typename S>
-static int bulk2d_typed(Domain& d, const void* in, void* out) {
-  dim3 grid((d.nx + kTX2 - 1) / kTX2, (d.ny + kTY2 - 1) / kTY2);
-  dim3 block(kTX2 * kTY2);
+static int bulk2d_typed(Domain& d, const void* in, void* out, const LinkArgs* la) {
+  const int tx = (d.nx + kTX2 - 1) / kTX2, ty = (d.ny + kTY2 - 1) / kTY2;
   const bool stage = d.options & FF_OPT_STAGE_FP16;
   const bool per = d.options & FF_OPT_PERIODIC;
   const S* i = (const S*)in;
   S* o = (S*)out;
-  if (stage && per) k_bulk2d<S, true, true><<<grid, block, 0, d.stream>>>(i, o, d.flags, d.nx, d.ny, d.plane, d.omega);
-  else if (stage) k_bulk2d<S, true, false><<<grid, block, 0, d.stream>>>(i, o, d.flags, d.nx, d.ny, d.plane, d.omega);
-  else if (per) k_bulk2d<S, false, true><<<grid, block, 0, d.stream>>>(i, o, d.flags, d.nx, d.ny, d.plane, d.omega);
-  else k_bulk2d<S, false, false><<<grid, block, 0, d.stream>>>(i, o, d.flags, d.nx, d.ny, d.plane, d.omega);
+  if (!la) {
+    dim3 grid(tx, ty), block(kTX2 * kTY2);
+#define FF_B(ST, PE) k_bulk2d<S, ST, PE><<<grid, block, 0, d.stream>>>(i, o, d.flags, d.nx, d.ny, d.plane, d.omega)
+    if (stage && per) FF_B(true, true);
+    else if (stage) FF_B(true, false);
+    else if (per) FF_B(false, true);
+    else FF_B(false, false);
+#undef FF_B
+  } else {
+    const int nbulk = tx * ty;
+    const long long pairs = (long long)d.ne * 8;
+    const int nbuild = (int)((pairs + 15) / 16);
+    const unsigned grid = (unsigned)(nbulk + nbuild);
+#define FF_B(ST, PE) k_bulk2d_links<S, ST, PE><<<grid, kTX2 * kTY2, 0, d.stream>>>(i, o, tx, nbuild, *la)
+    if (stage && per) FF_B(true, true);
+    else if (stage) FF_B(true, false);
+    else if (per) FF_B(false, true);
+    else FF_B(false, false);
+#undef FF_B
+  }
   FF_HIP(hipGetLastError());
   return FF_OK;
 }
 
 int launch_bulk2d(Domain& d, const void* in, void* out) {
   switch (d.storage) {
-    case FF_STORAGE_F32: return bulk2d_typed<float>(d, in, out);
-    case FF_STORAGE_F16: return bulk2d_typed<__half>(d, in, out);
-    default: return bulk2d_typed<__hip_bfloat16>(d, in, out);
+    case FF_STORAGE_F32: return bulk2d_typed<float>(d, in, out, nullptr);
+    case FF_STORAGE_F16: return bulk2d_typed<__half>(d, in, out, nullptr);
+    default: return bulk2d_typed<__hip_bfloat16>(d, in, out, nullptr);
+  }
+}
+
+int launch_bulk2d_links(Domain& d, const void* in, void* out, const LinkArgs& la) {
+  switch (d.storage) {
+    case FF_STORAGE_F32: return bulk2d_typed<float>(d, in, out, &la);
+    case FF_STORAGE_F16: return bulk2d_typed<__half>(d, in, out, &la);
+    default: return bulk2d_typed<__hip_bfloat16>(d, in, out, &la);
   }
 }
 
