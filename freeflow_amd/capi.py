@@ -28,7 +28,7 @@ FLAG = {
 EXPORTS = [
     "ff_domain_create", "ff_domain_destroy", "ff_set_flags", "ff_get_flags", "ff_init_state", "ff_set_moments",
     "ff_get_moments", "ff_set_boundary", "ff_update_boundary", "ff_set_body_ref", "ff_set_wave", "ff_step",
-    "ff_step_proxy", "ff_get_vertex_forces", "ff_get_body_wrench", "ff_set_wrench_history",
+    "ff_step_proxy", "ff_get_vertex_forces", "ff_get_step_vertex_forces", "ff_get_body_wrench", "ff_set_wrench_history",
     "ff_get_wrench_history", "ff_get_boundary", "ff_mass", "ff_fill_solid", "ff_sync", "ff_time_bulk_kernel",
     "ff_time_steps", "ff_last_error", "ff_device_count", "ff_version",
 ]
@@ -72,6 +72,7 @@ def lib():
         L.ff_step.argtypes = [vp, C.c_int]
         L.ff_step_proxy.argtypes = [vp, vp, vp, vp, vp, C.c_int, C.c_int]
         L.ff_get_vertex_forces.argtypes = [vp, vp, C.c_int]
+        L.ff_get_step_vertex_forces.argtypes = [vp, vp]
         L.ff_get_body_wrench.argtypes = [vp, vp]
         L.ff_set_wrench_history.argtypes = [vp, C.c_int]
         L.ff_get_wrench_history.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int)]
@@ -195,6 +196,11 @@ class Domain:
     def vertex_forces(self, reset=False):
         out = np.empty(self.nv * self.dim, np.float32)
         _check(lib().ff_get_vertex_forces(self.h, _ptr(out), int(reset)))
+        return out.reshape(self.nv, self.dim)
+
+    def step_vertex_forces(self):
+        out = np.empty(self.nv * self.dim, np.float32)
+        _check(lib().ff_get_step_vertex_forces(self.h, _ptr(out)))
         return out.reshape(self.nv, self.dim)
 
     def body_wrench(self):

@@ -637,6 +637,30 @@ int ff_get_vertex_forces(ff_domain* dd, float* out, int reset) {
   return check_overflow(d);
 }
 
+int ff_get_step_vertex_forces(ff_domain* dd, float* out) {
+  FF_CHECK_DOMAIN(dd);
+  Domain& d = *dd;
+  if (!d.has_boundary) return fail(FF_ERR_STATE, "no boundary set");
+  if (!out) return fail(FF_ERR_INVALID, "null output");
+  FF_HIP(hipSetDevice(d.device));
+  const size_t cnt = (size_t)d.nv * d.dim;
+  if (d.gen_host <= 1 || d.ne == 0) {  // no step has run on this boundary
+    std::memset(out, 0, cnt * sizeof(float));
+    return FF_OK;
+  }
+  float* tmp = nullptr;
+  int rc = dalloc(&tmp, cnt);
+  if (rc) return rc;
+  rc = launch_step_vertex_forces(d, tmp);
+  if (!rc) {
+    hipError_t e = hipStreamSynchronize(d.stream);
+    if (e == hipSuccess) e = hipMemcpy(out, tmp, cnt * sizeof(float), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = hip_fail(e, "step vertex forces");
+  }
+  dfree(tmp);
+  return rc;
+}
+
 int ff_get_body_wrench(ff_domain* dd, double* out) {
   FF_CHECK_DOMAIN(dd);
   Domain& d = *dd;

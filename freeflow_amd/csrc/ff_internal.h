@@ -145,6 +145,7 @@ int launch_boundary_cells(Domain& d, const void* in, void* out, const LinkArgs& 
 int launch_finalize(Domain& d, const LinkArgs& la, void* out, float* v_next0, float* v_next1, float* u_next0,
                     float* u_next1, float* proxy_forces);
 int launch_wave_now(Domain& d);
+int launch_step_vertex_forces(Domain& d, float* out_dev);
 int launch_convert_in(Domain& d, const float* aos_dev, void* buf);
 int launch_convert_out(Domain& d, const void* buf, float* aos_dev);
 int launch_mass(Domain& d, double* out_dev);

@@ -498,6 +498,34 @@ int launch_finalize(Domain& d, const LinkArgs& la, void* out, float* vn0, float*
   return FF_OK;
 }
 
+// Per-vertex forces of the most recent step from the element sums.
+template <int DIM>
+__global__ void k_step_vertex_forces(const double* __restrict__ elem_acc, const int* __restrict__ vptr,
+                                     const int* __restrict__ vinc, int nv, float* __restrict__ out) {
+  constexpr int STRIDE = DIM * DIM + DIM + (DIM == 2 ? 1 : 3);
+  const int vtx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (vtx >= nv) return;
+  double f[DIM];
+#pragma unroll
+  for (int c = 0; c < DIM; c++) f[c] = 0.0;
+  for (int k = vptr[vtx]; k < vptr[vtx + 1]; k++) {
+    const int e = vinc[k] >> 2, end = vinc[k] & 3;
+#pragma unroll
+    for (int c = 0; c < DIM; c++) f[c] += elem_acc[(size_t)e * STRIDE + end * DIM + c];
+  }
+#pragma unroll
+  for (int c = 0; c < DIM; c++) out[vtx * DIM + c] = (float)f[c];
+}
+
+int launch_step_vertex_forces(Domain& d, float* out) {
+  const int b = 256, g = (d.nv + b - 1) / b;
+  if (g == 0) return FF_OK;
+  if (d.dim == 2) k_step_vertex_forces<2><<<g, b, 0, d.stream>>>(d.elem_acc, d.vert_inc_ptr, d.vert_inc_idx, d.nv, out);
+  else k_step_vertex_forces<3><<<g, b, 0, d.stream>>>(d.elem_acc, d.vert_inc_ptr, d.vert_inc_idx, d.nv, out);
+  FF_HIP(hipGetLastError());
+  return FF_OK;
+}
+
 // Kinematics of the current step (used when the wave is (re)set).
 template <int DIM>
 __global__ void k_wave_now(const float* rest, const float* s, const int* vbody, const double* par, int nv, double t,

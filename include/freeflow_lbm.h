@@ -142,6 +142,10 @@ int ff_step_proxy(ff_domain* d, const float* d_vertices, const float* d_velociti
 /* Per-vertex hydrodynamic forces accumulated over steps (nv*dim floats);
    reset != 0 zeroes the accumulator after reading. */
 int ff_get_vertex_forces(ff_domain* d, float* out, int reset);
+/* Per-vertex forces of the most recent step only (nv*dim floats): what the
+   reference's VBD path scatters to the volume each step
+   (scatter_forces_kernel2d, src/fsi/FsiCouplingKernels.cu:77-91). */
+int ff_get_step_vertex_forces(ff_domain* d, float* out);
 /* Per-body wrench of the most recent step, FP64: 2D {Fx, Fy, Tz},
    3D {Fx, Fy, Fz, Tx, Ty, Tz} per body, torque about the body reference. */
 int ff_get_body_wrench(ff_domain* d, double* out);

@@ -1,0 +1,67 @@
+"""Small Medit .mesh files and JSON configs for the drop-in tests.
+
+The meshes are built here (a fan-triangulated disk, a Kuhn-triangulated
+cube) so that their boundaries are known in closed form.
+"""
+import json
+
+import numpy as np
+
+
+def write_mesh(path, dim, verts, edges=(), tris=(), tets=()):
+    with open(path, "w") as f:
+        f.write(f"MeshVersionFormatted 1\nDimension {dim}\n\nVertices\n{len(verts)}\n")
+        for v in verts:
+            f.write(" ".join(f"{x:.15f}" for x in v) + " 0\n")
+        for name, block in (("Edges", edges), ("Triangles", tris), ("Tetrahedra", tets)):
+            if len(block):
+                f.write(f"\n{name}\n{len(block)}\n")
+                for e in block:
+                    f.write(" ".join(str(int(k) + 1) for k in e) + " 0\n")
+        f.write("\nEnd\n")
+
+
+def disk(n, r=0.5):
+    """Centre vertex 0 plus n ring vertices, fan triangles (0, i, i+1)."""
+    a = 2 * np.pi * np.arange(n) / n
+    verts = np.vstack([[0.0, 0.0], np.stack([r * np.cos(a), r * np.sin(a)], 1)])
+    tris = [(0, 1 + i, 1 + (i + 1) % n) for i in range(n)]
+    return verts, tris
+
+
+def kuhn_cube():
+    """Unit cube as 6 positively oriented tetrahedra (one per axis order)."""
+    corners = np.array([[x, y, z] for z in (0, 1) for y in (0, 1) for x in (0, 1)], float)
+    idx = {tuple(c.astype(int)): i for i, c in enumerate(corners)}
+    tets = []
+    for perm in ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)):
+        p = np.zeros(3, int)
+        t = [idx[tuple(p)]]
+        for ax in perm:
+            p = p.copy()
+            p[ax] = 1
+            t.append(idx[tuple(p)])
+        v = corners[t]
+        if np.linalg.det(np.stack([v[1] - v[0], v[2] - v[0], v[3] - v[0]])) < 0:
+            t[1], t[2] = t[2], t[1]
+        tets.append(t)
+    return corners, tets
+
+
+def channel_boundaries(nx, ny):
+    """Walls on the bottom and top rows, then an inlet on the left column and an
+    outlet on the right one, listed in that order (later entries win)."""
+    b = []
+    for x in range(nx):
+        b.append({"type": "Wall", "pos": [x, 0]})
+        b.append({"type": "Wall", "pos": [x, ny - 1]})
+    for y in range(ny):
+        b.append({"type": "InletLeft", "pos": [0, y]})
+        b.append({"type": "OutletRight", "pos": [nx - 1, y]})
+    return b
+
+
+def write_config(path, **kw):
+    with open(path, "w") as f:
+        json.dump(kw, f, indent=1)
+    return str(path)
