@@ -20,7 +20,7 @@ LIB = os.path.join(PKG, "lib")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["capi.hip", "kernels2d.hip", "kernels3d.hip", "links.hip"]
+HIP_SOURCES = ["capi.hip", "kernels2d.hip", "kernels3d.hip", "links.hip", "batch.hip"]
 
 # -ffp-contract=off: the 2D path reproduces the reference's FP32 rounding
 # (including its FP16 tile staging) and the link predicates take discrete

@@ -30,7 +30,9 @@ EXPORTS = [
     "ff_get_moments", "ff_set_boundary", "ff_update_boundary", "ff_set_body_ref", "ff_set_wave", "ff_step",
     "ff_step_proxy", "ff_get_vertex_forces", "ff_get_step_vertex_forces", "ff_get_body_wrench", "ff_set_wrench_history",
     "ff_get_wrench_history", "ff_get_boundary", "ff_mass", "ff_fill_solid", "ff_sync", "ff_time_bulk_kernel",
-    "ff_time_steps", "ff_last_error", "ff_device_count", "ff_version",
+    "ff_time_steps", "ff_domain_footprint", "ff_set_tile", "ff_autotune", "ff_batch_create", "ff_batch_destroy", "ff_batch_add",
+    "ff_batch_remove", "ff_batch_size", "ff_batch_step", "ff_batch_sync", "ff_batch_set_tile", "ff_batch_get_tile",
+    "ff_batch_time_steps", "ff_last_error", "ff_device_count", "ff_version",
 ]
 
 
@@ -83,6 +85,19 @@ def lib():
         L.ff_time_bulk_kernel.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
         L.ff_time_steps.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
         L.ff_device_count.argtypes = [C.POINTER(C.c_int)]
+        L.ff_domain_footprint.argtypes = [C.POINTER(DomainDesc), C.c_int, C.c_int, vp, vp, C.POINTER(C.c_ulonglong)]
+        L.ff_set_tile.argtypes = [vp, C.c_int]
+        L.ff_autotune.argtypes = [vp, C.POINTER(C.c_int)]
+        L.ff_batch_create.argtypes = [C.c_int, C.c_int, C.c_uint, C.c_int, C.POINTER(vp)]
+        L.ff_batch_destroy.argtypes = [vp]
+        L.ff_batch_add.argtypes = [vp, vp]
+        L.ff_batch_remove.argtypes = [vp, vp]
+        L.ff_batch_size.argtypes = [vp, C.POINTER(C.c_int)]
+        L.ff_batch_step.argtypes = [vp, C.c_int]
+        L.ff_batch_sync.argtypes = [vp]
+        L.ff_batch_set_tile.argtypes = [vp, C.c_int]
+        L.ff_batch_get_tile.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.ff_batch_time_steps.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -100,6 +115,19 @@ def device_count():
     n = C.c_int(0)
     rc = lib().ff_device_count(C.byref(n))
     return n.value if rc == FF_OK else 0
+
+
+def footprint(dim, nx, ny, nz=1, storage="f32", verts=None, elems=None, dx=1.0):
+    """Device bytes a domain with this grid and boundary allocates (no GPU needed)."""
+    desc = DomainDesc(dim, nx, ny, nz if dim == 3 else 1, 0.01, dx, 1.0, 1.0, STORAGE[storage], 0, 0)
+    out = C.c_ulonglong(0)
+    if verts is None:
+        _check(lib().ff_domain_footprint(C.byref(desc), 0, 0, None, None, C.byref(out)))
+    else:
+        v = np.ascontiguousarray(verts, np.float32).ravel()
+        e = np.ascontiguousarray(elems, np.uint32).ravel()
+        _check(lib().ff_domain_footprint(C.byref(desc), v.size // dim, e.size // dim, _ptr(v), _ptr(e), C.byref(out)))
+    return out.value
 
 
 def cf_of(dim, rho, dx, dt):
@@ -241,4 +269,68 @@ class Domain:
     def time_steps(self, n):
         ms = C.c_double(0)
         _check(lib().ff_time_steps(self.h, n, C.byref(ms)))
+        return ms.value
+
+    def set_tile(self, tile):
+        _check(lib().ff_set_tile(self.h, tile))
+
+    def autotune(self):
+        t = C.c_int(0)
+        _check(lib().ff_autotune(self.h, C.byref(t)))
+        return t.value
+
+
+class Batch:
+    """Domains of one kind stepped together, one graph replay per step."""
+
+    def __init__(self, dim, storage="f32", stage_fp16=None, periodic=False, device=0):
+        if stage_fp16 is None:
+            stage_fp16 = dim == 2
+        opts = (OPT_STAGE_FP16 if (stage_fp16 and dim == 2) else 0) | (OPT_PERIODIC if periodic else 0)
+        self.h = C.c_void_p()
+        _check(lib().ff_batch_create(dim, STORAGE[storage], opts, device, C.byref(self.h)))
+        self.members = []
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().ff_batch_destroy(self.h)
+            self.h = C.c_void_p()
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, dom):
+        _check(lib().ff_batch_add(self.h, dom.h))
+        self.members.append(dom)
+
+    def remove(self, dom):
+        _check(lib().ff_batch_remove(self.h, dom.h))
+        self.members.remove(dom)
+
+    def __len__(self):
+        n = C.c_int(0)
+        _check(lib().ff_batch_size(self.h, C.byref(n)))
+        return n.value
+
+    def step(self, n=1):
+        _check(lib().ff_batch_step(self.h, n))
+
+    def sync(self):
+        _check(lib().ff_batch_sync(self.h))
+
+    def set_tile(self, tile):
+        _check(lib().ff_batch_set_tile(self.h, tile))
+
+    def tile(self):
+        t, x, y = C.c_int(0), C.c_int(0), C.c_int(0)
+        _check(lib().ff_batch_get_tile(self.h, C.byref(t), C.byref(x), C.byref(y)))
+        return t.value, x.value, y.value
+
+    def time_steps(self, n):
+        ms = C.c_double(0)
+        _check(lib().ff_batch_time_steps(self.h, n, C.byref(ms)))
         return ms.value

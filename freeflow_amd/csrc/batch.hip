@@ -1,0 +1,540 @@
+// batch.hip -- many fluid domains of uneven size stepped together.
+//
+// A batch holds domains of one kind (dimension, storage format, options).
+// One step of every member is one replay of a HIP graph with four kernel
+// nodes, the batched forms of the per-domain step (kernels2d/3d.hip,
+// links.hip):
+//
+//   kb_bulk      link detection of every domain + the stream-collide tiles
+//                of every domain (horizontally fused, one grid)
+//   kb_bcells    cells owning a moving-wall link, every domain
+//   kb_elem_sums per-element FP64 force sums, outlet cells, next-step wave
+//                kinematics, every domain
+//   kb_finish    per-vertex forces; per domain one block for the body wrench,
+//                the link-table generation and the domain's step counter
+//
+// Each block finds its domain and its block index within that domain in a
+// block map (int2 {domain, local block}) built on the host; per-domain
+// arguments live in a device array of DomDev records.  The ping-pong parity
+// of each domain is its own device step counter, so the graph is replayed
+// unchanged every step; it is rebuilt only when membership or a member's
+// buffers change.  The arithmetic per cell, link and element is that of the
+// per-domain kernels, so a domain's results are the same alone or in a batch.
+//
+// The 2D stream-collide tile shape is chosen per batch by timing the
+// candidates of kTiles2 on the batch itself (the bulk kernel only rewrites
+// cells every step rewrites, so timing it leaves the state unchanged).
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ff_internal.h"
+#include "geometry.h"
+#include "links_body.h"
+#include "stencil.h"
+
+namespace ff {
+
+struct DomDev {
+  LinkArgs la;
+  FinalArgs fa;
+  void* mom[2];      // mom[0] = the domain's current buffer when the batch was built
+  unsigned* step;    // steps since the batch was built (parity selects the buffer)
+  int tiles_x, tiles_y;
+  int nbc;           // boundary-cell blocks of this domain
+  int nvb;           // vertex blocks of the finish phase
+  int boundary;
+};
+
+struct Batch {
+  int dim = 2, storage = 0, device = 0;
+  unsigned options = 0;
+  hipStream_t stream = nullptr;
+  std::vector<Domain*> mem;
+  bool dirty = true;
+  int tile2 = -1;        // fixed 2D tile (index into kTiles2), -1 = autotune
+  int tuned_tile2 = -1;  // result of the last tile search
+  DomDev* d_doms = nullptr;
+  size_t doms_cap = 0;
+  int2* d_map = nullptr;
+  size_t map_cap = 0;
+  unsigned* d_steps = nullptr;
+  size_t steps_cap = 0;
+  int n_build = 0, n_bulk = 0, n_bc = 0, n_es = 0, n_fin = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+// ---------------------------------------------------------------- kernels
+template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT>
+__global__ void __launch_bounds__(256) kb_bulk2d(const DomDev* __restrict__ doms, const int2* __restrict__ map,
+                                                 int nbuild) {
+  const int2 m = map[blockIdx.x];
+  const DomDev* D = doms + m.x;
+  if ((int)blockIdx.x < nbuild) {
+    const LinkArgs la = D->la;
+    build_links_block<2>(la, (long long)m.y, la.st->gen);
+    return;
+  }
+  const unsigned p = *D->step & 1u;
+  bulk2d_tile<S, STAGE, PERIODIC, TX, TY, CPT>((const S*)D->mom[p], (S*)D->mom[p ^ 1u], D->la.flags, D->la.nx,
+                                               D->la.ny, D->la.plane, D->la.omega, m.y % D->tiles_x, m.y / D->tiles_x);
+}
+
+template <typename S, bool PERIODIC>
+__global__ void __launch_bounds__(256) kb_bulk3d(const DomDev* __restrict__ doms, const int2* __restrict__ map,
+                                                 int nbuild) {
+  const int2 m = map[blockIdx.x];
+  const DomDev* D = doms + m.x;
+  if ((int)blockIdx.x < nbuild) {
+    const LinkArgs la = D->la;
+    build_links_block<3>(la, (long long)m.y, la.st->gen);
+    return;
+  }
+  const unsigned p = *D->step & 1u;
+  const int tx = D->tiles_x, ty = D->tiles_y;
+  bulk3d_tile<S, PERIODIC>((const S*)D->mom[p], (S*)D->mom[p ^ 1u], D->la.flags, D->la.nx, D->la.ny, D->la.nz,
+                           D->la.plane, D->la.omega, m.y % tx, (m.y / tx) % ty, m.y / (tx * ty));
+}
+
+template <int DIM, typename S, bool STAGE, bool PERIODIC>
+__global__ void __launch_bounds__(256) kb_bcells(const DomDev* __restrict__ doms, const int2* __restrict__ map) {
+  const int2 m = map[blockIdx.x];
+  const DomDev* D = doms + m.x;
+  const LinkArgs la = D->la;
+  const unsigned p = *D->step & 1u;
+  bcells_body<DIM, S, STAGE, PERIODIC>((const S*)D->mom[p], (S*)D->mom[p ^ 1u], la, m.y, D->nbc);
+}
+
+template <int DIM, typename S, int FB>
+__global__ void __launch_bounds__(FB) kb_elem_sums(const DomDev* __restrict__ doms, const int2* __restrict__ map) {
+  const int2 m = map[blockIdx.x];
+  const DomDev* D = doms + m.x;
+  const LinkArgs la = D->la;
+  FinalArgs fa = D->fa;
+  fa.buf = D->mom[(*D->step & 1u) ^ 1u];
+  elem_sums_body<DIM, S, FB>(la, fa, m.y);
+}
+
+template <int DIM, int FB>
+__global__ void __launch_bounds__(FB) kb_finish(const DomDev* __restrict__ doms, const int2* __restrict__ map) {
+  const int2 m = map[blockIdx.x];
+  const DomDev* D = doms + m.x;
+  const LinkArgs la = D->la;
+  const FinalArgs fa = D->fa;
+  if (m.y < D->nvb) {
+    finish_vertex_body<DIM, FB>(la, fa, m.y);
+    return;
+  }
+  if (D->boundary) finish_last_body<DIM, FB>(la, fa);
+  if (threadIdx.x == 0) *D->step = *D->step + 1u;
+}
+
+// ---------------------------------------------------------------- launch helpers
+template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT>
+static void bulk2d_batch_shape(Batch& b, const int2* map, int nblocks, int nbuild) {
+  kb_bulk2d<S, STAGE, PERIODIC, TX, TY, CPT><<<nblocks, 256, 0, b.stream>>>(b.d_doms, map, nbuild);
+}
+
+template <typename S, bool STAGE, bool PERIODIC>
+static void bulk2d_batch_variant(Batch& b, int tile, const int2* map, int nblocks, int nbuild) {
+  switch (tile) {
+    case 1: bulk2d_batch_shape<S, STAGE, PERIODIC, 32, 8, 2>(b, map, nblocks, nbuild); break;
+    case 2: bulk2d_batch_shape<S, STAGE, PERIODIC, 64, 4, 2>(b, map, nblocks, nbuild); break;
+    case 3: bulk2d_batch_shape<S, STAGE, PERIODIC, 32, 8, 4>(b, map, nblocks, nbuild); break;
+    case 4: bulk2d_batch_shape<S, STAGE, PERIODIC, 64, 4, 4>(b, map, nblocks, nbuild); break;
+    default: bulk2d_batch_shape<S, STAGE, PERIODIC, 32, 8, 1>(b, map, nblocks, nbuild); break;
+  }
+}
+
+template <typename S>
+static void launch_bulk_batch(Batch& b, int tile, const int2* map, int nblocks, int nbuild) {
+  if (nblocks <= 0) return;
+  const bool stage = b.options & FF_OPT_STAGE_FP16, per = b.options & FF_OPT_PERIODIC;
+  if (b.dim == 2) {
+    if (stage && per) bulk2d_batch_variant<S, true, true>(b, tile, map, nblocks, nbuild);
+    else if (stage) bulk2d_batch_variant<S, true, false>(b, tile, map, nblocks, nbuild);
+    else if (per) bulk2d_batch_variant<S, false, true>(b, tile, map, nblocks, nbuild);
+    else bulk2d_batch_variant<S, false, false>(b, tile, map, nblocks, nbuild);
+  } else {
+    if (per) kb_bulk3d<S, true><<<nblocks, 256, 0, b.stream>>>(b.d_doms, map, nbuild);
+    else kb_bulk3d<S, false><<<nblocks, 256, 0, b.stream>>>(b.d_doms, map, nbuild);
+  }
+}
+
+template <int DIM, typename S>
+static void launch_phases(Batch& b, const int2* map_bc, const int2* map_es, const int2* map_fin) {
+  const bool stage = DIM == 2 && (b.options & FF_OPT_STAGE_FP16), per = b.options & FF_OPT_PERIODIC;
+  if (b.n_bc > 0) {
+    if (stage && per) kb_bcells<DIM, S, true, true><<<b.n_bc, 256, 0, b.stream>>>(b.d_doms, map_bc);
+    else if (stage) kb_bcells<DIM, S, true, false><<<b.n_bc, 256, 0, b.stream>>>(b.d_doms, map_bc);
+    else if (per) kb_bcells<DIM, S, false, true><<<b.n_bc, 256, 0, b.stream>>>(b.d_doms, map_bc);
+    else kb_bcells<DIM, S, false, false><<<b.n_bc, 256, 0, b.stream>>>(b.d_doms, map_bc);
+  }
+  constexpr int FB = DIM == 2 ? kElemBlock2 : kElemBlock3;
+  if (b.n_es > 0) kb_elem_sums<DIM, S, FB><<<b.n_es, FB, 0, b.stream>>>(b.d_doms, map_es);
+  if (b.n_fin > 0) kb_finish<DIM, 256><<<b.n_fin, 256, 0, b.stream>>>(b.d_doms, map_fin);
+}
+
+template <typename S>
+static void launch_step_typed(Batch& b, int tile) {
+  const int2* map_bulk = b.d_map;
+  const int2* map_bc = map_bulk + b.n_build + b.n_bulk;
+  const int2* map_es = map_bc + b.n_bc;
+  const int2* map_fin = map_es + b.n_es;
+  launch_bulk_batch<S>(b, tile, map_bulk, b.n_build + b.n_bulk, b.n_build);
+  if (b.dim == 2) launch_phases<2, S>(b, map_bc, map_es, map_fin);
+  else launch_phases<3, S>(b, map_bc, map_es, map_fin);
+}
+
+static void launch_step(Batch& b, int tile) {
+  if (b.storage == FF_STORAGE_F32) launch_step_typed<float>(b, tile);
+  else if (b.storage == FF_STORAGE_F16) launch_step_typed<__half>(b, tile);
+  else launch_step_typed<__hip_bfloat16>(b, tile);
+}
+
+static void launch_bulk_only(Batch& b, int tile, const int2* map, int nblocks) {
+  if (b.storage == FF_STORAGE_F32) launch_bulk_batch<float>(b, tile, map, nblocks, 0);
+  else if (b.storage == FF_STORAGE_F16) launch_bulk_batch<__half>(b, tile, map, nblocks, 0);
+  else launch_bulk_batch<__hip_bfloat16>(b, tile, map, nblocks, 0);
+}
+
+// ---------------------------------------------------------------- host side
+template <class T>
+static int grow(T** p, size_t* cap, size_t need) {
+  if (need <= *cap && *p) return FF_OK;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  size_t n = std::max<size_t>(need, 16);
+  n += n / 2;
+  FF_HIP(hipMalloc((void**)p, n * sizeof(T)));
+  *cap = n;
+  return FF_OK;
+}
+
+static void tiles_of(const Batch& b, const Domain& d, int tile, int* tx, int* ty, int* nt) {
+  if (b.dim == 2) {
+    const Tile2 t = kTiles2[tile];
+    *tx = (d.nx + t.tx - 1) / t.tx;
+    *ty = (d.ny + t.ty * t.cpt - 1) / (t.ty * t.cpt);
+    *nt = *tx * *ty;
+  } else {
+    *tx = (d.nx + kTX3 - 1) / kTX3;
+    *ty = (d.ny + kTY3 - 1) / kTY3;
+    *nt = *tx * *ty * ((d.nz + kTZ3 - 1) / kTZ3);
+  }
+}
+
+// Records and block maps for the current members and `tile`; uploads them.
+static int build_tables(Batch& b, int tile) {
+  const int n = (int)b.mem.size();
+  std::vector<DomDev> doms(n);
+  std::vector<int2> build, bulk, bc, es, fin;
+  const int K = b.dim == 2 ? 8 : 26;
+  const int fb = b.dim == 2 ? kElemBlock2 : kElemBlock3;
+  const int cpb = b.dim == 2 ? 16 : 8;  // boundary cells per block in bcells_body
+  int rc = grow(&b.d_steps, &b.steps_cap, (size_t)std::max(n, 1));
+  if (rc) return rc;
+  for (int i = 0; i < n; i++) {
+    Domain& d = *b.mem[i];
+    DomDev& r = doms[i];
+    std::memset(&r, 0, sizeof(r));
+    const bool bnd = d.has_boundary && d.ne > 0;
+    r.la = link_args(d, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], d.elems);
+    r.fa = make_final_args(d, nullptr, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], nullptr);
+    r.mom[0] = d.mom[d.cur];
+    r.mom[1] = d.mom[d.cur ^ 1];
+    r.step = b.d_steps + i;
+    r.boundary = bnd ? 1 : 0;
+    int tx, ty, nt;
+    tiles_of(b, d, tile, &tx, &ty, &nt);
+    r.tiles_x = tx;
+    r.tiles_y = ty;
+    for (int t = 0; t < nt; t++) bulk.push_back(make_int2(i, t));
+    if (bnd) {
+      const int nbuild = (int)(((long long)d.ne * K + 15) / 16);
+      for (int t = 0; t < nbuild; t++) build.push_back(make_int2(i, t));
+      r.nbc = (int)std::min<unsigned long long>(512, std::max<unsigned long long>(1, (d.lt_est / 4 + cpb - 1) / cpb));
+      for (int t = 0; t < r.nbc; t++) bc.push_back(make_int2(i, t));
+      r.nvb = (d.nv + 255) / 256;
+    }
+    const int wave_blocks = (bnd && d.wave) ? (d.nv + fb - 1) / fb : 0;
+    const int nes = (bnd ? d.ne : 0) + r.fa.outlet_blocks + wave_blocks;
+    if (!bnd) r.la.ne = 0;
+    for (int t = 0; t < nes; t++) es.push_back(make_int2(i, t));
+    for (int t = 0; t <= r.nvb; t++) fin.push_back(make_int2(i, t));
+  }
+  b.n_build = (int)build.size();
+  b.n_bulk = (int)bulk.size();
+  b.n_bc = (int)bc.size();
+  b.n_es = (int)es.size();
+  b.n_fin = (int)fin.size();
+  std::vector<int2> map;
+  map.reserve(build.size() + bulk.size() + bc.size() + es.size() + fin.size());
+  for (auto* v : {&build, &bulk, &bc, &es, &fin}) map.insert(map.end(), v->begin(), v->end());
+  if ((rc = grow(&b.d_doms, &b.doms_cap, (size_t)std::max(n, 1)))) return rc;
+  if ((rc = grow(&b.d_map, &b.map_cap, std::max<size_t>(map.size(), 1)))) return rc;
+  if (n) FF_HIP(hipMemcpyAsync(b.d_doms, doms.data(), n * sizeof(DomDev), hipMemcpyHostToDevice, b.stream));
+  if (!map.empty())
+    FF_HIP(hipMemcpyAsync(b.d_map, map.data(), map.size() * sizeof(int2), hipMemcpyHostToDevice, b.stream));
+  FF_HIP(hipMemsetAsync(b.d_steps, 0, std::max(n, 1) * sizeof(unsigned), b.stream));
+  FF_HIP(hipStreamSynchronize(b.stream));
+  return FF_OK;
+}
+
+// Time the stream-collide tiles of every member for each candidate shape
+// and keep the fastest.
+static int autotune(Batch& b) {
+  if (b.dim != 2 || b.mem.empty()) {
+    b.tuned_tile2 = 0;
+    return FF_OK;
+  }
+  hipEvent_t e0, e1;
+  FF_HIP(hipEventCreate(&e0));
+  FF_HIP(hipEventCreate(&e1));
+  double best = 1e30;
+  int best_t = 0;
+  int rc = FF_OK;
+  for (int t = 0; t < kNumTiles2 && !rc; t++) {
+    if ((rc = build_tables(b, t))) break;
+    const int2* map = b.d_map + b.n_build;
+    for (int w = 0; w < 3; w++) launch_bulk_only(b, t, map, b.n_bulk);
+    constexpr int iters = 10;
+    FF_HIP(hipEventRecord(e0, b.stream));
+    for (int k = 0; k < iters; k++) launch_bulk_only(b, t, map, b.n_bulk);
+    FF_HIP(hipEventRecord(e1, b.stream));
+    FF_HIP(hipEventSynchronize(e1));
+    FF_HIP(hipGetLastError());
+    float ms = 0.f;
+    FF_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) {
+      best = ms;
+      best_t = t;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b.tuned_tile2 = best_t;
+  return rc;
+}
+
+static void drop_graph(Batch& b) {
+  if (b.exec) (void)hipGraphExecDestroy(b.exec);
+  if (b.graph) (void)hipGraphDestroy(b.graph);
+  b.exec = nullptr;
+  b.graph = nullptr;
+}
+
+static int current_tile(const Batch& b) {
+  if (b.dim != 2) return 0;
+  return b.tile2 >= 0 ? b.tile2 : std::max(b.tuned_tile2, 0);
+}
+
+static int rebuild(Batch& b) {
+  FF_HIP(hipSetDevice(b.device));
+  FF_HIP(hipStreamSynchronize(b.stream));
+  drop_graph(b);
+  int rc;
+  if (b.dim == 2 && b.tile2 < 0 && b.tuned_tile2 < 0 && (rc = autotune(b))) return rc;
+  if ((rc = build_tables(b, current_tile(b)))) return rc;
+  FF_HIP(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
+  launch_step(b, current_tile(b));
+  hipError_t le = hipGetLastError();
+  hipGraph_t g = nullptr;
+  hipError_t ce = hipStreamEndCapture(b.stream, &g);
+  if (le != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return hip_fail(le, "batched step launch");
+  }
+  if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture");
+  b.graph = g;
+  FF_HIP(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
+  b.dirty = false;
+  return FF_OK;
+}
+
+void batch_mark_dirty(Batch* b) {
+  if (b) b->dirty = true;
+}
+
+static int detach(Batch& b, Domain& d) {
+  auto it = std::find(b.mem.begin(), b.mem.end(), &d);
+  if (it == b.mem.end()) return fail(FF_ERR_INVALID, "domain is not in this batch");
+  FF_HIP(hipSetDevice(b.device));
+  FF_HIP(hipStreamSynchronize(b.stream));
+  b.mem.erase(it);
+  d.stream = d.own_stream;
+  d.own_stream = nullptr;
+  d.batch = nullptr;
+  b.dirty = true;
+  return FF_OK;
+}
+
+int batch_detach(Domain& d) { return d.batch ? detach(*d.batch, d) : FF_OK; }
+
+// Host mirrors of the device-side step state after `nsteps` replays.
+static void advance_mirrors(Batch& b, int nsteps) {
+  for (Domain* dp : b.mem) {
+    Domain& d = *dp;
+    if (nsteps & 1) d.cur ^= 1;
+    if (d.has_boundary && d.ne > 0) {
+      d.gen_host += (unsigned)nsteps;
+      if (d.hist_cap > 0) d.hist_count += nsteps;
+    }
+  }
+}
+
+}  // namespace ff
+
+using namespace ff;
+
+struct ff_batch : public ff::Batch {};
+
+#define FF_CHECK_BATCH(b) \
+  if (!(b)) return fail(FF_ERR_INVALID, "null batch")
+
+extern "C" {
+
+int ff_batch_create(int dim, int storage, unsigned options, int device, ff_batch** out) {
+  if (!out) return fail(FF_ERR_INVALID, "null output");
+  *out = nullptr;
+  if (dim != 2 && dim != 3) return fail(FF_ERR_INVALID, "dim must be 2 or 3");
+  if (storage < 0 || storage > 2) return fail(FF_ERR_INVALID, "unknown storage format");
+  if (options & ~(FF_OPT_STAGE_FP16 | FF_OPT_PERIODIC)) return fail(FF_ERR_INVALID, "unknown option bits");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(FF_ERR_NODEVICE, "no HIP device available");
+  if (device < 0 || device >= ndev) return fail(FF_ERR_INVALID, "device ordinal out of range");
+  FF_HIP(hipSetDevice(device));
+  ff_batch* b = new ff_batch();
+  b->dim = dim;
+  b->storage = storage;
+  b->options = dim == 3 ? (options & ~FF_OPT_STAGE_FP16) : options;
+  b->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete b;
+    return hip_fail(e, "hipStreamCreate");
+  }
+  *out = b;
+  return FF_OK;
+}
+
+int ff_batch_destroy(ff_batch* b) {
+  if (!b) return FF_OK;
+  (void)hipSetDevice(b->device);
+  (void)hipStreamSynchronize(b->stream);
+  while (!b->mem.empty()) (void)detach(*b, *b->mem.back());
+  drop_graph(*b);
+  if (b->d_doms) (void)hipFree(b->d_doms);
+  if (b->d_map) (void)hipFree(b->d_map);
+  if (b->d_steps) (void)hipFree(b->d_steps);
+  (void)hipStreamDestroy(b->stream);
+  delete b;
+  return FF_OK;
+}
+
+int ff_batch_add(ff_batch* b, ff_domain* dom) {
+  FF_CHECK_BATCH(b);
+  if (!dom) return fail(FF_ERR_INVALID, "null domain");
+  Domain& d = *reinterpret_cast<Domain*>(dom);
+  if (d.batch) return fail(FF_ERR_STATE, "domain already belongs to a batch");
+  if (d.dim != b->dim || d.storage != b->storage || d.options != b->options || d.device != b->device)
+    return fail(FF_ERR_INVALID, "domain kind (dim, storage, options, device) differs from the batch");
+  FF_HIP(hipSetDevice(b->device));
+  FF_HIP(hipStreamSynchronize(d.stream));
+  FF_HIP(hipStreamSynchronize(b->stream));
+  d.own_stream = d.stream;
+  d.stream = b->stream;
+  d.batch = b;
+  b->mem.push_back(&d);
+  b->dirty = true;
+  return FF_OK;
+}
+
+int ff_batch_remove(ff_batch* b, ff_domain* dom) {
+  FF_CHECK_BATCH(b);
+  if (!dom) return fail(FF_ERR_INVALID, "null domain");
+  return detach(*b, *reinterpret_cast<Domain*>(dom));
+}
+
+int ff_batch_size(ff_batch* b, int* n) {
+  FF_CHECK_BATCH(b);
+  if (!n) return fail(FF_ERR_INVALID, "null output");
+  *n = (int)b->mem.size();
+  return FF_OK;
+}
+
+int ff_batch_step(ff_batch* b, int nsteps) {
+  FF_CHECK_BATCH(b);
+  if (nsteps < 0) return fail(FF_ERR_INVALID, "negative step count");
+  if (b->mem.empty() || nsteps == 0) return FF_OK;
+  FF_HIP(hipSetDevice(b->device));
+  int rc;
+  if (b->dirty && (rc = rebuild(*b))) return rc;
+  for (int s = 0; s < nsteps; s++) FF_HIP(hipGraphLaunch(b->exec, b->stream));
+  advance_mirrors(*b, nsteps);
+  return FF_OK;
+}
+
+int ff_batch_sync(ff_batch* b) {
+  FF_CHECK_BATCH(b);
+  FF_HIP(hipSetDevice(b->device));
+  FF_HIP(hipStreamSynchronize(b->stream));
+  for (Domain* d : b->mem) {
+    if (!d->has_boundary) continue;
+    StepState st;
+    FF_HIP(hipMemcpy(&st, d->st, sizeof(st), hipMemcpyDeviceToHost));
+    if (st.overflow) return fail(FF_ERR_CAPACITY, "boundary link table overflow in a batched domain");
+  }
+  return FF_OK;
+}
+
+int ff_batch_set_tile(ff_batch* b, int tile) {
+  FF_CHECK_BATCH(b);
+  if (tile < -1 || tile >= kNumTiles2) return fail(FF_ERR_INVALID, "tile index out of range");
+  b->tile2 = tile;
+  if (tile < 0) b->tuned_tile2 = -1;
+  b->dirty = true;
+  return FF_OK;
+}
+
+int ff_batch_get_tile(ff_batch* b, int* tile, int* tx, int* ty) {
+  FF_CHECK_BATCH(b);
+  if (!tile) return fail(FF_ERR_INVALID, "null output");
+  FF_HIP(hipSetDevice(b->device));
+  int rc;
+  if (b->dirty && !b->mem.empty() && (rc = rebuild(*b))) return rc;
+  const int t = current_tile(*b);
+  *tile = t;
+  if (tx) *tx = b->dim == 2 ? kTiles2[t].tx : kTX3;
+  if (ty) *ty = b->dim == 2 ? kTiles2[t].ty * kTiles2[t].cpt : kTY3;
+  return FF_OK;
+}
+
+int ff_batch_time_steps(ff_batch* b, int nsteps, double* total_ms) {
+  FF_CHECK_BATCH(b);
+  if (nsteps < 0 || !total_ms) return fail(FF_ERR_INVALID, "bad arguments");
+  FF_HIP(hipSetDevice(b->device));
+  int rc;
+  if (!b->mem.empty() && b->dirty && (rc = rebuild(*b))) return rc;
+  hipEvent_t e0, e1;
+  FF_HIP(hipEventCreate(&e0));
+  FF_HIP(hipEventCreate(&e1));
+  FF_HIP(hipEventRecord(e0, b->stream));
+  if (!b->mem.empty())
+    for (int s = 0; s < nsteps; s++) FF_HIP(hipGraphLaunch(b->exec, b->stream));
+  FF_HIP(hipEventRecord(e1, b->stream));
+  FF_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  FF_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (!b->mem.empty()) advance_mirrors(*b, nsteps);
+  *total_ms = ms;
+  return ff_batch_sync(b);
+}
+
+}  // extern "C"

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "ff_internal.h"
+#include "stencil.h"
 
 struct ff_domain : public ff::Domain {};
 
@@ -180,6 +181,39 @@ static unsigned next_pow2(unsigned long long v) {
   return p;
 }
 
+// Candidate link cells of a boundary: the cells of every element's box
+// (in cells, widened by 1.5 cells on each side), summed over elements.
+static unsigned long long link_estimate(int D, double dx, int ne, const float* verts, const uint32_t* elems) {
+  unsigned long long est = 0;
+  for (int k = 0; k < ne; k++) {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int j = 0; j < D; j++)
+      for (int c = 0; c < D; c++) {
+        const double x = verts[(size_t)elems[(size_t)k * D + j] * D + c] / dx;
+        lo[c] = std::min(lo[c], x);
+        hi[c] = std::max(hi[c], x);
+      }
+    unsigned long long cells = 1;
+    for (int c = 0; c < D; c++) cells *= (unsigned long long)(hi[c] - lo[c] + 4.0);
+    est += cells;
+  }
+  return est;
+}
+
+// Device bytes a domain allocates: moments, flags, scratch, and with a
+// boundary its geometry, reductions and link table (as ff_set_boundary sizes it).
+static unsigned long long footprint(int dim, long long n, int storage, int nv, int ne, unsigned long long est) {
+  const int nm = dim == 2 ? 6 : 10, q = dim == 2 ? 9 : 27, D = dim;
+  unsigned long long b = 2ull * plane_stride(n) * nm * storage_bytes(storage) + (unsigned long long)n + 1024 * 8;
+  if (nv > 0 || ne > 0) {
+    const unsigned long long K = q - 1, cap = next_pow2(est * 2 + 1024);
+    b += 5ull * nv * D * 4 + (unsigned long long)ne * (D * 4 + 4 + 4 + D * 4 + (D * D + D + 3) * 8);
+    b += (unsigned long long)(nv + 1) * 4 + 64;
+    b += cap * (8 + K * 8 + 3 * K * D * 4 + 2 * 4) + 64;
+  }
+  return b;
+}
+
 LinkArgs link_args(Domain& d, const float* v0, const float* v1, const float* u0, const float* u1, const uint32_t* e) {
   LinkArgs la;
   la.v[0] = v0;
@@ -345,6 +379,7 @@ int ff_domain_destroy(ff_domain* dd) {
   if (!dd) return FF_OK;
   Domain& d = *dd;
   (void)hipSetDevice(d.device);
+  (void)batch_detach(d);
   if (d.stream) (void)hipStreamSynchronize(d.stream);
   free_boundary(d);
   for (int b = 0; b < 2; b++) if (d.mom[b]) (void)hipFree(d.mom[b]);
@@ -363,6 +398,7 @@ int ff_set_flags(ff_domain* dd, const uint8_t* flags) {
   FF_HIP(hipSetDevice(d.device));
   std::memcpy(d.h_flags.data(), flags, (size_t)d.n);
   FF_HIP(hipStreamSynchronize(d.stream));
+  batch_mark_dirty(d.batch);
   FF_HIP(hipMemcpy(d.flags, flags, (size_t)d.n, hipMemcpyHostToDevice));
   std::vector<OutletCell> oc;
   build_outlets(d, oc);
@@ -395,7 +431,9 @@ int ff_init_state(ff_domain* dd, const float* bc, int n_bc) {
   if (bc && n_bc == need) for (int k = 0; k < need; k++) b[k] = bc[k];
   std::vector<float> aos;
   init_state_host(d, b, aos);
+  FF_HIP(hipStreamSynchronize(d.stream));
   d.cur = 0;
+  batch_mark_dirty(d.batch);
   return upload_aos(d, aos.data(), -1);
 }
 
@@ -448,6 +486,7 @@ int ff_set_boundary(ff_domain* dd, int nv, const float* verts, const float* vels
     }
   FF_HIP(hipSetDevice(d.device));
   FF_HIP(hipStreamSynchronize(d.stream));
+  batch_mark_dirty(d.batch);
   free_boundary(d);
   d.nv = nv;
   d.ne = ne;
@@ -516,20 +555,9 @@ int ff_set_boundary(ff_domain* dd, int nv, const float* verts, const float* vels
   FF_HIP(hipMemcpy(d.body_ref, ref.data(), ref.size() * sizeof(double), hipMemcpyHostToDevice));
   FF_HIP(hipMemset(d.wrench, 0, (size_t)n_bodies * wrench_width(D) * sizeof(double)));
   // link table capacity: candidate cells of every element's box, with margin
-  unsigned long long est = 0;
-  for (int k = 0; k < ne; k++) {
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    for (int j = 0; j < D; j++)
-      for (int c = 0; c < D; c++) {
-        const double x = verts[(size_t)elems[(size_t)k * D + j] * D + c] / d.dx;
-        lo[c] = std::min(lo[c], x);
-        hi[c] = std::max(hi[c], x);
-      }
-    unsigned long long cells = 1;
-    for (int c = 0; c < D; c++) cells *= (unsigned long long)(hi[c] - lo[c] + 4.0);
-    est += cells;
-  }
+  const unsigned long long est = link_estimate(D, d.dx, ne, verts, elems);
   d.lt_cap = (int)next_pow2(est * 2 + 1024);
+  d.lt_est = est;
   const size_t K = (size_t)(d.q - 1);
   const size_t cap = (size_t)d.lt_cap;
   if ((rc = dalloc(&d.lt_cells, cap)) || (rc = dalloc(&d.lt_keys, cap * K)) || (rc = dalloc(&d.lk_f, cap * K * D)) ||
@@ -561,6 +589,7 @@ int ff_update_boundary(ff_domain* dd, const float* verts, const float* vels) {
     FF_HIP(hipMemcpyAsync(d.ubuf[b], vels, bytes, hipMemcpyHostToDevice, d.stream));
   }
   FF_HIP(hipStreamSynchronize(d.stream));
+  if (d.wave) batch_mark_dirty(d.batch);
   d.wave = false;
   return FF_OK;
 }
@@ -590,6 +619,7 @@ int ff_set_wave(ff_domain* dd, const float* rest, const float* s, const double* 
   FF_HIP(hipMemcpy(d.rest_verts, rest, (size_t)d.nv * d.dim * sizeof(float), hipMemcpyHostToDevice));
   FF_HIP(hipMemcpy(d.s_coord, s, (size_t)d.nv * sizeof(float), hipMemcpyHostToDevice));
   FF_HIP(hipMemcpy(d.wave_params, params, (size_t)d.n_bodies * 4 * sizeof(double), hipMemcpyHostToDevice));
+  if (!d.wave) batch_mark_dirty(d.batch);
   d.wave = true;
   // kinematics of the next step to run; later steps are produced by the
   // finalize kernel of the step before
@@ -602,6 +632,7 @@ int ff_step(ff_domain* dd, int nsteps) {
   FF_CHECK_DOMAIN(dd);
   if (nsteps < 0) return fail(FF_ERR_INVALID, "negative step count");
   Domain& d = *dd;
+  if (d.batch) return fail(FF_ERR_STATE, "domain belongs to a batch: step it with ff_batch_step");
   FF_HIP(hipSetDevice(d.device));
   for (int s = 0; s < nsteps; s++) {
     int rc = one_step(d, nullptr, nullptr, nullptr, nullptr);
@@ -617,6 +648,7 @@ int ff_step_proxy(ff_domain* dd, const float* dv, const float* dvel, const uint3
   if (!d.has_boundary) return fail(FF_ERR_STATE, "ff_set_boundary must describe the proxy topology first");
   if (nv != d.nv || ne != d.ne) return fail(FF_ERR_INVALID, "proxy sizes differ from the boundary topology");
   if (!dv || !dvel || !de) return fail(FF_ERR_INVALID, "null proxy pointer");
+  if (d.batch) return fail(FF_ERR_STATE, "domain belongs to a batch: step it with ff_batch_step");
   FF_HIP(hipSetDevice(d.device));
   return one_step(d, dv, dvel, de, dforces);
 }
@@ -679,6 +711,7 @@ int ff_set_wrench_history(ff_domain* dd, int capacity) {
   if (capacity < 0) return fail(FF_ERR_INVALID, "negative capacity");
   FF_HIP(hipSetDevice(d.device));
   FF_HIP(hipStreamSynchronize(d.stream));
+  batch_mark_dirty(d.batch);
   dfree(d.wrench_hist);
   d.hist_cap = 0;
   d.hist_count = 0;
@@ -765,6 +798,7 @@ int ff_time_bulk_kernel(ff_domain* dd, int iters, double* avg_ms) {
   FF_CHECK_DOMAIN(dd);
   if (iters <= 0 || !avg_ms) return fail(FF_ERR_INVALID, "bad arguments");
   Domain& d = *dd;
+  if (d.batch) return fail(FF_ERR_STATE, "domain belongs to a batch");
   FF_HIP(hipSetDevice(d.device));
   std::vector<hipEvent_t> ev(2 * iters);
   for (auto& e : ev) FF_HIP(hipEventCreate(&e));
@@ -797,6 +831,7 @@ int ff_time_steps(ff_domain* dd, int nsteps, double* total_ms) {
   FF_CHECK_DOMAIN(dd);
   if (nsteps < 0 || !total_ms) return fail(FF_ERR_INVALID, "bad arguments");
   Domain& d = *dd;
+  if (d.batch) return fail(FF_ERR_STATE, "domain belongs to a batch: time it with ff_batch_time_steps");
   FF_HIP(hipSetDevice(d.device));
   hipEvent_t a, b;
   FF_HIP(hipEventCreate(&a));
@@ -813,6 +848,66 @@ int ff_time_steps(ff_domain* dd, int nsteps, double* total_ms) {
   *total_ms = ms;
   if (rc) return rc;
   return check_overflow(d);
+}
+
+int ff_domain_footprint(const ff_domain_desc* desc, int nv, int ne, const float* verts, const uint32_t* elems,
+                        unsigned long long* bytes) {
+  if (!desc || !bytes) return fail(FF_ERR_INVALID, "null argument");
+  if (desc->dim != 2 && desc->dim != 3) return fail(FF_ERR_INVALID, "dim must be 2 or 3");
+  if (nv < 0 || ne < 0 || (ne > 0 && (!verts || !elems))) return fail(FF_ERR_INVALID, "bad boundary arguments");
+  if (desc->storage < 0 || desc->storage > 2) return fail(FF_ERR_INVALID, "unknown storage format");
+  const long long n = (long long)desc->nx * desc->ny * (desc->dim == 3 ? desc->nz : 1);
+  for (long long k = 0; k < (long long)ne * desc->dim; k++)
+    if (elems[k] >= (uint32_t)nv) return fail(FF_ERR_INVALID, "element references a vertex out of range");
+  const unsigned long long est = ne ? link_estimate(desc->dim, desc->dx, ne, verts, elems) : 0;
+  *bytes = footprint(desc->dim, n, desc->storage, nv, ne, est);
+  return FF_OK;
+}
+
+int ff_set_tile(ff_domain* dd, int tile) {
+  FF_CHECK_DOMAIN(dd);
+  if (tile < 0 || tile >= kNumTiles2) return fail(FF_ERR_INVALID, "tile index out of range");
+  if (dd->dim != 2) return fail(FF_ERR_INVALID, "tile shapes apply to 2D domains");
+  dd->tile2 = tile;
+  batch_mark_dirty(dd->batch);
+  return FF_OK;
+}
+
+int ff_autotune(ff_domain* dd, int* tile) {
+  FF_CHECK_DOMAIN(dd);
+  Domain& d = *dd;
+  if (d.batch) return fail(FF_ERR_STATE, "domain belongs to a batch: its tile is tuned with the batch");
+  if (d.dim != 2) {
+    if (tile) *tile = 0;
+    return FF_OK;
+  }
+  FF_HIP(hipSetDevice(d.device));
+  hipEvent_t e0, e1;
+  FF_HIP(hipEventCreate(&e0));
+  FF_HIP(hipEventCreate(&e1));
+  // the stream-collide launch rewrites only cells every step rewrites, so
+  // timing it leaves the state unchanged
+  void* in = d.mom[d.cur];
+  void* out = d.mom[d.cur ^ 1];
+  double best = 1e30;
+  int best_t = 0, rc = FF_OK;
+  const int saved = d.tile2;
+  for (int t = 0; t < kNumTiles2 && !rc; t++) {
+    d.tile2 = t;
+    for (int w = 0; w < 3 && !rc; w++) rc = launch_bulk2d(d, in, out);
+    FF_HIP(hipEventRecord(e0, d.stream));
+    for (int k = 0; k < 20 && !rc; k++) rc = launch_bulk2d(d, in, out);
+    FF_HIP(hipEventRecord(e1, d.stream));
+    FF_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    FF_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) { best = ms; best_t = t; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  d.tile2 = rc ? saved : best_t;
+  if (tile) *tile = d.tile2;
+  return rc;
 }
 
 }  // extern "C"

@@ -11,6 +11,8 @@
 
 namespace ff {
 
+struct Batch;
+
 // Node flags: ordinals of fsi::lbm::LbmNodeFlag (LbmDataTypes.cuh:17-56).
 enum Flag : uint8_t {
   kInvalid = 0, kFluid, kFluidRest, kSolid, kSolidDynamic, kWall,
@@ -81,6 +83,7 @@ struct Domain {
   int q = 9;         // lattice directions
   int storage = FF_STORAGE_F32;
   unsigned options = 0;
+  int tile2 = 0;     // 2D stream-collide tile shape (index into kTiles2)
   double nu = 0, dx = 1, dt = 1, rho_f = 1;
   float nu_f = 0, cl = 1, ct = 1, cf = 1, omega = 1;
   size_t plane = 0;  // elements between moment planes
@@ -127,7 +130,16 @@ struct Domain {
   // scratch
   double* red = nullptr;          // reduction scratch
   bool has_boundary = false;
+  unsigned long long lt_est = 0;  // candidate link cells estimated at ff_set_boundary
+  // batch membership: while set, `stream` is the batch's stream and the
+  // domain's own stream is parked in `own_stream`
+  Batch* batch = nullptr;
+  hipStream_t own_stream = nullptr;
 };
+
+// Batch bookkeeping used by the per-domain entry points (batch.hip).
+void batch_mark_dirty(Batch* b);
+int batch_detach(Domain& d);
 
 // plane stride: moment planes aligned to 256 elements
 inline size_t plane_stride(long long n) { return (size_t)((n + 255) / 256 * 256); }

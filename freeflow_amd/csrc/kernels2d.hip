@@ -13,119 +13,70 @@
 // reference (FsiCoupling2D.cu:168-223: `is_blocked` stays false on the wall
 // branch), so the bulk kernel never needs neighbour flags.
 
-#include <type_traits>
-
 #include "ff_internal.h"
 #include "geometry.h"
-#include "hermite.h"
+#include "stencil.h"
 
 namespace ff {
 
-constexpr int kTX2 = 32, kTY2 = 8;
-
-// One 32x8 tile: stage the reconstructed populations of the tile and its
-// one-cell halo in LDS (clamped or periodic halo), then pull, collide and
-// store every updated cell.  `bx, by` select the tile.
-template <typename S, bool STAGE, bool PERIODIC>
-__device__ __forceinline__ void bulk2d_tile(const S* __restrict__ in, S* __restrict__ out,
-                                            const uint8_t* __restrict__ flags, int nx, int ny, size_t plane,
-                                            float omega, int bx, int by) {
-  constexpr int HX = kTX2 + 2, HY = kTY2 + 2, HN = HX * HY;
-  using T = typename std::conditional<STAGE, __half, float>::type;
-  __shared__ T sp[9][HN];
-  const int x0 = bx * kTX2, y0 = by * kTY2;
-  for (int k = threadIdx.x; k < HN; k += kTX2 * kTY2) {
-    const int hx = k % HX, hy = k / HX;
-    int gx = x0 + hx - 1, gy = y0 + hy - 1;
-    if (PERIODIC) {
-      gx = ((gx % nx) + nx) % nx;
-      gy = ((gy % ny) + ny) % ny;
-    } else {
-      gx = min(nx - 1, max(0, gx));
-      gy = min(ny - 1, max(0, gy));
-    }
-    const size_t c = (size_t)gy * nx + gx;
-    float m[6];
-#pragma unroll
-    for (int q = 0; q < 6; q++) m[q] = Store<S>::template load<2>(in + q * plane + c, q);
-    float f[9];
-    recon2_all(m[0], m[1], m[2], m[3], m[5], m[4], f);
-#pragma unroll
-    for (int i = 0; i < 9; i++) {
-      if constexpr (STAGE) sp[i][k] = to_half_rn(f[i]);
-      else sp[i][k] = f[i];
-    }
-  }
-  __syncthreads();
-  const int lx = threadIdx.x % kTX2, ly = threadIdx.x / kTX2;
-  const int x = x0 + lx, y = y0 + ly;
-  if (x >= nx || y >= ny) return;
-  const size_t cell = (size_t)y * nx + x;
-  if (!is_updated(flags[cell])) return;
-  float p[9];
-#pragma unroll
-  for (int i = 0; i < 9; i++) {
-    const int idx = (ly + 1 - D2::cy(i)) * HX + (lx + 1 - D2::cx(i));
-    float v;
-    if constexpr (STAGE) v = __half2float(sp[i][idx]);
-    else v = sp[i][idx];
-    p[i] = fmaxf(0.0f, v);
-  }
-  float o[6];
-  collide2(p, omega, o);
-#pragma unroll
-  for (int q = 0; q < 6; q++) Store<S>::template store<2>(out + q * plane + cell, q, o[q]);
-}
-
-template <typename S, bool STAGE, bool PERIODIC>
-__global__ void __launch_bounds__(kTX2* kTY2)
+template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT>
+__global__ void __launch_bounds__(256)
     k_bulk2d(const S* __restrict__ in, S* __restrict__ out, const uint8_t* __restrict__ flags, int nx, int ny,
              size_t plane, float omega) {
-  bulk2d_tile<S, STAGE, PERIODIC>(in, out, flags, nx, ny, plane, omega, blockIdx.x, blockIdx.y);
+  bulk2d_tile<S, STAGE, PERIODIC, TX, TY, CPT>(in, out, flags, nx, ny, plane, omega, blockIdx.x, blockIdx.y);
 }
 
 // Link detection and stream-collide tiles in one launch: the first `nbuild`
 // blocks detect the step's moving-wall links (dispatched first: they carry the
 // longest dependent chains), the rest are tiles.  The two parts are
 // independent (boundary cells are recomputed afterwards).
-template <typename S, bool STAGE, bool PERIODIC>
-__global__ void __launch_bounds__(kTX2* kTY2)
+template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT>
+__global__ void __launch_bounds__(256)
     k_bulk2d_links(const S* __restrict__ in, S* __restrict__ out, int tiles_x, int nbuild, LinkArgs la) {
   if ((int)blockIdx.x < nbuild) {
     build_links_block<2>(la, (long long)blockIdx.x, la.st->gen);
   } else {
     const int t = (int)blockIdx.x - nbuild;
-    bulk2d_tile<S, STAGE, PERIODIC>(in, out, la.flags, la.nx, la.ny, la.plane, la.omega, t % tiles_x, t / tiles_x);
+    bulk2d_tile<S, STAGE, PERIODIC, TX, TY, CPT>(in, out, la.flags, la.nx, la.ny, la.plane, la.omega, t % tiles_x,
+                                                 t / tiles_x);
+  }
+}
+
+template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT>
+static void bulk2d_shape(Domain& d, const S* i, S* o, const LinkArgs* la) {
+  const int tx = (d.nx + TX - 1) / TX, ty = (d.ny + TY * CPT - 1) / (TY * CPT);
+  if (!la) {
+    k_bulk2d<S, STAGE, PERIODIC, TX, TY, CPT><<<dim3(tx, ty), 256, 0, d.stream>>>(i, o, d.flags, d.nx, d.ny, d.plane,
+                                                                                  d.omega);
+  } else {
+    const long long pairs = (long long)d.ne * 8;
+    const int nbuild = (int)((pairs + 15) / 16);
+    k_bulk2d_links<S, STAGE, PERIODIC, TX, TY, CPT><<<(unsigned)(tx * ty + nbuild), 256, 0, d.stream>>>(i, o, tx, nbuild,
+                                                                                                      *la);
+  }
+}
+
+template <typename S, bool STAGE, bool PERIODIC>
+static void bulk2d_variant(Domain& d, const S* i, S* o, const LinkArgs* la) {
+  switch (d.tile2) {
+    case 1: bulk2d_shape<S, STAGE, PERIODIC, 32, 8, 2>(d, i, o, la); break;
+    case 2: bulk2d_shape<S, STAGE, PERIODIC, 64, 4, 2>(d, i, o, la); break;
+    case 3: bulk2d_shape<S, STAGE, PERIODIC, 32, 8, 4>(d, i, o, la); break;
+    case 4: bulk2d_shape<S, STAGE, PERIODIC, 64, 4, 4>(d, i, o, la); break;
+    default: bulk2d_shape<S, STAGE, PERIODIC, 32, 8, 1>(d, i, o, la); break;
   }
 }
 
 template <typename S>
 static int bulk2d_typed(Domain& d, const void* in, void* out, const LinkArgs* la) {
-  const int tx = (d.nx + kTX2 - 1) / kTX2, ty = (d.ny + kTY2 - 1) / kTY2;
   const bool stage = d.options & FF_OPT_STAGE_FP16;
   const bool per = d.options & FF_OPT_PERIODIC;
   const S* i = (const S*)in;
   S* o = (S*)out;
-  if (!la) {
-    dim3 grid(tx, ty), block(kTX2 * kTY2);
-#define FF_B(ST, PE) k_bulk2d<S, ST, PE><<<grid, block, 0, d.stream>>>(i, o, d.flags, d.nx, d.ny, d.plane, d.omega)
-    if (stage && per) FF_B(true, true);
-    else if (stage) FF_B(true, false);
-    else if (per) FF_B(false, true);
-    else FF_B(false, false);
-#undef FF_B
-  } else {
-    const int nbulk = tx * ty;
-    const long long pairs = (long long)d.ne * 8;
-    const int nbuild = (int)((pairs + 15) / 16);
-    const unsigned grid = (unsigned)(nbulk + nbuild);
-#define FF_B(ST, PE) k_bulk2d_links<S, ST, PE><<<grid, kTX2 * kTY2, 0, d.stream>>>(i, o, tx, nbuild, *la)
-    if (stage && per) FF_B(true, true);
-    else if (stage) FF_B(true, false);
-    else if (per) FF_B(false, true);
-    else FF_B(false, false);
-#undef FF_B
-  }
+  if (stage && per) bulk2d_variant<S, true, true>(d, i, o, la);
+  else if (stage) bulk2d_variant<S, true, false>(d, i, o, la);
+  else if (per) bulk2d_variant<S, false, true>(d, i, o, la);
+  else bulk2d_variant<S, false, false>(d, i, o, la);
   FF_HIP(hipGetLastError());
   return FF_OK;
 }

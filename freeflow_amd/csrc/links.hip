@@ -26,7 +26,7 @@
 
 #include "ff_internal.h"
 #include "geometry.h"
-#include "hermite.h"
+#include "links_body.h"
 
 namespace ff {
 
@@ -47,159 +47,23 @@ int launch_links_build(Domain& d, const LinkArgs& la) {
   return FF_OK;
 }
 
-// ---------------------------------------------------------------- boundary cells
-// Full update of every cell that owns a keyed link, one lane per lattice
-// direction: the per-direction logic of mrDeformableSolver2DKernel_Shrd
-// (FsiCoupling2D.cu:154-224) / mrDeformableSolver3DKernel_ShrdM
-// (FsiCoupling3D.cu:194-285), moving-wall value from the single-population
-// reconstruction, momentum-exchange force (FsiCoupling2D.cu:193-214,
-// FsiCoupling3D.cu:243-271).  Overwrites the bulk kernel's value for the cell.
-template <int DIM, typename S, bool STAGE>
-__device__ __forceinline__ void cell_pops(const S* in, size_t plane, size_t c, float* f) {
-  if constexpr (DIM == 2) {
-    float m[6];
-#pragma unroll
-    for (int q = 0; q < 6; q++) m[q] = Store<S>::template load<2>(in + q * plane + c, q);
-    recon2_all(m[0], m[1], m[2], m[3], m[5], m[4], f);
-    if constexpr (STAGE) {
-#pragma unroll
-      for (int i = 0; i < 9; i++) f[i] = __half2float(to_half_rn(f[i]));
-    }
-  } else {
-    float m[10];
-#pragma unroll
-    for (int q = 0; q < 10; q++) m[q] = Store<S>::template load<3>(in + q * plane + c, q);
-    recon3_all(m, f);
-  }
-}
-
+// ---------------------------------------------------------------- per-domain kernels
 template <int DIM, typename S, bool STAGE, bool PERIODIC>
 __global__ void __launch_bounds__(256) k_bcells(const S* __restrict__ in, S* __restrict__ out, LinkArgs la) {
-  constexpr int Q = DIM == 2 ? 9 : 27;
-  constexpr int K = Q - 1;
-  constexpr int NM = DIM == 2 ? 6 : 10;
-  constexpr int LPC = DIM == 2 ? 16 : 32;  // lanes per cell
-  constexpr int CPB = 256 / LPC;           // cells per block
-  __shared__ float sp[CPB][Q];
-  const unsigned gen = la.st->gen;
-  const int count = la.st->count[gen & 1];
-  const int* list = la.list + (size_t)(gen & 1) * la.cap;
-  const float* v = la.v[gen & 1];
-  const float* u = la.u[gen & 1];
-  const int lc = threadIdx.x / LPC, dir = threadIdx.x % LPC;
-  for (int base = blockIdx.x * CPB; base < count; base += gridDim.x * CPB) {
-    const int t = base + lc;
-    int cell = 0;
-    if (t < count && dir < Q) {
-      const int slot = list[t];
-      cell = (int)(uint32_t)la.cells[slot];
-      const int x = cell % la.nx, y = (cell / la.nx) % la.ny, z = DIM == 3 ? cell / (la.nx * la.ny) : 0;
-      const int dx = DIM == 2 ? D2::cx(dir) : D3::cx(dir);
-      const int dy = DIM == 2 ? D2::cy(dir) : D3::cy(dir);
-      const int dz = DIM == 2 ? 0 : D3::cz(dir);
-      const int inv = DIM == 2 ? D2::inv(dir) : D3::inv(dir);
-      const int x1 = x - dx, y1 = y - dy, z1 = z - dz;
-      const bool notCross = x1 >= 0 && x1 < la.nx && y1 >= 0 && y1 < la.ny && z1 >= 0 && z1 < la.nz;
-      const int xw = (x1 + la.nx) % la.nx, yw = (y1 + la.ny) % la.ny, zw = (z1 + la.nz) % la.nz;
-      int sx, sy, sz;  // pull source: clamped halo (reference) or periodic
-      if (PERIODIC) {
-        sx = xw; sy = yw; sz = zw;
-      } else {
-        sx = min(la.nx - 1, max(0, x1));
-        sy = min(la.ny - 1, max(0, y1));
-        sz = min(la.nz - 1, max(0, z1));
-      }
-      const size_t sc = ((size_t)sz * la.ny + sy) * la.nx + sx;
-      const size_t li = (size_t)slot * K + (dir - 1);
-      // every load that depends only on the cell index is issued up front
-      const uint8_t fup = la.flags[((size_t)zw * la.ny + yw) * la.nx + xw];
-      const uint32_t key = dir > 0 ? lt_key(la.keys, li, gen) : 0xFFFFFFFFu;
-      float mc[NM];
-#pragma unroll
-      for (int q = 0; q < NM; q++) mc[q] = Store<S>::template load<DIM>(in + q * la.plane + cell, q);
-      float fn[Q];
-      cell_pops<DIM, S, STAGE>(in, la.plane, sc, fn);
-      float fc[Q];
-      cell_pops<DIM, S, STAGE>(in, la.plane, (size_t)cell, fc);
-      float pop = 0.0f;
-      bool done = false;
-      if (DIM == 3 && is_wall3(fup)) {  // 3D keeps flag bounce-back (FsiCoupling3D.cu:209-222)
-        pop = fmaxf(0.0f, fc[inv]);
-        done = true;
-      }
-      if (dir > 0) la.lb[li * DIM] = -1.0f;
-      if (!done && dir > 0 && notCross && (DIM == 3 || !is_wall2(fup)) && key != 0xFFFFFFFFu) {
-        const int elem = (int)(key & 0xFFFFu);
-        Hit h;
-        bool hit;
-        if constexpr (DIM == 2) {
-          hit = hit_elem2(v, la.e, elem, (float)x * la.cl, (float)y * la.cl, (float)xw * la.cl, (float)yw * la.cl, h);
-        } else {
-          const float p1[3] = {(float)x * la.cl, (float)y * la.cl, (float)z * la.cl};
-          const float p2[3] = {(float)xw * la.cl, (float)yw * la.cl, (float)zw * la.cl};
-          hit = hit_elem3(v, la.e, elem, p1, p2, h);
-        }
-        if (hit) {
-          done = true;
-          const float fin = fmaxf(0.0f, fc[inv]);
-          float uw[3] = {0.f, 0.f, 0.f};
-          float fout;
-          if constexpr (DIM == 2) {
-            // getVelocityAt<2> (:173-180), then `ux *= ct / cl` (FsiCoupling2D.cu:200-201)
-            const uint32_t v1 = la.e[2 * elem], v2 = la.e[2 * elem + 1];
-            uw[0] = u[2 * v1] * h.b[0] + u[2 * v2] * h.b[1];
-            uw[1] = u[2 * v1 + 1] * h.b[0] + u[2 * v2 + 1] * h.b[1];
-            uw[0] = uw[0] * (la.ct / la.cl);
-            uw[1] = uw[1] * (la.ct / la.cl);
-            const float pixx = mc[3] + uw[0] * uw[0] - mc[1] * mc[1];
-            const float piyy = mc[4] + uw[1] * uw[1] - mc[2] * mc[2];
-            const float pixy = mc[5] + uw[1] * uw[0] - mc[1] * mc[2];
-            fout = recon2_at(mc[0], uw[0], uw[1], pixx, pixy, piyy, dir);
-          } else {
-            // getVelocityAt<3> (:385-398), then `u * ct / cl` (FsiCoupling3D.cu:244)
-            const uint32_t v1 = la.e[3 * elem], v2 = la.e[3 * elem + 1], v3 = la.e[3 * elem + 2];
-#pragma unroll
-            for (int c = 0; c < 3; c++)
-              uw[c] = ((u[3 * v1 + c] * h.b[0] + u[3 * v2 + c] * h.b[1] + u[3 * v3 + c] * h.b[2]) * la.ct) / la.cl;
-            float ms[10];
-            ms[0] = mc[0];
-            ms[1] = uw[0];
-            ms[2] = uw[1];
-            ms[3] = uw[2];
-            ms[4] = mc[4] + uw[0] * uw[0] - mc[1] * mc[1];
-            ms[5] = mc[5] + uw[1] * uw[1] - mc[2] * mc[2];
-            ms[6] = mc[6] + uw[2] * uw[2] - mc[3] * mc[3];
-            ms[7] = mc[7] + uw[0] * uw[1] - mc[1] * mc[2];
-            ms[8] = mc[8] + uw[1] * uw[2] - mc[2] * mc[3];
-            ms[9] = mc[9] + uw[0] * uw[2] - mc[1] * mc[3];
-            fout = recon3_at(ms, dir);
-          }
-          pop = fout;
-          const float cc[3] = {(float)dx, (float)dy, (float)dz};
-#pragma unroll
-          for (int c = 0; c < DIM; c++) {
-            la.lf[li * DIM + c] = (fin * (-cc[c] - uw[c]) - fout * (cc[c] - uw[c])) * la.cf;
-            la.lb[li * DIM + c] = h.b[c];
-            la.lh[li * DIM + c] = h.hit[c];
-          }
-        }
-      }
-      if (!done) pop = fmaxf(0.0f, fn[dir]);
-      sp[lc][dir] = pop;
-    }
-    __syncthreads();
-    if (t < count && dir == 0 && is_updated(la.flags[cell])) {
-      float p[Q];
-#pragma unroll
-      for (int i = 0; i < Q; i++) p[i] = sp[lc][i];
-      float o[NM];
-      if constexpr (DIM == 2) collide2(p, la.omega, o);
-      else collide3(p, la.omega, o);
-#pragma unroll
-      for (int q = 0; q < NM; q++) Store<S>::template store<DIM>(out + q * la.plane + cell, q, o[q]);
-    }
-    __syncthreads();
-  }
+  bcells_body<DIM, S, STAGE, PERIODIC>(in, out, la, blockIdx.x, gridDim.x);
+}
+
+template <int DIM, typename S, int FB>
+__global__ void __launch_bounds__(FB) k_elem_sums(LinkArgs la, FinalArgs fa) {
+  elem_sums_body<DIM, S, FB>(la, fa, blockIdx.x);
+}
+
+// Vertex blocks, then one last block for the body wrenches and the next
+// generation (only that block reads or writes the generation).
+template <int DIM, int FB>
+__global__ void __launch_bounds__(FB) k_finish(LinkArgs la, FinalArgs fa) {
+  if ((int)blockIdx.x < (int)gridDim.x - 1) finish_vertex_body<DIM, FB>(la, fa, blockIdx.x);
+  else finish_last_body<DIM, FB>(la, fa);
 }
 
 template <int DIM, typename S>
@@ -231,223 +95,7 @@ int launch_boundary_cells(Domain& d, const void* in, void* out, const LinkArgs& 
   return FF_OK;
 }
 
-// ---------------------------------------------------------------- finalize
-struct FinalArgs {
-  const int* elem_body;
-  const double* body_ref;
-  int n_bodies;
-  double* elem_acc;
-  double* wrench;
-  double* hist;
-  int hist_cap;
-  const int* bptr;
-  const int* bidx;
-  const int* vptr;
-  const int* vinc;
-  float* vforce;
-  float* proxy_forces;
-  int wave;
-  const float* rest;
-  const float* s;
-  const int* vbody;
-  const double* wpar;
-  double dt;
-  float* vn[2];
-  float* un[2];
-  void* buf;
-  const OutletCell* outlets;
-  int n_outlets;
-  int outlet_blocks;
-};
-
-// Travelling wave along the body axis, FP64:
-//   y = y_rest + A s sin(theta), v_y = -2 pi f A s cos(theta),
-//   theta = 2 pi (s L/lambda - f t) + phi.
-template <int DIM>
-__device__ __forceinline__ void wave_vertex(const float* rest, const float* s, const int* vbody, const double* par,
-                                            int k, double t, float* v, float* vel) {
-  const double* p = par + 4 * vbody[k];
-  const double A = p[0], f = p[1], phi = p[2], kl = p[3];
-  const double sk = (double)s[k];
-  const double two_pi = 6.283185307179586;
-  const double th = two_pi * (sk * kl - f * t) + phi;
-  double sn, cs;
-  sincos(th, &sn, &cs);
-#pragma unroll
-  for (int c = 0; c < DIM; c++) {
-    v[k * DIM + c] = rest[k * DIM + c];
-    vel[k * DIM + c] = 0.0f;
-  }
-  v[k * DIM + 1] = (float)((double)rest[k * DIM + 1] + A * sk * sn);
-  vel[k * DIM + 1] = (float)(-two_pi * f * A * sk * cs);
-}
-
-// Element blocks: each sums its own links (fixed order, FP64) into elem_acc.
-// The next blocks update the outlet cells of the freshly written buffer
-// (mrApplyOutlet2DKernel / mrApplyOutlet3DKernel); the last ones evaluate the
-// next step's wave kinematics.
-template <int DIM, typename S, int FB>
-__global__ void __launch_bounds__(FB) k_elem_sums(LinkArgs la, FinalArgs fa) {
-  constexpr int K = DIM == 2 ? 8 : 26;
-  constexpr int NT = DIM == 2 ? 1 : 3;
-  constexpr int STRIDE = DIM * DIM + DIM + NT;
-  constexpr int CH = 256;  // window cells resolved per chunk
-  if ((int)blockIdx.x >= la.ne + fa.outlet_blocks) {
-    // kinematics of the next step (its vertex buffer is not read in this step)
-    const int k = ((int)blockIdx.x - la.ne - fa.outlet_blocks) * FB + threadIdx.x;
-    if (fa.wave && k < la.nv) {
-      const unsigned g = la.st->gen;
-      wave_vertex<DIM>(fa.rest, fa.s, fa.vbody, fa.wpar, k, (double)g * fa.dt, fa.vn[(g + 1) & 1], fa.un[(g + 1) & 1]);
-    }
-    return;
-  }
-  if ((int)blockIdx.x >= la.ne) {
-    const int t = ((int)blockIdx.x - la.ne) * FB + threadIdx.x;
-    if (t >= fa.n_outlets) return;
-    S* buf = (S*)fa.buf;
-    const OutletCell c = fa.outlets[t];
-    if (la.flags[c.nb] != kFluid) return;
-    const float up = Store<S>::template load<DIM>(buf + c.comp * la.plane + c.nb, c.comp);
-    if ((c.sign > 0 && up > 0.0f) || (c.sign < 0 && up < 0.0f))
-      Store<S>::template store<DIM>(buf + c.comp * la.plane + c.cell, c.comp, up);
-    Store<S>::template store<DIM>(buf + c.pcomp * la.plane + c.cell, c.pcomp, up * up);
-    return;
-  }
-  __shared__ int s_slot[CH];
-  __shared__ double s_red[STRIDE][FB];
-  const unsigned gen = la.st->gen;
-  const int elem = blockIdx.x;
-  const float* v = la.v[gen & 1];
-  const int n[3] = {la.nx, la.ny, la.nz};
-  int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-  elem_window<DIM>(v, la.e, elem, la.cl, n, lo, hi);
-  const int wx = hi[0] - lo[0] + 1, wy = hi[1] - lo[1] + 1, wz = DIM == 3 ? hi[2] - lo[2] + 1 : 1;
-  const int ncell = (wx > 0 && wy > 0 && wz > 0) ? wx * wy * wz : 0;
-  const int b = fa.elem_body ? fa.elem_body[elem] : 0;
-  double ref[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-  for (int c = 0; c < DIM; c++) ref[c] = fa.body_ref[b * DIM + c];
-  double acc[STRIDE];
-#pragma unroll
-  for (int a = 0; a < STRIDE; a++) acc[a] = 0.0;
-  const uint32_t tag = (uint32_t)(uint16_t)elem;
-  for (int c0 = 0; c0 < ncell; c0 += CH) {
-    const int chunk = min(CH, ncell - c0);
-    for (int k = threadIdx.x; k < chunk; k += FB) {
-      const int kk = c0 + k;
-      const int i = lo[0] + kk % wx, j = lo[1] + (kk / wx) % wy, z = DIM == 3 ? lo[2] + kk / (wx * wy) : 0;
-      const int cell = (z * la.ny + j) * la.nx + i;
-      s_slot[k] = is_updated(la.flags[cell]) ? lt_find(la.cells, la.cap, gen, cell) : -1;
-    }
-    __syncthreads();
-    for (int task = threadIdx.x; task < K * chunk; task += FB) {
-      const int k = task % chunk, l = task / chunk;
-      const int slot = s_slot[k];
-      if (slot < 0) continue;
-      const size_t li = (size_t)slot * K + l;
-      const uint32_t key = lt_key(la.keys, li, gen);
-      if (key == 0xFFFFFFFFu || (key & 0xFFFFu) != tag) continue;
-      if (la.lb[li * DIM] < 0.0f) continue;
-      float F[DIM];
-#pragma unroll
-      for (int c = 0; c < DIM; c++) F[c] = la.lf[li * DIM + c];
-#pragma unroll
-      for (int jv = 0; jv < DIM; jv++) {
-        const float bj = la.lb[li * DIM + jv];
-#pragma unroll
-        for (int c = 0; c < DIM; c++) acc[jv * DIM + c] += (double)(F[c] * bj);  // addForceAt products
-      }
-#pragma unroll
-      for (int c = 0; c < DIM; c++) acc[DIM * DIM + c] += (double)F[c];
-      double r[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-      for (int c = 0; c < DIM; c++) r[c] = (double)la.lh[li * DIM + c] - ref[c];
-      if constexpr (DIM == 2) {
-        acc[DIM * DIM + DIM] += r[0] * (double)F[1] - r[1] * (double)F[0];
-      } else {
-        acc[DIM * DIM + DIM + 0] += r[1] * (double)F[2] - r[2] * (double)F[1];
-        acc[DIM * DIM + DIM + 1] += r[2] * (double)F[0] - r[0] * (double)F[2];
-        acc[DIM * DIM + DIM + 2] += r[0] * (double)F[1] - r[1] * (double)F[0];
-      }
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int a = 0; a < STRIDE; a++) s_red[a][threadIdx.x] = acc[a];
-  __syncthreads();
-  for (int w = FB / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w)
-#pragma unroll
-      for (int a = 0; a < STRIDE; a++) s_red[a][threadIdx.x] += s_red[a][threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x < STRIDE) fa.elem_acc[(size_t)elem * STRIDE + threadIdx.x] = s_red[threadIdx.x][0];
-}
-
-// Vertex blocks sum their incident element shares in CSR order (the
-// per-vertex forces addForceAt accumulates); the last block folds element
-// sums into body wrenches with a fixed tree and starts the next step's link
-// table generation.  Only that block reads or writes the generation.
-template <int DIM, int FB>
-__global__ void __launch_bounds__(FB) k_finish(LinkArgs la, FinalArgs fa) {
-  constexpr int NT = DIM == 2 ? 1 : 3;
-  constexpr int STRIDE = DIM * DIM + DIM + NT;
-  constexpr int NW = DIM + NT;
-  if ((int)blockIdx.x < (int)gridDim.x - 1) {
-    const int vtx = blockIdx.x * FB + threadIdx.x;
-    if (vtx >= la.nv) return;
-    double f[DIM];
-#pragma unroll
-    for (int c = 0; c < DIM; c++) f[c] = 0.0;
-    for (int k = fa.vptr[vtx]; k < fa.vptr[vtx + 1]; k++) {
-      const int code = fa.vinc[k];
-      const int e = code >> 2, end = code & 3;
-#pragma unroll
-      for (int c = 0; c < DIM; c++) f[c] += fa.elem_acc[(size_t)e * STRIDE + end * DIM + c];
-    }
-#pragma unroll
-    for (int c = 0; c < DIM; c++) {
-      fa.vforce[vtx * DIM + c] += (float)f[c];
-      if (fa.proxy_forces) fa.proxy_forces[vtx * DIM + c] += (float)f[c];
-    }
-    return;
-  }
-  __shared__ double s_red[NW][FB];
-  const unsigned gen = la.st->gen;
-  for (int bd = 0; bd < fa.n_bodies; bd++) {
-    double a[NW];
-#pragma unroll
-    for (int c = 0; c < NW; c++) a[c] = 0.0;
-    for (int k = fa.bptr[bd] + threadIdx.x; k < fa.bptr[bd + 1]; k += FB) {
-      const double* ea = fa.elem_acc + (size_t)fa.bidx[k] * STRIDE + DIM * DIM;
-#pragma unroll
-      for (int c = 0; c < NW; c++) a[c] += ea[c];
-    }
-#pragma unroll
-    for (int c = 0; c < NW; c++) s_red[c][threadIdx.x] = a[c];
-    __syncthreads();
-    for (int w = FB / 2; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w)
-#pragma unroll
-        for (int c = 0; c < NW; c++) s_red[c][threadIdx.x] += s_red[c][threadIdx.x + w];
-      __syncthreads();
-    }
-    if ((int)threadIdx.x < NW) {
-      fa.wrench[bd * NW + threadIdx.x] = s_red[threadIdx.x][0];
-      if (fa.hist)
-        fa.hist[((size_t)((gen - 1) % (unsigned)fa.hist_cap) * fa.n_bodies + bd) * NW + threadIdx.x] =
-            s_red[threadIdx.x][0];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    la.st->count[(gen + 1) & 1] = 0;
-    la.st->gen = gen + 1;
-  }
-}
-
-int launch_finalize(Domain& d, const LinkArgs& la, void* out, float* vn0, float* vn1, float* un0, float* un1,
-                    float* proxy_forces) {
+FinalArgs make_final_args(Domain& d, void* out, float* vn0, float* vn1, float* un0, float* un1, float* proxy_forces) {
   FinalArgs fa;
   fa.elem_body = d.elem_body;
   fa.body_ref = d.body_ref;
@@ -478,6 +126,14 @@ int launch_finalize(Domain& d, const LinkArgs& la, void* out, float* vn0, float*
   constexpr int FB2 = 128, FB3 = 256;
   const int fb = d.dim == 2 ? FB2 : FB3;
   fa.outlet_blocks = (d.n_outlets + fb - 1) / fb;
+  return fa;
+}
+
+int launch_finalize(Domain& d, const LinkArgs& la, void* out, float* vn0, float* vn1, float* un0, float* un1,
+                    float* proxy_forces) {
+  constexpr int FB2 = 128, FB3 = 256;
+  const int fb = d.dim == 2 ? FB2 : FB3;
+  const FinalArgs fa = make_final_args(d, out, vn0, vn1, un0, un1, proxy_forces);
   const int wave_blocks = fa.wave ? (d.nv + fb - 1) / fb : 0;
   const unsigned grid = (unsigned)(d.ne + fa.outlet_blocks + wave_blocks);
 #define FF_FIN(DIM, S, FB) k_elem_sums<DIM, S, FB><<<grid, FB, 0, d.stream>>>(la, fa)

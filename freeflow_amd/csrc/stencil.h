@@ -1,0 +1,142 @@
+// stencil.h -- the stream+collide tile bodies shared by the per-domain
+// kernels (kernels2d.hip, kernels3d.hip) and the batched kernels (batch.hip).
+//
+// The arithmetic is that of mrDeformableSolver2DKernel_Shrd
+// (src/core/FsiCoupling2D.cu:82-255) and mrDeformableSolver3DKernel_ShrdM
+// (src/core/FsiCoupling3D.cu:83-334) for cells without a moving-wall link.
+#pragma once
+
+#include <type_traits>
+
+#include "ff_internal.h"
+#include "hermite.h"
+
+namespace ff {
+
+// 2D tile shapes: TX x (TY * CPT) cells per 256-thread block, each thread
+// updating CPT cells of one column (rows ly, ly + TY, ...).  Taller tiles
+// re-read and re-reconstruct less halo per cell at the price of LDS.
+struct Tile2 {
+  int tx, ty, cpt;
+};
+constexpr Tile2 kTiles2[] = {{32, 8, 1}, {32, 8, 2}, {64, 4, 2}, {32, 8, 4}, {64, 4, 4}};
+constexpr int kNumTiles2 = 5;
+
+// Stage the reconstructed populations of the tile and its one-cell halo in
+// LDS (clamped or periodic halo; binary16 when the reference's half tile is
+// reproduced), then pull, collide and store every updated cell.
+template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT>
+__device__ __forceinline__ void bulk2d_tile(const S* __restrict__ in, S* __restrict__ out,
+                                            const uint8_t* __restrict__ flags, int nx, int ny, size_t plane,
+                                            float omega, int bx, int by) {
+  constexpr int NT = TX * TY, RY = TY * CPT;
+  constexpr int HX = TX + 2, HY = RY + 2, HN = HX * HY;
+  using T = typename std::conditional<STAGE, __half, float>::type;
+  __shared__ T sp[9][HN];
+  const int x0 = bx * TX, y0 = by * RY;
+  for (int k = threadIdx.x; k < HN; k += NT) {
+    const int hx = k % HX, hy = k / HX;
+    int gx = x0 + hx - 1, gy = y0 + hy - 1;
+    if (PERIODIC) {
+      gx = ((gx % nx) + nx) % nx;
+      gy = ((gy % ny) + ny) % ny;
+    } else {
+      gx = min(nx - 1, max(0, gx));
+      gy = min(ny - 1, max(0, gy));
+    }
+    const size_t c = (size_t)gy * nx + gx;
+    float m[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) m[q] = Store<S>::template load<2>(in + q * plane + c, q);
+    float f[9];
+    recon2_all(m[0], m[1], m[2], m[3], m[5], m[4], f);
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      if constexpr (STAGE) sp[i][k] = to_half_rn(f[i]);
+      else sp[i][k] = f[i];
+    }
+  }
+  __syncthreads();
+  const int lx = threadIdx.x % TX, ly0 = threadIdx.x / TX;
+  const int x = x0 + lx;
+  if (x >= nx) return;
+#pragma unroll
+  for (int r = 0; r < CPT; r++) {
+    const int ly = ly0 + r * TY;
+    const int y = y0 + ly;
+    if (y >= ny) break;
+    const size_t cell = (size_t)y * nx + x;
+    if (!is_updated(flags[cell])) continue;
+    float p[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      const int idx = (ly + 1 - D2::cy(i)) * HX + (lx + 1 - D2::cx(i));
+      float v;
+      if constexpr (STAGE) v = __half2float(sp[i][idx]);
+      else v = sp[i][idx];
+      p[i] = fmaxf(0.0f, v);
+    }
+    float o[6];
+    collide2(p, omega, o);
+#pragma unroll
+    for (int q = 0; q < 6; q++) Store<S>::template store<2>(out + q * plane + cell, q, o[q]);
+  }
+}
+
+constexpr int kTX3 = 8, kTY3 = 8, kTZ3 = 4;
+
+// 3D tile of 8x8x4 cells with a 10x10x6 halo in LDS (FP32 as the reference's
+// tile, FsiCoupling3D.cu:94).  The reference keeps the flag bounce-back value
+// in 3D (FsiCoupling3D.cu:209-222), so upstream flags are staged too.
+template <typename S, bool PERIODIC>
+__device__ __forceinline__ void bulk3d_tile(const S* __restrict__ in, S* __restrict__ out,
+                                            const uint8_t* __restrict__ flags, int nx, int ny, int nz, size_t plane,
+                                            float omega, int bx, int by, int bz) {
+  constexpr int HX = kTX3 + 2, HY = kTY3 + 2, HZ = kTZ3 + 2, HN = HX * HY * HZ;
+  constexpr int NT = kTX3 * kTY3 * kTZ3;
+  __shared__ float sp[27][HN];
+  __shared__ uint8_t sf[HN];
+  const int x0 = bx * kTX3, y0 = by * kTY3, z0 = bz * kTZ3;
+  for (int k = threadIdx.x; k < HN; k += NT) {
+    const int hx = k % HX, hy = (k / HX) % HY, hz = k / (HX * HY);
+    const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+    const int wx = ((gx % nx) + nx) % nx, wy = ((gy % ny) + ny) % ny, wz = ((gz % nz) + nz) % nz;
+    int sx, sy, sz;
+    if (PERIODIC) {
+      sx = wx; sy = wy; sz = wz;
+    } else {
+      sx = min(nx - 1, max(0, gx));
+      sy = min(ny - 1, max(0, gy));
+      sz = min(nz - 1, max(0, gz));
+    }
+    const size_t c = ((size_t)sz * ny + sy) * nx + sx;
+    float m[10];
+#pragma unroll
+    for (int q = 0; q < 10; q++) m[q] = Store<S>::template load<3>(in + q * plane + c, q);
+    float f[27];
+    recon3_all(m, f);
+#pragma unroll
+    for (int i = 0; i < 27; i++) sp[i][k] = f[i];
+    sf[k] = flags[((size_t)wz * ny + wy) * nx + wx];
+  }
+  __syncthreads();
+  const int lx = threadIdx.x % kTX3, ly = (threadIdx.x / kTX3) % kTY3, lz = threadIdx.x / (kTX3 * kTY3);
+  const int x = x0 + lx, y = y0 + ly, z = z0 + lz;
+  if (x >= nx || y >= ny || z >= nz) return;
+  const int ctr = ((lz + 1) * HY + (ly + 1)) * HX + (lx + 1);
+  if (!is_updated(sf[ctr])) return;
+  float p[27];
+#pragma unroll
+  for (int i = 0; i < 27; i++) {
+    const int up = ctr - (D3::cz(i) * HY + D3::cy(i)) * HX - D3::cx(i);
+    if (is_wall3(sf[up])) p[i] = fmaxf(0.0f, sp[D3::inv(i)][ctr]);
+    else p[i] = fmaxf(0.0f, sp[i][up]);
+  }
+  float o[10];
+  collide3(p, omega, o);
+  const size_t cell = ((size_t)z * ny + y) * nx + x;
+#pragma unroll
+  for (int q = 0; q < 10; q++) Store<S>::template store<3>(out + q * plane + cell, q, o[q]);
+}
+
+}  // namespace ff

@@ -95,11 +95,81 @@ def random_swimmer(rng, n_seg, cx, cy, a=0.3, b=0.08):
     return Swimmer2D(n_seg, cx, cy, a, b, A, f, phi)
 
 
+def icosphere(center, r, subdiv=1):
+    """Triangulated sphere (icosahedron refined `subdiv` times), outward faces."""
+    t = (1 + 5 ** 0.5) / 2
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], float)
+    f = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2], [10, 7, 6],
+         [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11], [6, 2, 10],
+         [8, 6, 7], [9, 8, 1]]
+    v = list(v / np.linalg.norm(v, axis=1)[:, None])
+    for _ in range(subdiv):
+        cache = {}
+        nf = []
+
+        def mid(a, b):
+            k = (min(a, b), max(a, b))
+            if k not in cache:
+                m = (v[a] + v[b]) / 2
+                v.append(m / np.linalg.norm(m))
+                cache[k] = len(v) - 1
+            return cache[k]
+
+        for a, b, c in f:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [[a, ab, ca], [b, bc, ab], [c, ca, bc], [ab, bc, ca]]
+        f = nf
+    return np.array(v) * r + np.asarray(center), np.array(f, np.uint32)
+
+
+class SwimmerScene2D:
+    """One 2D swimmer domain: outlet box, equilibrium at rest, elliptical
+    swimmer with travelling-wave kinematics (the layout of config 2 scaled to
+    the domain)."""
+
+    def __init__(self, nx, ny, n_seg, rng, dx=0.005, dt=0.0025, viscosity=0.005, rho=10.0):
+        self.nx, self.ny, self.dx, self.dt, self.viscosity, self.rho = nx, ny, dx, dt, viscosity, rho
+        lx, ly = nx * dx, ny * dx
+        a = min(0.3, 0.22 * lx)
+        b = min(0.27 * a, 0.12 * ly)
+        self.swimmer = random_swimmer(rng, n_seg, 0.45 * lx, 0.5 * ly, a, b)
+        self.swimmer.params[0] = min(self.swimmer.params[0], 0.2 * ly)
+        self.flags = outlet_box_flags(nx, ny)
+
+    def domain_args(self):
+        return dict(dim=2, nx=self.nx, ny=self.ny, viscosity=self.viscosity, dx=self.dx, dt=self.dt,
+                    fluid_density=self.rho)
+
+    def setup(self, dom, history=0):
+        """Flags, rest state, boundary and device-side wave kinematics on `dom`."""
+        dom.set_flags(self.flags)
+        dom.set_moments(equilibrium_moments(np.zeros((self.nx * self.ny, 2))))
+        v0, vel0 = self.swimmer.state(0.0)
+        dom.set_boundary(v0, vel0, self.swimmer.elems)
+        dom.set_wave(self.swimmer.rest, self.swimmer.s, self.swimmer.params[None, :])
+        if history:
+            dom.set_wrench_history(history)
+
+
+def batched_scenes(n, seed, nx_range=(256, 1200), ny_range=(128, 400), seg_range=(200, 800)):
+    """`n` uneven swimmer domains, sizes and swimmers from one seed (config 4)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        nx = int(rng.integers(nx_range[0], nx_range[1] + 1))
+        ny = int(rng.integers(ny_range[0], ny_range[1] + 1))
+        ns = int(rng.integers(seg_range[0], seg_range[1] + 1))
+        out.append(SwimmerScene2D(nx, ny, ns, rng))
+    return out
+
+
 # ---------------------------------------------------------------- configurations
 CONFIG1 = dict(nx=400, ny=200, viscosity=0.005, u_in=0.05, r=20.0, cx=100.0, cy=100.0, n_poly=128, seed=1234)
 CONFIG2 = dict(nx=1200, ny=400, dx=0.005, dt=0.0025, viscosity=0.005, rho=10.0, n_seg=600,
                cx=1.0, cy=1.0, a=0.3, b=0.08, seed=1234)
 CONFIG3 = dict(nx=256, ny=128, nz=128, viscosity=0.01, amp=0.04, ring_r=20.0, seed=1234)
+CONFIG4 = dict(n_domains=64, seed=1234, steps=1000, storage="f16")
 
 
 def config2_domain_args():

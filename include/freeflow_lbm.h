@@ -175,6 +175,48 @@ int ff_time_bulk_kernel(ff_domain* d, int iters, double* avg_ms);
    events on the domain stream (state advances). */
 int ff_time_steps(ff_domain* d, int nsteps, double* total_ms);
 
+/* Device bytes ff_domain_create + ff_set_boundary would allocate for this
+   descriptor and boundary (verts/elems may be NULL when ne == 0): the figure
+   an admission controller charges against the GPU's memory.  No device work. */
+int ff_domain_footprint(const ff_domain_desc* desc, int nv, int ne, const float* verts, const uint32_t* elems,
+                        unsigned long long* bytes);
+
+/* 2D stream-collide tile shape (0..4: 32x8, 32x16, 64x8, 32x32, 64x16 cells
+   per 256-thread block; 0 is the default).  Results do not depend on it.
+   No counterpart in the reference (its tile is fixed, CudaCommon.cuh:44-51). */
+int ff_set_tile(ff_domain* d, int tile);
+/* Time every tile shape on this domain (the state is left unchanged) and
+   keep the fastest; writes its index to *tile. */
+int ff_autotune(ff_domain* d, int* tile);
+
+/* ---- batches: many domains of one kind stepped together ----------------
+   No counterpart in the reference, which steps one flow field per Simulator
+   (src/core/Simulator2D.cpp:88-130).  A batch holds domains with the same
+   dim, storage, options and device.  One ff_batch_step step of every member
+   is one replay of a captured HIP graph; results of each member equal those
+   of ff_step on it alone.  While a domain is in a batch, its own ff_step /
+   ff_step_proxy refuse to run; every other per-domain call (getters,
+   ff_set_wave, ff_update_boundary, ...) keeps working and is ordered with
+   the batch's steps.  Changing a member's buffers (ff_set_boundary,
+   ff_set_flags, ...) or the membership rebuilds the graph at the next step. */
+typedef struct ff_batch ff_batch;
+int ff_batch_create(int dim, int storage, unsigned options, int device, ff_batch** out);
+/* Detaches remaining members (they stay valid domains). */
+int ff_batch_destroy(ff_batch* b);
+int ff_batch_add(ff_batch* b, ff_domain* d);
+int ff_batch_remove(ff_batch* b, ff_domain* d);
+int ff_batch_size(ff_batch* b, int* n);
+/* Asynchronous on the batch stream. */
+int ff_batch_step(ff_batch* b, int nsteps);
+/* Waits for the batch stream; reports a link-table overflow of any member. */
+int ff_batch_sync(ff_batch* b);
+/* Fix the 2D tile shape of the batch (-1: choose it by timing, the default). */
+int ff_batch_set_tile(ff_batch* b, int tile);
+/* Tile index in use and its cell extent. */
+int ff_batch_get_tile(ff_batch* b, int* tile, int* tile_x, int* tile_y);
+/* Run nsteps steps and return their GPU time in ms (HIP events). */
+int ff_batch_time_steps(ff_batch* b, int nsteps, double* total_ms);
+
 const char* ff_last_error(void);
 int ff_device_count(int* n);
 /* Library build identification string. */

@@ -41,3 +41,20 @@ def test_version_and_error_string_without_gpu():
     h = ctypes.c_void_p()
     rc = L.ff_domain_create(ctypes.byref(d), ctypes.byref(h))
     assert rc == 1 and b"dim" in L.ff_last_error()
+
+
+def test_footprint_without_gpu():
+    if not os.path.exists(capi.LIB_PATH):
+        pytest.skip("library not built")
+    from freeflow_amd import workloads as wl
+    nx, ny = 1200, 400
+    plane = (nx * ny + 255) // 256 * 256
+    base32 = capi.footprint(2, nx, ny, storage="f32")
+    base16 = capi.footprint(2, nx, ny, storage="f16")
+    assert base32 - base16 == 2 * plane * 6 * 2  # two ping-pong buffers, 6 planes, 2 bytes saved each
+    assert base32 == 2 * plane * 6 * 4 + nx * ny + 8192
+    sw = wl.config2_swimmer()
+    with_b = capi.footprint(2, nx, ny, storage="f32", verts=sw.rest, elems=sw.elems, dx=0.005)
+    assert with_b > base32
+    p3 = (96 ** 3 + 255) // 256 * 256
+    assert capi.footprint(3, 96, 96, 96, storage="bf16") == 2 * p3 * 10 * 2 + 96 ** 3 + 8192

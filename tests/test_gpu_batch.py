@@ -1,0 +1,164 @@
+"""Batched stepping (ff_batch_*): every member's results equal those of the
+same domain stepped alone with ff_step, bit for bit (moments, per-step FP64
+wrench, vertex forces), whatever the tile shape, the membership changes and
+the mix of domains with and without a boundary."""
+import sys
+
+import numpy as np
+import pytest
+
+from freeflow_amd import capi, workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def scenes(seed, n=3, nx=(96, 220), ny=(64, 120), seg=(60, 160)):
+    return wl.batched_scenes(n, seed, nx, ny, seg)
+
+
+def make(scene, storage="f32", history=64):
+    d = capi.Domain(**scene.domain_args(), storage=storage, stage_fp16=True)
+    scene.setup(d, history=history)
+    return d
+
+
+def same(a, b, what):
+    ma, mb = a.moments(0), b.moments(0)
+    assert np.array_equal(ma, mb), f"{what}: {np.sum(ma != mb)} moments differ"
+    assert np.array_equal(a.moments(1), b.moments(1)), f"{what}: other buffer differs"
+
+
+def test_batch_equals_individual_steps_2d():
+    sc = scenes(11)
+    note(f"case batch2d sizes {[(s.nx, s.ny, len(s.swimmer.elems)) for s in sc]}")
+    alone = [make(s) for s in sc]
+    mixed = [make(s) for s in sc]
+    b = capi.Batch(2, "f32", stage_fp16=True)
+    for d in mixed:
+        b.add(d)
+    for d in alone:
+        d.step(25)
+    b.step(25)
+    b.sync()
+    for i, (a, m) in enumerate(zip(alone, mixed)):
+        same(a, m, f"domain {i}")
+        assert np.array_equal(a.body_wrench(), m.body_wrench())
+        assert np.array_equal(a.wrench_history(25), m.wrench_history(25))
+        assert np.array_equal(a.vertex_forces(), m.vertex_forces())
+        assert np.array_equal(a.boundary()[0], m.boundary()[0])
+    # membership change: drop the middle swimmer, add a boundary-free vortex
+    # street; everybody keeps matching the solo runs
+    b.remove(mixed[1])
+    nx, ny = 120, 60
+    flags = wl.vortex_street_flags(nx, ny)
+    m0 = wl.noisy_inflow_moments(flags, nx, ny, 0.05, 1e-3, seed=5)
+    vs_alone = capi.Domain(2, nx, ny, viscosity=0.005)
+    vs_mixed = capi.Domain(2, nx, ny, viscosity=0.005)
+    for d in (vs_alone, vs_mixed):
+        d.set_flags(flags)
+        d.set_moments(m0)
+    b.add(vs_mixed)
+    for d in (alone[0], alone[2], vs_alone):
+        d.step(13)
+    mixed[1].step(13)  # detached again: steps on its own
+    alone[1].step(13)
+    b.step(13)
+    b.sync()
+    for a, m, what in ((alone[0], mixed[0], "d0"), (alone[2], mixed[2], "d2"), (vs_alone, vs_mixed, "street"),
+                       (alone[1], mixed[1], "detached")):
+        same(a, m, what)
+    with pytest.raises(capi.FreeFlowError):
+        mixed[0].step(1)
+
+
+@pytest.mark.parametrize("tile", [1, 2, 3, 4])
+def test_tile_shapes_give_identical_results(tile):
+    sc = scenes(21, n=2)
+    note(f"case tile {tile}")
+    ref = [make(s, history=0) for s in sc]
+    got = [make(s, history=0) for s in sc]
+    b0 = capi.Batch(2, "f32")
+    b0.set_tile(0)
+    bt = capi.Batch(2, "f32")
+    bt.set_tile(tile)
+    for d in ref:
+        b0.add(d)
+    for d in got:
+        bt.add(d)
+    b0.step(20)
+    bt.step(20)
+    assert bt.tile()[0] == tile
+    for a, m in zip(ref, got):
+        same(a, m, f"tile {tile}")
+        assert np.array_equal(a.body_wrench(), m.body_wrench())
+
+
+def test_autotune_and_single_domain_tiles():
+    s = scenes(31, n=1)[0]
+    note("case autotune")
+    a, m = make(s, history=0), make(s, history=0)
+    t = m.autotune()
+    assert 0 <= t < 5
+    a.step(15)
+    m.step(15)
+    same(a, m, f"autotuned tile {t}")
+    b = capi.Batch(2, "f32")
+    c = make(s, history=0)
+    b.add(c)
+    b.step(15)
+    tile, tx, ty = b.tile()
+    assert 0 <= tile < 5 and tx in (32, 64) and ty in (8, 16, 32)
+    same(a, c, "batch autotune")
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_batch_16bit_storage(storage):
+    sc = scenes(41, n=2)
+    note(f"case batch storage {storage}")
+    alone = [make(s, storage=storage) for s in sc]
+    mixed = [make(s, storage=storage) for s in sc]
+    b = capi.Batch(2, storage)
+    for d in mixed:
+        b.add(d)
+    for d in alone:
+        d.step(20)
+    b.step(20)
+    for a, m in zip(alone, mixed):
+        same(a, m, storage)
+        assert np.array_equal(a.body_wrench(), m.body_wrench())
+
+
+def test_batch_3d_sphere_and_periodic():
+    note("case batch3d")
+    nx, ny, nz = 40, 32, 24
+    m_tg = wl.taylor_green_ring(nx, ny, nz, amp=0.03, ring_r=6.0, seed=3)
+    sph_v, sph_f = wl.icosphere([20.0, 16.0, 12.0], 5.0, subdiv=2)
+    vel = np.zeros_like(sph_v)
+    vel[:, 0] = 0.02
+
+    def build(periodic):
+        d = capi.Domain(3, nx, ny, nz, viscosity=0.02, periodic=periodic)
+        d.set_moments(m_tg)
+        if not periodic:
+            d.set_boundary(sph_v, vel, sph_f)
+        return d
+
+    alone = [build(True), build(False)]
+    mixed_p, mixed_c = build(True), build(False)
+    bp = capi.Batch(3, "f32", periodic=True)
+    bc = capi.Batch(3, "f32", periodic=False)
+    bp.add(mixed_p)
+    bc.add(mixed_c)
+    for d in alone:
+        d.step(12)
+    bp.step(12)
+    bc.step(12)
+    same(alone[0], mixed_p, "3D periodic")
+    same(alone[1], mixed_c, "3D sphere")
+    assert np.array_equal(alone[1].body_wrench(), mixed_c.body_wrench())
+    assert np.abs(alone[1].body_wrench()).max() > 0

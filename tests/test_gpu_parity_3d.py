@@ -21,31 +21,7 @@ def note(msg):
     sys.stdout.flush()
 
 
-def icosphere(center, r, subdiv=1):
-    t = (1 + 5 ** 0.5) / 2
-    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
-                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], float)
-    f = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2], [10, 7, 6],
-         [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11], [6, 2, 10],
-         [8, 6, 7], [9, 8, 1]]
-    v = list(v / np.linalg.norm(v, axis=1)[:, None])
-    for _ in range(subdiv):
-        cache = {}
-        nf = []
-
-        def mid(a, b):
-            k = (min(a, b), max(a, b))
-            if k not in cache:
-                m = (v[a] + v[b]) / 2
-                v.append(m / np.linalg.norm(m))
-                cache[k] = len(v) - 1
-            return cache[k]
-
-        for a, b, c in f:
-            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
-            nf += [[a, ab, ca], [b, bc, ab], [c, ca, bc], [ab, bc, ca]]
-        f = nf
-    return np.array(v) * r + np.asarray(center), np.array(f, np.uint32)
+icosphere = wl.icosphere
 
 
 def pair3(nx, ny, nz, flags, m, periodic=False, storage="f32", nu=0.01, dx=1.0, dt=1.0, rho=1.0):
