@@ -16,6 +16,16 @@ inputs resident on the GPU.  `roofline` prices the stream-collide kernel
 update (6 FP32 moments read + 6 written + 1 flag byte); `cpu_baseline` is the
 CPU oracle (a C++ restatement of the reference update, OpenMP) timed on this
 host on a bounded number of steps of the same workload.
+
+Other workloads (one JSON line each, same keys):
+  --workload batched   64 uneven 2D swimmer domains (256x128 .. 1200x400,
+                       200-800 segments) in one batch, FP16 storage, one graph
+                       replay per step; also times the plain per-domain FP32
+                       form of the same domains and reports the ratio.
+  --workload config3   3D D3Q27 256x128x128 Taylor-Green + vortex ring,
+                       periodic, storage from --storage (f32, f16, bf16).
+  --workload served    Poisson-arriving 2D/3D rollout episodes at --rate per s
+                       for --duration s through the continuous-batching server.
 """
 import argparse
 import json
@@ -84,6 +94,131 @@ def cpu_baseline_config2(state, budget_s=12.0):
                       f"same 1200x400 swimmer workload, first {steps} steps ({el:.1f} s)"}
 
 
+def roofline(achieved_gbs, kernel, avg_ms, bpl, cells):
+    return {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
+            "avg_kernel_us": round(avg_ms * 1e3, 3), "bytes_per_lup": bpl, "cells_per_launch": cells}
+
+
+def base_line(value, K, W, ms_per_step, dtype, config, roof, cpu=None, **extra):
+    out = {"metric": METRIC, "value": round(value, 2), "unit": "MLUPS", "n_gpus": 1, "steps": K, "warmup": W,
+           "ms_per_step": round(ms_per_step, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": dtype, "data": "synthetic", "config": config, "roofline": roof, "cpu_baseline": cpu}
+    out.update(extra)
+    return out
+
+
+def run_batched(args, capi, wl):
+    c = wl.CONFIG4
+    scenes = wl.batched_scenes(c["n_domains"], c["seed"])
+    cells = sum(s.nx * s.ny for s in scenes)
+    K = args.steps if args.steps != 4000 else c["steps"]
+    W = min(args.warmup, 50)
+    # plain per-domain FP32 form: the reference's storage and FP16 tile, one
+    # domain per stream, stepped round-robin
+    plain = []
+    for sc in scenes:
+        d = capi.Domain(**sc.domain_args(), storage="f32", stage_fp16=True)
+        sc.setup(d)
+        plain.append(d)
+    for d in plain:
+        d.step(5)
+    for d in plain:
+        d.sync()
+    Kp = args.plain_steps
+    print(f"[bench] plain FP32 form: {len(plain)} domains, {cells} cells, {Kp} steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(Kp):
+        for d in plain:
+            d.step(1)
+    for d in plain:
+        d.sync()
+    t_plain = time.perf_counter() - t0
+    plain_mlups = cells * Kp / t_plain / 1e6
+    for d in plain:
+        d.close()
+    del plain
+    # batched FP16-storage form, FP32 arithmetic
+    b = capi.Batch(2, c["storage"], stage_fp16=False)
+    doms = []
+    for sc in scenes:
+        d = capi.Domain(**sc.domain_args(), storage=c["storage"], stage_fp16=False)
+        sc.setup(d)
+        b.add(d)
+        doms.append(d)
+    b.step(W)
+    b.sync()
+    print(f"[bench] batched FP16 form: warmup {W} done; timing {K} steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    b.step(K)
+    b.sync()
+    el = time.perf_counter() - t0
+    mlups = cells * K / el / 1e6
+    tile, tx, ty = b.tile()
+    avg_ms = b.time_bulk_kernel(100)
+    bpl = bytes_per_lup(2, 2)
+    roof = roofline(bpl * cells / (avg_ms * 1e-3) / 1e9, f"kb_bulk2d<__half> tile {tx}x{ty} (64 domains)", avg_ms,
+                    bpl, cells)
+    wr = np.concatenate([d.body_wrench() for d in doms])
+    out = base_line(mlups, K, W, el * 1e3 / K, "f32 compute, f16 storage",
+                    {"workload": "64 uneven 2D swimmer domains (256x128..1200x400, 200-800 segments) in one batch, "
+                                 "FP16 storage, one graph replay per step", "domains": len(scenes),
+                     "cells": cells, "tile": [tx, ty], "parallelism": "single-gpu"},
+                    roof, None, plain_fp32_mlups=round(plain_mlups, 2),
+                    speedup_vs_plain_fp32=round(mlups / plain_mlups, 3),
+                    healthy=bool(np.all(np.isfinite(wr))))
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def run_config3(args, capi, wl):
+    c = wl.CONFIG3
+    nx, ny, nz = c["nx"], c["ny"], c["nz"]
+    cells = nx * ny * nz
+    st = args.storage
+    d = capi.Domain(3, nx, ny, nz, viscosity=c["viscosity"], storage=st, periodic=True)
+    d.set_moments(wl.taylor_green_ring(nx, ny, nz, c["amp"], c["ring_r"], c["seed"]))
+    K = args.steps if args.steps != 4000 else 1000
+    W = min(args.warmup, 20)
+    m0 = d.mass()
+    d.step(W)
+    d.sync()
+    print(f"[bench] config3 {st}: timing {K} steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    d.step(K)
+    d.sync()
+    el = time.perf_counter() - t0
+    drift = abs(d.mass() - m0) / m0
+    avg_ms = d.time_bulk_kernel(50)
+    bpl = bytes_per_lup(3, 4 if st == "f32" else 2)
+    roof = roofline(bpl * cells / (avg_ms * 1e-3) / 1e9, f"k_bulk3d<{st}> 8x8x4 tile", avg_ms, bpl, cells)
+    out = base_line(cells * K / el / 1e6, K, W, el * 1e3 / K, f"f32 compute, {st} storage",
+                    {"workload": "3D D3Q27 256x128x128 Taylor-Green (amplitude 0.04) + vortex ring r=20, periodic",
+                     "nx": nx, "ny": ny, "nz": nz, "storage": st, "parallelism": "single-gpu"},
+                    roof, None, mass_drift=drift)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def run_served(args):
+    from freeflow_amd import serve
+    eps = serve.make_episodes(args.rate, args.duration, seed=1234)
+    srv = serve.Server(storage="f16")
+    print(f"[bench] served: {len(eps)} episodes at {args.rate}/s over {args.duration} s", file=sys.stderr, flush=True)
+    summ = srv.run(eps, realtime=True)
+    srv.close()
+    out = base_line(summ["aggregate_mlups"], summ["iterations"], 0, summ["wall_s"] * 1e3 / max(summ["iterations"], 1),
+                    "f32 compute, f16 storage",
+                    {"workload": f"Poisson rollout episodes at {args.rate}/s for {args.duration} s: 70% 2D "
+                                 "(600x200..1200x400 swimmer), 30% 3D (96..192 x 96 x 96 sphere), 500 steps, "
+                                 "action every 10 steps", "rate": args.rate, "duration": args.duration,
+                     "parallelism": "single-gpu"}, None, None,
+                    step_latency_ms={"p50": summ["p50_step_ms"], "p99": summ["p99_step_ms"]},
+                    served=summ)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +227,10 @@ def main():
     ap.add_argument("--workload", default="config2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--storage", default="f16", help="config3: f32, f16 or bf16")
+    ap.add_argument("--rate", type=float, default=50.0, help="served: episode arrivals per second")
+    ap.add_argument("--duration", type=float, default=5.0, help="served: arrival window in seconds")
+    ap.add_argument("--plain-steps", type=int, default=200, help="batched: steps of the per-domain FP32 form")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -100,6 +239,12 @@ def main():
         return 2
     from freeflow_amd import capi, workloads as wl
 
+    if args.workload == "batched":
+        return run_batched(args, capi, wl)
+    if args.workload == "config3":
+        return run_config3(args, capi, wl)
+    if args.workload == "served":
+        return run_served(args)
     if args.workload != "config2":
         print(f"unknown workload {args.workload}", file=sys.stderr)
         return 2

@@ -32,7 +32,7 @@ EXPORTS = [
     "ff_get_wrench_history", "ff_get_boundary", "ff_mass", "ff_fill_solid", "ff_sync", "ff_time_bulk_kernel",
     "ff_time_steps", "ff_domain_footprint", "ff_set_tile", "ff_autotune", "ff_batch_create", "ff_batch_destroy", "ff_batch_add",
     "ff_batch_remove", "ff_batch_size", "ff_batch_step", "ff_batch_sync", "ff_batch_set_tile", "ff_batch_get_tile",
-    "ff_batch_time_steps", "ff_last_error", "ff_device_count", "ff_version",
+    "ff_batch_time_steps", "ff_batch_time_bulk", "ff_last_error", "ff_device_count", "ff_version",
 ]
 
 
@@ -98,6 +98,7 @@ def lib():
         L.ff_batch_set_tile.argtypes = [vp, C.c_int]
         L.ff_batch_get_tile.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.ff_batch_time_steps.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
+        L.ff_batch_time_bulk.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -333,4 +334,9 @@ class Batch:
     def time_steps(self, n):
         ms = C.c_double(0)
         _check(lib().ff_batch_time_steps(self.h, n, C.byref(ms)))
+        return ms.value
+
+    def time_bulk_kernel(self, iters):
+        ms = C.c_double(0)
+        _check(lib().ff_batch_time_bulk(self.h, iters, C.byref(ms)))
         return ms.value

@@ -21,8 +21,8 @@
 // buffers change.  The arithmetic per cell, link and element is that of the
 // per-domain kernels, so a domain's results are the same alone or in a batch.
 //
-// The 2D stream-collide tile shape is chosen per batch by timing the
-// candidates of kTiles2 on the batch itself (the bulk kernel only rewrites
+// The stream-collide tile shape is chosen per batch by timing the
+// candidates of kTiles2 / kTiles3 on the batch itself (the bulk kernel only rewrites
 // cells every step rewrites, so timing it leaves the state unchanged).
 
 #include <algorithm>
@@ -54,8 +54,8 @@ struct Batch {
   hipStream_t stream = nullptr;
   std::vector<Domain*> mem;
   bool dirty = true;
-  int tile2 = -1;        // fixed 2D tile (index into kTiles2), -1 = autotune
-  int tuned_tile2 = -1;  // result of the last tile search
+  int tile = -1;        // fixed tile shape (index into kTiles2 / kTiles3), -1 = autotune
+  int tuned_tile = -1;  // result of the last tile search
   DomDev* d_doms = nullptr;
   size_t doms_cap = 0;
   int2* d_map = nullptr;
@@ -83,9 +83,10 @@ __global__ void __launch_bounds__(256) kb_bulk2d(const DomDev* __restrict__ doms
                                                D->la.ny, D->la.plane, D->la.omega, m.y % D->tiles_x, m.y / D->tiles_x);
 }
 
-template <typename S, bool PERIODIC>
-__global__ void __launch_bounds__(256) kb_bulk3d(const DomDev* __restrict__ doms, const int2* __restrict__ map,
-                                                 int nbuild) {
+// RING = 0: 8x8x4 box tiles; otherwise TX x TY columns marching ZC planes.
+template <typename S, bool PERIODIC, int RING, int TX, int TY, int ZC>
+__global__ void __launch_bounds__(RING ? TX * TY : 256) kb_bulk3d(const DomDev* __restrict__ doms,
+                                                                  const int2* __restrict__ map, int nbuild) {
   const int2 m = map[blockIdx.x];
   const DomDev* D = doms + m.x;
   if ((int)blockIdx.x < nbuild) {
@@ -95,8 +96,13 @@ __global__ void __launch_bounds__(256) kb_bulk3d(const DomDev* __restrict__ doms
   }
   const unsigned p = *D->step & 1u;
   const int tx = D->tiles_x, ty = D->tiles_y;
-  bulk3d_tile<S, PERIODIC>((const S*)D->mom[p], (S*)D->mom[p ^ 1u], D->la.flags, D->la.nx, D->la.ny, D->la.nz,
-                           D->la.plane, D->la.omega, m.y % tx, (m.y / tx) % ty, m.y / (tx * ty));
+  if constexpr (RING)
+    bulk3d_ring<S, PERIODIC, TX, TY, ZC>((const S*)D->mom[p], (S*)D->mom[p ^ 1u], D->la.flags, D->la.nx, D->la.ny,
+                                         D->la.nz, D->la.plane, D->la.omega, m.y % tx, (m.y / tx) % ty,
+                                         m.y / (tx * ty));
+  else
+    bulk3d_tile<S, PERIODIC>((const S*)D->mom[p], (S*)D->mom[p ^ 1u], D->la.flags, D->la.nx, D->la.ny, D->la.nz,
+                             D->la.plane, D->la.omega, m.y % tx, (m.y / tx) % ty, m.y / (tx * ty));
 }
 
 template <int DIM, typename S, bool STAGE, bool PERIODIC>
@@ -149,6 +155,18 @@ static void bulk2d_batch_variant(Batch& b, int tile, const int2* map, int nblock
   }
 }
 
+template <typename S, bool PERIODIC>
+static void bulk3d_batch_variant(Batch& b, int tile, const int2* map, int nblocks, int nbuild) {
+  // the link-detection blocks of a ring launch run with the ring's block size
+  // (build_links_block covers blockDim.x / 16 pairs), so their count depends on it
+  switch (tile) {
+    case 1: kb_bulk3d<S, PERIODIC, 1, 16, 8, 32><<<nblocks, 128, 0, b.stream>>>(b.d_doms, map, nbuild); break;
+    case 2: kb_bulk3d<S, PERIODIC, 1, 8, 8, 32><<<nblocks, 64, 0, b.stream>>>(b.d_doms, map, nbuild); break;
+    case 3: kb_bulk3d<S, PERIODIC, 1, 16, 16, 32><<<nblocks, 256, 0, b.stream>>>(b.d_doms, map, nbuild); break;
+    default: kb_bulk3d<S, PERIODIC, 0, 8, 8, 4><<<nblocks, 256, 0, b.stream>>>(b.d_doms, map, nbuild); break;
+  }
+}
+
 template <typename S>
 static void launch_bulk_batch(Batch& b, int tile, const int2* map, int nblocks, int nbuild) {
   if (nblocks <= 0) return;
@@ -158,9 +176,10 @@ static void launch_bulk_batch(Batch& b, int tile, const int2* map, int nblocks, 
     else if (stage) bulk2d_batch_variant<S, true, false>(b, tile, map, nblocks, nbuild);
     else if (per) bulk2d_batch_variant<S, false, true>(b, tile, map, nblocks, nbuild);
     else bulk2d_batch_variant<S, false, false>(b, tile, map, nblocks, nbuild);
+  } else if (per) {
+    bulk3d_batch_variant<S, true>(b, tile, map, nblocks, nbuild);
   } else {
-    if (per) kb_bulk3d<S, true><<<nblocks, 256, 0, b.stream>>>(b.d_doms, map, nbuild);
-    else kb_bulk3d<S, false><<<nblocks, 256, 0, b.stream>>>(b.d_doms, map, nbuild);
+    bulk3d_batch_variant<S, false>(b, tile, map, nblocks, nbuild);
   }
 }
 
@@ -222,9 +241,10 @@ static void tiles_of(const Batch& b, const Domain& d, int tile, int* tx, int* ty
     *ty = (d.ny + t.ty * t.cpt - 1) / (t.ty * t.cpt);
     *nt = *tx * *ty;
   } else {
-    *tx = (d.nx + kTX3 - 1) / kTX3;
-    *ty = (d.ny + kTY3 - 1) / kTY3;
-    *nt = *tx * *ty * ((d.nz + kTZ3 - 1) / kTZ3);
+    const Tile3 t = kTiles3[tile];
+    *tx = (d.nx + t.tx - 1) / t.tx;
+    *ty = (d.ny + t.ty - 1) / t.ty;
+    *nt = *tx * *ty * ((d.nz + t.tz - 1) / t.tz);
   }
 }
 
@@ -255,7 +275,8 @@ static int build_tables(Batch& b, int tile) {
     r.tiles_y = ty;
     for (int t = 0; t < nt; t++) bulk.push_back(make_int2(i, t));
     if (bnd) {
-      const int nbuild = (int)(((long long)d.ne * K + 15) / 16);
+      const int per_block = (b.dim == 3 && kTiles3[tile].ring ? kTiles3[tile].tx * kTiles3[tile].ty : 256) / 16;
+      const int nbuild = (int)(((long long)d.ne * K + per_block - 1) / per_block);
       for (int t = 0; t < nbuild; t++) build.push_back(make_int2(i, t));
       r.nbc = (int)std::min<unsigned long long>(512, std::max<unsigned long long>(1, (d.lt_est / 4 + cpb - 1) / cpb));
       for (int t = 0; t < r.nbc; t++) bc.push_back(make_int2(i, t));
@@ -288,17 +309,18 @@ static int build_tables(Batch& b, int tile) {
 // Time the stream-collide tiles of every member for each candidate shape
 // and keep the fastest.
 static int autotune(Batch& b) {
-  if (b.dim != 2 || b.mem.empty()) {
-    b.tuned_tile2 = 0;
+  if (b.mem.empty()) {
+    b.tuned_tile = 0;
     return FF_OK;
   }
+  const int ntiles = b.dim == 2 ? kNumTiles2 : kNumTiles3;
   hipEvent_t e0, e1;
   FF_HIP(hipEventCreate(&e0));
   FF_HIP(hipEventCreate(&e1));
   double best = 1e30;
   int best_t = 0;
   int rc = FF_OK;
-  for (int t = 0; t < kNumTiles2 && !rc; t++) {
+  for (int t = 0; t < ntiles && !rc; t++) {
     if ((rc = build_tables(b, t))) break;
     const int2* map = b.d_map + b.n_build;
     for (int w = 0; w < 3; w++) launch_bulk_only(b, t, map, b.n_bulk);
@@ -317,7 +339,7 @@ static int autotune(Batch& b) {
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  b.tuned_tile2 = best_t;
+  b.tuned_tile = best_t;
   return rc;
 }
 
@@ -328,17 +350,14 @@ static void drop_graph(Batch& b) {
   b.graph = nullptr;
 }
 
-static int current_tile(const Batch& b) {
-  if (b.dim != 2) return 0;
-  return b.tile2 >= 0 ? b.tile2 : std::max(b.tuned_tile2, 0);
-}
+static int current_tile(const Batch& b) { return b.tile >= 0 ? b.tile : std::max(b.tuned_tile, 0); }
 
 static int rebuild(Batch& b) {
   FF_HIP(hipSetDevice(b.device));
   FF_HIP(hipStreamSynchronize(b.stream));
   drop_graph(b);
   int rc;
-  if (b.dim == 2 && b.tile2 < 0 && b.tuned_tile2 < 0 && (rc = autotune(b))) return rc;
+  if (b.tile < 0 && b.tuned_tile < 0 && (rc = autotune(b))) return rc;
   if ((rc = build_tables(b, current_tile(b)))) return rc;
   FF_HIP(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
   launch_step(b, current_tile(b));
@@ -494,9 +513,9 @@ int ff_batch_sync(ff_batch* b) {
 
 int ff_batch_set_tile(ff_batch* b, int tile) {
   FF_CHECK_BATCH(b);
-  if (tile < -1 || tile >= kNumTiles2) return fail(FF_ERR_INVALID, "tile index out of range");
-  b->tile2 = tile;
-  if (tile < 0) b->tuned_tile2 = -1;
+  if (tile < -1 || tile >= (b->dim == 2 ? kNumTiles2 : kNumTiles3)) return fail(FF_ERR_INVALID, "tile index out of range");
+  b->tile = tile;
+  if (tile < 0) b->tuned_tile = -1;
   b->dirty = true;
   return FF_OK;
 }
@@ -509,8 +528,8 @@ int ff_batch_get_tile(ff_batch* b, int* tile, int* tx, int* ty) {
   if (b->dirty && !b->mem.empty() && (rc = rebuild(*b))) return rc;
   const int t = current_tile(*b);
   *tile = t;
-  if (tx) *tx = b->dim == 2 ? kTiles2[t].tx : kTX3;
-  if (ty) *ty = b->dim == 2 ? kTiles2[t].ty * kTiles2[t].cpt : kTY3;
+  if (tx) *tx = b->dim == 2 ? kTiles2[t].tx : kTiles3[t].tx;
+  if (ty) *ty = b->dim == 2 ? kTiles2[t].ty * kTiles2[t].cpt : kTiles3[t].ty;
   return FF_OK;
 }
 
@@ -535,6 +554,37 @@ int ff_batch_time_steps(ff_batch* b, int nsteps, double* total_ms) {
   if (!b->mem.empty()) advance_mirrors(*b, nsteps);
   *total_ms = ms;
   return ff_batch_sync(b);
+}
+
+int ff_batch_time_bulk(ff_batch* b, int iters, double* avg_ms) {
+  FF_CHECK_BATCH(b);
+  if (iters <= 0 || !avg_ms) return fail(FF_ERR_INVALID, "bad arguments");
+  FF_HIP(hipSetDevice(b->device));
+  int rc;
+  if (b->mem.empty()) return fail(FF_ERR_STATE, "empty batch");
+  if (b->dirty && (rc = rebuild(*b))) return rc;
+  // the stream-collide tiles of every member (the link-detection blocks of a
+  // step are excluded); they rewrite only cells every step rewrites
+  const int tile = current_tile(*b);
+  const int2* map = b->d_map + b->n_build;
+  std::vector<hipEvent_t> ev(2 * iters);
+  for (auto& e : ev) FF_HIP(hipEventCreate(&e));
+  for (int k = 0; k < iters; k++) {
+    FF_HIP(hipEventRecord(ev[2 * k], b->stream));
+    launch_bulk_only(*b, tile, map, b->n_bulk);
+    FF_HIP(hipEventRecord(ev[2 * k + 1], b->stream));
+  }
+  FF_HIP(hipGetLastError());
+  FF_HIP(hipStreamSynchronize(b->stream));
+  double tot = 0.0;
+  for (int k = 0; k < iters; k++) {
+    float ms = 0.f;
+    FF_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
+    tot += ms;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  *avg_ms = tot / iters;
+  return FF_OK;
 }
 
 }  // extern "C"

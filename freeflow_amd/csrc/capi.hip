@@ -866,9 +866,10 @@ int ff_domain_footprint(const ff_domain_desc* desc, int nv, int ne, const float*
 
 int ff_set_tile(ff_domain* dd, int tile) {
   FF_CHECK_DOMAIN(dd);
-  if (tile < 0 || tile >= kNumTiles2) return fail(FF_ERR_INVALID, "tile index out of range");
-  if (dd->dim != 2) return fail(FF_ERR_INVALID, "tile shapes apply to 2D domains");
-  dd->tile2 = tile;
+  const int n = dd->dim == 2 ? kNumTiles2 : kNumTiles3;
+  if (tile < 0 || tile >= n) return fail(FF_ERR_INVALID, "tile index out of range");
+  if (dd->dim == 2) dd->tile2 = tile;
+  else dd->tile3 = tile;
   batch_mark_dirty(dd->batch);
   return FF_OK;
 }
@@ -877,10 +878,6 @@ int ff_autotune(ff_domain* dd, int* tile) {
   FF_CHECK_DOMAIN(dd);
   Domain& d = *dd;
   if (d.batch) return fail(FF_ERR_STATE, "domain belongs to a batch: its tile is tuned with the batch");
-  if (d.dim != 2) {
-    if (tile) *tile = 0;
-    return FF_OK;
-  }
   FF_HIP(hipSetDevice(d.device));
   hipEvent_t e0, e1;
   FF_HIP(hipEventCreate(&e0));
@@ -891,22 +888,26 @@ int ff_autotune(ff_domain* dd, int* tile) {
   void* out = d.mom[d.cur ^ 1];
   double best = 1e30;
   int best_t = 0, rc = FF_OK;
-  const int saved = d.tile2;
-  for (int t = 0; t < kNumTiles2 && !rc; t++) {
-    d.tile2 = t;
-    for (int w = 0; w < 3 && !rc; w++) rc = launch_bulk2d(d, in, out);
+  int& cur = d.dim == 2 ? d.tile2 : d.tile3;
+  const int n = d.dim == 2 ? kNumTiles2 : kNumTiles3;
+  const int saved = cur;
+  for (int t = 0; t < n && !rc; t++) {
+    cur = t;
+    auto launch = [&]() { return d.dim == 2 ? launch_bulk2d(d, in, out) : launch_bulk3d(d, in, out); };
+    for (int w = 0; w < 3 && !rc; w++) rc = launch();
+    if (rc) break;
     FF_HIP(hipEventRecord(e0, d.stream));
-    for (int k = 0; k < 20 && !rc; k++) rc = launch_bulk2d(d, in, out);
+    for (int k = 0; k < 10 && !rc; k++) rc = launch();
     FF_HIP(hipEventRecord(e1, d.stream));
     FF_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
     FF_HIP(hipEventElapsedTime(&ms, e0, e1));
-    if (ms < best) { best = ms; best_t = t; }
+    if (!rc && ms < best) { best = ms; best_t = t; }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  d.tile2 = rc ? saved : best_t;
-  if (tile) *tile = d.tile2;
+  cur = rc ? saved : best_t;
+  if (tile) *tile = cur;
   return rc;
 }
 

@@ -84,6 +84,7 @@ struct Domain {
   int storage = FF_STORAGE_F32;
   unsigned options = 0;
   int tile2 = 0;     // 2D stream-collide tile shape (index into kTiles2)
+  int tile3 = 0;     // 3D stream-collide tile shape (index into kTiles3)
   double nu = 0, dx = 1, dt = 1, rho_f = 1;
   float nu_f = 0, cl = 1, ct = 1, cf = 1, omega = 1;
   size_t plane = 0;  // elements between moment planes

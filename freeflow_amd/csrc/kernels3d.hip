@@ -19,14 +19,29 @@ __global__ void __launch_bounds__(kTX3* kTY3* kTZ3)
   bulk3d_tile<S, PERIODIC>(in, out, flags, nx, ny, nz, plane, omega, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
+template <typename S, bool PERIODIC, int TX, int TY, int ZC>
+__global__ void __launch_bounds__(TX* TY)
+    k_bulk3d_ring(const S* __restrict__ in, S* __restrict__ out, const uint8_t* __restrict__ flags, int nx, int ny,
+                  int nz, size_t plane, float omega) {
+  bulk3d_ring<S, PERIODIC, TX, TY, ZC>(in, out, flags, nx, ny, nz, plane, omega, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+template <typename S, bool PERIODIC>
+static void bulk3d_variant(Domain& d, const S* i, S* o) {
+  const Tile3 t = kTiles3[d.tile3];
+  dim3 grid((d.nx + t.tx - 1) / t.tx, (d.ny + t.ty - 1) / t.ty, (d.nz + t.tz - 1) / t.tz);
+  switch (d.tile3) {
+    case 1: k_bulk3d_ring<S, PERIODIC, 16, 8, 32><<<grid, 128, 0, d.stream>>>(i, o, d.flags, d.nx, d.ny, d.nz, d.plane, d.omega); break;
+    case 2: k_bulk3d_ring<S, PERIODIC, 8, 8, 32><<<grid, 64, 0, d.stream>>>(i, o, d.flags, d.nx, d.ny, d.nz, d.plane, d.omega); break;
+    case 3: k_bulk3d_ring<S, PERIODIC, 16, 16, 32><<<grid, 256, 0, d.stream>>>(i, o, d.flags, d.nx, d.ny, d.nz, d.plane, d.omega); break;
+    default: k_bulk3d<S, PERIODIC><<<grid, kTX3 * kTY3 * kTZ3, 0, d.stream>>>(i, o, d.flags, d.nx, d.ny, d.nz, d.plane, d.omega); break;
+  }
+}
+
 template <typename S>
 static int bulk3d_typed(Domain& d, const void* in, void* out) {
-  dim3 grid((d.nx + kTX3 - 1) / kTX3, (d.ny + kTY3 - 1) / kTY3, (d.nz + kTZ3 - 1) / kTZ3);
-  dim3 block(kTX3 * kTY3 * kTZ3);
-  if (d.options & FF_OPT_PERIODIC)
-    k_bulk3d<S, true><<<grid, block, 0, d.stream>>>((const S*)in, (S*)out, d.flags, d.nx, d.ny, d.nz, d.plane, d.omega);
-  else
-    k_bulk3d<S, false><<<grid, block, 0, d.stream>>>((const S*)in, (S*)out, d.flags, d.nx, d.ny, d.nz, d.plane, d.omega);
+  if (d.options & FF_OPT_PERIODIC) bulk3d_variant<S, true>(d, (const S*)in, (S*)out);
+  else bulk3d_variant<S, false>(d, (const S*)in, (S*)out);
   FF_HIP(hipGetLastError());
   return FF_OK;
 }

@@ -139,4 +139,103 @@ __device__ __forceinline__ void bulk3d_tile(const S* __restrict__ in, S* __restr
   for (int q = 0; q < 10; q++) Store<S>::template store<3>(out + q * plane + cell, q, o[q]);
 }
 
+// 3D ring variant: a TX x TY column of cells per block marching along z over
+// ZC planes.  Three planes of reconstructed populations (z-1, z, z+1, each
+// with a one-cell xy halo) rotate through LDS, so each halo cell is loaded
+// and reconstructed once per column instead of once per 8x8x4 box (1.41x
+// instead of 2.34x for 16x8).  The next plane's moments are fetched into
+// registers before the current plane is computed, hiding their latency.
+// Arithmetic per cell is that of bulk3d_tile.
+struct Tile3 {
+  int tx, ty, tz;  // tz: planes per block (box variant: 4, ring: ZC)
+  bool ring;
+};
+constexpr Tile3 kTiles3[] = {{8, 8, 4, false}, {16, 8, 32, true}, {8, 8, 32, true}, {16, 16, 32, true}};
+constexpr int kNumTiles3 = 4;
+
+template <typename S, bool PERIODIC, int TX, int TY, int ZC>
+__device__ __forceinline__ void bulk3d_ring(const S* __restrict__ in, S* __restrict__ out,
+                                            const uint8_t* __restrict__ flags, int nx, int ny, int nz, size_t plane,
+                                            float omega, int bx, int by, int bz) {
+  constexpr int NT = TX * TY, HX = TX + 2, HY = TY + 2, HA = HX * HY;
+  constexpr int PER = (HA + NT - 1) / NT;  // halo cells per thread per plane
+  __shared__ float sp[3][27][HA];
+  __shared__ uint8_t sf[3][HA];
+  const int x0 = bx * TX, y0 = by * TY, z0 = bz * ZC;
+  const int zend = min(z0 + ZC, nz);
+  float mreg[PER][10];
+  uint8_t freg[PER];
+  // fetch the moments and flags of plane gz (may be -1 or nz: halo) into registers
+  auto fetch = [&](int gz) {
+    const int wz = ((gz % nz) + nz) % nz;
+    const int sz = PERIODIC ? wz : min(nz - 1, max(0, gz));
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+      const int k = threadIdx.x + r * NT;
+      if (k < HA) {
+        const int gx = x0 + k % HX - 1, gy = y0 + k / HX - 1;
+        const int wx = ((gx % nx) + nx) % nx, wy = ((gy % ny) + ny) % ny;
+        int sx, sy;
+        if (PERIODIC) {
+          sx = wx; sy = wy;
+        } else {
+          sx = min(nx - 1, max(0, gx));
+          sy = min(ny - 1, max(0, gy));
+        }
+        const size_t c = ((size_t)sz * ny + sy) * nx + sx;
+#pragma unroll
+        for (int q = 0; q < 10; q++) mreg[r][q] = Store<S>::template load<3>(in + q * plane + c, q);
+        freg[r] = flags[((size_t)wz * ny + wy) * nx + wx];
+      }
+    }
+  };
+  // reconstruct the fetched plane into ring slot `slot`
+  auto put = [&](int slot) {
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+      const int k = threadIdx.x + r * NT;
+      if (k < HA) {
+        float f[27];
+        recon3_all(mreg[r], f);
+#pragma unroll
+        for (int i = 0; i < 27; i++) sp[slot][i][k] = f[i];
+        sf[slot][k] = freg[r];
+      }
+    }
+  };
+  const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
+  const int x = x0 + lx, y = y0 + ly;
+  const bool inside = x < nx && y < ny;
+  const int ctr = (ly + 1) * HX + (lx + 1);
+  // slot of plane z: (z - z0 + 1) % 3, so z0-1 -> 0, z0 -> 1, z0+1 -> 2
+  fetch(z0 - 1);
+  put(0);
+  fetch(z0);
+  put(1);
+  fetch(z0 + 1);
+  for (int z = z0; z < zend; z++) {
+    const int s_m = (z - z0) % 3, s_c = (z - z0 + 1) % 3, s_p = (z - z0 + 2) % 3;
+    put(s_p);  // plane z+1 (fetched during the previous iteration)
+    __syncthreads();
+    if (z + 2 <= zend) fetch(z + 2);  // overlaps the compute below
+    if (inside && is_updated(sf[s_c][ctr])) {
+      float p[27];
+#pragma unroll
+      for (int i = 0; i < 27; i++) {
+        const int cz = D3::cz(i);
+        const int sl = cz > 0 ? s_m : (cz < 0 ? s_p : s_c);
+        const int up = ctr - D3::cy(i) * HX - D3::cx(i);
+        if (is_wall3(sf[sl][up])) p[i] = fmaxf(0.0f, sp[s_c][D3::inv(i)][ctr]);
+        else p[i] = fmaxf(0.0f, sp[sl][i][up]);
+      }
+      float o[10];
+      collide3(p, omega, o);
+      const size_t cell = ((size_t)z * ny + y) * nx + x;
+#pragma unroll
+      for (int q = 0; q < 10; q++) Store<S>::template store<3>(out + q * plane + cell, q, o[q]);
+    }
+    __syncthreads();  // slot s_m is overwritten by the next iteration's put
+  }
+}
+
 }  // namespace ff

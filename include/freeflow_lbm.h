@@ -181,8 +181,9 @@ int ff_time_steps(ff_domain* d, int nsteps, double* total_ms);
 int ff_domain_footprint(const ff_domain_desc* desc, int nv, int ne, const float* verts, const uint32_t* elems,
                         unsigned long long* bytes);
 
-/* 2D stream-collide tile shape (0..4: 32x8, 32x16, 64x8, 32x32, 64x16 cells
-   per 256-thread block; 0 is the default).  Results do not depend on it.
+/* Stream-collide tile shape.  2D (0..4): 32x8, 32x16, 64x8, 32x32, 64x16
+   cells per 256-thread block.  3D (0..3): 8x8x4 boxes, or 16x8, 8x8, 16x16
+   columns marching 32 planes.  0 is the default; results do not depend on it.
    No counterpart in the reference (its tile is fixed, CudaCommon.cuh:44-51). */
 int ff_set_tile(ff_domain* d, int tile);
 /* Time every tile shape on this domain (the state is left unchanged) and
@@ -210,12 +211,17 @@ int ff_batch_size(ff_batch* b, int* n);
 int ff_batch_step(ff_batch* b, int nsteps);
 /* Waits for the batch stream; reports a link-table overflow of any member. */
 int ff_batch_sync(ff_batch* b);
-/* Fix the 2D tile shape of the batch (-1: choose it by timing, the default). */
+/* Fix the tile shape of the batch (index as for ff_set_tile; -1: choose it
+   by timing the candidates on the members, the default). */
 int ff_batch_set_tile(ff_batch* b, int tile);
 /* Tile index in use and its cell extent. */
 int ff_batch_get_tile(ff_batch* b, int* tile, int* tile_x, int* tile_y);
 /* Run nsteps steps and return their GPU time in ms (HIP events). */
 int ff_batch_time_steps(ff_batch* b, int nsteps, double* total_ms);
+/* Average duration (ms, HIP events) of the batched stream-collide kernel
+   over `iters` launches, without the link-detection blocks; the state is
+   left unchanged. */
+int ff_batch_time_bulk(ff_batch* b, int iters, double* avg_ms);
 
 const char* ff_last_error(void);
 int ff_device_count(int* n);
