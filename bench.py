@@ -40,6 +40,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "MLUPS + % HBM peak; batched/served vs plain at ≤1e-3 rel err; p99 step ms"
 HBM_PEAK_GBS = 8000.0
+# tile shapes by index, as in the library (ff_set_tile)
+TILES2 = ["32x8", "32x16", "64x8", "32x32", "64x16"]
+TILES3 = ["8x8x4 box", "16x8 column x32", "8x8 column x32", "16x16 column x32"]
 
 
 def bytes_per_lup(dim, storage_bytes):
@@ -180,6 +183,7 @@ def run_config3(args, capi, wl):
     d.set_moments(wl.taylor_green_ring(nx, ny, nz, c["amp"], c["ring_r"], c["seed"]))
     K = args.steps if args.steps != 4000 else 1000
     W = min(args.warmup, 20)
+    tile = d.autotune() if not args.no_tune else 0
     m0 = d.mass()
     d.step(W)
     d.sync()
@@ -191,10 +195,10 @@ def run_config3(args, capi, wl):
     drift = abs(d.mass() - m0) / m0
     avg_ms = d.time_bulk_kernel(50)
     bpl = bytes_per_lup(3, 4 if st == "f32" else 2)
-    roof = roofline(bpl * cells / (avg_ms * 1e-3) / 1e9, f"k_bulk3d<{st}> 8x8x4 tile", avg_ms, bpl, cells)
+    roof = roofline(bpl * cells / (avg_ms * 1e-3) / 1e9, f"k_bulk3d<{st}> tile {TILES3[tile]}", avg_ms, bpl, cells)
     out = base_line(cells * K / el / 1e6, K, W, el * 1e3 / K, f"f32 compute, {st} storage",
                     {"workload": "3D D3Q27 256x128x128 Taylor-Green (amplitude 0.04) + vortex ring r=20, periodic",
-                     "nx": nx, "ny": ny, "nz": nz, "storage": st, "parallelism": "single-gpu"},
+                     "nx": nx, "ny": ny, "nz": nz, "storage": st, "tile": TILES3[tile], "parallelism": "single-gpu"},
                     roof, None, mass_drift=drift)
     print(json.dumps(out), flush=True)
     return 0
@@ -226,6 +230,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--workload", default="config2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default tile shape instead of timing the candidates")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--storage", default="f16", help="config3: f32, f16 or bf16")
     ap.add_argument("--rate", type=float, default=50.0, help="served: episode arrivals per second")
@@ -249,6 +254,7 @@ def main():
         print(f"unknown workload {args.workload}", file=sys.stderr)
         return 2
     d, state = setup_config2(capi, wl)
+    tile = d.autotune() if not args.no_tune else 0
     c = wl.CONFIG2
     cells = c["nx"] * c["ny"]
     K, W = args.steps, args.warmup
@@ -279,7 +285,8 @@ def main():
     achieved = bpl * cells / (avg_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_bulk2d_links<float,stage_fp16> (stream-collide tiles + link detection blocks)", "avg_kernel_us": round(avg_ms * 1e3, 3),
+            "kernel": f"k_bulk2d_links<float,stage_fp16> tile {TILES2[tile]} (stream-collide tiles + link detection blocks)",
+            "avg_kernel_us": round(avg_ms * 1e3, 3),
             "bytes_per_lup": bpl, "cells_per_launch": cells}
     cpu = None if args.no_cpu else cpu_baseline_config2(state, args.cpu_budget)
     out = {
@@ -289,7 +296,8 @@ def main():
         "config": {"workload": "2D FSI plain FP32: D2Q9 1200x400, 600-segment elliptical swimmer with seeded "
                                "travelling wave, outlets on all sides, per-step FP64 wrench readback",
                    "nx": c["nx"], "ny": c["ny"], "dx": c["dx"], "dt": c["dt"], "viscosity": c["viscosity"],
-                   "segments": c["n_seg"], "storage": "f32", "stage_fp16": True, "parallelism": "single-gpu"},
+                   "segments": c["n_seg"], "storage": "f32", "stage_fp16": True, "tile": TILES2[tile],
+                   "parallelism": "single-gpu"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "step_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),

@@ -10,7 +10,9 @@ import os
 import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "lib", "libfreeflow_lbm.so")
+# FREEFLOW_LBM_LIB selects another build of the same library (e.g. an
+# installed copy); default: the in-tree build.
+LIB_PATH = os.environ.get("FREEFLOW_LBM_LIB") or os.path.join(PKG, "lib", "libfreeflow_lbm.so")
 
 FF_OK = 0
 STORAGE = {"f32": 0, "f16": 1, "bf16": 2}

@@ -74,8 +74,7 @@ __global__ void __launch_bounds__(256) kb_bulk2d(const DomDev* __restrict__ doms
   const int2 m = map[blockIdx.x];
   const DomDev* D = doms + m.x;
   if ((int)blockIdx.x < nbuild) {
-    const LinkArgs la = D->la;
-    build_links_block<2>(la, (long long)m.y, la.st->gen);
+    build_links_block<2>(D->la, (long long)m.y, D->la.st->gen);
     return;
   }
   const unsigned p = *D->step & 1u;
@@ -90,8 +89,7 @@ __global__ void __launch_bounds__(RING ? TX * TY : 256) kb_bulk3d(const DomDev* 
   const int2 m = map[blockIdx.x];
   const DomDev* D = doms + m.x;
   if ((int)blockIdx.x < nbuild) {
-    const LinkArgs la = D->la;
-    build_links_block<3>(la, (long long)m.y, la.st->gen);
+    build_links_block<3>(D->la, (long long)m.y, D->la.st->gen);
     return;
   }
   const unsigned p = *D->step & 1u;
@@ -109,32 +107,26 @@ template <int DIM, typename S, bool STAGE, bool PERIODIC>
 __global__ void __launch_bounds__(256) kb_bcells(const DomDev* __restrict__ doms, const int2* __restrict__ map) {
   const int2 m = map[blockIdx.x];
   const DomDev* D = doms + m.x;
-  const LinkArgs la = D->la;
   const unsigned p = *D->step & 1u;
-  bcells_body<DIM, S, STAGE, PERIODIC>((const S*)D->mom[p], (S*)D->mom[p ^ 1u], la, m.y, D->nbc);
+  bcells_body<DIM, S, STAGE, PERIODIC>((const S*)D->mom[p], (S*)D->mom[p ^ 1u], D->la, m.y, D->nbc);
 }
 
 template <int DIM, typename S, int FB>
 __global__ void __launch_bounds__(FB) kb_elem_sums(const DomDev* __restrict__ doms, const int2* __restrict__ map) {
   const int2 m = map[blockIdx.x];
   const DomDev* D = doms + m.x;
-  const LinkArgs la = D->la;
-  FinalArgs fa = D->fa;
-  fa.buf = D->mom[(*D->step & 1u) ^ 1u];
-  elem_sums_body<DIM, S, FB>(la, fa, m.y);
+  elem_sums_body<DIM, S, FB>(D->la, D->fa, m.y, D->mom[(*D->step & 1u) ^ 1u]);
 }
 
 template <int DIM, int FB>
 __global__ void __launch_bounds__(FB) kb_finish(const DomDev* __restrict__ doms, const int2* __restrict__ map) {
   const int2 m = map[blockIdx.x];
   const DomDev* D = doms + m.x;
-  const LinkArgs la = D->la;
-  const FinalArgs fa = D->fa;
   if (m.y < D->nvb) {
-    finish_vertex_body<DIM, FB>(la, fa, m.y);
+    finish_vertex_body<DIM, FB>(D->la, D->fa, m.y);
     return;
   }
-  if (D->boundary) finish_last_body<DIM, FB>(la, fa);
+  if (D->boundary) finish_last_body<DIM, FB>(D->la, D->fa);
   if (threadIdx.x == 0) *D->step = *D->step + 1u;
 }
 

@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(256) k_bcells(const S* __restrict__ in, S* __r
 
 template <int DIM, typename S, int FB>
 __global__ void __launch_bounds__(FB) k_elem_sums(LinkArgs la, FinalArgs fa) {
-  elem_sums_body<DIM, S, FB>(la, fa, blockIdx.x);
+  elem_sums_body<DIM, S, FB>(la, fa, blockIdx.x, fa.buf);
 }
 
 // Vertex blocks, then one last block for the body wrenches and the next

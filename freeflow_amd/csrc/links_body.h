@@ -11,6 +11,11 @@
 namespace ff {
 
 // ---------------------------------------------------------------- boundary cells
+// First barycentric of a (slot, direction) whose link was not applied this
+// step.  Applied 3D links can carry a slightly negative barycentric (the
+// triangle test accepts u + v <= 1 + 1e-6), so the marker is far outside.
+constexpr float kLinkUnused = -1.0e30f;
+
 // Full update of every cell that owns a keyed link, one lane per lattice
 // direction: the per-direction logic of mrDeformableSolver2DKernel_Shrd
 // (FsiCoupling2D.cu:154-224) / mrDeformableSolver3DKernel_ShrdM
@@ -91,7 +96,7 @@ __device__ __forceinline__ void bcells_body(const S* __restrict__ in, S* __restr
         pop = fmaxf(0.0f, fc[inv]);
         done = true;
       }
-      if (dir > 0) la.lb[li * DIM] = -1.0f;
+      if (dir > 0) la.lb[li * DIM] = kLinkUnused;
       if (!done && dir > 0 && notCross && (DIM == 3 || !is_wall2(fup)) && key != 0xFFFFFFFFu) {
         const int elem = (int)(key & 0xFFFFu);
         Hit h;
@@ -229,7 +234,7 @@ __device__ __forceinline__ void wave_vertex(const float* rest, const float* s, c
 // (mrApplyOutlet2DKernel / mrApplyOutlet3DKernel); the last ones evaluate the
 // next step's wave kinematics.
 template <int DIM, typename S, int FB>
-__device__ __forceinline__ void elem_sums_body(const LinkArgs& la, const FinalArgs& fa, int blk) {
+__device__ __forceinline__ void elem_sums_body(const LinkArgs& la, const FinalArgs& fa, int blk, void* out) {
   constexpr int K = DIM == 2 ? 8 : 26;
   constexpr int NT = DIM == 2 ? 1 : 3;
   constexpr int STRIDE = DIM * DIM + DIM + NT;
@@ -246,7 +251,7 @@ __device__ __forceinline__ void elem_sums_body(const LinkArgs& la, const FinalAr
   if (blk >= la.ne) {
     const int t = (blk - la.ne) * FB + threadIdx.x;
     if (t >= fa.n_outlets) return;
-    S* buf = (S*)fa.buf;
+    S* buf = (S*)out;
     const OutletCell c = fa.outlets[t];
     if (la.flags[c.nb] != kFluid) return;
     const float up = Store<S>::template load<DIM>(buf + c.comp * la.plane + c.nb, c.comp);
@@ -289,7 +294,7 @@ __device__ __forceinline__ void elem_sums_body(const LinkArgs& la, const FinalAr
       const size_t li = (size_t)slot * K + l;
       const uint32_t key = lt_key(la.keys, li, gen);
       if (key == 0xFFFFFFFFu || (key & 0xFFFFu) != tag) continue;
-      if (la.lb[li * DIM] < 0.0f) continue;
+      if (la.lb[li * DIM] == kLinkUnused) continue;
       float F[DIM];
 #pragma unroll
       for (int c = 0; c < DIM; c++) F[c] = la.lf[li * DIM + c];

@@ -12,7 +12,7 @@ def write_mesh(path, dim, verts, edges=(), tris=(), tets=()):
     with open(path, "w") as f:
         f.write(f"MeshVersionFormatted 1\nDimension {dim}\n\nVertices\n{len(verts)}\n")
         for v in verts:
-            f.write(" ".join(f"{x:.15f}" for x in v) + " 0\n")
+            f.write(" ".join(repr(float(x)) for x in v) + " 0\n")  # exact round trip
         for name, block in (("Edges", edges), ("Triangles", tris), ("Tetrahedra", tets)):
             if len(block):
                 f.write(f"\n{name}\n{len(block)}\n")
