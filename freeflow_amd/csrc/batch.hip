@@ -1,17 +1,17 @@
 // batch.hip -- many fluid domains of uneven size stepped together.
 //
 // A batch holds domains of one kind (dimension, storage format, options).
-// One step of every member is one replay of a HIP graph with four kernel
+// One step of every member is one replay of a HIP graph with three kernel
 // nodes, the batched forms of the per-domain step (kernels2d/3d.hip,
 // links.hip):
 //
-//   kb_bulk      link detection of every domain + the stream-collide tiles
-//                of every domain (horizontally fused, one grid)
-//   kb_bcells    cells owning a moving-wall link, every domain
-//   kb_elem_sums per-element FP64 force sums, outlet cells, next-step wave
-//                kinematics, every domain
-//   kb_finish    per-vertex forces; per domain one block for the body wrench,
-//                the link-table generation and the domain's step counter
+//   kb_step      boundary cells (into side buffers), next-step wave
+//                kinematics and the stream-collide tiles of every domain
+//   kb_sums      per-element FP64 force sums, outlet cells and the
+//                boundary-cell copy-back of every domain
+//   kb_finish    per-vertex forces and the next step's link detection; per
+//                domain one block for the body wrench, the link-table tag
+//                and the domain's step counter
 //
 // Each block finds its domain and its block index within that domain in a
 // block map (int2 {domain, local block}) built on the host; per-domain
@@ -45,6 +45,7 @@ struct DomDev {
   int tiles_x, tiles_y;
   int nbc;           // boundary-cell blocks of this domain
   int nvb;           // vertex blocks of the finish phase
+  int ndet;          // next-step detection blocks of the finish phase
   int boundary;
 };
 
@@ -62,62 +63,64 @@ struct Batch {
   size_t map_cap = 0;
   unsigned* d_steps = nullptr;
   size_t steps_cap = 0;
-  int n_build = 0, n_bulk = 0, n_bc = 0, n_es = 0, n_fin = 0;
+  int n_bc = 0, n_wave = 0, n_bulk = 0, n_es = 0, n_fin = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
 
 // ---------------------------------------------------------------- kernels
+// Stream launch: [boundary-cell blocks of every domain][wave blocks][tiles].
 template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT>
-__global__ void __launch_bounds__(256) kb_bulk2d(const DomDev* __restrict__ doms, const int2* __restrict__ map,
-                                                 int nbuild) {
+__global__ void __launch_bounds__(256) kb_step2d(const DomDev* __restrict__ doms, const int2* __restrict__ map,
+                                                 int n_bc, int n_wave) {
   const int2 m = map[blockIdx.x];
   const DomDev* D = doms + m.x;
-  if ((int)blockIdx.x < nbuild) {
-    build_links_block<2>(D->la, (long long)m.y, D->la.st->gen);
-    return;
-  }
   const unsigned p = *D->step & 1u;
-  bulk2d_tile<S, STAGE, PERIODIC, TX, TY, CPT>((const S*)D->mom[p], (S*)D->mom[p ^ 1u], D->la.flags, D->la.nx,
-                                               D->la.ny, D->la.plane, D->la.omega, m.y % D->tiles_x, m.y / D->tiles_x);
+  const int b = blockIdx.x;
+  if (b < n_bc) {
+    bcells_body<2, S, STAGE, PERIODIC, 256>((const S*)D->mom[p], D->la, m.y, D->nbc);
+  } else if (b < n_bc + n_wave) {
+    wave_block_body<2, 256>(D->la, D->fa, m.y);
+  } else {
+    bulk2d_tile<S, STAGE, PERIODIC, TX, TY, CPT>((const S*)D->mom[p], (S*)D->mom[p ^ 1u], D->la.flags, D->la.nx,
+                                                 D->la.ny, D->la.plane, D->la.omega, m.y % D->tiles_x,
+                                                 m.y / D->tiles_x);
+  }
 }
 
-// RING = 0: 8x8x4 box tiles; otherwise TX x TY columns marching ZC planes.
-template <typename S, bool PERIODIC, int RING, int TX, int TY, int ZC>
-__global__ void __launch_bounds__(RING ? TX * TY : 256) kb_bulk3d(const DomDev* __restrict__ doms,
-                                                                  const int2* __restrict__ map, int nbuild) {
+// RING = 0: 8x8x4 box tiles; otherwise TX x TY columns marching ZC planes; NT = block size.
+template <typename S, bool PERIODIC, int RING, int TX, int TY, int ZC, int NT>
+__global__ void __launch_bounds__(NT) kb_step3d(const DomDev* __restrict__ doms, const int2* __restrict__ map,
+                                                int n_bc, int n_wave) {
   const int2 m = map[blockIdx.x];
   const DomDev* D = doms + m.x;
-  if ((int)blockIdx.x < nbuild) {
-    build_links_block<3>(D->la, (long long)m.y, D->la.st->gen);
-    return;
+  const unsigned p = *D->step & 1u;
+  const int b = blockIdx.x;
+  if (b < n_bc) {
+    bcells_body<3, S, false, PERIODIC, NT>((const S*)D->mom[p], D->la, m.y, D->nbc);
+  } else if (b < n_bc + n_wave) {
+    wave_block_body<3, NT>(D->la, D->fa, m.y);
+  } else {
+    const int tx = D->tiles_x, ty = D->tiles_y;
+    if constexpr (RING)
+      bulk3d_ring<S, PERIODIC, TX, TY, ZC>((const S*)D->mom[p], (S*)D->mom[p ^ 1u], D->la.flags, D->la.nx, D->la.ny,
+                                           D->la.nz, D->la.plane, D->la.omega, m.y % tx, (m.y / tx) % ty,
+                                           m.y / (tx * ty));
+    else
+      bulk3d_tile<S, PERIODIC>((const S*)D->mom[p], (S*)D->mom[p ^ 1u], D->la.flags, D->la.nx, D->la.ny, D->la.nz,
+                               D->la.plane, D->la.omega, m.y % tx, (m.y / tx) % ty, m.y / (tx * ty));
   }
-  const unsigned p = *D->step & 1u;
-  const int tx = D->tiles_x, ty = D->tiles_y;
-  if constexpr (RING)
-    bulk3d_ring<S, PERIODIC, TX, TY, ZC>((const S*)D->mom[p], (S*)D->mom[p ^ 1u], D->la.flags, D->la.nx, D->la.ny,
-                                         D->la.nz, D->la.plane, D->la.omega, m.y % tx, (m.y / tx) % ty,
-                                         m.y / (tx * ty));
-  else
-    bulk3d_tile<S, PERIODIC>((const S*)D->mom[p], (S*)D->mom[p ^ 1u], D->la.flags, D->la.nx, D->la.ny, D->la.nz,
-                             D->la.plane, D->la.omega, m.y % tx, (m.y / tx) % ty, m.y / (tx * ty));
-}
-
-template <int DIM, typename S, bool STAGE, bool PERIODIC>
-__global__ void __launch_bounds__(256) kb_bcells(const DomDev* __restrict__ doms, const int2* __restrict__ map) {
-  const int2 m = map[blockIdx.x];
-  const DomDev* D = doms + m.x;
-  const unsigned p = *D->step & 1u;
-  bcells_body<DIM, S, STAGE, PERIODIC>((const S*)D->mom[p], (S*)D->mom[p ^ 1u], D->la, m.y, D->nbc);
 }
 
 template <int DIM, typename S, int FB>
-__global__ void __launch_bounds__(FB) kb_elem_sums(const DomDev* __restrict__ doms, const int2* __restrict__ map) {
+__global__ void __launch_bounds__(FB) kb_sums(const DomDev* __restrict__ doms, const int2* __restrict__ map) {
   const int2 m = map[blockIdx.x];
   const DomDev* D = doms + m.x;
   elem_sums_body<DIM, S, FB>(D->la, D->fa, m.y, D->mom[(*D->step & 1u) ^ 1u]);
 }
 
+// Per domain: vertex blocks, detection blocks of the next step, then one
+// block that folds the wrench, advances gen/tag and the domain's step counter.
 template <int DIM, int FB>
 __global__ void __launch_bounds__(FB) kb_finish(const DomDev* __restrict__ doms, const int2* __restrict__ map) {
   const int2 m = map[blockIdx.x];
@@ -126,78 +129,81 @@ __global__ void __launch_bounds__(FB) kb_finish(const DomDev* __restrict__ doms,
     finish_vertex_body<DIM, FB>(D->la, D->fa, m.y);
     return;
   }
+  if (m.y < D->nvb + D->ndet) {
+    detect_next_body<DIM>(D->la, m.y - D->nvb);
+    return;
+  }
   if (D->boundary) finish_last_body<DIM, FB>(D->la, D->fa);
   if (threadIdx.x == 0) *D->step = *D->step + 1u;
 }
 
 // ---------------------------------------------------------------- launch helpers
+// Block size of the stream launch for a tile index.
+static int stream_threads(int dim, int tile) {
+  if (dim == 2) return 256;
+  const Tile3 t = kTiles3[tile];
+  return t.ring ? t.tx * t.ty : 256;
+}
+
 template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT>
-static void bulk2d_batch_shape(Batch& b, const int2* map, int nblocks, int nbuild) {
-  kb_bulk2d<S, STAGE, PERIODIC, TX, TY, CPT><<<nblocks, 256, 0, b.stream>>>(b.d_doms, map, nbuild);
+static void step2d_batch_shape(Batch& b, const int2* map, int nblocks, int n_bc, int n_wave) {
+  kb_step2d<S, STAGE, PERIODIC, TX, TY, CPT><<<nblocks, 256, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave);
 }
 
 template <typename S, bool STAGE, bool PERIODIC>
-static void bulk2d_batch_variant(Batch& b, int tile, const int2* map, int nblocks, int nbuild) {
+static void step2d_batch_variant(Batch& b, int tile, const int2* map, int nblocks, int n_bc, int n_wave) {
   switch (tile) {
-    case 1: bulk2d_batch_shape<S, STAGE, PERIODIC, 32, 8, 2>(b, map, nblocks, nbuild); break;
-    case 2: bulk2d_batch_shape<S, STAGE, PERIODIC, 64, 4, 2>(b, map, nblocks, nbuild); break;
-    case 3: bulk2d_batch_shape<S, STAGE, PERIODIC, 32, 8, 4>(b, map, nblocks, nbuild); break;
-    case 4: bulk2d_batch_shape<S, STAGE, PERIODIC, 64, 4, 4>(b, map, nblocks, nbuild); break;
-    default: bulk2d_batch_shape<S, STAGE, PERIODIC, 32, 8, 1>(b, map, nblocks, nbuild); break;
+    case 1: step2d_batch_shape<S, STAGE, PERIODIC, 32, 8, 2>(b, map, nblocks, n_bc, n_wave); break;
+    case 2: step2d_batch_shape<S, STAGE, PERIODIC, 64, 4, 2>(b, map, nblocks, n_bc, n_wave); break;
+    case 3: step2d_batch_shape<S, STAGE, PERIODIC, 32, 8, 4>(b, map, nblocks, n_bc, n_wave); break;
+    case 4: step2d_batch_shape<S, STAGE, PERIODIC, 64, 4, 4>(b, map, nblocks, n_bc, n_wave); break;
+    default: step2d_batch_shape<S, STAGE, PERIODIC, 32, 8, 1>(b, map, nblocks, n_bc, n_wave); break;
   }
 }
 
 template <typename S, bool PERIODIC>
-static void bulk3d_batch_variant(Batch& b, int tile, const int2* map, int nblocks, int nbuild) {
-  // the link-detection blocks of a ring launch run with the ring's block size
-  // (build_links_block covers blockDim.x / 16 pairs), so their count depends on it
+static void step3d_batch_variant(Batch& b, int tile, const int2* map, int nblocks, int n_bc, int n_wave) {
   switch (tile) {
-    case 1: kb_bulk3d<S, PERIODIC, 1, 16, 8, 32><<<nblocks, 128, 0, b.stream>>>(b.d_doms, map, nbuild); break;
-    case 2: kb_bulk3d<S, PERIODIC, 1, 8, 8, 32><<<nblocks, 64, 0, b.stream>>>(b.d_doms, map, nbuild); break;
-    case 3: kb_bulk3d<S, PERIODIC, 1, 16, 16, 32><<<nblocks, 256, 0, b.stream>>>(b.d_doms, map, nbuild); break;
-    default: kb_bulk3d<S, PERIODIC, 0, 8, 8, 4><<<nblocks, 256, 0, b.stream>>>(b.d_doms, map, nbuild); break;
+    case 1: kb_step3d<S, PERIODIC, 1, 16, 8, 32, 128><<<nblocks, 128, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
+    case 2: kb_step3d<S, PERIODIC, 1, 8, 8, 32, 64><<<nblocks, 64, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
+    case 3: kb_step3d<S, PERIODIC, 1, 16, 16, 32, 256><<<nblocks, 256, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
+    default: kb_step3d<S, PERIODIC, 0, 8, 8, 4, 256><<<nblocks, 256, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
   }
 }
 
+// The stream launch over `map` (n_bc boundary-cell blocks, n_wave wave blocks, then tiles).
 template <typename S>
-static void launch_bulk_batch(Batch& b, int tile, const int2* map, int nblocks, int nbuild) {
+static void launch_stream_batch(Batch& b, int tile, const int2* map, int nblocks, int n_bc, int n_wave) {
   if (nblocks <= 0) return;
   const bool stage = b.options & FF_OPT_STAGE_FP16, per = b.options & FF_OPT_PERIODIC;
   if (b.dim == 2) {
-    if (stage && per) bulk2d_batch_variant<S, true, true>(b, tile, map, nblocks, nbuild);
-    else if (stage) bulk2d_batch_variant<S, true, false>(b, tile, map, nblocks, nbuild);
-    else if (per) bulk2d_batch_variant<S, false, true>(b, tile, map, nblocks, nbuild);
-    else bulk2d_batch_variant<S, false, false>(b, tile, map, nblocks, nbuild);
+    if (stage && per) step2d_batch_variant<S, true, true>(b, tile, map, nblocks, n_bc, n_wave);
+    else if (stage) step2d_batch_variant<S, true, false>(b, tile, map, nblocks, n_bc, n_wave);
+    else if (per) step2d_batch_variant<S, false, true>(b, tile, map, nblocks, n_bc, n_wave);
+    else step2d_batch_variant<S, false, false>(b, tile, map, nblocks, n_bc, n_wave);
   } else if (per) {
-    bulk3d_batch_variant<S, true>(b, tile, map, nblocks, nbuild);
+    step3d_batch_variant<S, true>(b, tile, map, nblocks, n_bc, n_wave);
   } else {
-    bulk3d_batch_variant<S, false>(b, tile, map, nblocks, nbuild);
+    step3d_batch_variant<S, false>(b, tile, map, nblocks, n_bc, n_wave);
   }
 }
 
 template <int DIM, typename S>
-static void launch_phases(Batch& b, const int2* map_bc, const int2* map_es, const int2* map_fin) {
-  const bool stage = DIM == 2 && (b.options & FF_OPT_STAGE_FP16), per = b.options & FF_OPT_PERIODIC;
-  if (b.n_bc > 0) {
-    if (stage && per) kb_bcells<DIM, S, true, true><<<b.n_bc, 256, 0, b.stream>>>(b.d_doms, map_bc);
-    else if (stage) kb_bcells<DIM, S, true, false><<<b.n_bc, 256, 0, b.stream>>>(b.d_doms, map_bc);
-    else if (per) kb_bcells<DIM, S, false, true><<<b.n_bc, 256, 0, b.stream>>>(b.d_doms, map_bc);
-    else kb_bcells<DIM, S, false, false><<<b.n_bc, 256, 0, b.stream>>>(b.d_doms, map_bc);
-  }
+static void launch_tail(Batch& b, const int2* map_sums, const int2* map_fin) {
   constexpr int FB = DIM == 2 ? kElemBlock2 : kElemBlock3;
-  if (b.n_es > 0) kb_elem_sums<DIM, S, FB><<<b.n_es, FB, 0, b.stream>>>(b.d_doms, map_es);
+  if (b.n_es > 0) kb_sums<DIM, S, FB><<<b.n_es, FB, 0, b.stream>>>(b.d_doms, map_sums);
   if (b.n_fin > 0) kb_finish<DIM, 256><<<b.n_fin, 256, 0, b.stream>>>(b.d_doms, map_fin);
 }
 
 template <typename S>
 static void launch_step_typed(Batch& b, int tile) {
-  const int2* map_bulk = b.d_map;
-  const int2* map_bc = map_bulk + b.n_build + b.n_bulk;
-  const int2* map_es = map_bc + b.n_bc;
-  const int2* map_fin = map_es + b.n_es;
-  launch_bulk_batch<S>(b, tile, map_bulk, b.n_build + b.n_bulk, b.n_build);
-  if (b.dim == 2) launch_phases<2, S>(b, map_bc, map_es, map_fin);
-  else launch_phases<3, S>(b, map_bc, map_es, map_fin);
+  const int n_stream = b.n_bc + b.n_wave + b.n_bulk;
+  const int2* map_stream = b.d_map;
+  const int2* map_sums = map_stream + n_stream;
+  const int2* map_fin = map_sums + b.n_es;
+  launch_stream_batch<S>(b, tile, map_stream, n_stream, b.n_bc, b.n_wave);
+  if (b.dim == 2) launch_tail<2, S>(b, map_sums, map_fin);
+  else launch_tail<3, S>(b, map_sums, map_fin);
 }
 
 static void launch_step(Batch& b, int tile) {
@@ -206,10 +212,11 @@ static void launch_step(Batch& b, int tile) {
   else launch_step_typed<__hip_bfloat16>(b, tile);
 }
 
+// Tiles only (the map's tile range), for timing.
 static void launch_bulk_only(Batch& b, int tile, const int2* map, int nblocks) {
-  if (b.storage == FF_STORAGE_F32) launch_bulk_batch<float>(b, tile, map, nblocks, 0);
-  else if (b.storage == FF_STORAGE_F16) launch_bulk_batch<__half>(b, tile, map, nblocks, 0);
-  else launch_bulk_batch<__hip_bfloat16>(b, tile, map, nblocks, 0);
+  if (b.storage == FF_STORAGE_F32) launch_stream_batch<float>(b, tile, map, nblocks, 0, 0);
+  else if (b.storage == FF_STORAGE_F16) launch_stream_batch<__half>(b, tile, map, nblocks, 0, 0);
+  else launch_stream_batch<__hip_bfloat16>(b, tile, map, nblocks, 0, 0);
 }
 
 // ---------------------------------------------------------------- host side
@@ -241,13 +248,16 @@ static void tiles_of(const Batch& b, const Domain& d, int tile, int* tx, int* ty
 }
 
 // Records and block maps for the current members and `tile`; uploads them.
+// Map layout: stream launch [boundary cells][wave][tiles], sums launch,
+// finish launch.
 static int build_tables(Batch& b, int tile) {
   const int n = (int)b.mem.size();
   std::vector<DomDev> doms(n);
-  std::vector<int2> build, bulk, bc, es, fin;
+  std::vector<int2> bc, wave, bulk, es, fin;
   const int K = b.dim == 2 ? 8 : 26;
   const int fb = b.dim == 2 ? kElemBlock2 : kElemBlock3;
-  const int cpb = b.dim == 2 ? 16 : 8;  // boundary cells per block in bcells_body
+  const int nt = stream_threads(b.dim, tile);
+  const int cpb = nt / (b.dim == 2 ? 16 : 32);  // boundary cells per block in bcells_body
   int rc = grow(&b.d_steps, &b.steps_cap, (size_t)std::max(n, 1));
   if (rc) return rc;
   for (int i = 0; i < n; i++) {
@@ -261,33 +271,37 @@ static int build_tables(Batch& b, int tile) {
     r.mom[1] = d.mom[d.cur ^ 1];
     r.step = b.d_steps + i;
     r.boundary = bnd ? 1 : 0;
-    int tx, ty, nt;
-    tiles_of(b, d, tile, &tx, &ty, &nt);
+    int tx, ty, ntl;
+    tiles_of(b, d, tile, &tx, &ty, &ntl);
     r.tiles_x = tx;
     r.tiles_y = ty;
-    for (int t = 0; t < nt; t++) bulk.push_back(make_int2(i, t));
+    for (int t = 0; t < ntl; t++) bulk.push_back(make_int2(i, t));
     if (bnd) {
-      const int per_block = (b.dim == 3 && kTiles3[tile].ring ? kTiles3[tile].tx * kTiles3[tile].ty : 256) / 16;
-      const int nbuild = (int)(((long long)d.ne * K + per_block - 1) / per_block);
-      for (int t = 0; t < nbuild; t++) build.push_back(make_int2(i, t));
-      r.nbc = (int)std::min<unsigned long long>(512, std::max<unsigned long long>(1, (d.lt_est / 4 + cpb - 1) / cpb));
+      r.nbc = boundary_cell_blocks(d, cpb);
       for (int t = 0; t < r.nbc; t++) bc.push_back(make_int2(i, t));
+      if (d.wave) {
+        const int nw = (d.nv + nt - 1) / nt;
+        for (int t = 0; t < nw; t++) wave.push_back(make_int2(i, t));
+      }
       r.nvb = (d.nv + 255) / 256;
+      r.ndet = (int)(((long long)d.ne * K + 15) / 16);
+    } else {
+      r.la.ne = 0;
+      r.fa.copy_blocks = 0;
     }
-    const int wave_blocks = (bnd && d.wave) ? (d.nv + fb - 1) / fb : 0;
-    const int nes = (bnd ? d.ne : 0) + r.fa.outlet_blocks + wave_blocks;
-    if (!bnd) r.la.ne = 0;
+    const int nes = (bnd ? d.ne : 0) + r.fa.outlet_blocks + r.fa.copy_blocks;
     for (int t = 0; t < nes; t++) es.push_back(make_int2(i, t));
-    for (int t = 0; t <= r.nvb; t++) fin.push_back(make_int2(i, t));
+    for (int t = 0; t <= r.nvb + r.ndet; t++) fin.push_back(make_int2(i, t));
   }
-  b.n_build = (int)build.size();
-  b.n_bulk = (int)bulk.size();
   b.n_bc = (int)bc.size();
+  b.n_wave = (int)wave.size();
+  b.n_bulk = (int)bulk.size();
   b.n_es = (int)es.size();
   b.n_fin = (int)fin.size();
+  (void)fb;
   std::vector<int2> map;
-  map.reserve(build.size() + bulk.size() + bc.size() + es.size() + fin.size());
-  for (auto* v : {&build, &bulk, &bc, &es, &fin}) map.insert(map.end(), v->begin(), v->end());
+  map.reserve(bc.size() + wave.size() + bulk.size() + es.size() + fin.size());
+  for (auto* v : {&bc, &wave, &bulk, &es, &fin}) map.insert(map.end(), v->begin(), v->end());
   if ((rc = grow(&b.d_doms, &b.doms_cap, (size_t)std::max(n, 1)))) return rc;
   if ((rc = grow(&b.d_map, &b.map_cap, std::max<size_t>(map.size(), 1)))) return rc;
   if (n) FF_HIP(hipMemcpyAsync(b.d_doms, doms.data(), n * sizeof(DomDev), hipMemcpyHostToDevice, b.stream));
@@ -314,7 +328,7 @@ static int autotune(Batch& b) {
   int rc = FF_OK;
   for (int t = 0; t < ntiles && !rc; t++) {
     if ((rc = build_tables(b, t))) break;
-    const int2* map = b.d_map + b.n_build;
+    const int2* map = b.d_map + b.n_bc + b.n_wave;
     for (int w = 0; w < 3; w++) launch_bulk_only(b, t, map, b.n_bulk);
     constexpr int iters = 10;
     FF_HIP(hipEventRecord(e0, b.stream));
@@ -386,6 +400,21 @@ static int detach(Batch& b, Domain& d) {
 
 int batch_detach(Domain& d) { return d.batch ? detach(*d.batch, d) : FF_OK; }
 
+// Members whose links were not detected ahead (new boundary, or a boundary
+// moved by the host since the last step) detect them before the replay.
+static int detect_pending(Batch& b) {
+  for (Domain* dp : b.mem) {
+    Domain& d = *dp;
+    if (!(d.has_boundary && d.ne > 0) || d.predetected) continue;
+    const LinkArgs la = link_args(d, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], d.elems);
+    int rc = launch_detect(d, la, d.stale);
+    if (rc) return rc;
+    d.stale = false;
+    d.predetected = true;
+  }
+  return FF_OK;
+}
+
 // Host mirrors of the device-side step state after `nsteps` replays.
 static void advance_mirrors(Batch& b, int nsteps) {
   for (Domain* dp : b.mem) {
@@ -394,6 +423,7 @@ static void advance_mirrors(Batch& b, int nsteps) {
     if (d.has_boundary && d.ne > 0) {
       d.gen_host += (unsigned)nsteps;
       if (d.hist_cap > 0) d.hist_count += nsteps;
+      d.predetected = true;
     }
   }
 }
@@ -485,6 +515,7 @@ int ff_batch_step(ff_batch* b, int nsteps) {
   FF_HIP(hipSetDevice(b->device));
   int rc;
   if (b->dirty && (rc = rebuild(*b))) return rc;
+  if ((rc = detect_pending(*b))) return rc;
   for (int s = 0; s < nsteps; s++) FF_HIP(hipGraphLaunch(b->exec, b->stream));
   advance_mirrors(*b, nsteps);
   return FF_OK;
@@ -534,6 +565,7 @@ int ff_batch_time_steps(ff_batch* b, int nsteps, double* total_ms) {
   hipEvent_t e0, e1;
   FF_HIP(hipEventCreate(&e0));
   FF_HIP(hipEventCreate(&e1));
+  if (!b->mem.empty() && (rc = detect_pending(*b))) return rc;
   FF_HIP(hipEventRecord(e0, b->stream));
   if (!b->mem.empty())
     for (int s = 0; s < nsteps; s++) FF_HIP(hipGraphLaunch(b->exec, b->stream));
@@ -558,7 +590,7 @@ int ff_batch_time_bulk(ff_batch* b, int iters, double* avg_ms) {
   // the stream-collide tiles of every member (the link-detection blocks of a
   // step are excluded); they rewrite only cells every step rewrites
   const int tile = current_tile(*b);
-  const int2* map = b->d_map + b->n_build;
+  const int2* map = b->d_map + b->n_bc + b->n_wave;
   std::vector<hipEvent_t> ev(2 * iters);
   for (auto& e : ev) FF_HIP(hipEventCreate(&e));
   for (int k = 0; k < iters; k++) {

@@ -46,13 +46,15 @@ static void free_boundary(Domain& d) {
   dfree(d.elems); dfree(d.elem_body); dfree(d.vforce);
   dfree(d.body_elem_ptr); dfree(d.body_elem_idx); dfree(d.vert_inc_ptr); dfree(d.vert_inc_idx);
   dfree(d.body_ref); dfree(d.elem_acc); dfree(d.wrench); dfree(d.wrench_hist);
-  dfree(d.lt_cells); dfree(d.lt_keys); dfree(d.lk_f); dfree(d.lk_b); dfree(d.lk_h); dfree(d.lt_list);
+  dfree(d.lt_cells); dfree(d.lt_keys); dfree(d.lk_f); dfree(d.lk_b); dfree(d.lk_h); dfree(d.lt_list); dfree(d.lk_sb);
   dfree(d.st); dfree(d.rest_verts); dfree(d.s_coord); dfree(d.wave_params); dfree(d.vert_body);
   d.nv = d.ne = d.n_bodies = 0;
   d.gen_host = 1;
   d.has_boundary = false;
   d.wave = false;
   d.hist_cap = 0;
+  d.predetected = false;
+  d.stale = false;
 }
 
 static int elem_stride(int dim) { return dim == 2 ? 7 : 15; }
@@ -229,6 +231,7 @@ LinkArgs link_args(Domain& d, const float* v0, const float* v1, const float* u0,
   la.lb = d.lk_b;
   la.lh = d.lk_h;
   la.list = d.lt_list;
+  la.sb = d.lk_sb;
   la.cap = d.lt_cap;
   la.st = d.st;
   la.flags = d.flags;
@@ -243,9 +246,18 @@ LinkArgs link_args(Domain& d, const float* v0, const float* v1, const float* u0,
   return la;
 }
 
-// One step: [stream-collide (+ link detection)] -> [boundary cells] ->
-// [finalize: forces, outlet, next-step kinematics].  Without a boundary:
-// [stream-collide] -> [outlet].  `pv` != null selects caller-owned proxy arrays.
+// The links detected ahead by the last step no longer describe the boundary.
+static void links_moved(Domain& d) {
+  if (d.predetected) d.stale = true;
+  d.predetected = false;
+}
+
+// One step.  Without a boundary: [stream-collide] -> [outlet].  With one:
+// [detect (first step or after the boundary moved)] -> [stream: tiles +
+// boundary cells + next wave kinematics] -> [sums: element forces, outlets,
+// boundary-cell copy-back] -> [finish: vertex forces, wrench, next step's
+// links].  `pv` != null selects caller-owned proxy arrays (positions not
+// known ahead: detected every step).
 static int one_step(Domain& d, const float* pv, const float* pu, const uint32_t* pe, float* proxy_forces) {
   void* in = d.mom[d.cur];
   void* out = d.mom[d.cur ^ 1];
@@ -255,20 +267,17 @@ static int one_step(Domain& d, const float* pv, const float* pu, const uint32_t*
     d.cur ^= 1;
     return launch_outlet(d, d.mom[d.cur]);
   }
-  const LinkArgs la = pv ? link_args(d, pv, pv, pu, pu, pe) : link_args(d, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], d.elems);
-  if (d.dim == 2) {
-    if ((rc = launch_bulk2d_links(d, in, out, la))) return rc;
-  } else {
-    if ((rc = launch_links_build(d, la))) return rc;
-    if ((rc = launch_bulk3d(d, in, out))) return rc;
+  const bool proxy = pv != nullptr;
+  const LinkArgs la = proxy ? link_args(d, pv, pv, pu, pu, pe) : link_args(d, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], d.elems);
+  if (proxy) links_moved(d);
+  if (!d.predetected) {
+    if ((rc = launch_detect(d, la, d.stale))) return rc;
+    d.stale = false;
   }
-  if ((rc = launch_boundary_cells(d, in, out, la))) return rc;
-  const bool wave = d.wave && !pv;
-  const bool saved = d.wave;
-  d.wave = wave;
-  rc = launch_finalize(d, la, out, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], proxy_forces);
-  d.wave = saved;
-  if (rc) return rc;
+  if ((rc = launch_step_stream(d, in, out, la, d.wave && !proxy))) return rc;
+  if ((rc = launch_step_sums(d, la, out))) return rc;
+  if ((rc = launch_step_finish(d, la, !proxy, proxy_forces))) return rc;
+  d.predetected = !proxy;
   d.cur ^= 1;
   d.gen_host++;
   if (d.hist_cap > 0) d.hist_count++;
@@ -399,6 +408,7 @@ int ff_set_flags(ff_domain* dd, const uint8_t* flags) {
   std::memcpy(d.h_flags.data(), flags, (size_t)d.n);
   FF_HIP(hipStreamSynchronize(d.stream));
   batch_mark_dirty(d.batch);
+  links_moved(d);
   FF_HIP(hipMemcpy(d.flags, flags, (size_t)d.n, hipMemcpyHostToDevice));
   std::vector<OutletCell> oc;
   build_outlets(d, oc);
@@ -562,13 +572,14 @@ int ff_set_boundary(ff_domain* dd, int nv, const float* verts, const float* vels
   const size_t cap = (size_t)d.lt_cap;
   if ((rc = dalloc(&d.lt_cells, cap)) || (rc = dalloc(&d.lt_keys, cap * K)) || (rc = dalloc(&d.lk_f, cap * K * D)) ||
       (rc = dalloc(&d.lk_b, cap * K * D)) || (rc = dalloc(&d.lk_h, cap * K * D)) ||
-      (rc = dalloc(&d.lt_list, 2 * cap)) || (rc = dalloc(&d.st, 1)))
+      (rc = dalloc(&d.lt_list, 2 * cap)) || (rc = dalloc(&d.lk_sb, cap * d.nm)) || (rc = dalloc(&d.st, 1)))
     return rc;
   // generation 0 marks every entry empty; steps run from generation 1
   FF_HIP(hipMemset(d.lt_cells, 0, cap * sizeof(unsigned long long)));
   FF_HIP(hipMemset(d.lt_keys, 0, cap * K * sizeof(unsigned long long)));
   StepState st0{};
   st0.gen = 1;
+  st0.tag = 1;
   FF_HIP(hipMemcpy(d.st, &st0, sizeof(st0), hipMemcpyHostToDevice));
   d.gen_host = 1;
   // the memsets above run on the null stream; the domain stream is non-blocking
@@ -591,6 +602,7 @@ int ff_update_boundary(ff_domain* dd, const float* verts, const float* vels) {
   FF_HIP(hipStreamSynchronize(d.stream));
   if (d.wave) batch_mark_dirty(d.batch);
   d.wave = false;
+  links_moved(d);
   return FF_OK;
 }
 
@@ -621,6 +633,7 @@ int ff_set_wave(ff_domain* dd, const float* rest, const float* s, const double* 
   FF_HIP(hipMemcpy(d.wave_params, params, (size_t)d.n_bodies * 4 * sizeof(double), hipMemcpyHostToDevice));
   if (!d.wave) batch_mark_dirty(d.batch);
   d.wave = true;
+  links_moved(d);
   // kinematics of the next step to run; later steps are produced by the
   // finalize kernel of the step before
   if ((rc = launch_wave_now(d))) return rc;
@@ -804,14 +817,14 @@ int ff_time_bulk_kernel(ff_domain* dd, int iters, double* avg_ms) {
   for (auto& e : ev) FF_HIP(hipEventCreate(&e));
   void* in = d.mom[d.cur];
   void* out = d.mom[d.cur ^ 1];
-  // the stream-collide launch of a step (in 2D with a boundary it also runs
-  // the link detection, which is idempotent within a generation)
-  const bool links = d.has_boundary && d.ne > 0 && d.dim == 2;
+  // the stream launch of a step (with a boundary it also runs the boundary
+  // cells and the next wave kinematics; all of it is idempotent within a step)
+  const bool links = d.has_boundary && d.ne > 0;
   const LinkArgs la = links ? link_args(d, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], d.elems) : LinkArgs{};
   int rc = FF_OK;
   for (int k = 0; k < iters && !rc; k++) {
     FF_HIP(hipEventRecord(ev[2 * k], d.stream));
-    if (links) rc = launch_bulk2d_links(d, in, out, la);
+    if (links) rc = launch_step_stream(d, in, out, la, d.wave);
     else rc = d.dim == 2 ? launch_bulk2d(d, in, out) : launch_bulk3d(d, in, out);
     FF_HIP(hipEventRecord(ev[2 * k + 1], d.stream));
   }

@@ -46,11 +46,13 @@ struct OutletCell {
 // Step state kept on the device so a step's launches need no host-side
 // arguments that change from step to step (they can be replayed as a graph).
 struct StepState {
-  unsigned gen;      // generation of the current step: steps since the boundary was set + 1
-  int count[2];      // link-table occupancy, by generation parity
-  int overflow;      // sticky: link table full
-  unsigned done;     // unused
-  int pad[3];
+  unsigned gen;       // steps since the boundary was set + 1 (vertex-buffer parity, wrench ring slot)
+  int count[2];       // link-table occupancy (boundary cells), by tag parity
+  int overflow;       // sticky: link table full
+  unsigned tag;       // link-table tag of the current step's links
+  unsigned tag_next;  // tag the next step's links are detected under
+  unsigned vnext;     // vertex-buffer parity of the next step (set with tag_next)
+  int pad;
 };
 
 // Everything the link kernels read, passed by value.
@@ -64,7 +66,8 @@ struct LinkArgs {
   float* lf;                  // cap * (Q-1) * DIM  link force
   float* lb;                  // cap * (Q-1) * DIM  barycentrics, lb[0] < 0: link not applied
   float* lh;                  // cap * (Q-1) * DIM  hit point
-  int* list;                  // 2 * cap occupied slots, by generation parity
+  int* list;                  // 2 * cap occupied slots, by tag parity
+  float* sb;                  // cap * NM  updated moments of each boundary cell (copied back after the step)
   int cap;
   StepState* st;
   const uint8_t* flags;
@@ -120,8 +123,13 @@ struct Domain {
   float* lk_b = nullptr;
   float* lk_h = nullptr;
   int* lt_list = nullptr;
+  float* lk_sb = nullptr;         // boundary-cell results (lt_cap * nm)
   StepState* st = nullptr;
   unsigned gen_host = 1;          // mirror of st->gen
+  // the links of the next step were detected by the previous step's finish
+  // launch; `stale` when the boundary moved since (detect again, new tag)
+  bool predetected = false;
+  bool stale = false;
   // wave kinematics
   bool wave = false;
   float* rest_verts = nullptr;
@@ -150,13 +158,14 @@ LinkArgs link_args(Domain& d, const float* v0, const float* v1, const float* u0,
 
 // launchers (defined in the .hip units)
 int launch_bulk2d(Domain& d, const void* in, void* out);
-int launch_bulk2d_links(Domain& d, const void* in, void* out, const LinkArgs& la);
 int launch_bulk3d(Domain& d, const void* in, void* out);
-int launch_links_build(Domain& d, const LinkArgs& la);
+// boundary step: [detect] -> stream (tiles + boundary cells + next kinematics)
+// -> element sums + outlets + boundary-cell copy-back -> finish (+ next detection)
+int launch_detect(Domain& d, const LinkArgs& la, bool bump);
+int launch_step_stream(Domain& d, const void* in, void* out, const LinkArgs& la, bool wave);
+int launch_step_sums(Domain& d, const LinkArgs& la, void* out);
+int launch_step_finish(Domain& d, const LinkArgs& la, bool detect_next, float* proxy_forces);
 int launch_outlet(Domain& d, void* buf);
-int launch_boundary_cells(Domain& d, const void* in, void* out, const LinkArgs& la);
-int launch_finalize(Domain& d, const LinkArgs& la, void* out, float* v_next0, float* v_next1, float* u_next0,
-                    float* u_next1, float* proxy_forces);
 int launch_wave_now(Domain& d);
 int launch_step_vertex_forces(Domain& d, float* out_dev);
 int launch_convert_in(Domain& d, const float* aos_dev, void* buf);

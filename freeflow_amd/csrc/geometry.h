@@ -196,16 +196,17 @@ namespace ff {
 // pair walk the element's candidate window (the loops of
 // mrSolver2DComputeOccupancyFlagKernel_v3, FsiCoupling2D.cu:44-74, and its 3D
 // twin, FsiCoupling3D.cu:43-76) and offer depthIndexCompact keys for every
-// hit of an updated cell.  `block` indexes the pair groups.
+// hit of an updated cell.  `block` indexes the pair groups; the links are
+// entered under `tag` for the vertex positions `v`.
 template <int DIM>
-__device__ __forceinline__ void build_links_block(const LinkArgs& la, long long block, unsigned gen) {
+__device__ __forceinline__ void build_links_block(const LinkArgs& la, long long block, unsigned tag, const float* v) {
+  const unsigned gen = tag;
   constexpr int K = DIM == 2 ? 8 : 26;
   const int lane = threadIdx.x & 15;
   const long long pair = block * (blockDim.x >> 4) + (threadIdx.x >> 4);
   if (pair >= (long long)la.ne * K) return;
   const int elem = (int)(pair / K);
   const int l = (int)(pair % K) + 1;
-  const float* v = la.v[gen & 1];
   const int n[3] = {la.nx, la.ny, la.nz};
   int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
   elem_window<DIM>(v, la.e, elem, la.cl, n, lo, hi);

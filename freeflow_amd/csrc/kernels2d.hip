@@ -7,7 +7,7 @@
 // moving-wall link are recomputed afterwards by the boundary-cell kernel
 // (links.hip), so the bulk pass stays a pure stencil with one byte of flag
 // traffic per cell.  With a boundary present the same launch also runs the
-// step's link detection in extra workgroups.
+// boundary cells and the next step's wave kinematics in extra workgroups.
 //
 // In 2D a wall-upstream population is overwritten by the ordinary pull in the
 // reference (FsiCoupling2D.cu:168-223: `is_blocked` stays false on the wall
@@ -15,6 +15,7 @@
 
 #include "ff_internal.h"
 #include "geometry.h"
+#include "links_body.h"
 #include "stencil.h"
 
 namespace ff {
@@ -26,75 +27,89 @@ __global__ void __launch_bounds__(256)
   bulk2d_tile<S, STAGE, PERIODIC, TX, TY, CPT>(in, out, flags, nx, ny, plane, omega, blockIdx.x, blockIdx.y);
 }
 
-// Link detection and stream-collide tiles in one launch: the first `nbuild`
-// blocks detect the step's moving-wall links (dispatched first: they carry the
-// longest dependent chains), the rest are tiles.  The two parts are
-// independent (boundary cells are recomputed afterwards).
+// Stream launch of a step with a boundary: the first `nbc` blocks update the
+// cells owning a moving-wall link (into the side buffer), the next `nwave`
+// blocks evaluate the next step's wave kinematics, the rest are the
+// stream-collide tiles (which also write the boundary cells; the copy-back of
+// the sums launch overwrites them).
 template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT>
 __global__ void __launch_bounds__(256)
-    k_bulk2d_links(const S* __restrict__ in, S* __restrict__ out, int tiles_x, int nbuild, LinkArgs la) {
-  if ((int)blockIdx.x < nbuild) {
-    build_links_block<2>(la, (long long)blockIdx.x, la.st->gen);
+    k_step2d(const S* __restrict__ in, S* __restrict__ out, int tiles_x, int nbc, int nwave, LinkArgs la,
+             FinalArgs fa) {
+  const int b = blockIdx.x;
+  if (b < nbc) {
+    bcells_body<2, S, STAGE, PERIODIC, 256>(in, la, b, nbc);
+  } else if (b < nbc + nwave) {
+    wave_block_body<2, 256>(la, fa, b - nbc);
   } else {
-    const int t = (int)blockIdx.x - nbuild;
+    const int t = b - nbc - nwave;
     bulk2d_tile<S, STAGE, PERIODIC, TX, TY, CPT>(in, out, la.flags, la.nx, la.ny, la.plane, la.omega, t % tiles_x,
                                                  t / tiles_x);
   }
 }
 
+// Step launch parameters beyond the tiles (null: plain stream-collide).
+struct StreamExtra {
+  const LinkArgs* la;
+  const FinalArgs* fa;
+  int nbc, nwave;
+};
+
 template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT>
-static void bulk2d_shape(Domain& d, const S* i, S* o, const LinkArgs* la) {
+static void bulk2d_shape(Domain& d, const S* i, S* o, const StreamExtra* x) {
   const int tx = (d.nx + TX - 1) / TX, ty = (d.ny + TY * CPT - 1) / (TY * CPT);
-  if (!la) {
+  if (!x) {
     k_bulk2d<S, STAGE, PERIODIC, TX, TY, CPT><<<dim3(tx, ty), 256, 0, d.stream>>>(i, o, d.flags, d.nx, d.ny, d.plane,
                                                                                   d.omega);
   } else {
-    const long long pairs = (long long)d.ne * 8;
-    const int nbuild = (int)((pairs + 15) / 16);
-    k_bulk2d_links<S, STAGE, PERIODIC, TX, TY, CPT><<<(unsigned)(tx * ty + nbuild), 256, 0, d.stream>>>(i, o, tx, nbuild,
-                                                                                                      *la);
+    const unsigned grid = (unsigned)(tx * ty + x->nbc + x->nwave);
+    k_step2d<S, STAGE, PERIODIC, TX, TY, CPT><<<grid, 256, 0, d.stream>>>(i, o, tx, x->nbc, x->nwave, *x->la, *x->fa);
   }
 }
 
 template <typename S, bool STAGE, bool PERIODIC>
-static void bulk2d_variant(Domain& d, const S* i, S* o, const LinkArgs* la) {
+static void bulk2d_variant(Domain& d, const S* i, S* o, const StreamExtra* x) {
   switch (d.tile2) {
-    case 1: bulk2d_shape<S, STAGE, PERIODIC, 32, 8, 2>(d, i, o, la); break;
-    case 2: bulk2d_shape<S, STAGE, PERIODIC, 64, 4, 2>(d, i, o, la); break;
-    case 3: bulk2d_shape<S, STAGE, PERIODIC, 32, 8, 4>(d, i, o, la); break;
-    case 4: bulk2d_shape<S, STAGE, PERIODIC, 64, 4, 4>(d, i, o, la); break;
-    default: bulk2d_shape<S, STAGE, PERIODIC, 32, 8, 1>(d, i, o, la); break;
+    case 1: bulk2d_shape<S, STAGE, PERIODIC, 32, 8, 2>(d, i, o, x); break;
+    case 2: bulk2d_shape<S, STAGE, PERIODIC, 64, 4, 2>(d, i, o, x); break;
+    case 3: bulk2d_shape<S, STAGE, PERIODIC, 32, 8, 4>(d, i, o, x); break;
+    case 4: bulk2d_shape<S, STAGE, PERIODIC, 64, 4, 4>(d, i, o, x); break;
+    default: bulk2d_shape<S, STAGE, PERIODIC, 32, 8, 1>(d, i, o, x); break;
   }
 }
 
 template <typename S>
-static int bulk2d_typed(Domain& d, const void* in, void* out, const LinkArgs* la) {
+static int bulk2d_typed(Domain& d, const void* in, void* out, const StreamExtra* x) {
   const bool stage = d.options & FF_OPT_STAGE_FP16;
   const bool per = d.options & FF_OPT_PERIODIC;
   const S* i = (const S*)in;
   S* o = (S*)out;
-  if (stage && per) bulk2d_variant<S, true, true>(d, i, o, la);
-  else if (stage) bulk2d_variant<S, true, false>(d, i, o, la);
-  else if (per) bulk2d_variant<S, false, true>(d, i, o, la);
-  else bulk2d_variant<S, false, false>(d, i, o, la);
+  if (stage && per) bulk2d_variant<S, true, true>(d, i, o, x);
+  else if (stage) bulk2d_variant<S, true, false>(d, i, o, x);
+  else if (per) bulk2d_variant<S, false, true>(d, i, o, x);
+  else bulk2d_variant<S, false, false>(d, i, o, x);
   FF_HIP(hipGetLastError());
   return FF_OK;
 }
 
-int launch_bulk2d(Domain& d, const void* in, void* out) {
+static int bulk2d_dispatch(Domain& d, const void* in, void* out, const StreamExtra* x) {
   switch (d.storage) {
-    case FF_STORAGE_F32: return bulk2d_typed<float>(d, in, out, nullptr);
-    case FF_STORAGE_F16: return bulk2d_typed<__half>(d, in, out, nullptr);
-    default: return bulk2d_typed<__hip_bfloat16>(d, in, out, nullptr);
+    case FF_STORAGE_F32: return bulk2d_typed<float>(d, in, out, x);
+    case FF_STORAGE_F16: return bulk2d_typed<__half>(d, in, out, x);
+    default: return bulk2d_typed<__hip_bfloat16>(d, in, out, x);
   }
 }
 
-int launch_bulk2d_links(Domain& d, const void* in, void* out, const LinkArgs& la) {
-  switch (d.storage) {
-    case FF_STORAGE_F32: return bulk2d_typed<float>(d, in, out, &la);
-    case FF_STORAGE_F16: return bulk2d_typed<__half>(d, in, out, &la);
-    default: return bulk2d_typed<__hip_bfloat16>(d, in, out, &la);
-  }
+int launch_bulk2d(Domain& d, const void* in, void* out) { return bulk2d_dispatch(d, in, out, nullptr); }
+
+int launch_step_stream3d(Domain& d, const void* in, void* out, const LinkArgs& la, const FinalArgs& fa, int nwave);
+
+int launch_step_stream(Domain& d, const void* in, void* out, const LinkArgs& la, bool wave) {
+  const FinalArgs fa = make_final_args(d, out, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], nullptr);
+  const int nwave = wave ? (d.nv + 255) / 256 : 0;
+  if (d.dim == 3) return launch_step_stream3d(d, in, out, la, fa, nwave);
+  const StreamExtra x{&la, &fa, boundary_cell_blocks(d, 16), nwave};
+  return bulk2d_dispatch(d, in, out, &x);
 }
 
 // ---------------------------------------------------------------- outlet

@@ -8,6 +8,7 @@
 // reference is identically zero (LbmFlowField3D.cpp:35-36) and is not read.
 
 #include "ff_internal.h"
+#include "links_body.h"
 #include "stencil.h"
 
 namespace ff {
@@ -35,6 +36,62 @@ static void bulk3d_variant(Domain& d, const S* i, S* o) {
     case 2: k_bulk3d_ring<S, PERIODIC, 8, 8, 32><<<grid, 64, 0, d.stream>>>(i, o, d.flags, d.nx, d.ny, d.nz, d.plane, d.omega); break;
     case 3: k_bulk3d_ring<S, PERIODIC, 16, 16, 32><<<grid, 256, 0, d.stream>>>(i, o, d.flags, d.nx, d.ny, d.nz, d.plane, d.omega); break;
     default: k_bulk3d<S, PERIODIC><<<grid, kTX3 * kTY3 * kTZ3, 0, d.stream>>>(i, o, d.flags, d.nx, d.ny, d.nz, d.plane, d.omega); break;
+  }
+}
+
+// Stream launch of a 3D step with a boundary (see k_step2d): boundary-cell
+// blocks, wave blocks, then the tiles of the chosen shape; NT = block size.
+template <typename S, bool PERIODIC, int RING, int TX, int TY, int ZC, int NT>
+__global__ void __launch_bounds__(NT)
+    k_step3d(const S* __restrict__ in, S* __restrict__ out, int tiles_x, int tiles_y, int nbc, int nwave, LinkArgs la,
+             FinalArgs fa) {
+  const int b = blockIdx.x;
+  if (b < nbc) {
+    bcells_body<3, S, false, PERIODIC, NT>(in, la, b, nbc);
+  } else if (b < nbc + nwave) {
+    wave_block_body<3, NT>(la, fa, b - nbc);
+  } else {
+    const int t = b - nbc - nwave;
+    const int bx = t % tiles_x, by = (t / tiles_x) % tiles_y, bz = t / (tiles_x * tiles_y);
+    if constexpr (RING)
+      bulk3d_ring<S, PERIODIC, TX, TY, ZC>(in, out, la.flags, la.nx, la.ny, la.nz, la.plane, la.omega, bx, by, bz);
+    else
+      bulk3d_tile<S, PERIODIC>(in, out, la.flags, la.nx, la.ny, la.nz, la.plane, la.omega, bx, by, bz);
+  }
+}
+
+template <typename S, bool PERIODIC, int RING, int TX, int TY, int ZC, int NT>
+static void step3d_shape(Domain& d, const S* i, S* o, const LinkArgs& la, const FinalArgs& fa, int nwave) {
+  const int tx = (d.nx + TX - 1) / TX, ty = (d.ny + TY - 1) / TY, tz = (d.nz + ZC - 1) / ZC;
+  const int nbc = boundary_cell_blocks(d, NT / 32);
+  const int nw = nwave > 0 ? (d.nv + NT - 1) / NT : 0;
+  const unsigned grid = (unsigned)(tx * ty * tz + nbc + nw);
+  k_step3d<S, PERIODIC, RING, TX, TY, ZC, NT><<<grid, NT, 0, d.stream>>>(i, o, tx, ty, nbc, nw, la, fa);
+}
+
+template <typename S, bool PERIODIC>
+static void step3d_variant(Domain& d, const S* i, S* o, const LinkArgs& la, const FinalArgs& fa, int nwave) {
+  switch (d.tile3) {
+    case 1: step3d_shape<S, PERIODIC, 1, 16, 8, 32, 128>(d, i, o, la, fa, nwave); break;
+    case 2: step3d_shape<S, PERIODIC, 1, 8, 8, 32, 64>(d, i, o, la, fa, nwave); break;
+    case 3: step3d_shape<S, PERIODIC, 1, 16, 16, 32, 256>(d, i, o, la, fa, nwave); break;
+    default: step3d_shape<S, PERIODIC, 0, 8, 8, 4, 256>(d, i, o, la, fa, nwave); break;
+  }
+}
+
+template <typename S>
+static int step3d_typed(Domain& d, const void* in, void* out, const LinkArgs& la, const FinalArgs& fa, int nwave) {
+  if (d.options & FF_OPT_PERIODIC) step3d_variant<S, true>(d, (const S*)in, (S*)out, la, fa, nwave);
+  else step3d_variant<S, false>(d, (const S*)in, (S*)out, la, fa, nwave);
+  FF_HIP(hipGetLastError());
+  return FF_OK;
+}
+
+int launch_step_stream3d(Domain& d, const void* in, void* out, const LinkArgs& la, const FinalArgs& fa, int nwave) {
+  switch (d.storage) {
+    case FF_STORAGE_F32: return step3d_typed<float>(d, in, out, la, fa, nwave);
+    case FF_STORAGE_F16: return step3d_typed<__half>(d, in, out, la, fa, nwave);
+    default: return step3d_typed<__hip_bfloat16>(d, in, out, la, fa, nwave);
   }
 }
 

@@ -18,11 +18,20 @@
 // element), and one block sums elements into bodies and vertices in CSR
 // order, all in FP64.  Results do not depend on scheduling.
 //
-// A step with a boundary is four launches: stream-collide + link detection
-// (kernels2d.hip / kernels3d.hip), boundary cells (k_bcells), element sums +
-// outlet (k_elem_sums) and the one-block finish (k_finish).  Per-step values
-// (generation, counters, vertex buffer parity) live in StepState on the
-// device, so the launches take identical arguments every step.
+// A step with a boundary is three launches (four when the links must be
+// detected first):
+//   [detect]  links of this step under the current tag (first step, or after
+//             the boundary moved by a host update)
+//   stream    stream-collide tiles + boundary cells (into a side buffer) +
+//             next-step wave kinematics (kernels2d.hip / kernels3d.hip)
+//   sums      per-element force sums, outlet cells, boundary-cell copy-back
+//   finish    per-vertex forces, body wrench, and the detection of the next
+//             step's links (positions of the next step are already known)
+// Per-step values (gen, tags, counters, vertex-buffer parity) live in
+// StepState on the device, so the launches take identical arguments every
+// step.
+
+#include <algorithm>
 
 #include "ff_internal.h"
 #include "geometry.h"
@@ -30,67 +39,90 @@
 
 namespace ff {
 
-// ---------------------------------------------------------------- build (standalone)
+// ---------------------------------------------------------------- detect
+// Links of the current step under the current tag and vertex buffer.
 template <int DIM>
 __global__ void __launch_bounds__(256) k_links_build(LinkArgs la) {
-  build_links_block<DIM>(la, blockIdx.x, la.st->gen);
+  build_links_block<DIM>(la, blockIdx.x, la.st->tag, la.v[la.st->gen & 1]);
 }
 
-int launch_links_build(Domain& d, const LinkArgs& la) {
+// Invalidate links detected ahead for a boundary that has since moved: open a
+// fresh tag (entries under older tags read as absent).
+__global__ void k_tag_bump(StepState* st) {
+  if (threadIdx.x == 0) {
+    st->tag = st->tag + 1;
+    st->count[st->tag & 1] = 0;
+  }
+}
+
+int launch_detect(Domain& d, const LinkArgs& la, bool bump) {
+  if (bump) k_tag_bump<<<1, 64, 0, d.stream>>>(d.st);
   const int K = d.q - 1;
   const long long pairs = (long long)d.ne * K;
   const unsigned g = (unsigned)((pairs + 15) / 16);
-  if (g == 0) return FF_OK;
-  if (d.dim == 2) k_links_build<2><<<g, 256, 0, d.stream>>>(la);
-  else k_links_build<3><<<g, 256, 0, d.stream>>>(la);
+  if (g > 0) {
+    if (d.dim == 2) k_links_build<2><<<g, 256, 0, d.stream>>>(la);
+    else k_links_build<3><<<g, 256, 0, d.stream>>>(la);
+  }
   FF_HIP(hipGetLastError());
   return FF_OK;
 }
 
-// ---------------------------------------------------------------- per-domain kernels
-template <int DIM, typename S, bool STAGE, bool PERIODIC>
-__global__ void __launch_bounds__(256) k_bcells(const S* __restrict__ in, S* __restrict__ out, LinkArgs la) {
-  bcells_body<DIM, S, STAGE, PERIODIC>(in, out, la, blockIdx.x, gridDim.x);
-}
-
+// ---------------------------------------------------------------- sums, finish
 template <int DIM, typename S, int FB>
-__global__ void __launch_bounds__(FB) k_elem_sums(LinkArgs la, FinalArgs fa) {
-  elem_sums_body<DIM, S, FB>(la, fa, blockIdx.x, fa.buf);
+__global__ void __launch_bounds__(FB) k_step_sums(LinkArgs la, FinalArgs fa, void* out) {
+  elem_sums_body<DIM, S, FB>(la, fa, blockIdx.x, out);
 }
 
-// Vertex blocks, then one last block for the body wrenches and the next
-// generation (only that block reads or writes the generation).
+// Vertex blocks, then `ndet` detection blocks for the next step, then one
+// last block for the body wrenches and the advance of gen and tag.
 template <int DIM, int FB>
-__global__ void __launch_bounds__(FB) k_finish(LinkArgs la, FinalArgs fa) {
-  if ((int)blockIdx.x < (int)gridDim.x - 1) finish_vertex_body<DIM, FB>(la, fa, blockIdx.x);
+__global__ void __launch_bounds__(FB) k_step_finish(LinkArgs la, FinalArgs fa, int nvb, int ndet) {
+  const int b = blockIdx.x;
+  if (b < nvb) finish_vertex_body<DIM, FB>(la, fa, b);
+  else if (b < nvb + ndet) detect_next_body<DIM>(la, b - nvb);
   else finish_last_body<DIM, FB>(la, fa);
 }
 
-template <int DIM, typename S>
-static void bcells_launch(Domain& d, const void* in, void* out, const LinkArgs& la) {
-  constexpr int CPB = DIM == 2 ? 16 : 8;
-  int blocks = (d.lt_cap + CPB - 1) / CPB;
-  blocks = blocks > 512 ? 512 : blocks;
-  const bool stage = DIM == 2 && (d.options & FF_OPT_STAGE_FP16);
-  const bool per = d.options & FF_OPT_PERIODIC;
-  const S* i = (const S*)in;
-  S* o = (S*)out;
-  if (stage && per) k_bcells<DIM, S, true, true><<<blocks, 256, 0, d.stream>>>(i, o, la);
-  else if (stage) k_bcells<DIM, S, true, false><<<blocks, 256, 0, d.stream>>>(i, o, la);
-  else if (per) k_bcells<DIM, S, false, true><<<blocks, 256, 0, d.stream>>>(i, o, la);
-  else k_bcells<DIM, S, false, false><<<blocks, 256, 0, d.stream>>>(i, o, la);
+// Copy-back blocks of the sums launch: grid-stride over the step's boundary
+// cells, at least one block (it also opens the next tag).
+int copyback_blocks(const Domain& d, int fb) {
+  const unsigned long long est = std::max<unsigned long long>(d.lt_est, 1);
+  return (int)std::min<unsigned long long>(256, (est + fb - 1) / fb);
 }
 
-int launch_boundary_cells(Domain& d, const void* in, void* out, const LinkArgs& la) {
+int boundary_cell_blocks(const Domain& d, int cells_per_block) {
+  return (int)std::min<unsigned long long>(512, std::max<unsigned long long>(1, (d.lt_est / 4 + cells_per_block - 1) /
+                                                                                    cells_per_block));
+}
+
+int launch_step_sums(Domain& d, const LinkArgs& la, void* out) {
+  constexpr int FB2 = kElemBlock2, FB3 = kElemBlock3;
+  const int fb = d.dim == 2 ? FB2 : FB3;
+  const FinalArgs fa = make_final_args(d, out, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], nullptr);
+  const unsigned grid = (unsigned)(d.ne + fa.outlet_blocks + fa.copy_blocks);
+#define FF_SUM(DIM, S, FB) k_step_sums<DIM, S, FB><<<grid, FB, 0, d.stream>>>(la, fa, out)
   if (d.dim == 2) {
-    if (d.storage == FF_STORAGE_F32) bcells_launch<2, float>(d, in, out, la);
-    else if (d.storage == FF_STORAGE_F16) bcells_launch<2, __half>(d, in, out, la);
-    else bcells_launch<2, __hip_bfloat16>(d, in, out, la);
+    if (d.storage == FF_STORAGE_F32) FF_SUM(2, float, FB2);
+    else if (d.storage == FF_STORAGE_F16) FF_SUM(2, __half, FB2);
+    else FF_SUM(2, __hip_bfloat16, FB2);
   } else {
-    if (d.storage == FF_STORAGE_F32) bcells_launch<3, float>(d, in, out, la);
-    else if (d.storage == FF_STORAGE_F16) bcells_launch<3, __half>(d, in, out, la);
-    else bcells_launch<3, __hip_bfloat16>(d, in, out, la);
+    if (d.storage == FF_STORAGE_F32) FF_SUM(3, float, FB3);
+    else if (d.storage == FF_STORAGE_F16) FF_SUM(3, __half, FB3);
+    else FF_SUM(3, __hip_bfloat16, FB3);
   }
+#undef FF_SUM
+  FF_HIP(hipGetLastError());
+  return FF_OK;
+}
+
+int launch_step_finish(Domain& d, const LinkArgs& la, bool detect_next, float* proxy_forces) {
+  const FinalArgs fa = make_final_args(d, nullptr, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], proxy_forces);
+  const int nvb = (d.nv + 255) / 256;
+  const int ndet = detect_next ? (int)(((long long)d.ne * (d.q - 1) + 15) / 16) : 0;
+  const unsigned grid = (unsigned)(nvb + ndet + 1);
+  if (d.dim == 2) k_step_finish<2, 256><<<grid, 256, 0, d.stream>>>(la, fa, nvb, ndet);
+  else k_step_finish<3, 256><<<grid, 256, 0, d.stream>>>(la, fa, nvb, ndet);
   FF_HIP(hipGetLastError());
   return FF_OK;
 }
@@ -126,32 +158,8 @@ FinalArgs make_final_args(Domain& d, void* out, float* vn0, float* vn1, float* u
   constexpr int FB2 = 128, FB3 = 256;
   const int fb = d.dim == 2 ? FB2 : FB3;
   fa.outlet_blocks = (d.n_outlets + fb - 1) / fb;
+  fa.copy_blocks = (d.has_boundary && d.ne > 0) ? copyback_blocks(d, fb) : 0;
   return fa;
-}
-
-int launch_finalize(Domain& d, const LinkArgs& la, void* out, float* vn0, float* vn1, float* un0, float* un1,
-                    float* proxy_forces) {
-  constexpr int FB2 = 128, FB3 = 256;
-  const int fb = d.dim == 2 ? FB2 : FB3;
-  const FinalArgs fa = make_final_args(d, out, vn0, vn1, un0, un1, proxy_forces);
-  const int wave_blocks = fa.wave ? (d.nv + fb - 1) / fb : 0;
-  const unsigned grid = (unsigned)(d.ne + fa.outlet_blocks + wave_blocks);
-#define FF_FIN(DIM, S, FB) k_elem_sums<DIM, S, FB><<<grid, FB, 0, d.stream>>>(la, fa)
-  if (d.dim == 2) {
-    if (d.storage == FF_STORAGE_F32) FF_FIN(2, float, FB2);
-    else if (d.storage == FF_STORAGE_F16) FF_FIN(2, __half, FB2);
-    else FF_FIN(2, __hip_bfloat16, FB2);
-  } else {
-    if (d.storage == FF_STORAGE_F32) FF_FIN(3, float, FB3);
-    else if (d.storage == FF_STORAGE_F16) FF_FIN(3, __half, FB3);
-    else FF_FIN(3, __hip_bfloat16, FB3);
-  }
-#undef FF_FIN
-  const unsigned fgrid = (unsigned)((d.nv + 255) / 256 + 1);
-  if (d.dim == 2) k_finish<2, 256><<<fgrid, 256, 0, d.stream>>>(la, fa);
-  else k_finish<3, 256><<<fgrid, 256, 0, d.stream>>>(la, fa);
-  FF_HIP(hipGetLastError());
-  return FF_OK;
 }
 
 // Per-vertex forces of the most recent step from the element sums.

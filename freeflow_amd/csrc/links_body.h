@@ -41,18 +41,21 @@ __device__ __forceinline__ void cell_pops(const S* in, size_t plane, size_t c, f
   }
 }
 
-template <int DIM, typename S, bool STAGE, bool PERIODIC>
-__device__ __forceinline__ void bcells_body(const S* __restrict__ in, S* __restrict__ out, const LinkArgs& la, int blk,
-                                            int nblk) {
+// The result is written to the cell's side-buffer row (la.sb), not to the
+// output moments: the stream-collide tiles of the same launch also write the
+// cell, and the element-sum launch copies the side buffer over them.
+template <int DIM, typename S, bool STAGE, bool PERIODIC, int NT>
+__device__ __forceinline__ void bcells_body(const S* __restrict__ in, const LinkArgs& la, int blk, int nblk) {
   constexpr int Q = DIM == 2 ? 9 : 27;
   constexpr int K = Q - 1;
   constexpr int NM = DIM == 2 ? 6 : 10;
   constexpr int LPC = DIM == 2 ? 16 : 32;  // lanes per cell
-  constexpr int CPB = 256 / LPC;           // cells per block
+  constexpr int CPB = NT / LPC;            // cells per block
   __shared__ float sp[CPB][Q];
   const unsigned gen = la.st->gen;
-  const int count = la.st->count[gen & 1];
-  const int* list = la.list + (size_t)(gen & 1) * la.cap;
+  const unsigned tag = la.st->tag;
+  const int count = la.st->count[tag & 1];
+  const int* list = la.list + (size_t)(tag & 1) * la.cap;
   const float* v = la.v[gen & 1];
   const float* u = la.u[gen & 1];
   const int lc = threadIdx.x / LPC, dir = threadIdx.x % LPC;
@@ -82,7 +85,7 @@ __device__ __forceinline__ void bcells_body(const S* __restrict__ in, S* __restr
       const size_t li = (size_t)slot * K + (dir - 1);
       // every load that depends only on the cell index is issued up front
       const uint8_t fup = la.flags[((size_t)zw * la.ny + yw) * la.nx + xw];
-      const uint32_t key = dir > 0 ? lt_key(la.keys, li, gen) : 0xFFFFFFFFu;
+      const uint32_t key = dir > 0 ? lt_key(la.keys, li, tag) : 0xFFFFFFFFu;
       float mc[NM];
 #pragma unroll
       for (int q = 0; q < NM; q++) mc[q] = Store<S>::template load<DIM>(in + q * la.plane + cell, q);
@@ -164,8 +167,9 @@ __device__ __forceinline__ void bcells_body(const S* __restrict__ in, S* __restr
       float o[NM];
       if constexpr (DIM == 2) collide2(p, la.omega, o);
       else collide3(p, la.omega, o);
+      float* row = la.sb + (size_t)list[t] * NM;
 #pragma unroll
-      for (int q = 0; q < NM; q++) Store<S>::template store<DIM>(out + q * la.plane + cell, q, o[q]);
+      for (int q = 0; q < NM; q++) row[q] = o[q];
     }
     __syncthreads();
   }
@@ -198,6 +202,7 @@ struct FinalArgs {
   const OutletCell* outlets;
   int n_outlets;
   int outlet_blocks;
+  int copy_blocks;  // copy-back blocks of the sums launch
 };
 
 // Element-sum block size: 2D 128, 3D 256.
@@ -206,6 +211,9 @@ constexpr int kElemBlock2 = 128, kElemBlock3 = 256;
 // Host: the FinalArgs of a domain (outlets written into `out`, next-step
 // kinematics into vn/un, per-vertex forces optionally added to proxy_forces).
 FinalArgs make_final_args(Domain& d, void* out, float* vn0, float* vn1, float* un0, float* un1, float* proxy_forces);
+// Host: grid sizes of the boundary-cell and copy-back parts of a step.
+int boundary_cell_blocks(const Domain& d, int cells_per_block);
+int copyback_blocks(const Domain& d, int fb);
 
 // Travelling wave along the body axis, FP64:
 //   y = y_rest + A s sin(theta), v_y = -2 pi f A s cos(theta),
@@ -229,22 +237,49 @@ __device__ __forceinline__ void wave_vertex(const float* rest, const float* s, c
   vel[k * DIM + 1] = (float)(-two_pi * f * A * sk * cs);
 }
 
-// Element blocks: each sums its own links (fixed order, FP64) into elem_acc.
-// The next blocks update the outlet cells of the freshly written buffer
-// (mrApplyOutlet2DKernel / mrApplyOutlet3DKernel); the last ones evaluate the
-// next step's wave kinematics.
+// Next-step kinematics for one block of vertices (stream launch): the vertex
+// buffers of the next step are not read during this step.
+template <int DIM, int FB>
+__device__ __forceinline__ void wave_block_body(const LinkArgs& la, const FinalArgs& fa, int blk) {
+  const int k = blk * FB + threadIdx.x;
+  if (k < la.nv) {
+    const unsigned g = la.st->gen;
+    wave_vertex<DIM>(fa.rest, fa.s, fa.vbody, fa.wpar, k, (double)g * fa.dt, fa.vn[(g + 1) & 1], fa.un[(g + 1) & 1]);
+  }
+}
+
+// Second launch of a boundary step.  Blocks [0, ne): each element sums its
+// own links (fixed order, FP64) into elem_acc.  Then outlet blocks: outlet
+// cells of the freshly written buffer (mrApplyOutlet2DKernel /
+// mrApplyOutlet3DKernel; a neighbour that is a boundary cell is read from its
+// side-buffer row, which the copy-back of this launch writes).  Then
+// copy-back blocks: boundary-cell rows into the output; block `ne +
+// outlet_blocks` also opens the next step's tag and clears its list.
 template <int DIM, typename S, int FB>
 __device__ __forceinline__ void elem_sums_body(const LinkArgs& la, const FinalArgs& fa, int blk, void* out) {
   constexpr int K = DIM == 2 ? 8 : 26;
   constexpr int NT = DIM == 2 ? 1 : 3;
+  constexpr int NM = DIM == 2 ? 6 : 10;
   constexpr int STRIDE = DIM * DIM + DIM + NT;
   constexpr int CH = 256;  // window cells resolved per chunk
   if (blk >= la.ne + fa.outlet_blocks) {
-    // kinematics of the next step (its vertex buffer is not read in this step)
-    const int k = (blk - la.ne - fa.outlet_blocks) * FB + threadIdx.x;
-    if (fa.wave && k < la.nv) {
-      const unsigned g = la.st->gen;
-      wave_vertex<DIM>(fa.rest, fa.s, fa.vbody, fa.wpar, k, (double)g * fa.dt, fa.vn[(g + 1) & 1], fa.un[(g + 1) & 1]);
+    const unsigned tag = la.st->tag;
+    const int b = blk - la.ne - fa.outlet_blocks;
+    if (b == 0 && threadIdx.x == 0) {
+      la.st->tag_next = tag + 1;
+      la.st->vnext = (la.st->gen + 1) & 1;
+      la.st->count[(tag + 1) & 1] = 0;
+    }
+    const int count = la.st->count[tag & 1];
+    const int nb = fa.copy_blocks;
+    S* o = (S*)out;
+    for (int t = b * FB + threadIdx.x; t < count; t += nb * FB) {
+      const int slot = la.list[(size_t)(tag & 1) * la.cap + t];
+      const int cell = (int)(uint32_t)la.cells[slot];
+      if (!is_updated(la.flags[cell])) continue;
+      const float* row = la.sb + (size_t)slot * NM;
+#pragma unroll
+      for (int q = 0; q < NM; q++) Store<S>::template store<DIM>(o + q * la.plane + cell, q, row[q]);
     }
     return;
   }
@@ -254,7 +289,9 @@ __device__ __forceinline__ void elem_sums_body(const LinkArgs& la, const FinalAr
     S* buf = (S*)out;
     const OutletCell c = fa.outlets[t];
     if (la.flags[c.nb] != kFluid) return;
-    const float up = Store<S>::template load<DIM>(buf + c.comp * la.plane + c.nb, c.comp);
+    const int nslot = la.ne > 0 ? lt_find(la.cells, la.cap, la.st->tag, c.nb) : -1;
+    const float up = nslot >= 0 ? la.sb[(size_t)nslot * NM + c.comp]
+                                : Store<S>::template load<DIM>(buf + c.comp * la.plane + c.nb, c.comp);
     if ((c.sign > 0 && up > 0.0f) || (c.sign < 0 && up < 0.0f))
       Store<S>::template store<DIM>(buf + c.comp * la.plane + c.cell, c.comp, up);
     Store<S>::template store<DIM>(buf + c.pcomp * la.plane + c.cell, c.pcomp, up * up);
@@ -263,6 +300,7 @@ __device__ __forceinline__ void elem_sums_body(const LinkArgs& la, const FinalAr
   __shared__ int s_slot[CH];
   __shared__ double s_red[STRIDE][FB];
   const unsigned gen = la.st->gen;
+  const unsigned tag = la.st->tag;
   const int elem = blk;
   const float* v = la.v[gen & 1];
   const int n[3] = {la.nx, la.ny, la.nz};
@@ -277,14 +315,14 @@ __device__ __forceinline__ void elem_sums_body(const LinkArgs& la, const FinalAr
   double acc[STRIDE];
 #pragma unroll
   for (int a = 0; a < STRIDE; a++) acc[a] = 0.0;
-  const uint32_t tag = (uint32_t)(uint16_t)elem;
+  const uint32_t etag = (uint32_t)(uint16_t)elem;
   for (int c0 = 0; c0 < ncell; c0 += CH) {
     const int chunk = min(CH, ncell - c0);
     for (int k = threadIdx.x; k < chunk; k += FB) {
       const int kk = c0 + k;
       const int i = lo[0] + kk % wx, j = lo[1] + (kk / wx) % wy, z = DIM == 3 ? lo[2] + kk / (wx * wy) : 0;
       const int cell = (z * la.ny + j) * la.nx + i;
-      s_slot[k] = is_updated(la.flags[cell]) ? lt_find(la.cells, la.cap, gen, cell) : -1;
+      s_slot[k] = is_updated(la.flags[cell]) ? lt_find(la.cells, la.cap, tag, cell) : -1;
     }
     __syncthreads();
     for (int task = threadIdx.x; task < K * chunk; task += FB) {
@@ -292,8 +330,8 @@ __device__ __forceinline__ void elem_sums_body(const LinkArgs& la, const FinalAr
       const int slot = s_slot[k];
       if (slot < 0) continue;
       const size_t li = (size_t)slot * K + l;
-      const uint32_t key = lt_key(la.keys, li, gen);
-      if (key == 0xFFFFFFFFu || (key & 0xFFFFu) != tag) continue;
+      const uint32_t key = lt_key(la.keys, li, tag);
+      if (key == 0xFFFFFFFFu || (key & 0xFFFFu) != etag) continue;
       if (la.lb[li * DIM] == kLinkUnused) continue;
       float F[DIM];
 #pragma unroll
@@ -333,8 +371,7 @@ __device__ __forceinline__ void elem_sums_body(const LinkArgs& la, const FinalAr
 
 // Vertex blocks sum their incident element shares in CSR order (the
 // per-vertex forces addForceAt accumulates); the last block folds element
-// sums into body wrenches with a fixed tree and starts the next step's link
-// table generation.  Only that block reads or writes the generation.
+// sums into body wrenches with a fixed tree and advances gen and tag.
 template <int DIM, int FB>
 __device__ __forceinline__ void finish_vertex_body(const LinkArgs& la, const FinalArgs& fa, int blk) {
   constexpr int NT = DIM == 2 ? 1 : 3;
@@ -391,9 +428,18 @@ __device__ __forceinline__ void finish_last_body(const LinkArgs& la, const Final
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    la.st->count[(gen + 1) & 1] = 0;
     la.st->gen = gen + 1;
+    la.st->tag = la.st->tag_next;
   }
+}
+
+// Detection of the next step's links in the finish launch, under tag_next
+// (set by the element-sum launch) and the next step's vertex positions.
+// (tag_next and vnext were written by the element-sum launch; the last block
+// of the finish launch advances gen and tag, which these blocks do not read.)
+template <int DIM>
+__device__ __forceinline__ void detect_next_body(const LinkArgs& la, int blk) {
+  build_links_block<DIM>(la, (long long)blk, la.st->tag_next, la.v[la.st->vnext & 1]);
 }
 
 }  // namespace ff

@@ -32,28 +32,41 @@ __device__ __forceinline__ void bulk2d_tile(const S* __restrict__ in, S* __restr
   constexpr int NT = TX * TY, RY = TY * CPT;
   constexpr int HX = TX + 2, HY = RY + 2, HN = HX * HY;
   using T = typename std::conditional<STAGE, __half, float>::type;
+  constexpr int PER = (HN + NT - 1) / NT;  // halo cells per thread
   __shared__ T sp[9][HN];
   const int x0 = bx * TX, y0 = by * RY;
-  for (int k = threadIdx.x; k < HN; k += NT) {
-    const int hx = k % HX, hy = k / HX;
-    int gx = x0 + hx - 1, gy = y0 + hy - 1;
-    if (PERIODIC) {
-      gx = ((gx % nx) + nx) % nx;
-      gy = ((gy % ny) + ny) % ny;
-    } else {
-      gx = min(nx - 1, max(0, gx));
-      gy = min(ny - 1, max(0, gy));
+  // every load of the tile is issued before the first reconstruction, so the
+  // block waits for memory once
+  float m[PER][6];
+#pragma unroll
+  for (int r = 0; r < PER; r++) {
+    const int k = threadIdx.x + r * NT;
+    if (k < HN) {
+      const int hx = k % HX, hy = k / HX;
+      int gx = x0 + hx - 1, gy = y0 + hy - 1;
+      if (PERIODIC) {
+        gx = ((gx % nx) + nx) % nx;
+        gy = ((gy % ny) + ny) % ny;
+      } else {
+        gx = min(nx - 1, max(0, gx));
+        gy = min(ny - 1, max(0, gy));
+      }
+      const size_t c = (size_t)gy * nx + gx;
+#pragma unroll
+      for (int q = 0; q < 6; q++) m[r][q] = Store<S>::template load<2>(in + q * plane + c, q);
     }
-    const size_t c = (size_t)gy * nx + gx;
-    float m[6];
+  }
 #pragma unroll
-    for (int q = 0; q < 6; q++) m[q] = Store<S>::template load<2>(in + q * plane + c, q);
-    float f[9];
-    recon2_all(m[0], m[1], m[2], m[3], m[5], m[4], f);
+  for (int r = 0; r < PER; r++) {
+    const int k = threadIdx.x + r * NT;
+    if (k < HN) {
+      float f[9];
+      recon2_all(m[r][0], m[r][1], m[r][2], m[r][3], m[r][5], m[r][4], f);
 #pragma unroll
-    for (int i = 0; i < 9; i++) {
-      if constexpr (STAGE) sp[i][k] = to_half_rn(f[i]);
-      else sp[i][k] = f[i];
+      for (int i = 0; i < 9; i++) {
+        if constexpr (STAGE) sp[i][k] = to_half_rn(f[i]);
+        else sp[i][k] = f[i];
+      }
     }
   }
   __syncthreads();
@@ -94,30 +107,44 @@ __device__ __forceinline__ void bulk3d_tile(const S* __restrict__ in, S* __restr
                                             float omega, int bx, int by, int bz) {
   constexpr int HX = kTX3 + 2, HY = kTY3 + 2, HZ = kTZ3 + 2, HN = HX * HY * HZ;
   constexpr int NT = kTX3 * kTY3 * kTZ3;
+  constexpr int PER = (HN + NT - 1) / NT;  // halo cells per thread
   __shared__ float sp[27][HN];
   __shared__ uint8_t sf[HN];
   const int x0 = bx * kTX3, y0 = by * kTY3, z0 = bz * kTZ3;
-  for (int k = threadIdx.x; k < HN; k += NT) {
-    const int hx = k % HX, hy = (k / HX) % HY, hz = k / (HX * HY);
-    const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
-    const int wx = ((gx % nx) + nx) % nx, wy = ((gy % ny) + ny) % ny, wz = ((gz % nz) + nz) % nz;
-    int sx, sy, sz;
-    if (PERIODIC) {
-      sx = wx; sy = wy; sz = wz;
-    } else {
-      sx = min(nx - 1, max(0, gx));
-      sy = min(ny - 1, max(0, gy));
-      sz = min(nz - 1, max(0, gz));
+  // all loads first, then the reconstructions (one wait for memory)
+  float m[PER][10];
+  uint8_t fl[PER];
+#pragma unroll
+  for (int r = 0; r < PER; r++) {
+    const int k = threadIdx.x + r * NT;
+    if (k < HN) {
+      const int hx = k % HX, hy = (k / HX) % HY, hz = k / (HX * HY);
+      const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+      const int wx = ((gx % nx) + nx) % nx, wy = ((gy % ny) + ny) % ny, wz = ((gz % nz) + nz) % nz;
+      int sx, sy, sz;
+      if (PERIODIC) {
+        sx = wx; sy = wy; sz = wz;
+      } else {
+        sx = min(nx - 1, max(0, gx));
+        sy = min(ny - 1, max(0, gy));
+        sz = min(nz - 1, max(0, gz));
+      }
+      const size_t c = ((size_t)sz * ny + sy) * nx + sx;
+#pragma unroll
+      for (int q = 0; q < 10; q++) m[r][q] = Store<S>::template load<3>(in + q * plane + c, q);
+      fl[r] = flags[((size_t)wz * ny + wy) * nx + wx];
     }
-    const size_t c = ((size_t)sz * ny + sy) * nx + sx;
-    float m[10];
+  }
 #pragma unroll
-    for (int q = 0; q < 10; q++) m[q] = Store<S>::template load<3>(in + q * plane + c, q);
-    float f[27];
-    recon3_all(m, f);
+  for (int r = 0; r < PER; r++) {
+    const int k = threadIdx.x + r * NT;
+    if (k < HN) {
+      float f[27];
+      recon3_all(m[r], f);
 #pragma unroll
-    for (int i = 0; i < 27; i++) sp[i][k] = f[i];
-    sf[k] = flags[((size_t)wz * ny + wy) * nx + wx];
+      for (int i = 0; i < 27; i++) sp[i][k] = f[i];
+      sf[k] = fl[r];
+    }
   }
   __syncthreads();
   const int lx = threadIdx.x % kTX3, ly = (threadIdx.x / kTX3) % kTY3, lz = threadIdx.x / (kTX3 * kTY3);
