@@ -42,7 +42,7 @@ METRIC = "MLUPS + % HBM peak; batched/served vs plain at ≤1e-3 rel err; p99 st
 HBM_PEAK_GBS = 8000.0
 # tile shapes by index, as in the library (ff_set_tile)
 TILES2 = ["32x8", "32x16", "64x8", "32x32", "64x16"]
-TILES3 = ["8x8x4 box", "16x8 column x32", "8x8 column x32", "16x16 column x32"]
+TILES3 = ["8x8x4 box", "16x4x4 box", "16x8 column x32", "8x8 column x32", "16x16 column x32"]
 
 
 def bytes_per_lup(dim, storage_bytes):
@@ -95,6 +95,13 @@ def cpu_baseline_config2(state, budget_s=12.0):
     return {"value": round(mlups, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
             "sample": f"CPU oracle (FP32 restatement of the reference update, OpenMP {threads} threads) on the "
                       f"same 1200x400 swimmer workload, first {steps} steps ({el:.1f} s)"}
+
+
+def pick_tile(d, args):
+    if args.tile >= 0:
+        d.set_tile(args.tile)
+        return args.tile
+    return 0 if args.no_tune else d.autotune()
 
 
 def roofline(achieved_gbs, kernel, avg_ms, bpl, cells):
@@ -183,7 +190,7 @@ def run_config3(args, capi, wl):
     d.set_moments(wl.taylor_green_ring(nx, ny, nz, c["amp"], c["ring_r"], c["seed"]))
     K = args.steps if args.steps != 4000 else 1000
     W = min(args.warmup, 20)
-    tile = d.autotune() if not args.no_tune else 0
+    tile = pick_tile(d, args)
     m0 = d.mass()
     d.step(W)
     d.sync()
@@ -223,6 +230,20 @@ def run_served(args):
     return 0
 
 
+def copy_probe(mib):
+    """A plain copy of known size: its FETCH_SIZE / WRITE_SIZE readings turn
+    the counters of the stream-collide kernel into bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4, "copies": 5}), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -231,12 +252,18 @@ def main():
     ap.add_argument("--workload", default="config2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tile shape instead of timing the candidates")
+    ap.add_argument("--tile", type=int, default=-1, help="use this tile index instead of timing the candidates")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--storage", default="f16", help="config3: f32, f16 or bf16")
     ap.add_argument("--rate", type=float, default=50.0, help="served: episode arrivals per second")
     ap.add_argument("--duration", type=float, default=5.0, help="served: arrival window in seconds")
     ap.add_argument("--plain-steps", type=int, default=200, help="batched: steps of the per-domain FP32 form")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (the reference kernel that calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters), then exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(independent domains shard across GPUs without communication); use --gpus 1",
@@ -254,7 +281,7 @@ def main():
         print(f"unknown workload {args.workload}", file=sys.stderr)
         return 2
     d, state = setup_config2(capi, wl)
-    tile = d.autotune() if not args.no_tune else 0
+    tile = pick_tile(d, args)
     c = wl.CONFIG2
     cells = c["nx"] * c["ny"]
     K, W = args.steps, args.warmup

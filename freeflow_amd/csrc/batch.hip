@@ -74,6 +74,7 @@ template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT>
 __global__ void __launch_bounds__(256) kb_step2d(const DomDev* __restrict__ doms, const int2* __restrict__ map,
                                                  int n_bc, int n_wave) {
   const int2 m = map[blockIdx.x];
+  if (m.x < 0) return;  // padding of the XCD-aware tile order
   const DomDev* D = doms + m.x;
   const unsigned p = *D->step & 1u;
   const int b = blockIdx.x;
@@ -88,11 +89,12 @@ __global__ void __launch_bounds__(256) kb_step2d(const DomDev* __restrict__ doms
   }
 }
 
-// RING = 0: 8x8x4 box tiles; otherwise TX x TY columns marching ZC planes; NT = block size.
+// RING = 0: TX x TY x ZC box tiles; otherwise TX x TY columns marching ZC planes; NT = block size.
 template <typename S, bool PERIODIC, int RING, int TX, int TY, int ZC, int NT>
 __global__ void __launch_bounds__(NT) kb_step3d(const DomDev* __restrict__ doms, const int2* __restrict__ map,
                                                 int n_bc, int n_wave) {
   const int2 m = map[blockIdx.x];
+  if (m.x < 0) return;  // padding of the XCD-aware tile order
   const DomDev* D = doms + m.x;
   const unsigned p = *D->step & 1u;
   const int b = blockIdx.x;
@@ -107,8 +109,9 @@ __global__ void __launch_bounds__(NT) kb_step3d(const DomDev* __restrict__ doms,
                                            D->la.nz, D->la.plane, D->la.omega, m.y % tx, (m.y / tx) % ty,
                                            m.y / (tx * ty));
     else
-      bulk3d_tile<S, PERIODIC>((const S*)D->mom[p], (S*)D->mom[p ^ 1u], D->la.flags, D->la.nx, D->la.ny, D->la.nz,
-                               D->la.plane, D->la.omega, m.y % tx, (m.y / tx) % ty, m.y / (tx * ty));
+      bulk3d_tile<S, PERIODIC, TX, TY, ZC>((const S*)D->mom[p], (S*)D->mom[p ^ 1u], D->la.flags, D->la.nx, D->la.ny,
+                                           D->la.nz, D->la.plane, D->la.omega, m.y % tx, (m.y / tx) % ty,
+                                           m.y / (tx * ty));
   }
 }
 
@@ -164,9 +167,10 @@ static void step2d_batch_variant(Batch& b, int tile, const int2* map, int nblock
 template <typename S, bool PERIODIC>
 static void step3d_batch_variant(Batch& b, int tile, const int2* map, int nblocks, int n_bc, int n_wave) {
   switch (tile) {
-    case 1: kb_step3d<S, PERIODIC, 1, 16, 8, 32, 128><<<nblocks, 128, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
-    case 2: kb_step3d<S, PERIODIC, 1, 8, 8, 32, 64><<<nblocks, 64, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
-    case 3: kb_step3d<S, PERIODIC, 1, 16, 16, 32, 256><<<nblocks, 256, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
+    case 1: kb_step3d<S, PERIODIC, 0, 16, 4, 4, 256><<<nblocks, 256, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
+    case 2: kb_step3d<S, PERIODIC, 1, 16, 8, 32, 128><<<nblocks, 128, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
+    case 3: kb_step3d<S, PERIODIC, 1, 8, 8, 32, 64><<<nblocks, 64, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
+    case 4: kb_step3d<S, PERIODIC, 1, 16, 16, 32, 256><<<nblocks, 256, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
     default: kb_step3d<S, PERIODIC, 0, 8, 8, 4, 256><<<nblocks, 256, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
   }
 }
@@ -292,6 +296,19 @@ static int build_tables(Batch& b, int tile) {
     const int nes = (bnd ? d.ne : 0) + r.fa.outlet_blocks + r.fa.copy_blocks;
     for (int t = 0; t < nes; t++) es.push_back(make_int2(i, t));
     for (int t = 0; t <= r.nvb + r.ndet; t++) fin.push_back(make_int2(i, t));
+  }
+  // XCD-aware order of the tiles (see xcd_tile): the tile range starts at a
+  // multiple of 8 blocks (the wave range is padded) and block p of the range
+  // takes tile (p % 8) * ceil(n / 8) + p / 8 of the concatenated list.
+  while ((bc.size() + wave.size()) % kXcds) wave.push_back(make_int2(-1, -1));
+  {
+    const size_t nt = bulk.size(), per = (nt + kXcds - 1) / kXcds;
+    std::vector<int2> order(per * kXcds, make_int2(-1, -1));
+    for (size_t p = 0; p < order.size(); p++) {
+      const size_t t = (p % kXcds) * per + p / kXcds;
+      if (t < nt) order[p] = bulk[t];
+    }
+    bulk.swap(order);
   }
   b.n_bc = (int)bc.size();
   b.n_wave = (int)wave.size();

@@ -23,18 +23,21 @@ namespace ff {
 template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT>
 __global__ void __launch_bounds__(256)
     k_bulk2d(const S* __restrict__ in, S* __restrict__ out, const uint8_t* __restrict__ flags, int nx, int ny,
-             size_t plane, float omega) {
-  bulk2d_tile<S, STAGE, PERIODIC, TX, TY, CPT>(in, out, flags, nx, ny, plane, omega, blockIdx.x, blockIdx.y);
+             size_t plane, float omega, int tiles_x, int ntiles) {
+  const int t = xcd_tile(blockIdx.x, ntiles);
+  if (t < 0) return;
+  bulk2d_tile<S, STAGE, PERIODIC, TX, TY, CPT>(in, out, flags, nx, ny, plane, omega, t % tiles_x, t / tiles_x);
 }
 
 // Stream launch of a step with a boundary: the first `nbc` blocks update the
 // cells owning a moving-wall link (into the side buffer), the next `nwave`
 // blocks evaluate the next step's wave kinematics, the rest are the
 // stream-collide tiles (which also write the boundary cells; the copy-back of
-// the sums launch overwrites them).
+// the sums launch overwrites them).  nbc + nwave is a multiple of 8, so the
+// tiles keep the XCD-aware order.
 template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT>
 __global__ void __launch_bounds__(256)
-    k_step2d(const S* __restrict__ in, S* __restrict__ out, int tiles_x, int nbc, int nwave, LinkArgs la,
+    k_step2d(const S* __restrict__ in, S* __restrict__ out, int tiles_x, int ntiles, int nbc, int nwave, LinkArgs la,
              FinalArgs fa) {
   const int b = blockIdx.x;
   if (b < nbc) {
@@ -42,7 +45,8 @@ __global__ void __launch_bounds__(256)
   } else if (b < nbc + nwave) {
     wave_block_body<2, 256>(la, fa, b - nbc);
   } else {
-    const int t = b - nbc - nwave;
+    const int t = xcd_tile(b - nbc - nwave, ntiles);
+    if (t < 0) return;
     bulk2d_tile<S, STAGE, PERIODIC, TX, TY, CPT>(in, out, la.flags, la.nx, la.ny, la.plane, la.omega, t % tiles_x,
                                                  t / tiles_x);
   }
@@ -58,12 +62,14 @@ struct StreamExtra {
 template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT>
 static void bulk2d_shape(Domain& d, const S* i, S* o, const StreamExtra* x) {
   const int tx = (d.nx + TX - 1) / TX, ty = (d.ny + TY * CPT - 1) / (TY * CPT);
+  const int nt = tx * ty;
   if (!x) {
-    k_bulk2d<S, STAGE, PERIODIC, TX, TY, CPT><<<dim3(tx, ty), 256, 0, d.stream>>>(i, o, d.flags, d.nx, d.ny, d.plane,
-                                                                                  d.omega);
+    k_bulk2d<S, STAGE, PERIODIC, TX, TY, CPT><<<xcd_grid(nt), 256, 0, d.stream>>>(i, o, d.flags, d.nx, d.ny, d.plane,
+                                                                                  d.omega, tx, nt);
   } else {
-    const unsigned grid = (unsigned)(tx * ty + x->nbc + x->nwave);
-    k_step2d<S, STAGE, PERIODIC, TX, TY, CPT><<<grid, 256, 0, d.stream>>>(i, o, tx, x->nbc, x->nwave, *x->la, *x->fa);
+    const int nbc = xcd_grid(x->nbc + x->nwave) - x->nwave;  // extra boundary-cell blocks share the grid stride
+    const unsigned grid = (unsigned)(nbc + x->nwave + xcd_grid(nt));
+    k_step2d<S, STAGE, PERIODIC, TX, TY, CPT><<<grid, 256, 0, d.stream>>>(i, o, tx, nt, nbc, x->nwave, *x->la, *x->fa);
   }
 }
 

@@ -13,29 +13,41 @@
 
 namespace ff {
 
-template <typename S, bool PERIODIC>
-__global__ void __launch_bounds__(kTX3* kTY3* kTZ3)
-    k_bulk3d(const S* __restrict__ in, S* __restrict__ out, const uint8_t* __restrict__ flags, int nx, int ny,
-             int nz, size_t plane, float omega) {
-  bulk3d_tile<S, PERIODIC>(in, out, flags, nx, ny, nz, plane, omega, blockIdx.x, blockIdx.y, blockIdx.z);
+// One tile per block (XCD-aware order): RING = 0 boxes of TX x TY x ZC cells,
+// otherwise TX x TY columns marching ZC planes; NT = block size.
+template <typename S, bool PERIODIC, int RING, int TX, int TY, int ZC, int NT>
+__device__ __forceinline__ void tile3d(const S* in, S* out, const uint8_t* flags, int nx, int ny, int nz, size_t plane,
+                                       float omega, int t, int tiles_x, int tiles_y) {
+  const int bx = t % tiles_x, by = (t / tiles_x) % tiles_y, bz = t / (tiles_x * tiles_y);
+  if constexpr (RING) bulk3d_ring<S, PERIODIC, TX, TY, ZC>(in, out, flags, nx, ny, nz, plane, omega, bx, by, bz);
+  else bulk3d_tile<S, PERIODIC, TX, TY, ZC>(in, out, flags, nx, ny, nz, plane, omega, bx, by, bz);
 }
 
-template <typename S, bool PERIODIC, int TX, int TY, int ZC>
-__global__ void __launch_bounds__(TX* TY)
-    k_bulk3d_ring(const S* __restrict__ in, S* __restrict__ out, const uint8_t* __restrict__ flags, int nx, int ny,
-                  int nz, size_t plane, float omega) {
-  bulk3d_ring<S, PERIODIC, TX, TY, ZC>(in, out, flags, nx, ny, nz, plane, omega, blockIdx.x, blockIdx.y, blockIdx.z);
+template <typename S, bool PERIODIC, int RING, int TX, int TY, int ZC, int NT>
+__global__ void __launch_bounds__(NT)
+    k_bulk3d(const S* __restrict__ in, S* __restrict__ out, const uint8_t* __restrict__ flags, int nx, int ny,
+             int nz, size_t plane, float omega, int tiles_x, int tiles_y, int ntiles) {
+  const int t = xcd_tile(blockIdx.x, ntiles);
+  if (t < 0) return;
+  tile3d<S, PERIODIC, RING, TX, TY, ZC, NT>(in, out, flags, nx, ny, nz, plane, omega, t, tiles_x, tiles_y);
+}
+
+template <typename S, bool PERIODIC, int RING, int TX, int TY, int ZC, int NT>
+static void bulk3d_shape(Domain& d, const S* i, S* o) {
+  const int tx = (d.nx + TX - 1) / TX, ty = (d.ny + TY - 1) / TY, tz = (d.nz + ZC - 1) / ZC;
+  const int nt = tx * ty * tz;
+  k_bulk3d<S, PERIODIC, RING, TX, TY, ZC, NT><<<xcd_grid(nt), NT, 0, d.stream>>>(i, o, d.flags, d.nx, d.ny, d.nz,
+                                                                                 d.plane, d.omega, tx, ty, nt);
 }
 
 template <typename S, bool PERIODIC>
 static void bulk3d_variant(Domain& d, const S* i, S* o) {
-  const Tile3 t = kTiles3[d.tile3];
-  dim3 grid((d.nx + t.tx - 1) / t.tx, (d.ny + t.ty - 1) / t.ty, (d.nz + t.tz - 1) / t.tz);
   switch (d.tile3) {
-    case 1: k_bulk3d_ring<S, PERIODIC, 16, 8, 32><<<grid, 128, 0, d.stream>>>(i, o, d.flags, d.nx, d.ny, d.nz, d.plane, d.omega); break;
-    case 2: k_bulk3d_ring<S, PERIODIC, 8, 8, 32><<<grid, 64, 0, d.stream>>>(i, o, d.flags, d.nx, d.ny, d.nz, d.plane, d.omega); break;
-    case 3: k_bulk3d_ring<S, PERIODIC, 16, 16, 32><<<grid, 256, 0, d.stream>>>(i, o, d.flags, d.nx, d.ny, d.nz, d.plane, d.omega); break;
-    default: k_bulk3d<S, PERIODIC><<<grid, kTX3 * kTY3 * kTZ3, 0, d.stream>>>(i, o, d.flags, d.nx, d.ny, d.nz, d.plane, d.omega); break;
+    case 1: bulk3d_shape<S, PERIODIC, 0, 16, 4, 4, 256>(d, i, o); break;
+    case 2: bulk3d_shape<S, PERIODIC, 1, 16, 8, 32, 128>(d, i, o); break;
+    case 3: bulk3d_shape<S, PERIODIC, 1, 8, 8, 32, 64>(d, i, o); break;
+    case 4: bulk3d_shape<S, PERIODIC, 1, 16, 16, 32, 256>(d, i, o); break;
+    default: bulk3d_shape<S, PERIODIC, 0, 8, 8, 4, 256>(d, i, o); break;
   }
 }
 
@@ -43,38 +55,38 @@ static void bulk3d_variant(Domain& d, const S* i, S* o) {
 // blocks, wave blocks, then the tiles of the chosen shape; NT = block size.
 template <typename S, bool PERIODIC, int RING, int TX, int TY, int ZC, int NT>
 __global__ void __launch_bounds__(NT)
-    k_step3d(const S* __restrict__ in, S* __restrict__ out, int tiles_x, int tiles_y, int nbc, int nwave, LinkArgs la,
-             FinalArgs fa) {
+    k_step3d(const S* __restrict__ in, S* __restrict__ out, int tiles_x, int tiles_y, int ntiles, int nbc, int nwave,
+             LinkArgs la, FinalArgs fa) {
   const int b = blockIdx.x;
   if (b < nbc) {
     bcells_body<3, S, false, PERIODIC, NT>(in, la, b, nbc);
   } else if (b < nbc + nwave) {
     wave_block_body<3, NT>(la, fa, b - nbc);
   } else {
-    const int t = b - nbc - nwave;
-    const int bx = t % tiles_x, by = (t / tiles_x) % tiles_y, bz = t / (tiles_x * tiles_y);
-    if constexpr (RING)
-      bulk3d_ring<S, PERIODIC, TX, TY, ZC>(in, out, la.flags, la.nx, la.ny, la.nz, la.plane, la.omega, bx, by, bz);
-    else
-      bulk3d_tile<S, PERIODIC>(in, out, la.flags, la.nx, la.ny, la.nz, la.plane, la.omega, bx, by, bz);
+    const int t = xcd_tile(b - nbc - nwave, ntiles);
+    if (t < 0) return;
+    tile3d<S, PERIODIC, RING, TX, TY, ZC, NT>(in, out, la.flags, la.nx, la.ny, la.nz, la.plane, la.omega, t, tiles_x,
+                                              tiles_y);
   }
 }
 
 template <typename S, bool PERIODIC, int RING, int TX, int TY, int ZC, int NT>
 static void step3d_shape(Domain& d, const S* i, S* o, const LinkArgs& la, const FinalArgs& fa, int nwave) {
   const int tx = (d.nx + TX - 1) / TX, ty = (d.ny + TY - 1) / TY, tz = (d.nz + ZC - 1) / ZC;
-  const int nbc = boundary_cell_blocks(d, NT / 32);
+  const int nt = tx * ty * tz;
   const int nw = nwave > 0 ? (d.nv + NT - 1) / NT : 0;
-  const unsigned grid = (unsigned)(tx * ty * tz + nbc + nw);
-  k_step3d<S, PERIODIC, RING, TX, TY, ZC, NT><<<grid, NT, 0, d.stream>>>(i, o, tx, ty, nbc, nw, la, fa);
+  const int nbc = xcd_grid(boundary_cell_blocks(d, NT / 32) + nw) - nw;
+  const unsigned grid = (unsigned)(nbc + nw + xcd_grid(nt));
+  k_step3d<S, PERIODIC, RING, TX, TY, ZC, NT><<<grid, NT, 0, d.stream>>>(i, o, tx, ty, nt, nbc, nw, la, fa);
 }
 
 template <typename S, bool PERIODIC>
 static void step3d_variant(Domain& d, const S* i, S* o, const LinkArgs& la, const FinalArgs& fa, int nwave) {
   switch (d.tile3) {
-    case 1: step3d_shape<S, PERIODIC, 1, 16, 8, 32, 128>(d, i, o, la, fa, nwave); break;
-    case 2: step3d_shape<S, PERIODIC, 1, 8, 8, 32, 64>(d, i, o, la, fa, nwave); break;
-    case 3: step3d_shape<S, PERIODIC, 1, 16, 16, 32, 256>(d, i, o, la, fa, nwave); break;
+    case 1: step3d_shape<S, PERIODIC, 0, 16, 4, 4, 256>(d, i, o, la, fa, nwave); break;
+    case 2: step3d_shape<S, PERIODIC, 1, 16, 8, 32, 128>(d, i, o, la, fa, nwave); break;
+    case 3: step3d_shape<S, PERIODIC, 1, 8, 8, 32, 64>(d, i, o, la, fa, nwave); break;
+    case 4: step3d_shape<S, PERIODIC, 1, 16, 16, 32, 256>(d, i, o, la, fa, nwave); break;
     default: step3d_shape<S, PERIODIC, 0, 8, 8, 4, 256>(d, i, o, la, fa, nwave); break;
   }
 }

@@ -13,6 +13,20 @@
 
 namespace ff {
 
+// XCD-aware tile order.  Workgroups are dispatched to the 8 XCDs round-robin
+// by block index and every XCD has its own L2, so block p runs tile
+// (p % 8) * ceil(n / 8) + p / 8: each XCD works through one contiguous run of
+// tiles (whole rows), and the halo rows and columns its tiles share hit in
+// its L2 instead of being fetched once per XCD.  Grids are rounded up to a
+// multiple of 8 blocks; -1: no tile for this block.
+constexpr int kXcds = 8;
+__host__ __device__ inline int xcd_grid(int n) { return (n + kXcds - 1) / kXcds * kXcds; }
+__device__ __forceinline__ int xcd_tile(int p, int n) {
+  const int per = (n + kXcds - 1) / kXcds;
+  const int t = (p % kXcds) * per + p / kXcds;
+  return t < n ? t : -1;
+}
+
 // 2D tile shapes: TX x (TY * CPT) cells per 256-thread block, each thread
 // updating CPT cells of one column (rows ly, ly + TY, ...).  Taller tiles
 // re-read and re-reconstruct less halo per cell at the price of LDS.
@@ -96,15 +110,15 @@ __device__ __forceinline__ void bulk2d_tile(const S* __restrict__ in, S* __restr
   }
 }
 
-constexpr int kTX3 = 8, kTY3 = 8, kTZ3 = 4;
-
-// 3D tile of 8x8x4 cells with a 10x10x6 halo in LDS (FP32 as the reference's
-// tile, FsiCoupling3D.cu:94).  The reference keeps the flag bounce-back value
-// in 3D (FsiCoupling3D.cu:209-222), so upstream flags are staged too.
-template <typename S, bool PERIODIC>
+// 3D box tile of TX x TY x TZ cells (one per thread) with a one-cell halo in
+// LDS (FP32 as the reference's tile, FsiCoupling3D.cu:94).  The reference
+// keeps the flag bounce-back value in 3D (FsiCoupling3D.cu:209-222), so
+// upstream flags are staged too.
+template <typename S, bool PERIODIC, int TX, int TY, int TZ>
 __device__ __forceinline__ void bulk3d_tile(const S* __restrict__ in, S* __restrict__ out,
                                             const uint8_t* __restrict__ flags, int nx, int ny, int nz, size_t plane,
                                             float omega, int bx, int by, int bz) {
+  constexpr int kTX3 = TX, kTY3 = TY, kTZ3 = TZ;
   constexpr int HX = kTX3 + 2, HY = kTY3 + 2, HZ = kTZ3 + 2, HN = HX * HY * HZ;
   constexpr int NT = kTX3 * kTY3 * kTZ3;
   constexpr int PER = (HN + NT - 1) / NT;  // halo cells per thread
@@ -177,8 +191,8 @@ struct Tile3 {
   int tx, ty, tz;  // tz: planes per block (box variant: 4, ring: ZC)
   bool ring;
 };
-constexpr Tile3 kTiles3[] = {{8, 8, 4, false}, {16, 8, 32, true}, {8, 8, 32, true}, {16, 16, 32, true}};
-constexpr int kNumTiles3 = 4;
+constexpr Tile3 kTiles3[] = {{8, 8, 4, false}, {16, 4, 4, false}, {16, 8, 32, true}, {8, 8, 32, true}, {16, 16, 32, true}};
+constexpr int kNumTiles3 = 5;
 
 template <typename S, bool PERIODIC, int TX, int TY, int ZC>
 __device__ __forceinline__ void bulk3d_ring(const S* __restrict__ in, S* __restrict__ out,
