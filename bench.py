@@ -97,6 +97,37 @@ def cpu_baseline_config2(state, budget_s=12.0):
                       f"same 1200x400 swimmer workload, first {steps} steps ({el:.1f} s)"}
 
 
+# (tx, ty, cells per thread) of the 2D tiles and (tx, ty, planes, ring, threads)
+# of the 3D tiles, by tile index (ff_set_tile)
+SHAPES2 = [(32, 8, 1), (32, 8, 2), (64, 4, 2), (32, 8, 4), (64, 4, 4)]
+SHAPES3 = [(8, 8, 4, 0, 256), (16, 4, 4, 0, 256), (16, 8, 32, 1, 128), (8, 8, 32, 1, 64), (16, 16, 32, 1, 256)]
+
+
+def kernel2d(prefix, s, stage, periodic, tile):
+    tx, ty, cpt = SHAPES2[tile]
+    return f"ff::{prefix}<{s}, {str(stage).lower()}, {str(periodic).lower()}, {tx}, {ty}, {cpt}>"
+
+
+def kernel3d(s, periodic, tile):
+    tx, ty, tz, ring, nt = SHAPES3[tile]
+    return f"ff::k_bulk3d<{s}, {str(periodic).lower()}, {ring}, {tx}, {ty}, {tz}, {nt}>"
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE measurement (profiles/traffic.json, counters
+    scaled by a 256 MiB copy), or None when that kernel was not measured."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            t = json.load(f)
+    except OSError:
+        return None
+    for k in t.get("kernels", []):
+        if k["kernel"] == kernel:
+            return k["traffic_bytes"]
+    return None
+
+
 def pick_tile(d, args):
     if args.tile >= 0:
         d.set_tile(args.tile)
@@ -106,7 +137,7 @@ def pick_tile(d, args):
 
 def roofline(achieved_gbs, kernel, avg_ms, bpl, cells):
     return {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
+            "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kernel), "kernel": kernel,
             "avg_kernel_us": round(avg_ms * 1e3, 3), "bytes_per_lup": bpl, "cells_per_launch": cells}
 
 
@@ -167,7 +198,7 @@ def run_batched(args, capi, wl):
     tile, tx, ty = b.tile()
     avg_ms = b.time_bulk_kernel(100)
     bpl = bytes_per_lup(2, 2)
-    roof = roofline(bpl * cells / (avg_ms * 1e-3) / 1e9, f"kb_bulk2d<__half> tile {tx}x{ty} (64 domains)", avg_ms,
+    roof = roofline(bpl * cells / (avg_ms * 1e-3) / 1e9, kernel2d("kb_step2d", "__half", False, False, tile), avg_ms,
                     bpl, cells)
     wr = np.concatenate([d.body_wrench() for d in doms])
     out = base_line(mlups, K, W, el * 1e3 / K, "f32 compute, f16 storage",
@@ -202,7 +233,8 @@ def run_config3(args, capi, wl):
     drift = abs(d.mass() - m0) / m0
     avg_ms = d.time_bulk_kernel(50)
     bpl = bytes_per_lup(3, 4 if st == "f32" else 2)
-    roof = roofline(bpl * cells / (avg_ms * 1e-3) / 1e9, f"k_bulk3d<{st}> tile {TILES3[tile]}", avg_ms, bpl, cells)
+    sname = {"f32": "float", "f16": "__half", "bf16": "__hip_bfloat16"}[st]
+    roof = roofline(bpl * cells / (avg_ms * 1e-3) / 1e9, kernel3d(sname, True, tile), avg_ms, bpl, cells)
     out = base_line(cells * K / el / 1e6, K, W, el * 1e3 / K, f"f32 compute, {st} storage",
                     {"workload": "3D D3Q27 256x128x128 Taylor-Green (amplitude 0.04) + vortex ring r=20, periodic",
                      "nx": nx, "ny": ny, "nz": nz, "storage": st, "tile": TILES3[tile], "parallelism": "single-gpu"},
@@ -310,9 +342,10 @@ def main():
     avg_ms = d.time_bulk_kernel(iters)
     bpl = bytes_per_lup(2, 4)
     achieved = bpl * cells / (avg_ms * 1e-3) / 1e9
+    kname = kernel2d("k_step2d", "float", True, False, tile)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": f"k_bulk2d_links<float,stage_fp16> tile {TILES2[tile]} (stream-collide tiles + link detection blocks)",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kname),
+            "kernel": kname + " (stream-collide tiles + boundary cells + next-step kinematics)",
             "avg_kernel_us": round(avg_ms * 1e3, 3),
             "bytes_per_lup": bpl, "cells_per_launch": cells}
     cpu = None if args.no_cpu else cpu_baseline_config2(state, args.cpu_budget)
