@@ -145,7 +145,7 @@ __global__ void __launch_bounds__(FB) kb_finish(const DomDev* __restrict__ doms,
 static int stream_threads(int dim, int tile) {
   if (dim == 2) return 256;
   const Tile3 t = kTiles3[tile];
-  return t.ring ? t.tx * t.ty : 256;
+  return t.ring ? t.tx * t.ty : t.tx * t.ty * t.tz;
 }
 
 template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT>
@@ -169,8 +169,9 @@ static void step3d_batch_variant(Batch& b, int tile, const int2* map, int nblock
   switch (tile) {
     case 1: kb_step3d<S, PERIODIC, 0, 16, 4, 4, 256><<<nblocks, 256, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
     case 2: kb_step3d<S, PERIODIC, 1, 16, 8, 32, 128><<<nblocks, 128, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
-    case 3: kb_step3d<S, PERIODIC, 1, 8, 8, 32, 64><<<nblocks, 64, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
-    case 4: kb_step3d<S, PERIODIC, 1, 16, 16, 32, 256><<<nblocks, 256, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
+    case 3: kb_step3d<S, PERIODIC, 1, 16, 8, 16, 128><<<nblocks, 128, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
+    case 4: kb_step3d<S, PERIODIC, 1, 16, 8, 8, 128><<<nblocks, 128, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
+    case 5: kb_step3d<S, PERIODIC, 0, 8, 8, 8, 512><<<nblocks, 512, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
     default: kb_step3d<S, PERIODIC, 0, 8, 8, 4, 256><<<nblocks, 256, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
   }
 }

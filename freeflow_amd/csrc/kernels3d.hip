@@ -45,8 +45,9 @@ static void bulk3d_variant(Domain& d, const S* i, S* o) {
   switch (d.tile3) {
     case 1: bulk3d_shape<S, PERIODIC, 0, 16, 4, 4, 256>(d, i, o); break;
     case 2: bulk3d_shape<S, PERIODIC, 1, 16, 8, 32, 128>(d, i, o); break;
-    case 3: bulk3d_shape<S, PERIODIC, 1, 8, 8, 32, 64>(d, i, o); break;
-    case 4: bulk3d_shape<S, PERIODIC, 1, 16, 16, 32, 256>(d, i, o); break;
+    case 3: bulk3d_shape<S, PERIODIC, 1, 16, 8, 16, 128>(d, i, o); break;
+    case 4: bulk3d_shape<S, PERIODIC, 1, 16, 8, 8, 128>(d, i, o); break;
+    case 5: bulk3d_shape<S, PERIODIC, 0, 8, 8, 8, 512>(d, i, o); break;
     default: bulk3d_shape<S, PERIODIC, 0, 8, 8, 4, 256>(d, i, o); break;
   }
 }
@@ -85,8 +86,9 @@ static void step3d_variant(Domain& d, const S* i, S* o, const LinkArgs& la, cons
   switch (d.tile3) {
     case 1: step3d_shape<S, PERIODIC, 0, 16, 4, 4, 256>(d, i, o, la, fa, nwave); break;
     case 2: step3d_shape<S, PERIODIC, 1, 16, 8, 32, 128>(d, i, o, la, fa, nwave); break;
-    case 3: step3d_shape<S, PERIODIC, 1, 8, 8, 32, 64>(d, i, o, la, fa, nwave); break;
-    case 4: step3d_shape<S, PERIODIC, 1, 16, 16, 32, 256>(d, i, o, la, fa, nwave); break;
+    case 3: step3d_shape<S, PERIODIC, 1, 16, 8, 16, 128>(d, i, o, la, fa, nwave); break;
+    case 4: step3d_shape<S, PERIODIC, 1, 16, 8, 8, 128>(d, i, o, la, fa, nwave); break;
+    case 5: step3d_shape<S, PERIODIC, 0, 8, 8, 8, 512>(d, i, o, la, fa, nwave); break;
     default: step3d_shape<S, PERIODIC, 0, 8, 8, 4, 256>(d, i, o, la, fa, nwave); break;
   }
 }

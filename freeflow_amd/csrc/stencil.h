@@ -122,8 +122,10 @@ __device__ __forceinline__ void bulk3d_tile(const S* __restrict__ in, S* __restr
   constexpr int HX = kTX3 + 2, HY = kTY3 + 2, HZ = kTZ3 + 2, HN = HX * HY * HZ;
   constexpr int NT = kTX3 * kTY3 * kTZ3;
   constexpr int PER = (HN + NT - 1) / NT;  // halo cells per thread
+  constexpr int NW = (NT + 63) / 64;
   __shared__ float sp[27][HN];
   __shared__ uint8_t sf[HN];
+  __shared__ int swall[NW];  // per wave: a staged cell is a bounce-back wall
   const int x0 = bx * kTX3, y0 = by * kTY3, z0 = bz * kTZ3;
   // all loads first, then the reconstructions (one wait for memory)
   float m[PER][10];
@@ -149,6 +151,7 @@ __device__ __forceinline__ void bulk3d_tile(const S* __restrict__ in, S* __restr
       fl[r] = flags[((size_t)wz * ny + wy) * nx + wx];
     }
   }
+  bool wall = false;
 #pragma unroll
   for (int r = 0; r < PER; r++) {
     const int k = threadIdx.x + r * NT;
@@ -158,20 +161,33 @@ __device__ __forceinline__ void bulk3d_tile(const S* __restrict__ in, S* __restr
 #pragma unroll
       for (int i = 0; i < 27; i++) sp[i][k] = f[i];
       sf[k] = fl[r];
+      wall = wall || is_wall3(fl[r]);
     }
   }
+  const unsigned long long vote = __ballot(wall);
+  if ((threadIdx.x & 63) == 0) swall[threadIdx.x >> 6] = vote != 0ull;
   __syncthreads();
+  // without a wall anywhere in the tile and halo every population is pulled,
+  // and the upstream flags need not be read (a block-uniform branch)
+  int walls = 0;
+#pragma unroll
+  for (int w = 0; w < NW; w++) walls |= swall[w];
   const int lx = threadIdx.x % kTX3, ly = (threadIdx.x / kTX3) % kTY3, lz = threadIdx.x / (kTX3 * kTY3);
   const int x = x0 + lx, y = y0 + ly, z = z0 + lz;
   if (x >= nx || y >= ny || z >= nz) return;
   const int ctr = ((lz + 1) * HY + (ly + 1)) * HX + (lx + 1);
   if (!is_updated(sf[ctr])) return;
   float p[27];
+  if (walls) {
 #pragma unroll
-  for (int i = 0; i < 27; i++) {
-    const int up = ctr - (D3::cz(i) * HY + D3::cy(i)) * HX - D3::cx(i);
-    if (is_wall3(sf[up])) p[i] = fmaxf(0.0f, sp[D3::inv(i)][ctr]);
-    else p[i] = fmaxf(0.0f, sp[i][up]);
+    for (int i = 0; i < 27; i++) {
+      const int up = ctr - (D3::cz(i) * HY + D3::cy(i)) * HX - D3::cx(i);
+      if (is_wall3(sf[up])) p[i] = fmaxf(0.0f, sp[D3::inv(i)][ctr]);
+      else p[i] = fmaxf(0.0f, sp[i][up]);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 27; i++) p[i] = fmaxf(0.0f, sp[i][ctr - (D3::cz(i) * HY + D3::cy(i)) * HX - D3::cx(i)]);
   }
   float o[10];
   collide3(p, omega, o);
@@ -181,18 +197,33 @@ __device__ __forceinline__ void bulk3d_tile(const S* __restrict__ in, S* __restr
 }
 
 // 3D ring variant: a TX x TY column of cells per block marching along z over
-// ZC planes.  Three planes of reconstructed populations (z-1, z, z+1, each
+// ZC planes.  Reconstructed populations of three planes (z-1, z, z+1, each
 // with a one-cell xy halo) rotate through LDS, so each halo cell is loaded
-// and reconstructed once per column instead of once per 8x8x4 box (1.41x
-// instead of 2.34x for 16x8).  The next plane's moments are fetched into
-// registers before the current plane is computed, hiding their latency.
-// Arithmetic per cell is that of bulk3d_tile.
+// and reconstructed once per column instead of once per box (1.41x instead
+// of 2.53x for 16x8).  The 27 directions fall into three classes of nine by
+// c_z, and each class is kept only as long as a pull or the centre cell's
+// own bounce-back needs it (plane z+1 is written before plane z is computed):
+//   c_z = +1 (pulled from the plane below): planes z-1, z, z+1 -> 3 slots
+//   c_z =  0 (pulled within the plane):      planes z, z+1      -> 2 slots
+//   c_z = -1 (pulled from the plane above):  planes z, z+1      -> 2 slots
+// 63 instead of 81 population planes in LDS (16x8: 46 KB, three blocks per
+// CU instead of two).  The next plane's moments are fetched into registers
+// before the current plane is computed, hiding their latency.  Arithmetic
+// per cell is that of bulk3d_tile.
 struct Tile3 {
   int tx, ty, tz;  // tz: planes per block (box variant: 4, ring: ZC)
   bool ring;
 };
-constexpr Tile3 kTiles3[] = {{8, 8, 4, false}, {16, 4, 4, false}, {16, 8, 32, true}, {8, 8, 32, true}, {16, 16, 32, true}};
-constexpr int kNumTiles3 = 5;
+constexpr Tile3 kTiles3[] = {{8, 8, 4, false}, {16, 4, 4, false}, {16, 8, 32, true},
+                             {16, 8, 16, true}, {16, 8, 8, true},  {8, 8, 8, false}};
+constexpr int kNumTiles3 = 6;
+
+// position of direction i among the nine directions sharing its c_z
+__host__ __device__ constexpr int zclass_index(int i) {
+  int n = 0;
+  for (int j = 0; j < i; j++) n += D3::cz(j) == D3::cz(i) ? 1 : 0;
+  return n;
+}
 
 template <typename S, bool PERIODIC, int TX, int TY, int ZC>
 __device__ __forceinline__ void bulk3d_ring(const S* __restrict__ in, S* __restrict__ out,
@@ -200,8 +231,13 @@ __device__ __forceinline__ void bulk3d_ring(const S* __restrict__ in, S* __restr
                                             float omega, int bx, int by, int bz) {
   constexpr int NT = TX * TY, HX = TX + 2, HY = TY + 2, HA = HX * HY;
   constexpr int PER = (HA + NT - 1) / NT;  // halo cells per thread per plane
-  __shared__ float sp[3][27][HA];
-  __shared__ uint8_t sf[3][HA];
+  __shared__ float sup[3][9][HA];  // c_z = +1 class, slot (rel % 3)
+  __shared__ float sin_[2][9][HA];  // c_z = 0 class, slot (rel % 2)
+  __shared__ float sdn[2][9][HA];  // c_z = -1 class, slot (rel % 2)
+  __shared__ uint8_t sf[3][HA];    // flags, slot (rel % 3)
+  constexpr int NW = (NT + 63) / 64;
+  __shared__ int swall[3][NW];     // per plane slot and wave: a staged cell is a wall
+  // rel = z - z0 + 1 numbers the planes from the lower halo plane
   const int x0 = bx * TX, y0 = by * TY, z0 = bz * ZC;
   const int zend = min(z0 + ZC, nz);
   float mreg[PER][10];
@@ -230,8 +266,11 @@ __device__ __forceinline__ void bulk3d_ring(const S* __restrict__ in, S* __restr
       }
     }
   };
-  // reconstruct the fetched plane into ring slot `slot`
-  auto put = [&](int slot) {
+  // reconstruct the fetched plane (number rel) into its class slots; the
+  // lower halo plane (rel 0) only ever serves c_z = +1 pulls
+  auto put = [&](int rel, bool only_up) {
+    const int s3 = rel % 3, s2 = rel % 2;
+    bool wall = false;
 #pragma unroll
     for (int r = 0; r < PER; r++) {
       const int k = threadIdx.x + r * NT;
@@ -239,35 +278,63 @@ __device__ __forceinline__ void bulk3d_ring(const S* __restrict__ in, S* __restr
         float f[27];
         recon3_all(mreg[r], f);
 #pragma unroll
-        for (int i = 0; i < 27; i++) sp[slot][i][k] = f[i];
-        sf[slot][k] = freg[r];
+        for (int i = 0; i < 27; i++) {
+          const int cz = D3::cz(i), j = zclass_index(i);
+          if (cz > 0) sup[s3][j][k] = f[i];
+          else if (!only_up) {
+            if (cz == 0) sin_[s2][j][k] = f[i];
+            else sdn[s2][j][k] = f[i];
+          }
+        }
+        sf[s3][k] = freg[r];
+        wall = wall || is_wall3(freg[r]);
       }
     }
+    const unsigned long long vote = __ballot(wall);
+    if ((threadIdx.x & 63) == 0) swall[s3][threadIdx.x >> 6] = vote != 0ull;
   };
   const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
   const int x = x0 + lx, y = y0 + ly;
   const bool inside = x < nx && y < ny;
   const int ctr = (ly + 1) * HX + (lx + 1);
-  // slot of plane z: (z - z0 + 1) % 3, so z0-1 -> 0, z0 -> 1, z0+1 -> 2
   fetch(z0 - 1);
-  put(0);
+  put(0, true);
   fetch(z0);
-  put(1);
+  put(1, false);
   fetch(z0 + 1);
   for (int z = z0; z < zend; z++) {
-    const int s_m = (z - z0) % 3, s_c = (z - z0 + 1) % 3, s_p = (z - z0 + 2) % 3;
-    put(s_p);  // plane z+1 (fetched during the previous iteration)
+    const int rel = z - z0 + 1;
+    const int s_m = (rel - 1) % 3, s_c = rel % 3, s2 = rel % 2;
+    put(rel + 1, false);  // plane z+1 (fetched during the previous iteration)
     __syncthreads();
     if (z + 2 <= zend) fetch(z + 2);  // overlaps the compute below
+    // a wall in any of the three planes' staged cells (block-uniform)
+    int walls = 0;
+#pragma unroll
+    for (int w = 0; w < NW; w++) walls |= swall[0][w] | swall[1][w] | swall[2][w];
     if (inside && is_updated(sf[s_c][ctr])) {
       float p[27];
+      if (walls) {
 #pragma unroll
-      for (int i = 0; i < 27; i++) {
-        const int cz = D3::cz(i);
-        const int sl = cz > 0 ? s_m : (cz < 0 ? s_p : s_c);
-        const int up = ctr - D3::cy(i) * HX - D3::cx(i);
-        if (is_wall3(sf[sl][up])) p[i] = fmaxf(0.0f, sp[s_c][D3::inv(i)][ctr]);
-        else p[i] = fmaxf(0.0f, sp[sl][i][up]);
+        for (int i = 0; i < 27; i++) {
+          const int cz = D3::cz(i), j = zclass_index(i), up = ctr - D3::cy(i) * HX - D3::cx(i);
+          const int sl = cz > 0 ? s_m : (cz < 0 ? (rel + 1) % 3 : s_c);
+          const int ib = D3::inv(i), jb = zclass_index(ib);
+          float v;
+          if (is_wall3(sf[sl][up])) {
+            // bounce-back: the centre's own population in direction inv(i), plane z
+            v = cz > 0 ? sdn[s2][jb][ctr] : (cz < 0 ? sup[s_c][jb][ctr] : sin_[s2][jb][ctr]);
+          } else {
+            v = cz > 0 ? sup[s_m][j][up] : (cz < 0 ? sdn[s2 ^ 1][j][up] : sin_[s2][j][up]);
+          }
+          p[i] = fmaxf(0.0f, v);
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < 27; i++) {
+          const int cz = D3::cz(i), j = zclass_index(i), up = ctr - D3::cy(i) * HX - D3::cx(i);
+          p[i] = fmaxf(0.0f, cz > 0 ? sup[s_m][j][up] : (cz < 0 ? sdn[s2 ^ 1][j][up] : sin_[s2][j][up]));
+        }
       }
       float o[10];
       collide3(p, omega, o);
@@ -275,7 +342,7 @@ __device__ __forceinline__ void bulk3d_ring(const S* __restrict__ in, S* __restr
 #pragma unroll
       for (int q = 0; q < 10; q++) Store<S>::template store<3>(out + q * plane + cell, q, o[q]);
     }
-    __syncthreads();  // slot s_m is overwritten by the next iteration's put
+    __syncthreads();  // the slots of plane z-1 (c_z = +1) and plane z (c_z = 0, -1) are overwritten next
   }
 }
 
