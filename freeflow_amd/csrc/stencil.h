@@ -201,15 +201,17 @@ __device__ __forceinline__ void bulk3d_tile(const S* __restrict__ in, S* __restr
 // with a one-cell xy halo) rotate through LDS, so each halo cell is loaded
 // and reconstructed once per column instead of once per box (1.41x instead
 // of 2.53x for 16x8).  The 27 directions fall into three classes of nine by
-// c_z, and each class is kept only as long as a pull or the centre cell's
-// own bounce-back needs it (plane z+1 is written before plane z is computed):
-//   c_z = +1 (pulled from the plane below): planes z-1, z, z+1 -> 3 slots
-//   c_z =  0 (pulled within the plane):      planes z, z+1      -> 2 slots
-//   c_z = -1 (pulled from the plane above):  planes z, z+1      -> 2 slots
-// 63 instead of 81 population planes in LDS (16x8: 46 KB, three blocks per
-// CU instead of two).  The next plane's moments are fetched into registers
-// before the current plane is computed, hiding their latency.  Arithmetic
-// per cell is that of bulk3d_tile.
+// c_z, and each class is kept only while a pull or the centre cell's own
+// bounce-back needs it.  Plane z+1 is reconstructed before plane z is
+// computed; its c_z = -1 class goes to LDS at once (plane z pulls it), its
+// c_z = +1 and c_z = 0 classes wait in registers until plane z is done:
+//   c_z = +1 (pulled from the plane below): planes z-1, z -> 2 slots
+//   c_z =  0 (pulled within the plane):      plane z       -> 1 slot
+//   c_z = -1 (pulled from the plane above):  planes z, z+1 -> 2 slots
+// 45 instead of 81 population planes in LDS (16x8: 33 KB, four blocks per
+// CU).  The next plane's moments are fetched into registers before the
+// current plane is computed, hiding their latency.  Arithmetic per cell is
+// that of bulk3d_tile.
 struct Tile3 {
   int tx, ty, tz;  // tz: planes per block (box variant: 4, ring: ZC)
   bool ring;
@@ -231,17 +233,18 @@ __device__ __forceinline__ void bulk3d_ring(const S* __restrict__ in, S* __restr
                                             float omega, int bx, int by, int bz) {
   constexpr int NT = TX * TY, HX = TX + 2, HY = TY + 2, HA = HX * HY;
   constexpr int PER = (HA + NT - 1) / NT;  // halo cells per thread per plane
-  __shared__ float sup[3][9][HA];  // c_z = +1 class, slot (rel % 3)
-  __shared__ float sin_[2][9][HA];  // c_z = 0 class, slot (rel % 2)
-  __shared__ float sdn[2][9][HA];  // c_z = -1 class, slot (rel % 2)
-  __shared__ uint8_t sf[3][HA];    // flags, slot (rel % 3)
   constexpr int NW = (NT + 63) / 64;
-  __shared__ int swall[3][NW];     // per plane slot and wave: a staged cell is a wall
   // rel = z - z0 + 1 numbers the planes from the lower halo plane
+  __shared__ float sup[2][9][HA];  // c_z = +1 class, slot rel % 2
+  __shared__ float szc[9][HA];     // c_z = 0 class of the current plane
+  __shared__ float sdn[2][9][HA];  // c_z = -1 class, slot rel % 2
+  __shared__ uint8_t sf[3][HA];    // flags, slot rel % 3
+  __shared__ int swall[3][NW];     // per flag slot and wave: a staged cell is a wall
   const int x0 = bx * TX, y0 = by * TY, z0 = bz * ZC;
   const int zend = min(z0 + ZC, nz);
   float mreg[PER][10];
   uint8_t freg[PER];
+  float hold[PER][18];  // c_z = +1 (0..8) and c_z = 0 (9..17) of the plane in flight
   // fetch the moments and flags of plane gz (may be -1 or nz: halo) into registers
   auto fetch = [&](int gz) {
     const int wz = ((gz % nz) + nz) % nz;
@@ -266,9 +269,9 @@ __device__ __forceinline__ void bulk3d_ring(const S* __restrict__ in, S* __restr
       }
     }
   };
-  // reconstruct the fetched plane (number rel) into its class slots; the
-  // lower halo plane (rel 0) only ever serves c_z = +1 pulls
-  auto put = [&](int rel, bool only_up) {
+  // reconstruct the fetched plane (number rel): c_z = -1 class and flags to
+  // LDS, the other two classes to `hold`
+  auto put_dn = [&](int rel) {
     const int s3 = rel % 3, s2 = rel % 2;
     bool wall = false;
 #pragma unroll
@@ -280,11 +283,8 @@ __device__ __forceinline__ void bulk3d_ring(const S* __restrict__ in, S* __restr
 #pragma unroll
         for (int i = 0; i < 27; i++) {
           const int cz = D3::cz(i), j = zclass_index(i);
-          if (cz > 0) sup[s3][j][k] = f[i];
-          else if (!only_up) {
-            if (cz == 0) sin_[s2][j][k] = f[i];
-            else sdn[s2][j][k] = f[i];
-          }
+          if (cz < 0) sdn[s2][j][k] = f[i];
+          else hold[r][cz > 0 ? j : 9 + j] = f[i];
         }
         sf[s3][k] = freg[r];
         wall = wall || is_wall3(freg[r]);
@@ -293,19 +293,36 @@ __device__ __forceinline__ void bulk3d_ring(const S* __restrict__ in, S* __restr
     const unsigned long long vote = __ballot(wall);
     if ((threadIdx.x & 63) == 0) swall[s3][threadIdx.x >> 6] = vote != 0ull;
   };
+  // the held classes of plane rel to LDS
+  auto put_rest = [&](int rel) {
+    const int s2 = rel % 2;
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+      const int k = threadIdx.x + r * NT;
+      if (k < HA) {
+#pragma unroll
+        for (int j = 0; j < 9; j++) {
+          sup[s2][j][k] = hold[r][j];
+          szc[j][k] = hold[r][9 + j];
+        }
+      }
+    }
+  };
   const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
   const int x = x0 + lx, y = y0 + ly;
   const bool inside = x < nx && y < ny;
   const int ctr = (ly + 1) * HX + (lx + 1);
   fetch(z0 - 1);
-  put(0, true);
+  put_dn(0);
+  put_rest(0);
   fetch(z0);
-  put(1, false);
+  put_dn(1);
+  put_rest(1);
   fetch(z0 + 1);
   for (int z = z0; z < zend; z++) {
     const int rel = z - z0 + 1;
-    const int s_m = (rel - 1) % 3, s_c = rel % 3, s2 = rel % 2;
-    put(rel + 1, false);  // plane z+1 (fetched during the previous iteration)
+    const int s_m = (rel - 1) % 3, s_c = rel % 3, s_p = (rel + 1) % 3, s2 = rel % 2;
+    put_dn(rel + 1);  // plane z+1 (fetched during the previous iteration)
     __syncthreads();
     if (z + 2 <= zend) fetch(z + 2);  // overlaps the compute below
     // a wall in any of the three planes' staged cells (block-uniform)
@@ -318,14 +335,14 @@ __device__ __forceinline__ void bulk3d_ring(const S* __restrict__ in, S* __restr
 #pragma unroll
         for (int i = 0; i < 27; i++) {
           const int cz = D3::cz(i), j = zclass_index(i), up = ctr - D3::cy(i) * HX - D3::cx(i);
-          const int sl = cz > 0 ? s_m : (cz < 0 ? (rel + 1) % 3 : s_c);
+          const int sl = cz > 0 ? s_m : (cz < 0 ? s_p : s_c);
           const int ib = D3::inv(i), jb = zclass_index(ib);
           float v;
           if (is_wall3(sf[sl][up])) {
             // bounce-back: the centre's own population in direction inv(i), plane z
-            v = cz > 0 ? sdn[s2][jb][ctr] : (cz < 0 ? sup[s_c][jb][ctr] : sin_[s2][jb][ctr]);
+            v = cz > 0 ? sdn[s2][jb][ctr] : (cz < 0 ? sup[s2][jb][ctr] : szc[jb][ctr]);
           } else {
-            v = cz > 0 ? sup[s_m][j][up] : (cz < 0 ? sdn[s2 ^ 1][j][up] : sin_[s2][j][up]);
+            v = cz > 0 ? sup[s2 ^ 1][j][up] : (cz < 0 ? sdn[s2 ^ 1][j][up] : szc[j][up]);
           }
           p[i] = fmaxf(0.0f, v);
         }
@@ -333,7 +350,7 @@ __device__ __forceinline__ void bulk3d_ring(const S* __restrict__ in, S* __restr
 #pragma unroll
         for (int i = 0; i < 27; i++) {
           const int cz = D3::cz(i), j = zclass_index(i), up = ctr - D3::cy(i) * HX - D3::cx(i);
-          p[i] = fmaxf(0.0f, cz > 0 ? sup[s_m][j][up] : (cz < 0 ? sdn[s2 ^ 1][j][up] : sin_[s2][j][up]));
+          p[i] = fmaxf(0.0f, cz > 0 ? sup[s2 ^ 1][j][up] : (cz < 0 ? sdn[s2 ^ 1][j][up] : szc[j][up]));
         }
       }
       float o[10];
@@ -342,7 +359,8 @@ __device__ __forceinline__ void bulk3d_ring(const S* __restrict__ in, S* __restr
 #pragma unroll
       for (int q = 0; q < 10; q++) Store<S>::template store<3>(out + q * plane + cell, q, o[q]);
     }
-    __syncthreads();  // the slots of plane z-1 (c_z = +1) and plane z (c_z = 0, -1) are overwritten next
+    __syncthreads();  // plane z-1's c_z = +1 slot and plane z's c_z = 0 slot are free
+    put_rest(rel + 1);
   }
 }
 
