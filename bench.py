@@ -42,7 +42,8 @@ METRIC = "MLUPS + % HBM peak; batched/served vs plain at ≤1e-3 rel err; p99 st
 HBM_PEAK_GBS = 8000.0
 # tile shapes by index, as in the library (ff_set_tile)
 TILES2 = ["32x8", "32x16", "64x8", "32x32", "64x16"]
-TILES3 = ["8x8x4 box", "16x4x4 box", "16x8 column x32", "16x8 column x16", "16x8 column x8", "8x8x8 box"]
+TILES3 = ["8x8x4 box", "16x4x4 box", "16x8 column x32", "16x8 column x16", "16x8 column x8", "16x16 column x32",
+          "16x16 column x16"]
 
 
 def bytes_per_lup(dim, storage_bytes):
@@ -101,7 +102,7 @@ def cpu_baseline_config2(state, budget_s=12.0):
 # of the 3D tiles, by tile index (ff_set_tile)
 SHAPES2 = [(32, 8, 1), (32, 8, 2), (64, 4, 2), (32, 8, 4), (64, 4, 4)]
 SHAPES3 = [(8, 8, 4, 0, 256), (16, 4, 4, 0, 256), (16, 8, 32, 1, 128), (16, 8, 16, 1, 128), (16, 8, 8, 1, 128),
-           (8, 8, 8, 0, 512)]
+           (16, 16, 32, 1, 256), (16, 16, 16, 1, 256)]
 
 
 def kernel2d(prefix, s, stage, periodic, tile):

@@ -171,7 +171,8 @@ static void step3d_batch_variant(Batch& b, int tile, const int2* map, int nblock
     case 2: kb_step3d<S, PERIODIC, 1, 16, 8, 32, 128><<<nblocks, 128, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
     case 3: kb_step3d<S, PERIODIC, 1, 16, 8, 16, 128><<<nblocks, 128, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
     case 4: kb_step3d<S, PERIODIC, 1, 16, 8, 8, 128><<<nblocks, 128, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
-    case 5: kb_step3d<S, PERIODIC, 0, 8, 8, 8, 512><<<nblocks, 512, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
+    case 5: kb_step3d<S, PERIODIC, 1, 16, 16, 32, 256><<<nblocks, 256, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
+    case 6: kb_step3d<S, PERIODIC, 1, 16, 16, 16, 256><<<nblocks, 256, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
     default: kb_step3d<S, PERIODIC, 0, 8, 8, 4, 256><<<nblocks, 256, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
   }
 }

@@ -47,7 +47,8 @@ static void bulk3d_variant(Domain& d, const S* i, S* o) {
     case 2: bulk3d_shape<S, PERIODIC, 1, 16, 8, 32, 128>(d, i, o); break;
     case 3: bulk3d_shape<S, PERIODIC, 1, 16, 8, 16, 128>(d, i, o); break;
     case 4: bulk3d_shape<S, PERIODIC, 1, 16, 8, 8, 128>(d, i, o); break;
-    case 5: bulk3d_shape<S, PERIODIC, 0, 8, 8, 8, 512>(d, i, o); break;
+    case 5: bulk3d_shape<S, PERIODIC, 1, 16, 16, 32, 256>(d, i, o); break;
+    case 6: bulk3d_shape<S, PERIODIC, 1, 16, 16, 16, 256>(d, i, o); break;
     default: bulk3d_shape<S, PERIODIC, 0, 8, 8, 4, 256>(d, i, o); break;
   }
 }
@@ -88,7 +89,8 @@ static void step3d_variant(Domain& d, const S* i, S* o, const LinkArgs& la, cons
     case 2: step3d_shape<S, PERIODIC, 1, 16, 8, 32, 128>(d, i, o, la, fa, nwave); break;
     case 3: step3d_shape<S, PERIODIC, 1, 16, 8, 16, 128>(d, i, o, la, fa, nwave); break;
     case 4: step3d_shape<S, PERIODIC, 1, 16, 8, 8, 128>(d, i, o, la, fa, nwave); break;
-    case 5: step3d_shape<S, PERIODIC, 0, 8, 8, 8, 512>(d, i, o, la, fa, nwave); break;
+    case 5: step3d_shape<S, PERIODIC, 1, 16, 16, 32, 256>(d, i, o, la, fa, nwave); break;
+    case 6: step3d_shape<S, PERIODIC, 1, 16, 16, 16, 256>(d, i, o, la, fa, nwave); break;
     default: step3d_shape<S, PERIODIC, 0, 8, 8, 4, 256>(d, i, o, la, fa, nwave); break;
   }
 }

@@ -216,9 +216,9 @@ struct Tile3 {
   int tx, ty, tz;  // tz: planes per block (box variant: 4, ring: ZC)
   bool ring;
 };
-constexpr Tile3 kTiles3[] = {{8, 8, 4, false}, {16, 4, 4, false}, {16, 8, 32, true},
-                             {16, 8, 16, true}, {16, 8, 8, true},  {8, 8, 8, false}};
-constexpr int kNumTiles3 = 6;
+constexpr Tile3 kTiles3[] = {{8, 8, 4, false}, {16, 4, 4, false}, {16, 8, 32, true}, {16, 8, 16, true},
+                             {16, 8, 8, true},  {16, 16, 32, true}, {16, 16, 16, true}};
+constexpr int kNumTiles3 = 7;
 
 // position of direction i among the nine directions sharing its c_z
 __host__ __device__ constexpr int zclass_index(int i) {

@@ -182,9 +182,9 @@ int ff_domain_footprint(const ff_domain_desc* desc, int nv, int ne, const float*
                         unsigned long long* bytes);
 
 /* Stream-collide tile shape.  2D (0..4): 32x8, 32x16, 64x8, 32x32, 64x16
-   cells per 256-thread block.  3D (0..5): 8x8x4 and 16x4x4 boxes, 16x8
-   columns marching 32, 16 or 8 planes, and an 8x8x8 box (512 threads).  0 is
-   the default; results do not depend on it.
+   cells per 256-thread block.  3D (0..6): 8x8x4 and 16x4x4 boxes, 16x8
+   columns marching 32, 16 or 8 planes, 16x16 columns marching 32 or 16
+   planes.  0 is the default; results do not depend on it.
    No counterpart in the reference (its tile is fixed, CudaCommon.cuh:44-51). */
 int ff_set_tile(ff_domain* d, int tile);
 /* Time every tile shape on this domain (the state is left unchanged) and

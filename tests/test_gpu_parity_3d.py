@@ -126,7 +126,7 @@ def test_fp16_storage_3d():
     compare(g, o, tol=1e-5)
 
 
-@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5, 6])
 def test_tile_variants_match_oracle(tile):
     # walls on every side (3D flag bounce-back), inlet/outlet, nz = 40 so the
     # 32-plane columns have a full and a partial chunk
