@@ -45,7 +45,7 @@ struct DomDev {
   int tiles_x, tiles_y;
   int nbc;           // boundary-cell blocks of this domain
   int nvb;           // vertex blocks of the finish phase
-  int ndet;          // next-step detection blocks of the finish phase
+  int ndet;          // next-step detection blocks of the element-sum phase
   int boundary;
 };
 
@@ -122,18 +122,14 @@ __global__ void __launch_bounds__(FB) kb_sums(const DomDev* __restrict__ doms, c
   elem_sums_body<DIM, S, FB>(D->la, D->fa, m.y, D->mom[(*D->step & 1u) ^ 1u]);
 }
 
-// Per domain: vertex blocks, detection blocks of the next step, then one
-// block that folds the wrench, advances gen/tag and the domain's step counter.
+// Per domain: vertex blocks, then one block that folds the wrench, advances
+// gen/tag and the domain's step counter.
 template <int DIM, int FB>
 __global__ void __launch_bounds__(FB) kb_finish(const DomDev* __restrict__ doms, const int2* __restrict__ map) {
   const int2 m = map[blockIdx.x];
   const DomDev* D = doms + m.x;
   if (m.y < D->nvb) {
     finish_vertex_body<DIM, FB>(D->la, D->fa, m.y);
-    return;
-  }
-  if (m.y < D->nvb + D->ndet) {
-    detect_next_body<DIM>(D->la, m.y - D->nvb);
     return;
   }
   if (D->boundary) finish_last_body<DIM, FB>(D->la, D->fa);
@@ -196,7 +192,7 @@ static void launch_stream_batch(Batch& b, int tile, const int2* map, int nblocks
 
 template <int DIM, typename S>
 static void launch_tail(Batch& b, const int2* map_sums, const int2* map_fin) {
-  constexpr int FB = DIM == 2 ? kElemBlock2 : kElemBlock3;
+  constexpr int FB = kSumBlock;
   if (b.n_es > 0) kb_sums<DIM, S, FB><<<b.n_es, FB, 0, b.stream>>>(b.d_doms, map_sums);
   if (b.n_fin > 0) kb_finish<DIM, 256><<<b.n_fin, 256, 0, b.stream>>>(b.d_doms, map_fin);
 }
@@ -261,7 +257,7 @@ static int build_tables(Batch& b, int tile) {
   std::vector<DomDev> doms(n);
   std::vector<int2> bc, wave, bulk, es, fin;
   const int K = b.dim == 2 ? 8 : 26;
-  const int fb = b.dim == 2 ? kElemBlock2 : kElemBlock3;
+  const int fb = kSumBlock;
   const int nt = stream_threads(b.dim, tile);
   const int cpb = nt / (b.dim == 2 ? 16 : 32);  // boundary cells per block in bcells_body
   int rc = grow(&b.d_steps, &b.steps_cap, (size_t)std::max(n, 1));
@@ -290,14 +286,14 @@ static int build_tables(Batch& b, int tile) {
         for (int t = 0; t < nw; t++) wave.push_back(make_int2(i, t));
       }
       r.nvb = (d.nv + 255) / 256;
-      r.ndet = (int)(((long long)d.ne * K + 15) / 16);
+      r.ndet = detect_blocks(d);
     } else {
       r.la.ne = 0;
       r.fa.copy_blocks = 0;
     }
-    const int nes = (bnd ? d.ne : 0) + r.fa.outlet_blocks + r.fa.copy_blocks;
+    const int nes = (bnd ? elem_blocks(d.ne) : 0) + r.fa.outlet_blocks + r.fa.copy_blocks + r.ndet;
     for (int t = 0; t < nes; t++) es.push_back(make_int2(i, t));
-    for (int t = 0; t <= r.nvb + r.ndet; t++) fin.push_back(make_int2(i, t));
+    for (int t = 0; t <= r.nvb; t++) fin.push_back(make_int2(i, t));
   }
   // XCD-aware order of the tiles (see xcd_tile): the tile range starts at a
   // multiple of 8 blocks (the wave range is padded) and block p of the range

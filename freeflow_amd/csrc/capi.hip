@@ -211,7 +211,7 @@ static unsigned long long footprint(int dim, long long n, int storage, int nv, i
     const unsigned long long K = q - 1, cap = next_pow2(est * 2 + 1024);
     b += 5ull * nv * D * 4 + (unsigned long long)ne * (D * 4 + 4 + 4 + D * 4 + (D * D + D + 3) * 8);
     b += (unsigned long long)(nv + 1) * 4 + 64;
-    b += cap * (8 + K * 8 + 3 * K * D * 4 + 2 * 4) + 64;
+    b += cap * (2 * 8 + 2 * K * 8 + 3 * K * D * 4 + 2 * 4 + nm * 4) + 64;
   }
   return b;
 }
@@ -255,8 +255,8 @@ static void links_moved(Domain& d) {
 // One step.  Without a boundary: [stream-collide] -> [outlet].  With one:
 // [detect (first step or after the boundary moved)] -> [stream: tiles +
 // boundary cells + next wave kinematics] -> [sums: element forces, outlets,
-// boundary-cell copy-back] -> [finish: vertex forces, wrench, next step's
-// links].  `pv` != null selects caller-owned proxy arrays (positions not
+// boundary-cell copy-back, next step's links] -> [finish: vertex forces,
+// wrench].  `pv` != null selects caller-owned proxy arrays (positions not
 // known ahead: detected every step).
 static int one_step(Domain& d, const float* pv, const float* pu, const uint32_t* pe, float* proxy_forces) {
   void* in = d.mom[d.cur];
@@ -275,8 +275,8 @@ static int one_step(Domain& d, const float* pv, const float* pu, const uint32_t*
     d.stale = false;
   }
   if ((rc = launch_step_stream(d, in, out, la, d.wave && !proxy))) return rc;
-  if ((rc = launch_step_sums(d, la, out))) return rc;
-  if ((rc = launch_step_finish(d, la, !proxy, proxy_forces))) return rc;
+  if ((rc = launch_step_sums(d, la, out, !proxy))) return rc;
+  if ((rc = launch_step_finish(d, la, proxy_forces))) return rc;
   d.predetected = !proxy;
   d.cur ^= 1;
   d.gen_host++;
@@ -570,13 +570,14 @@ int ff_set_boundary(ff_domain* dd, int nv, const float* verts, const float* vels
   d.lt_est = est;
   const size_t K = (size_t)(d.q - 1);
   const size_t cap = (size_t)d.lt_cap;
-  if ((rc = dalloc(&d.lt_cells, cap)) || (rc = dalloc(&d.lt_keys, cap * K)) || (rc = dalloc(&d.lk_f, cap * K * D)) ||
+  // two tables (cells and keys), alternating by tag parity
+  if ((rc = dalloc(&d.lt_cells, 2 * cap)) || (rc = dalloc(&d.lt_keys, 2 * cap * K)) || (rc = dalloc(&d.lk_f, cap * K * D)) ||
       (rc = dalloc(&d.lk_b, cap * K * D)) || (rc = dalloc(&d.lk_h, cap * K * D)) ||
       (rc = dalloc(&d.lt_list, 2 * cap)) || (rc = dalloc(&d.lk_sb, cap * d.nm)) || (rc = dalloc(&d.st, 1)))
     return rc;
   // generation 0 marks every entry empty; steps run from generation 1
-  FF_HIP(hipMemset(d.lt_cells, 0, cap * sizeof(unsigned long long)));
-  FF_HIP(hipMemset(d.lt_keys, 0, cap * K * sizeof(unsigned long long)));
+  FF_HIP(hipMemset(d.lt_cells, 0, 2 * cap * sizeof(unsigned long long)));
+  FF_HIP(hipMemset(d.lt_keys, 0, 2 * cap * K * sizeof(unsigned long long)));
   StepState st0{};
   st0.gen = 1;
   st0.tag = 1;

@@ -61,8 +61,8 @@ struct LinkArgs {
   const float* u[2];   // vertex velocities by generation parity
   const uint32_t* e;
   int ne, nv;
-  unsigned long long* cells;  // cap
-  unsigned long long* keys;   // cap * (Q-1)
+  unsigned long long* cells;  // 2 * cap: two tables, by tag parity
+  unsigned long long* keys;   // 2 * cap * (Q-1), by tag parity
   float* lf;                  // cap * (Q-1) * DIM  link force
   float* lb;                  // cap * (Q-1) * DIM  barycentrics, lb[0] < 0: link not applied
   float* lh;                  // cap * (Q-1) * DIM  hit point
@@ -160,11 +160,12 @@ LinkArgs link_args(Domain& d, const float* v0, const float* v1, const float* u0,
 int launch_bulk2d(Domain& d, const void* in, void* out);
 int launch_bulk3d(Domain& d, const void* in, void* out);
 // boundary step: [detect] -> stream (tiles + boundary cells + next kinematics)
-// -> element sums + outlets + boundary-cell copy-back -> finish (+ next detection)
+// -> element sums + outlets + boundary-cell copy-back + next step's detection
+// -> finish (vertex forces, wrench)
 int launch_detect(Domain& d, const LinkArgs& la, bool bump);
 int launch_step_stream(Domain& d, const void* in, void* out, const LinkArgs& la, bool wave);
-int launch_step_sums(Domain& d, const LinkArgs& la, void* out);
-int launch_step_finish(Domain& d, const LinkArgs& la, bool detect_next, float* proxy_forces);
+int launch_step_sums(Domain& d, const LinkArgs& la, void* out, bool detect_next);
+int launch_step_finish(Domain& d, const LinkArgs& la, float* proxy_forces);
 int launch_outlet(Domain& d, void* buf);
 int launch_wave_now(Domain& d);
 int launch_step_vertex_forces(Domain& d, float* out_dev);

@@ -33,6 +33,16 @@ __device__ __forceinline__ unsigned mix32(unsigned h) {
 
 // Returns the slot of `cell` in this generation, inserting it if absent
 // (the first inserter appends the slot to `list`).  -1 on a full table.
+// Two link tables alternate by tag parity: the next step's links are detected
+// into one while the current step's are read from the other.
+__device__ __forceinline__ unsigned long long* table_cells(const LinkArgs& la, unsigned tag) {
+  return la.cells + (size_t)(tag & 1u) * la.cap;
+}
+template <int DIM>
+__device__ __forceinline__ unsigned long long* table_keys(const LinkArgs& la, unsigned tag) {
+  return la.keys + (size_t)(tag & 1u) * la.cap * (DIM == 2 ? 8 : 26);
+}
+
 __device__ __forceinline__ int lt_insert(unsigned long long* cells, int cap, unsigned gen, int cell, int* list,
                                          int* count, int* overflow) {
   const unsigned long long want = ((unsigned long long)gen << 32) | (unsigned)cell;
@@ -236,9 +246,9 @@ __device__ __forceinline__ void build_links_block(const LinkArgs& la, long long 
       hit = hit_elem3(v, la.e, elem, p1, p2, h);
     }
     if (!hit) continue;
-    const int slot = lt_insert(la.cells, la.cap, gen, cell, list, count, &la.st->overflow);
+    const int slot = lt_insert(table_cells(la, gen), la.cap, gen, cell, list, count, &la.st->overflow);
     if (slot < 0) return;
-    lt_offer_key(la.keys, (size_t)slot * K + (l - 1), gen, depth_key(h.depth, elem));
+    lt_offer_key(table_keys<DIM>(la, gen), (size_t)slot * K + (l - 1), gen, depth_key(h.depth, elem));
   }
 }
 

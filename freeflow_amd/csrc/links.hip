@@ -69,19 +69,28 @@ int launch_detect(Domain& d, const LinkArgs& la, bool bump) {
 }
 
 // ---------------------------------------------------------------- sums, finish
+// Element sums, outlets, copy-back and next-step detection.
 template <int DIM, typename S, int FB>
 __global__ void __launch_bounds__(FB) k_step_sums(LinkArgs la, FinalArgs fa, void* out) {
   elem_sums_body<DIM, S, FB>(la, fa, blockIdx.x, out);
 }
 
-// Vertex blocks, then `ndet` detection blocks for the next step, then one
-// last block for the body wrenches and the advance of gen and tag.
+// Vertex blocks, then one last block for the body wrenches and the advance
+// of gen and tag.  (A separate launch: folding it into the element-sum launch
+// would need device-scope release/acquire between blocks on different XCDs,
+// i.e. L2 write-back and invalidation, which costs more than the launch.)
 template <int DIM, int FB>
-__global__ void __launch_bounds__(FB) k_step_finish(LinkArgs la, FinalArgs fa, int nvb, int ndet) {
+__global__ void __launch_bounds__(FB) k_step_finish(LinkArgs la, FinalArgs fa, int nvb) {
   const int b = blockIdx.x;
   if (b < nvb) finish_vertex_body<DIM, FB>(la, fa, b);
-  else if (b < nvb + ndet) detect_next_body<DIM>(la, b - nvb);
   else finish_last_body<DIM, FB>(la, fa);
+}
+
+// Detection blocks of the element-sum launch (kSumBlock threads): one
+// (element, direction) pair per 16 threads.
+int detect_blocks(const Domain& d) {
+  const long long per = kSumBlock / 16;
+  return (int)(((long long)d.ne * (d.q - 1) + per - 1) / per);
 }
 
 // Copy-back blocks of the sums launch: grid-stride over the step's boundary
@@ -96,33 +105,32 @@ int boundary_cell_blocks(const Domain& d, int cells_per_block) {
                                                                                     cells_per_block));
 }
 
-int launch_step_sums(Domain& d, const LinkArgs& la, void* out) {
-  constexpr int FB2 = kElemBlock2, FB3 = kElemBlock3;
-  const int fb = d.dim == 2 ? FB2 : FB3;
+int launch_step_sums(Domain& d, const LinkArgs& la, void* out, bool detect_next) {
+  constexpr int FB = kSumBlock;
   const FinalArgs fa = make_final_args(d, out, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], nullptr);
-  const unsigned grid = (unsigned)(d.ne + fa.outlet_blocks + fa.copy_blocks);
-#define FF_SUM(DIM, S, FB) k_step_sums<DIM, S, FB><<<grid, FB, 0, d.stream>>>(la, fa, out)
+  const unsigned grid =
+      (unsigned)(elem_blocks(d.ne) + fa.outlet_blocks + fa.copy_blocks + (detect_next ? detect_blocks(d) : 0));
+#define FF_SUM(DIM, S) k_step_sums<DIM, S, FB><<<grid, FB, 0, d.stream>>>(la, fa, out)
   if (d.dim == 2) {
-    if (d.storage == FF_STORAGE_F32) FF_SUM(2, float, FB2);
-    else if (d.storage == FF_STORAGE_F16) FF_SUM(2, __half, FB2);
-    else FF_SUM(2, __hip_bfloat16, FB2);
+    if (d.storage == FF_STORAGE_F32) FF_SUM(2, float);
+    else if (d.storage == FF_STORAGE_F16) FF_SUM(2, __half);
+    else FF_SUM(2, __hip_bfloat16);
   } else {
-    if (d.storage == FF_STORAGE_F32) FF_SUM(3, float, FB3);
-    else if (d.storage == FF_STORAGE_F16) FF_SUM(3, __half, FB3);
-    else FF_SUM(3, __hip_bfloat16, FB3);
+    if (d.storage == FF_STORAGE_F32) FF_SUM(3, float);
+    else if (d.storage == FF_STORAGE_F16) FF_SUM(3, __half);
+    else FF_SUM(3, __hip_bfloat16);
   }
 #undef FF_SUM
   FF_HIP(hipGetLastError());
   return FF_OK;
 }
 
-int launch_step_finish(Domain& d, const LinkArgs& la, bool detect_next, float* proxy_forces) {
+int launch_step_finish(Domain& d, const LinkArgs& la, float* proxy_forces) {
   const FinalArgs fa = make_final_args(d, nullptr, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], proxy_forces);
   const int nvb = (d.nv + 255) / 256;
-  const int ndet = detect_next ? (int)(((long long)d.ne * (d.q - 1) + 15) / 16) : 0;
-  const unsigned grid = (unsigned)(nvb + ndet + 1);
-  if (d.dim == 2) k_step_finish<2, 256><<<grid, 256, 0, d.stream>>>(la, fa, nvb, ndet);
-  else k_step_finish<3, 256><<<grid, 256, 0, d.stream>>>(la, fa, nvb, ndet);
+  const unsigned grid = (unsigned)(nvb + 1);
+  if (d.dim == 2) k_step_finish<2, 256><<<grid, 256, 0, d.stream>>>(la, fa, nvb);
+  else k_step_finish<3, 256><<<grid, 256, 0, d.stream>>>(la, fa, nvb);
   FF_HIP(hipGetLastError());
   return FF_OK;
 }
@@ -155,8 +163,7 @@ FinalArgs make_final_args(Domain& d, void* out, float* vn0, float* vn1, float* u
   fa.buf = out;
   fa.outlets = d.outlets;
   fa.n_outlets = d.n_outlets;
-  constexpr int FB2 = 128, FB3 = 256;
-  const int fb = d.dim == 2 ? FB2 : FB3;
+  const int fb = kSumBlock;
   fa.outlet_blocks = (d.n_outlets + fb - 1) / fb;
   fa.copy_blocks = (d.has_boundary && d.ne > 0) ? copyback_blocks(d, fb) : 0;
   return fa;

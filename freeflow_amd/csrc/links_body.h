@@ -59,12 +59,21 @@ __device__ __forceinline__ void bcells_body(const S* __restrict__ in, const Link
   const float* v = la.v[gen & 1];
   const float* u = la.u[gen & 1];
   const int lc = threadIdx.x / LPC, dir = threadIdx.x % LPC;
+  const unsigned long long* cells = table_cells(la, tag);
+  const unsigned long long* keys = table_keys<DIM>(la, tag);
+  if (blk == 0 && threadIdx.x == 0) {
+    // open the next step's tag: its links are detected in the element-sum
+    // launch into the other table (this step's list parity is untouched)
+    la.st->tag_next = tag + 1;
+    la.st->vnext = (gen + 1) & 1;
+    la.st->count[(tag + 1) & 1] = 0;
+  }
   for (int base = blk * CPB; base < count; base += nblk * CPB) {
     const int t = base + lc;
     int cell = 0;
     if (t < count && dir < Q) {
       const int slot = list[t];
-      cell = (int)(uint32_t)la.cells[slot];
+      cell = (int)(uint32_t)cells[slot];
       const int x = cell % la.nx, y = (cell / la.nx) % la.ny, z = DIM == 3 ? cell / (la.nx * la.ny) : 0;
       const int dx = DIM == 2 ? D2::cx(dir) : D3::cx(dir);
       const int dy = DIM == 2 ? D2::cy(dir) : D3::cy(dir);
@@ -85,7 +94,7 @@ __device__ __forceinline__ void bcells_body(const S* __restrict__ in, const Link
       const size_t li = (size_t)slot * K + (dir - 1);
       // every load that depends only on the cell index is issued up front
       const uint8_t fup = la.flags[((size_t)zw * la.ny + yw) * la.nx + xw];
-      const uint32_t key = dir > 0 ? lt_key(la.keys, li, tag) : 0xFFFFFFFFu;
+      const uint32_t key = dir > 0 ? lt_key(keys, li, tag) : 0xFFFFFFFFu;
       float mc[NM];
 #pragma unroll
       for (int q = 0; q < NM; q++) mc[q] = Store<S>::template load<DIM>(in + q * la.plane + cell, q);
@@ -205,8 +214,10 @@ struct FinalArgs {
   int copy_blocks;  // copy-back blocks of the sums launch
 };
 
-// Element-sum block size: 2D 128, 3D 256.
-constexpr int kElemBlock2 = 128, kElemBlock3 = 256;
+// Block size of the element-sum launch (2D and 3D) and elements per block
+// (one per wave).
+constexpr int kSumBlock = 256, kElemPerBlock = kSumBlock / 64;
+__host__ __device__ inline int elem_blocks(int ne) { return (ne + kElemPerBlock - 1) / kElemPerBlock; }
 
 // Host: the FinalArgs of a domain (outlets written into `out`, next-step
 // kinematics into vn/un, per-vertex forces optionally added to proxy_forces).
@@ -214,6 +225,7 @@ FinalArgs make_final_args(Domain& d, void* out, float* vn0, float* vn1, float* u
 // Host: grid sizes of the boundary-cell and copy-back parts of a step.
 int boundary_cell_blocks(const Domain& d, int cells_per_block);
 int copyback_blocks(const Domain& d, int fb);
+int detect_blocks(const Domain& d);
 
 // Travelling wave along the body axis, FP64:
 //   y = y_rest + A s sin(theta), v_y = -2 pi f A s cos(theta),
@@ -248,60 +260,31 @@ __device__ __forceinline__ void wave_block_body(const LinkArgs& la, const FinalA
   }
 }
 
-// Second launch of a boundary step.  Blocks [0, ne): each element sums its
-// own links (fixed order, FP64) into elem_acc.  Then outlet blocks: outlet
-// cells of the freshly written buffer (mrApplyOutlet2DKernel /
-// mrApplyOutlet3DKernel; a neighbour that is a boundary cell is read from its
-// side-buffer row, which the copy-back of this launch writes).  Then
-// copy-back blocks: boundary-cell rows into the output; block `ne +
-// outlet_blocks` also opens the next step's tag and clears its list.
-template <int DIM, typename S, int FB>
-__device__ __forceinline__ void elem_sums_body(const LinkArgs& la, const FinalArgs& fa, int blk, void* out) {
+// Detection of the next step's links under tag_next (opened by the stream
+// launch) and the next step's vertex positions, into the other link table.
+template <int DIM>
+__device__ __forceinline__ void detect_next_body(const LinkArgs& la, int blk) {
+  build_links_block<DIM>(la, (long long)blk, la.st->tag_next, la.v[la.st->vnext & 1]);
+}
+
+// Second launch of a boundary step, kSumBlock threads per block.  The first
+// ceil(ne / kElemPerBlock) blocks sum links per element, one element per
+// wave: each sums its own links (fixed order, FP64) into elem_acc.  Then
+// outlet blocks: outlet cells of the freshly written buffer
+// (mrApplyOutlet2DKernel / mrApplyOutlet3DKernel; a neighbour that is a
+// boundary cell is read from its side-buffer row, which the copy-back of this
+// launch writes).  Then copy-back blocks: boundary-cell rows into the output.
+// Any further blocks detect the next step's links.
+// Sum of one element's links by one wave (lane = lane index, sl = 64 ints of
+// LDS for the wave): window cells are resolved 64 at a time, then every lane
+// takes (cell, direction) tasks; a fixed shuffle tree folds the FP64 sums.
+template <int DIM>
+__device__ __forceinline__ void elem_sum_wave(const LinkArgs& la, const FinalArgs& fa, int elem, int lane, int* sl) {
   constexpr int K = DIM == 2 ? 8 : 26;
   constexpr int NT = DIM == 2 ? 1 : 3;
-  constexpr int NM = DIM == 2 ? 6 : 10;
   constexpr int STRIDE = DIM * DIM + DIM + NT;
-  constexpr int CH = 256;  // window cells resolved per chunk
-  if (blk >= la.ne + fa.outlet_blocks) {
-    const unsigned tag = la.st->tag;
-    const int b = blk - la.ne - fa.outlet_blocks;
-    if (b == 0 && threadIdx.x == 0) {
-      la.st->tag_next = tag + 1;
-      la.st->vnext = (la.st->gen + 1) & 1;
-      la.st->count[(tag + 1) & 1] = 0;
-    }
-    const int count = la.st->count[tag & 1];
-    const int nb = fa.copy_blocks;
-    S* o = (S*)out;
-    for (int t = b * FB + threadIdx.x; t < count; t += nb * FB) {
-      const int slot = la.list[(size_t)(tag & 1) * la.cap + t];
-      const int cell = (int)(uint32_t)la.cells[slot];
-      if (!is_updated(la.flags[cell])) continue;
-      const float* row = la.sb + (size_t)slot * NM;
-#pragma unroll
-      for (int q = 0; q < NM; q++) Store<S>::template store<DIM>(o + q * la.plane + cell, q, row[q]);
-    }
-    return;
-  }
-  if (blk >= la.ne) {
-    const int t = (blk - la.ne) * FB + threadIdx.x;
-    if (t >= fa.n_outlets) return;
-    S* buf = (S*)out;
-    const OutletCell c = fa.outlets[t];
-    if (la.flags[c.nb] != kFluid) return;
-    const int nslot = la.ne > 0 ? lt_find(la.cells, la.cap, la.st->tag, c.nb) : -1;
-    const float up = nslot >= 0 ? la.sb[(size_t)nslot * NM + c.comp]
-                                : Store<S>::template load<DIM>(buf + c.comp * la.plane + c.nb, c.comp);
-    if ((c.sign > 0 && up > 0.0f) || (c.sign < 0 && up < 0.0f))
-      Store<S>::template store<DIM>(buf + c.comp * la.plane + c.cell, c.comp, up);
-    Store<S>::template store<DIM>(buf + c.pcomp * la.plane + c.cell, c.pcomp, up * up);
-    return;
-  }
-  __shared__ int s_slot[CH];
-  __shared__ double s_red[STRIDE][FB];
   const unsigned gen = la.st->gen;
   const unsigned tag = la.st->tag;
-  const int elem = blk;
   const float* v = la.v[gen & 1];
   const int n[3] = {la.nx, la.ny, la.nz};
   int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
@@ -316,21 +299,25 @@ __device__ __forceinline__ void elem_sums_body(const LinkArgs& la, const FinalAr
 #pragma unroll
   for (int a = 0; a < STRIDE; a++) acc[a] = 0.0;
   const uint32_t etag = (uint32_t)(uint16_t)elem;
-  for (int c0 = 0; c0 < ncell; c0 += CH) {
-    const int chunk = min(CH, ncell - c0);
-    for (int k = threadIdx.x; k < chunk; k += FB) {
-      const int kk = c0 + k;
+  const unsigned long long* cells = table_cells(la, tag);
+  const unsigned long long* keys = table_keys<DIM>(la, tag);
+  for (int c0 = 0; c0 < ncell; c0 += 64) {
+    const int chunk = min(64, ncell - c0);
+    if (lane < chunk) {
+      const int kk = c0 + lane;
       const int i = lo[0] + kk % wx, j = lo[1] + (kk / wx) % wy, z = DIM == 3 ? lo[2] + kk / (wx * wy) : 0;
       const int cell = (z * la.ny + j) * la.nx + i;
-      s_slot[k] = is_updated(la.flags[cell]) ? lt_find(la.cells, la.cap, tag, cell) : -1;
+      sl[lane] = is_updated(la.flags[cell]) ? lt_find(cells, la.cap, tag, cell) : -1;
     }
-    __syncthreads();
-    for (int task = threadIdx.x; task < K * chunk; task += FB) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int task = lane; task < K * chunk; task += 64) {
       const int k = task % chunk, l = task / chunk;
-      const int slot = s_slot[k];
+      const int slot = sl[k];
       if (slot < 0) continue;
       const size_t li = (size_t)slot * K + l;
-      const uint32_t key = lt_key(la.keys, li, tag);
+      const uint32_t key = lt_key(keys, li, tag);
       if (key == 0xFFFFFFFFu || (key & 0xFFFFu) != etag) continue;
       if (la.lb[li * DIM] == kLinkUnused) continue;
       float F[DIM];
@@ -355,18 +342,72 @@ __device__ __forceinline__ void elem_sums_body(const LinkArgs& la, const FinalAr
         acc[DIM * DIM + DIM + 2] += r[0] * (double)F[1] - r[1] * (double)F[0];
       }
     }
-    __syncthreads();
+    // the next chunk overwrites sl[] only after every lane's reads above
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
+  // fixed shuffle tree over the wave's 64 lanes
 #pragma unroll
-  for (int a = 0; a < STRIDE; a++) s_red[a][threadIdx.x] = acc[a];
-  __syncthreads();
-  for (int w = FB / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w)
+  for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
-      for (int a = 0; a < STRIDE; a++) s_red[a][threadIdx.x] += s_red[a][threadIdx.x + w];
-    __syncthreads();
+    for (int a = 0; a < STRIDE; a++) acc[a] += __shfl_down(acc[a], off, 64);
   }
-  if (threadIdx.x < STRIDE) fa.elem_acc[(size_t)elem * STRIDE + threadIdx.x] = s_red[threadIdx.x][0];
+  if (lane == 0) {
+#pragma unroll
+    for (int a = 0; a < STRIDE; a++) fa.elem_acc[(size_t)elem * STRIDE + a] = acc[a];
+  }
+}
+
+
+template <int DIM, typename S, int FB>
+__device__ __forceinline__ void elem_sums_body(const LinkArgs& la, const FinalArgs& fa, int blk, void* out) {
+  static_assert(FB == kSumBlock, "element sums run one element per wave of a kSumBlock block");
+  constexpr int K = DIM == 2 ? 8 : 26;
+  constexpr int NT = DIM == 2 ? 1 : 3;
+  constexpr int NM = DIM == 2 ? 6 : 10;
+  constexpr int STRIDE = DIM * DIM + DIM + NT;
+  const int neb = elem_blocks(la.ne);
+  if (blk >= neb + fa.outlet_blocks + fa.copy_blocks) {
+    detect_next_body<DIM>(la, blk - neb - fa.outlet_blocks - fa.copy_blocks);
+    return;
+  }
+  if (blk >= neb + fa.outlet_blocks) {
+    const unsigned tag = la.st->tag;
+    const int b = blk - neb - fa.outlet_blocks;
+    const int count = la.st->count[tag & 1];
+    const int nb = fa.copy_blocks;
+    const unsigned long long* cells = table_cells(la, tag);
+    S* o = (S*)out;
+    for (int t = b * FB + threadIdx.x; t < count; t += nb * FB) {
+      const int slot = la.list[(size_t)(tag & 1) * la.cap + t];
+      const int cell = (int)(uint32_t)cells[slot];
+      if (!is_updated(la.flags[cell])) continue;
+      const float* row = la.sb + (size_t)slot * NM;
+#pragma unroll
+      for (int q = 0; q < NM; q++) Store<S>::template store<DIM>(o + q * la.plane + cell, q, row[q]);
+    }
+    return;
+  }
+  if (blk >= neb) {
+    const int t = (blk - neb) * FB + threadIdx.x;
+    if (t >= fa.n_outlets) return;
+    S* buf = (S*)out;
+    const OutletCell c = fa.outlets[t];
+    if (la.flags[c.nb] != kFluid) return;
+    const unsigned tag = la.st->tag;
+    const int nslot = la.ne > 0 ? lt_find(table_cells(la, tag), la.cap, tag, c.nb) : -1;
+    const float up = nslot >= 0 ? la.sb[(size_t)nslot * NM + c.comp]
+                                : Store<S>::template load<DIM>(buf + c.comp * la.plane + c.nb, c.comp);
+    if ((c.sign > 0 && up > 0.0f) || (c.sign < 0 && up < 0.0f))
+      Store<S>::template store<DIM>(buf + c.comp * la.plane + c.cell, c.comp, up);
+    Store<S>::template store<DIM>(buf + c.pcomp * la.plane + c.cell, c.pcomp, up * up);
+    return;
+  }
+  __shared__ int s_slot[kElemPerBlock][64];
+  const int wv = threadIdx.x >> 6;
+  const int elem = blk * kElemPerBlock + wv;
+  if (elem < la.ne) elem_sum_wave<DIM>(la, fa, elem, threadIdx.x & 63, s_slot[wv]);
 }
 
 // Vertex blocks sum their incident element shares in CSR order (the
@@ -431,15 +472,6 @@ __device__ __forceinline__ void finish_last_body(const LinkArgs& la, const Final
     la.st->gen = gen + 1;
     la.st->tag = la.st->tag_next;
   }
-}
-
-// Detection of the next step's links in the finish launch, under tag_next
-// (set by the element-sum launch) and the next step's vertex positions.
-// (tag_next and vnext were written by the element-sum launch; the last block
-// of the finish launch advances gen and tag, which these blocks do not read.)
-template <int DIM>
-__device__ __forceinline__ void detect_next_body(const LinkArgs& la, int blk) {
-  build_links_block<DIM>(la, (long long)blk, la.st->tag_next, la.v[la.st->vnext & 1]);
 }
 
 }  // namespace ff
