@@ -259,7 +259,6 @@ static int build_tables(Batch& b, int tile) {
   const int K = b.dim == 2 ? 8 : 26;
   const int fb = kSumBlock;
   const int nt = stream_threads(b.dim, tile);
-  const int cpb = nt / (b.dim == 2 ? 16 : 32);  // boundary cells per block in bcells_body
   int rc = grow(&b.d_steps, &b.steps_cap, (size_t)std::max(n, 1));
   if (rc) return rc;
   for (int i = 0; i < n; i++) {
@@ -279,7 +278,7 @@ static int build_tables(Batch& b, int tile) {
     r.tiles_y = ty;
     for (int t = 0; t < ntl; t++) bulk.push_back(make_int2(i, t));
     if (bnd) {
-      r.nbc = boundary_cell_blocks(d, cpb);
+      r.nbc = boundary_cell_blocks(d, nt);
       for (int t = 0; t < r.nbc; t++) bc.push_back(make_int2(i, t));
       if (d.wave) {
         const int nw = (d.nv + nt - 1) / nt;

@@ -46,7 +46,7 @@ static void free_boundary(Domain& d) {
   dfree(d.elems); dfree(d.elem_body); dfree(d.vforce);
   dfree(d.body_elem_ptr); dfree(d.body_elem_idx); dfree(d.vert_inc_ptr); dfree(d.vert_inc_idx);
   dfree(d.body_ref); dfree(d.elem_acc); dfree(d.wrench); dfree(d.wrench_hist);
-  dfree(d.lt_cells); dfree(d.lt_keys); dfree(d.lk_f); dfree(d.lk_b); dfree(d.lk_h); dfree(d.lt_list); dfree(d.lk_sb);
+  dfree(d.lt_cells); dfree(d.lt_keys); dfree(d.lk_f); dfree(d.lk_b); dfree(d.lk_h); dfree(d.lk_sb);
   dfree(d.st); dfree(d.rest_verts); dfree(d.s_coord); dfree(d.wave_params); dfree(d.vert_body);
   d.nv = d.ne = d.n_bodies = 0;
   d.gen_host = 1;
@@ -211,7 +211,7 @@ static unsigned long long footprint(int dim, long long n, int storage, int nv, i
     const unsigned long long K = q - 1, cap = next_pow2(est * 2 + 1024);
     b += 5ull * nv * D * 4 + (unsigned long long)ne * (D * 4 + 4 + 4 + D * 4 + (D * D + D + 3) * 8);
     b += (unsigned long long)(nv + 1) * 4 + 64;
-    b += cap * (2 * 8 + 2 * K * 8 + 3 * K * D * 4 + 2 * 4 + nm * 4) + 64;
+    b += cap * (2 * 8 + 2 * K * 8 + 3 * K * D * 4 + nm * 4) + 64;
   }
   return b;
 }
@@ -230,7 +230,6 @@ LinkArgs link_args(Domain& d, const float* v0, const float* v1, const float* u0,
   la.lf = d.lk_f;
   la.lb = d.lk_b;
   la.lh = d.lk_h;
-  la.list = d.lt_list;
   la.sb = d.lk_sb;
   la.cap = d.lt_cap;
   la.st = d.st;
@@ -573,7 +572,7 @@ int ff_set_boundary(ff_domain* dd, int nv, const float* verts, const float* vels
   // two tables (cells and keys), alternating by tag parity
   if ((rc = dalloc(&d.lt_cells, 2 * cap)) || (rc = dalloc(&d.lt_keys, 2 * cap * K)) || (rc = dalloc(&d.lk_f, cap * K * D)) ||
       (rc = dalloc(&d.lk_b, cap * K * D)) || (rc = dalloc(&d.lk_h, cap * K * D)) ||
-      (rc = dalloc(&d.lt_list, 2 * cap)) || (rc = dalloc(&d.lk_sb, cap * d.nm)) || (rc = dalloc(&d.st, 1)))
+      (rc = dalloc(&d.lk_sb, cap * d.nm)) || (rc = dalloc(&d.st, 1)))
     return rc;
   // generation 0 marks every entry empty; steps run from generation 1
   FF_HIP(hipMemset(d.lt_cells, 0, 2 * cap * sizeof(unsigned long long)));

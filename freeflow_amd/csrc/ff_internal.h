@@ -47,7 +47,6 @@ struct OutletCell {
 // arguments that change from step to step (they can be replayed as a graph).
 struct StepState {
   unsigned gen;       // steps since the boundary was set + 1 (vertex-buffer parity, wrench ring slot)
-  int count[2];       // link-table occupancy (boundary cells), by tag parity
   int overflow;       // sticky: link table full
   unsigned tag;       // link-table tag of the current step's links
   unsigned tag_next;  // tag the next step's links are detected under
@@ -66,7 +65,6 @@ struct LinkArgs {
   float* lf;                  // cap * (Q-1) * DIM  link force
   float* lb;                  // cap * (Q-1) * DIM  barycentrics, lb[0] < 0: link not applied
   float* lh;                  // cap * (Q-1) * DIM  hit point
-  int* list;                  // 2 * cap occupied slots, by tag parity
   float* sb;                  // cap * NM  updated moments of each boundary cell (copied back after the step)
   int cap;
   StepState* st;
@@ -122,7 +120,6 @@ struct Domain {
   float* lk_f = nullptr;
   float* lk_b = nullptr;
   float* lk_h = nullptr;
-  int* lt_list = nullptr;
   float* lk_sb = nullptr;         // boundary-cell results (lt_cap * nm)
   StepState* st = nullptr;
   unsigned gen_host = 1;          // mirror of st->gen

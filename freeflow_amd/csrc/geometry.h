@@ -43,18 +43,17 @@ __device__ __forceinline__ unsigned long long* table_keys(const LinkArgs& la, un
   return la.keys + (size_t)(tag & 1u) * la.cap * (DIM == 2 ? 8 : 26);
 }
 
-__device__ __forceinline__ int lt_insert(unsigned long long* cells, int cap, unsigned gen, int cell, int* list,
-                                         int* count, int* overflow) {
+// Slot of `cell` under generation `gen`, entering it if absent.  The cells of
+// a generation are found again by scanning the slots for its tag (no list, so
+// no counter that every detecting lane would serialise on).
+__device__ __forceinline__ int lt_claim(unsigned long long* cells, int cap, unsigned gen, int cell, int* overflow) {
   const unsigned long long want = ((unsigned long long)gen << 32) | (unsigned)cell;
   unsigned h = mix32((unsigned)cell) & (unsigned)(cap - 1);
   for (int probe = 0; probe < cap; probe++) {
     unsigned long long cur = cells[h];
     while ((unsigned)(cur >> 32) != gen) {  // stale or never used: try to claim
       const unsigned long long prev = atomicCAS(&cells[h], cur, want);
-      if (prev == cur) {
-        list[atomicAdd(count, 1)] = (int)h;
-        return (int)h;
-      }
+      if (prev == cur) return (int)h;
       cur = prev;
     }
     if (cur == want) return (int)h;
@@ -226,29 +225,26 @@ __device__ __forceinline__ void build_links_block(const LinkArgs& la, long long 
   const int cx = DIM == 2 ? D2::cx(l) : D3::cx(l);
   const int cy = DIM == 2 ? D2::cy(l) : D3::cy(l);
   const int cz = DIM == 2 ? 0 : D3::cz(l);
-  int* list = la.list + (size_t)(gen & 1) * la.cap;
-  int* count = &la.st->count[gen & 1];
   for (long long k = lane; k < ncell; k += 16) {
     const int i = lo[0] + (int)(k % wx);
     const int j = lo[1] + (int)((k / wx) % wy);
     const int z = DIM == 3 ? lo[2] + (int)(k / ((long long)wx * wy)) : 0;
     const int cell = (z * la.ny + j) * la.nx + i;
-    if (!is_updated(la.flags[cell])) continue;
     Hit h;
-    bool hit;
-    if constexpr (DIM == 2) {
-      hit = hit_elem2(v, la.e, elem, (float)i * la.cl, (float)j * la.cl, ((float)i - (float)cx) * la.cl,
-                      ((float)j - (float)cy) * la.cl, h);
-    } else {
-      const float p1[3] = {(float)i * la.cl, (float)j * la.cl, (float)z * la.cl};
-      const float p2[3] = {((float)i - (float)cx) * la.cl, ((float)j - (float)cy) * la.cl,
-                           ((float)z - (float)cz) * la.cl};
-      hit = hit_elem3(v, la.e, elem, p1, p2, h);
+    bool hit = false;
+    if (is_updated(la.flags[cell])) {
+      if constexpr (DIM == 2) {
+        hit = hit_elem2(v, la.e, elem, (float)i * la.cl, (float)j * la.cl, ((float)i - (float)cx) * la.cl,
+                        ((float)j - (float)cy) * la.cl, h);
+      } else {
+        const float p1[3] = {(float)i * la.cl, (float)j * la.cl, (float)z * la.cl};
+        const float p2[3] = {((float)i - (float)cx) * la.cl, ((float)j - (float)cy) * la.cl,
+                             ((float)z - (float)cz) * la.cl};
+        hit = hit_elem3(v, la.e, elem, p1, p2, h);
+      }
     }
-    if (!hit) continue;
-    const int slot = lt_insert(table_cells(la, gen), la.cap, gen, cell, list, count, &la.st->overflow);
-    if (slot < 0) return;
-    lt_offer_key(table_keys<DIM>(la, gen), (size_t)slot * K + (l - 1), gen, depth_key(h.depth, elem));
+    const int slot = hit ? lt_claim(table_cells(la, gen), la.cap, gen, cell, &la.st->overflow) : -1;
+    if (slot >= 0) lt_offer_key(table_keys<DIM>(la, gen), (size_t)slot * K + (l - 1), gen, depth_key(h.depth, elem));
   }
 }
 

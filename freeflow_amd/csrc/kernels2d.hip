@@ -114,7 +114,7 @@ int launch_step_stream(Domain& d, const void* in, void* out, const LinkArgs& la,
   const FinalArgs fa = make_final_args(d, out, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], nullptr);
   const int nwave = wave ? (d.nv + 255) / 256 : 0;
   if (d.dim == 3) return launch_step_stream3d(d, in, out, la, fa, nwave);
-  const StreamExtra x{&la, &fa, boundary_cell_blocks(d, 16), nwave};
+  const StreamExtra x{&la, &fa, boundary_cell_blocks(d, 256), nwave};
   return bulk2d_dispatch(d, in, out, &x);
 }
 

@@ -77,7 +77,7 @@ static void step3d_shape(Domain& d, const S* i, S* o, const LinkArgs& la, const 
   const int tx = (d.nx + TX - 1) / TX, ty = (d.ny + TY - 1) / TY, tz = (d.nz + ZC - 1) / ZC;
   const int nt = tx * ty * tz;
   const int nw = nwave > 0 ? (d.nv + NT - 1) / NT : 0;
-  const int nbc = xcd_grid(boundary_cell_blocks(d, NT / 32) + nw) - nw;
+  const int nbc = xcd_grid(boundary_cell_blocks(d, NT) + nw) - nw;
   const unsigned grid = (unsigned)(nbc + nw + xcd_grid(nt));
   k_step3d<S, PERIODIC, RING, TX, TY, ZC, NT><<<grid, NT, 0, d.stream>>>(i, o, tx, ty, nt, nbc, nw, la, fa);
 }

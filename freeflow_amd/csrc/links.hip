@@ -49,10 +49,7 @@ __global__ void __launch_bounds__(256) k_links_build(LinkArgs la) {
 // Invalidate links detected ahead for a boundary that has since moved: open a
 // fresh tag (entries under older tags read as absent).
 __global__ void k_tag_bump(StepState* st) {
-  if (threadIdx.x == 0) {
-    st->tag = st->tag + 1;
-    st->count[st->tag & 1] = 0;
-  }
+  if (threadIdx.x == 0) st->tag = st->tag + 1;
 }
 
 int launch_detect(Domain& d, const LinkArgs& la, bool bump) {
@@ -93,16 +90,15 @@ int detect_blocks(const Domain& d) {
   return (int)(((long long)d.ne * (d.q - 1) + per - 1) / per);
 }
 
-// Copy-back blocks of the sums launch: grid-stride over the step's boundary
-// cells, at least one block (it also opens the next tag).
+// Copy-back blocks of the sums launch and boundary-cell blocks of the stream
+// launch: both scan the link table's slots (grid stride), so the block
+// counts follow its capacity.
 int copyback_blocks(const Domain& d, int fb) {
-  const unsigned long long est = std::max<unsigned long long>(d.lt_est, 1);
-  return (int)std::min<unsigned long long>(256, (est + fb - 1) / fb);
+  return (int)std::min<long long>(256, std::max<long long>(1, ((long long)d.lt_cap + fb - 1) / fb));
 }
 
-int boundary_cell_blocks(const Domain& d, int cells_per_block) {
-  return (int)std::min<unsigned long long>(512, std::max<unsigned long long>(1, (d.lt_est / 4 + cells_per_block - 1) /
-                                                                                    cells_per_block));
+int boundary_cell_blocks(const Domain& d, int nt) {
+  return (int)std::min<long long>(512, std::max<long long>(1, ((long long)d.lt_cap + nt - 1) / nt));
 }
 
 int launch_step_sums(Domain& d, const LinkArgs& la, void* out, bool detect_next) {

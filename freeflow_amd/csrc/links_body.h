@@ -44,6 +44,8 @@ __device__ __forceinline__ void cell_pops(const S* in, size_t plane, size_t c, f
 // The result is written to the cell's side-buffer row (la.sb), not to the
 // output moments: the stream-collide tiles of the same launch also write the
 // cell, and the element-sum launch copies the side buffer over them.
+// Each block scans NT link-table slots at a time (grid stride nblk * NT),
+// compacts the occupied ones in LDS and updates those cells CPB at a time.
 template <int DIM, typename S, bool STAGE, bool PERIODIC, int NT>
 __device__ __forceinline__ void bcells_body(const S* __restrict__ in, const LinkArgs& la, int blk, int nblk) {
   constexpr int Q = DIM == 2 ? 9 : 27;
@@ -51,11 +53,11 @@ __device__ __forceinline__ void bcells_body(const S* __restrict__ in, const Link
   constexpr int NM = DIM == 2 ? 6 : 10;
   constexpr int LPC = DIM == 2 ? 16 : 32;  // lanes per cell
   constexpr int CPB = NT / LPC;            // cells per block
+  constexpr int NW = NT / 64;
   __shared__ float sp[CPB][Q];
+  __shared__ int s_slot[NT], s_cell[NT], s_wn[NW];
   const unsigned gen = la.st->gen;
   const unsigned tag = la.st->tag;
-  const int count = la.st->count[tag & 1];
-  const int* list = la.list + (size_t)(tag & 1) * la.cap;
   const float* v = la.v[gen & 1];
   const float* u = la.u[gen & 1];
   const int lc = threadIdx.x / LPC, dir = threadIdx.x % LPC;
@@ -63,124 +65,145 @@ __device__ __forceinline__ void bcells_body(const S* __restrict__ in, const Link
   const unsigned long long* keys = table_keys<DIM>(la, tag);
   if (blk == 0 && threadIdx.x == 0) {
     // open the next step's tag: its links are detected in the element-sum
-    // launch into the other table (this step's list parity is untouched)
+    // launch into the other table
     la.st->tag_next = tag + 1;
     la.st->vnext = (gen + 1) & 1;
-    la.st->count[(tag + 1) & 1] = 0;
   }
-  for (int base = blk * CPB; base < count; base += nblk * CPB) {
-    const int t = base + lc;
-    int cell = 0;
-    if (t < count && dir < Q) {
-      const int slot = list[t];
-      cell = (int)(uint32_t)cells[slot];
-      const int x = cell % la.nx, y = (cell / la.nx) % la.ny, z = DIM == 3 ? cell / (la.nx * la.ny) : 0;
-      const int dx = DIM == 2 ? D2::cx(dir) : D3::cx(dir);
-      const int dy = DIM == 2 ? D2::cy(dir) : D3::cy(dir);
-      const int dz = DIM == 2 ? 0 : D3::cz(dir);
-      const int inv = DIM == 2 ? D2::inv(dir) : D3::inv(dir);
-      const int x1 = x - dx, y1 = y - dy, z1 = z - dz;
-      const bool notCross = x1 >= 0 && x1 < la.nx && y1 >= 0 && y1 < la.ny && z1 >= 0 && z1 < la.nz;
-      const int xw = (x1 + la.nx) % la.nx, yw = (y1 + la.ny) % la.ny, zw = (z1 + la.nz) % la.nz;
-      int sx, sy, sz;  // pull source: clamped halo (reference) or periodic
-      if (PERIODIC) {
-        sx = xw; sy = yw; sz = zw;
-      } else {
-        sx = min(la.nx - 1, max(0, x1));
-        sy = min(la.ny - 1, max(0, y1));
-        sz = min(la.nz - 1, max(0, z1));
-      }
-      const size_t sc = ((size_t)sz * la.ny + sy) * la.nx + sx;
-      const size_t li = (size_t)slot * K + (dir - 1);
-      // every load that depends only on the cell index is issued up front
-      const uint8_t fup = la.flags[((size_t)zw * la.ny + yw) * la.nx + xw];
-      const uint32_t key = dir > 0 ? lt_key(keys, li, tag) : 0xFFFFFFFFu;
-      float mc[NM];
+  for (int base = blk * NT; base < la.cap; base += nblk * NT) {
+    // occupied slots of this chunk, in slot order
+    const int s = base + (int)threadIdx.x;
+    const unsigned long long ent = s < la.cap ? cells[s] : 0ull;
+    const bool occ = s < la.cap && (unsigned)(ent >> 32) == tag;
+    const unsigned long long m = __ballot(occ);
+    const int w = (int)(threadIdx.x >> 6);
+    if ((threadIdx.x & 63u) == 0) s_wn[w] = (int)__popcll(m);
+    __syncthreads();
+    int off = 0, count = 0;
 #pragma unroll
-      for (int q = 0; q < NM; q++) mc[q] = Store<S>::template load<DIM>(in + q * la.plane + cell, q);
-      float fn[Q];
-      cell_pops<DIM, S, STAGE>(in, la.plane, sc, fn);
-      float fc[Q];
-      cell_pops<DIM, S, STAGE>(in, la.plane, (size_t)cell, fc);
-      float pop = 0.0f;
-      bool done = false;
-      if (DIM == 3 && is_wall3(fup)) {  // 3D keeps flag bounce-back (FsiCoupling3D.cu:209-222)
-        pop = fmaxf(0.0f, fc[inv]);
-        done = true;
-      }
-      if (dir > 0) la.lb[li * DIM] = kLinkUnused;
-      if (!done && dir > 0 && notCross && (DIM == 3 || !is_wall2(fup)) && key != 0xFFFFFFFFu) {
-        const int elem = (int)(key & 0xFFFFu);
-        Hit h;
-        bool hit;
-        if constexpr (DIM == 2) {
-          hit = hit_elem2(v, la.e, elem, (float)x * la.cl, (float)y * la.cl, (float)xw * la.cl, (float)yw * la.cl, h);
+    for (int i = 0; i < NW; i++) {
+      off += i < w ? s_wn[i] : 0;
+      count += s_wn[i];
+    }
+    if (occ) {
+      const int r = off + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+      s_slot[r] = s;
+      s_cell[r] = (int)(uint32_t)ent;
+    }
+    __syncthreads();
+    for (int c0 = 0; c0 < count; c0 += CPB) {
+      const int t = c0 + lc;
+      int cell = 0;
+      if (t < count && dir < Q) {
+        const int slot = s_slot[t];
+        cell = s_cell[t];
+        const int x = cell % la.nx, y = (cell / la.nx) % la.ny, z = DIM == 3 ? cell / (la.nx * la.ny) : 0;
+        const int dx = DIM == 2 ? D2::cx(dir) : D3::cx(dir);
+        const int dy = DIM == 2 ? D2::cy(dir) : D3::cy(dir);
+        const int dz = DIM == 2 ? 0 : D3::cz(dir);
+        const int inv = DIM == 2 ? D2::inv(dir) : D3::inv(dir);
+        const int x1 = x - dx, y1 = y - dy, z1 = z - dz;
+        const bool notCross = x1 >= 0 && x1 < la.nx && y1 >= 0 && y1 < la.ny && z1 >= 0 && z1 < la.nz;
+        const int xw = (x1 + la.nx) % la.nx, yw = (y1 + la.ny) % la.ny, zw = (z1 + la.nz) % la.nz;
+        int sx, sy, sz;  // pull source: clamped halo (reference) or periodic
+        if (PERIODIC) {
+          sx = xw; sy = yw; sz = zw;
         } else {
-          const float p1[3] = {(float)x * la.cl, (float)y * la.cl, (float)z * la.cl};
-          const float p2[3] = {(float)xw * la.cl, (float)yw * la.cl, (float)zw * la.cl};
-          hit = hit_elem3(v, la.e, elem, p1, p2, h);
+          sx = min(la.nx - 1, max(0, x1));
+          sy = min(la.ny - 1, max(0, y1));
+          sz = min(la.nz - 1, max(0, z1));
         }
-        if (hit) {
+        const size_t sc = ((size_t)sz * la.ny + sy) * la.nx + sx;
+        const size_t li = (size_t)slot * K + (dir - 1);
+        // every load that depends only on the cell index is issued up front
+        const uint8_t fup = la.flags[((size_t)zw * la.ny + yw) * la.nx + xw];
+        const uint32_t key = dir > 0 ? lt_key(keys, li, tag) : 0xFFFFFFFFu;
+        float mc[NM];
+#pragma unroll
+        for (int q = 0; q < NM; q++) mc[q] = Store<S>::template load<DIM>(in + q * la.plane + cell, q);
+        float fn[Q];
+        cell_pops<DIM, S, STAGE>(in, la.plane, sc, fn);
+        float fc[Q];
+        cell_pops<DIM, S, STAGE>(in, la.plane, (size_t)cell, fc);
+        float pop = 0.0f;
+        bool done = false;
+        if (DIM == 3 && is_wall3(fup)) {  // 3D keeps flag bounce-back (FsiCoupling3D.cu:209-222)
+          pop = fmaxf(0.0f, fc[inv]);
           done = true;
-          const float fin = fmaxf(0.0f, fc[inv]);
-          float uw[3] = {0.f, 0.f, 0.f};
-          float fout;
+        }
+        if (dir > 0) la.lb[li * DIM] = kLinkUnused;
+        if (!done && dir > 0 && notCross && (DIM == 3 || !is_wall2(fup)) && key != 0xFFFFFFFFu) {
+          const int elem = (int)(key & 0xFFFFu);
+          Hit h;
+          bool hit;
           if constexpr (DIM == 2) {
-            // getVelocityAt<2> (:173-180), then `ux *= ct / cl` (FsiCoupling2D.cu:200-201)
-            const uint32_t v1 = la.e[2 * elem], v2 = la.e[2 * elem + 1];
-            uw[0] = u[2 * v1] * h.b[0] + u[2 * v2] * h.b[1];
-            uw[1] = u[2 * v1 + 1] * h.b[0] + u[2 * v2 + 1] * h.b[1];
-            uw[0] = uw[0] * (la.ct / la.cl);
-            uw[1] = uw[1] * (la.ct / la.cl);
-            const float pixx = mc[3] + uw[0] * uw[0] - mc[1] * mc[1];
-            const float piyy = mc[4] + uw[1] * uw[1] - mc[2] * mc[2];
-            const float pixy = mc[5] + uw[1] * uw[0] - mc[1] * mc[2];
-            fout = recon2_at(mc[0], uw[0], uw[1], pixx, pixy, piyy, dir);
+            hit = hit_elem2(v, la.e, elem, (float)x * la.cl, (float)y * la.cl, (float)xw * la.cl, (float)yw * la.cl, h);
           } else {
-            // getVelocityAt<3> (:385-398), then `u * ct / cl` (FsiCoupling3D.cu:244)
-            const uint32_t v1 = la.e[3 * elem], v2 = la.e[3 * elem + 1], v3 = la.e[3 * elem + 2];
-#pragma unroll
-            for (int c = 0; c < 3; c++)
-              uw[c] = ((u[3 * v1 + c] * h.b[0] + u[3 * v2 + c] * h.b[1] + u[3 * v3 + c] * h.b[2]) * la.ct) / la.cl;
-            float ms[10];
-            ms[0] = mc[0];
-            ms[1] = uw[0];
-            ms[2] = uw[1];
-            ms[3] = uw[2];
-            ms[4] = mc[4] + uw[0] * uw[0] - mc[1] * mc[1];
-            ms[5] = mc[5] + uw[1] * uw[1] - mc[2] * mc[2];
-            ms[6] = mc[6] + uw[2] * uw[2] - mc[3] * mc[3];
-            ms[7] = mc[7] + uw[0] * uw[1] - mc[1] * mc[2];
-            ms[8] = mc[8] + uw[1] * uw[2] - mc[2] * mc[3];
-            ms[9] = mc[9] + uw[0] * uw[2] - mc[1] * mc[3];
-            fout = recon3_at(ms, dir);
+            const float p1[3] = {(float)x * la.cl, (float)y * la.cl, (float)z * la.cl};
+            const float p2[3] = {(float)xw * la.cl, (float)yw * la.cl, (float)zw * la.cl};
+            hit = hit_elem3(v, la.e, elem, p1, p2, h);
           }
-          pop = fout;
-          const float cc[3] = {(float)dx, (float)dy, (float)dz};
+          if (hit) {
+            done = true;
+            const float fin = fmaxf(0.0f, fc[inv]);
+            float uw[3] = {0.f, 0.f, 0.f};
+            float fout;
+            if constexpr (DIM == 2) {
+              // getVelocityAt<2> (:173-180), then `ux *= ct / cl` (FsiCoupling2D.cu:200-201)
+              const uint32_t v1 = la.e[2 * elem], v2 = la.e[2 * elem + 1];
+              uw[0] = u[2 * v1] * h.b[0] + u[2 * v2] * h.b[1];
+              uw[1] = u[2 * v1 + 1] * h.b[0] + u[2 * v2 + 1] * h.b[1];
+              uw[0] = uw[0] * (la.ct / la.cl);
+              uw[1] = uw[1] * (la.ct / la.cl);
+              const float pixx = mc[3] + uw[0] * uw[0] - mc[1] * mc[1];
+              const float piyy = mc[4] + uw[1] * uw[1] - mc[2] * mc[2];
+              const float pixy = mc[5] + uw[1] * uw[0] - mc[1] * mc[2];
+              fout = recon2_at(mc[0], uw[0], uw[1], pixx, pixy, piyy, dir);
+            } else {
+              // getVelocityAt<3> (:385-398), then `u * ct / cl` (FsiCoupling3D.cu:244)
+              const uint32_t v1 = la.e[3 * elem], v2 = la.e[3 * elem + 1], v3 = la.e[3 * elem + 2];
 #pragma unroll
-          for (int c = 0; c < DIM; c++) {
-            la.lf[li * DIM + c] = (fin * (-cc[c] - uw[c]) - fout * (cc[c] - uw[c])) * la.cf;
-            la.lb[li * DIM + c] = h.b[c];
-            la.lh[li * DIM + c] = h.hit[c];
+              for (int c = 0; c < 3; c++)
+                uw[c] = ((u[3 * v1 + c] * h.b[0] + u[3 * v2 + c] * h.b[1] + u[3 * v3 + c] * h.b[2]) * la.ct) / la.cl;
+              float ms[10];
+              ms[0] = mc[0];
+              ms[1] = uw[0];
+              ms[2] = uw[1];
+              ms[3] = uw[2];
+              ms[4] = mc[4] + uw[0] * uw[0] - mc[1] * mc[1];
+              ms[5] = mc[5] + uw[1] * uw[1] - mc[2] * mc[2];
+              ms[6] = mc[6] + uw[2] * uw[2] - mc[3] * mc[3];
+              ms[7] = mc[7] + uw[0] * uw[1] - mc[1] * mc[2];
+              ms[8] = mc[8] + uw[1] * uw[2] - mc[2] * mc[3];
+              ms[9] = mc[9] + uw[0] * uw[2] - mc[1] * mc[3];
+              fout = recon3_at(ms, dir);
+            }
+            pop = fout;
+            const float cc[3] = {(float)dx, (float)dy, (float)dz};
+#pragma unroll
+            for (int c = 0; c < DIM; c++) {
+              la.lf[li * DIM + c] = (fin * (-cc[c] - uw[c]) - fout * (cc[c] - uw[c])) * la.cf;
+              la.lb[li * DIM + c] = h.b[c];
+              la.lh[li * DIM + c] = h.hit[c];
+            }
           }
         }
+        if (!done) pop = fmaxf(0.0f, fn[dir]);
+        sp[lc][dir] = pop;
       }
-      if (!done) pop = fmaxf(0.0f, fn[dir]);
-      sp[lc][dir] = pop;
-    }
-    __syncthreads();
-    if (t < count && dir == 0 && is_updated(la.flags[cell])) {
-      float p[Q];
+      __syncthreads();
+      if (t < count && dir == 0 && is_updated(la.flags[cell])) {
+        float p[Q];
 #pragma unroll
-      for (int i = 0; i < Q; i++) p[i] = sp[lc][i];
-      float o[NM];
-      if constexpr (DIM == 2) collide2(p, la.omega, o);
-      else collide3(p, la.omega, o);
-      float* row = la.sb + (size_t)list[t] * NM;
+        for (int i = 0; i < Q; i++) p[i] = sp[lc][i];
+        float o[NM];
+        if constexpr (DIM == 2) collide2(p, la.omega, o);
+        else collide3(p, la.omega, o);
+        float* row = la.sb + (size_t)s_slot[t] * NM;
 #pragma unroll
-      for (int q = 0; q < NM; q++) row[q] = o[q];
+        for (int q = 0; q < NM; q++) row[q] = o[q];
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 
@@ -223,7 +246,7 @@ __host__ __device__ inline int elem_blocks(int ne) { return (ne + kElemPerBlock 
 // kinematics into vn/un, per-vertex forces optionally added to proxy_forces).
 FinalArgs make_final_args(Domain& d, void* out, float* vn0, float* vn1, float* un0, float* un1, float* proxy_forces);
 // Host: grid sizes of the boundary-cell and copy-back parts of a step.
-int boundary_cell_blocks(const Domain& d, int cells_per_block);
+int boundary_cell_blocks(const Domain& d, int nt);
 int copyback_blocks(const Domain& d, int fb);
 int detect_blocks(const Domain& d);
 
@@ -375,13 +398,13 @@ __device__ __forceinline__ void elem_sums_body(const LinkArgs& la, const FinalAr
   if (blk >= neb + fa.outlet_blocks) {
     const unsigned tag = la.st->tag;
     const int b = blk - neb - fa.outlet_blocks;
-    const int count = la.st->count[tag & 1];
     const int nb = fa.copy_blocks;
     const unsigned long long* cells = table_cells(la, tag);
     S* o = (S*)out;
-    for (int t = b * FB + threadIdx.x; t < count; t += nb * FB) {
-      const int slot = la.list[(size_t)(tag & 1) * la.cap + t];
-      const int cell = (int)(uint32_t)cells[slot];
+    for (int slot = b * FB + threadIdx.x; slot < la.cap; slot += nb * FB) {
+      const unsigned long long ent = cells[slot];
+      if ((unsigned)(ent >> 32) != tag) continue;
+      const int cell = (int)(uint32_t)ent;
       if (!is_updated(la.flags[cell])) continue;
       const float* row = la.sb + (size_t)slot * NM;
 #pragma unroll
