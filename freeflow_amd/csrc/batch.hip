@@ -132,7 +132,7 @@ __global__ void __launch_bounds__(FB) kb_finish(const DomDev* __restrict__ doms,
     finish_vertex_body<DIM, FB>(D->la, D->fa, m.y);
     return;
   }
-  if (D->boundary) finish_last_body<DIM, FB>(D->la, D->fa);
+  if (D->boundary) finish_last_body<DIM, FB>(D->la, D->fa, D->la.st->gen, true);
   if (threadIdx.x == 0) *D->step = *D->step + 1u;
 }
 
@@ -267,6 +267,7 @@ static int build_tables(Batch& b, int tile) {
     std::memset(&r, 0, sizeof(r));
     const bool bnd = d.has_boundary && d.ne > 0;
     r.la = link_args(d, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], d.elems);
+    r.la.dev_state = 1;  // graph replays advance gen and tag on the device
     r.fa = make_final_args(d, nullptr, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], nullptr);
     r.mom[0] = d.mom[d.cur];
     r.mom[1] = d.mom[d.cur ^ 1];
@@ -420,9 +421,11 @@ static int detect_pending(Batch& b) {
   for (Domain* dp : b.mem) {
     Domain& d = *dp;
     if (!(d.has_boundary && d.ne > 0) || d.predetected) continue;
-    const LinkArgs la = link_args(d, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], d.elems);
+    LinkArgs la = link_args(d, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], d.elems);
+    la.dev_state = 1;
     int rc = launch_detect(d, la, d.stale);
     if (rc) return rc;
+    if (d.stale) d.tag_host++;  // mirror of the device bump
     d.stale = false;
     d.predetected = true;
   }
@@ -436,6 +439,7 @@ static void advance_mirrors(Batch& b, int nsteps) {
     if (nsteps & 1) d.cur ^= 1;
     if (d.has_boundary && d.ne > 0) {
       d.gen_host += (unsigned)nsteps;
+      d.tag_host += (unsigned)nsteps;
       if (d.hist_cap > 0) d.hist_count += nsteps;
       d.predetected = true;
     }
@@ -499,8 +503,17 @@ int ff_batch_add(ff_batch* b, ff_domain* dom) {
   if (d.dim != b->dim || d.storage != b->storage || d.options != b->options || d.device != b->device)
     return fail(FF_ERR_INVALID, "domain kind (dim, storage, options, device) differs from the batch");
   FF_HIP(hipSetDevice(b->device));
+  if (int rc = flush_finish(d)) return rc;
   FF_HIP(hipStreamSynchronize(d.stream));
   FF_HIP(hipStreamSynchronize(b->stream));
+  if (d.has_boundary) {
+    // batched steps read and advance gen and tag on the device
+    StepState st;
+    FF_HIP(hipMemcpy(&st, d.st, sizeof(st), hipMemcpyDeviceToHost));
+    st.gen = d.gen_host;
+    st.tag = d.tag_host;
+    FF_HIP(hipMemcpy(d.st, &st, sizeof(st), hipMemcpyHostToDevice));
+  }
   d.own_stream = d.stream;
   d.stream = b->stream;
   d.batch = b;

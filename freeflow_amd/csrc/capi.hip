@@ -50,6 +50,8 @@ static void free_boundary(Domain& d) {
   dfree(d.st); dfree(d.rest_verts); dfree(d.s_coord); dfree(d.wave_params); dfree(d.vert_body);
   d.nv = d.ne = d.n_bodies = 0;
   d.gen_host = 1;
+  d.tag_host = 1;
+  d.finish_pending = false;
   d.has_boundary = false;
   d.wave = false;
   d.hist_cap = 0;
@@ -233,6 +235,9 @@ LinkArgs link_args(Domain& d, const float* v0, const float* v1, const float* u0,
   la.sb = d.lk_sb;
   la.cap = d.lt_cap;
   la.st = d.st;
+  la.gen = d.gen_host;
+  la.tag = d.tag_host;
+  la.dev_state = 0;
   la.flags = d.flags;
   la.nx = d.nx;
   la.ny = d.ny;
@@ -255,8 +260,10 @@ static void links_moved(Domain& d) {
 // [detect (first step or after the boundary moved)] -> [stream: tiles +
 // boundary cells + next wave kinematics] -> [sums: element forces, outlets,
 // boundary-cell copy-back, next step's links] -> [finish: vertex forces,
-// wrench].  `pv` != null selects caller-owned proxy arrays (positions not
-// known ahead: detected every step).
+// wrench], the finish deferred into the next step's stream launch (or run
+// before anything reads its results).  `pv` != null selects caller-owned
+// proxy arrays (positions not known ahead: detected every step; finish run
+// at once).
 static int one_step(Domain& d, const float* pv, const float* pu, const uint32_t* pe, float* proxy_forces) {
   void* in = d.mom[d.cur];
   void* out = d.mom[d.cur ^ 1];
@@ -267,18 +274,29 @@ static int one_step(Domain& d, const float* pv, const float* pu, const uint32_t*
     return launch_outlet(d, d.mom[d.cur]);
   }
   const bool proxy = pv != nullptr;
-  const LinkArgs la = proxy ? link_args(d, pv, pv, pu, pu, pe) : link_args(d, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], d.elems);
   if (proxy) links_moved(d);
+  // links detected ahead for a boundary moved since: a fresh tag drops them
+  if (!d.predetected && d.stale) d.tag_host++;
+  const LinkArgs la = proxy ? link_args(d, pv, pv, pu, pu, pe) : link_args(d, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], d.elems);
   if (!d.predetected) {
-    if ((rc = launch_detect(d, la, d.stale))) return rc;
+    if ((rc = launch_detect(d, la, false))) return rc;
     d.stale = false;
   }
+  // the stream launch also runs the previous step's deferred finish
   if ((rc = launch_step_stream(d, in, out, la, d.wave && !proxy))) return rc;
+  d.finish_pending = false;
   if ((rc = launch_step_sums(d, la, out, !proxy))) return rc;
-  if ((rc = launch_step_finish(d, la, proxy_forces))) return rc;
+  if (proxy) {
+    // caller-owned force arrays are filled by the step itself
+    if ((rc = launch_step_finish(d, la, d.gen_host, proxy_forces))) return rc;
+  } else {
+    d.finish_pending = true;
+    d.finish_gen = d.gen_host;
+  }
   d.predetected = !proxy;
   d.cur ^= 1;
   d.gen_host++;
+  d.tag_host++;
   if (d.hist_cap > 0) d.hist_count++;
   return FF_OK;
 }
@@ -582,6 +600,7 @@ int ff_set_boundary(ff_domain* dd, int nv, const float* verts, const float* vels
   st0.tag = 1;
   FF_HIP(hipMemcpy(d.st, &st0, sizeof(st0), hipMemcpyHostToDevice));
   d.gen_host = 1;
+  d.tag_host = 1;
   // the memsets above run on the null stream; the domain stream is non-blocking
   FF_HIP(hipDeviceSynchronize());
   d.has_boundary = true;
@@ -672,6 +691,7 @@ int ff_get_vertex_forces(ff_domain* dd, float* out, int reset) {
   if (!d.has_boundary) return fail(FF_ERR_STATE, "no boundary set");
   if (!out) return fail(FF_ERR_INVALID, "null output");
   FF_HIP(hipSetDevice(d.device));
+  if (int frc = flush_finish(d)) return frc;  // the last step's deferred finish
   FF_HIP(hipStreamSynchronize(d.stream));
   const size_t bytes = (size_t)d.nv * d.dim * sizeof(float);
   FF_HIP(hipMemcpy(out, d.vforce, bytes, hipMemcpyDeviceToHost));
@@ -712,6 +732,7 @@ int ff_get_body_wrench(ff_domain* dd, double* out) {
   if (!d.has_boundary) return fail(FF_ERR_STATE, "no boundary set");
   if (!out) return fail(FF_ERR_INVALID, "null output");
   FF_HIP(hipSetDevice(d.device));
+  if (int frc = flush_finish(d)) return frc;  // the last step's deferred finish
   FF_HIP(hipStreamSynchronize(d.stream));
   FF_HIP(hipMemcpy(out, d.wrench, (size_t)d.n_bodies * wrench_width(d.dim) * sizeof(double), hipMemcpyDeviceToHost));
   return check_overflow(d);
@@ -723,6 +744,7 @@ int ff_set_wrench_history(ff_domain* dd, int capacity) {
   if (!d.has_boundary) return fail(FF_ERR_STATE, "no boundary set");
   if (capacity < 0) return fail(FF_ERR_INVALID, "negative capacity");
   FF_HIP(hipSetDevice(d.device));
+  if (int frc = flush_finish(d)) return frc;  // the last step's deferred finish
   FF_HIP(hipStreamSynchronize(d.stream));
   batch_mark_dirty(d.batch);
   dfree(d.wrench_hist);
@@ -740,6 +762,7 @@ int ff_get_wrench_history(ff_domain* dd, double* out, int count, int* n_out) {
   Domain& d = *dd;
   if (!out || !n_out || count < 0) return fail(FF_ERR_INVALID, "bad output");
   FF_HIP(hipSetDevice(d.device));
+  if (int frc = flush_finish(d)) return frc;  // the last step's deferred finish
   FF_HIP(hipStreamSynchronize(d.stream));
   const long long avail = std::min<long long>(d.hist_count, d.hist_cap);
   const int n = (int)std::min<long long>(avail, count);
@@ -813,6 +836,7 @@ int ff_time_bulk_kernel(ff_domain* dd, int iters, double* avg_ms) {
   Domain& d = *dd;
   if (d.batch) return fail(FF_ERR_STATE, "domain belongs to a batch");
   FF_HIP(hipSetDevice(d.device));
+  if (int frc = flush_finish(d)) return frc;  // the last step's deferred finish
   std::vector<hipEvent_t> ev(2 * iters);
   for (auto& e : ev) FF_HIP(hipEventCreate(&e));
   void* in = d.mom[d.cur];
@@ -852,6 +876,7 @@ int ff_time_steps(ff_domain* dd, int nsteps, double* total_ms) {
   FF_HIP(hipEventRecord(a, d.stream));
   int rc = FF_OK;
   for (int s = 0; s < nsteps && !rc; s++) rc = one_step(d, nullptr, nullptr, nullptr, nullptr);
+  if (!rc) rc = flush_finish(d);  // the timed work includes the last step's finish
   FF_HIP(hipEventRecord(b, d.stream));
   FF_HIP(hipEventSynchronize(b));
   float ms = 0.f;

@@ -46,11 +46,9 @@ struct OutletCell {
 // Step state kept on the device so a step's launches need no host-side
 // arguments that change from step to step (they can be replayed as a graph).
 struct StepState {
-  unsigned gen;       // steps since the boundary was set + 1 (vertex-buffer parity, wrench ring slot)
+  unsigned gen;       // batched domains: steps since the boundary was set + 1
+  unsigned tag;       // batched domains: link-table tag of the current step's links
   int overflow;       // sticky: link table full
-  unsigned tag;       // link-table tag of the current step's links
-  unsigned tag_next;  // tag the next step's links are detected under
-  unsigned vnext;     // vertex-buffer parity of the next step (set with tag_next)
   int pad;
 };
 
@@ -68,6 +66,12 @@ struct LinkArgs {
   float* sb;                  // cap * NM  updated moments of each boundary cell (copied back after the step)
   int cap;
   StepState* st;
+  // Step generation (vertex-buffer parity gen & 1, wrench ring slot) and
+  // link-table tag of the step: passed by value for a domain stepped on its
+  // own (the host keeps them), read from st for batched domains (graph
+  // replays advance them on the device).
+  unsigned gen, tag;
+  int dev_state;
   const uint8_t* flags;
   int nx, ny, nz;
   float cl, ct, cf, omega;
@@ -122,7 +126,10 @@ struct Domain {
   float* lk_h = nullptr;
   float* lk_sb = nullptr;         // boundary-cell results (lt_cap * nm)
   StepState* st = nullptr;
-  unsigned gen_host = 1;          // mirror of st->gen
+  unsigned gen_host = 1;          // generation of the next step (st->gen while batched)
+  unsigned tag_host = 1;          // link-table tag of the next step (st->tag while batched)
+  bool finish_pending = false;    // the last step's vertex forces / wrench are not yet summed
+  unsigned finish_gen = 0;        // generation of that step
   // the links of the next step were detected by the previous step's finish
   // launch; `stale` when the boundary moved since (detect again, new tag)
   bool predetected = false;
@@ -162,7 +169,8 @@ int launch_bulk3d(Domain& d, const void* in, void* out);
 int launch_detect(Domain& d, const LinkArgs& la, bool bump);
 int launch_step_stream(Domain& d, const void* in, void* out, const LinkArgs& la, bool wave);
 int launch_step_sums(Domain& d, const LinkArgs& la, void* out, bool detect_next);
-int launch_step_finish(Domain& d, const LinkArgs& la, float* proxy_forces);
+int launch_step_finish(Domain& d, const LinkArgs& la, unsigned gen, float* proxy_forces);
+int flush_finish(Domain& d);
 int launch_outlet(Domain& d, void* buf);
 int launch_wave_now(Domain& d);
 int launch_step_vertex_forces(Domain& d, float* out_dev);

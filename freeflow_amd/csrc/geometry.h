@@ -33,6 +33,10 @@ __device__ __forceinline__ unsigned mix32(unsigned h) {
 
 // Returns the slot of `cell` in this generation, inserting it if absent
 // (the first inserter appends the slot to `list`).  -1 on a full table.
+// Generation and tag of the step being run (see LinkArgs).
+__device__ __forceinline__ unsigned step_gen(const LinkArgs& la) { return la.dev_state ? la.st->gen : la.gen; }
+__device__ __forceinline__ unsigned step_tag(const LinkArgs& la) { return la.dev_state ? la.st->tag : la.tag; }
+
 // Two link tables alternate by tag parity: the next step's links are detected
 // into one while the current step's are read from the other.
 __device__ __forceinline__ unsigned long long* table_cells(const LinkArgs& la, unsigned tag) {

@@ -43,7 +43,7 @@ __global__ void __launch_bounds__(256)
   if (b < nbc) {
     bcells_body<2, S, STAGE, PERIODIC, 256>(in, la, b, nbc);
   } else if (b < nbc + nwave) {
-    wave_block_body<2, 256>(la, fa, b - nbc);
+    stream_extra_body<2, 256>(la, fa, b - nbc);
   } else {
     const int t = xcd_tile(b - nbc - nwave, ntiles);
     if (t < 0) return;
@@ -108,13 +108,15 @@ static int bulk2d_dispatch(Domain& d, const void* in, void* out, const StreamExt
 
 int launch_bulk2d(Domain& d, const void* in, void* out) { return bulk2d_dispatch(d, in, out, nullptr); }
 
-int launch_step_stream3d(Domain& d, const void* in, void* out, const LinkArgs& la, const FinalArgs& fa, int nwave);
+int launch_step_stream3d(Domain& d, const void* in, void* out, const LinkArgs& la, FinalArgs fa, bool wave);
 
 int launch_step_stream(Domain& d, const void* in, void* out, const LinkArgs& la, bool wave) {
-  const FinalArgs fa = make_final_args(d, out, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], nullptr);
-  const int nwave = wave ? (d.nv + 255) / 256 : 0;
-  if (d.dim == 3) return launch_step_stream3d(d, in, out, la, fa, nwave);
-  const StreamExtra x{&la, &fa, boundary_cell_blocks(d, 256), nwave};
+  FinalArgs fa = make_final_args(d, out, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], nullptr);
+  fa.fin_gen = d.finish_gen;
+  if (d.dim == 3) return launch_step_stream3d(d, in, out, la, fa, wave);
+  fa.kin_blocks = wave ? (d.nv + 255) / 256 : 0;
+  fa.fin_blocks = d.finish_pending ? (d.nv + 255) / 256 + 1 : 0;
+  const StreamExtra x{&la, &fa, boundary_cell_blocks(d, 256), fa.kin_blocks + fa.fin_blocks};
   return bulk2d_dispatch(d, in, out, &x);
 }
 

@@ -63,7 +63,7 @@ __global__ void __launch_bounds__(NT)
   if (b < nbc) {
     bcells_body<3, S, false, PERIODIC, NT>(in, la, b, nbc);
   } else if (b < nbc + nwave) {
-    wave_block_body<3, NT>(la, fa, b - nbc);
+    stream_extra_body<3, NT>(la, fa, b - nbc);
   } else {
     const int t = xcd_tile(b - nbc - nwave, ntiles);
     if (t < 0) return;
@@ -73,41 +73,43 @@ __global__ void __launch_bounds__(NT)
 }
 
 template <typename S, bool PERIODIC, int RING, int TX, int TY, int ZC, int NT>
-static void step3d_shape(Domain& d, const S* i, S* o, const LinkArgs& la, const FinalArgs& fa, int nwave) {
+static void step3d_shape(Domain& d, const S* i, S* o, const LinkArgs& la, FinalArgs fa, bool wave) {
   const int tx = (d.nx + TX - 1) / TX, ty = (d.ny + TY - 1) / TY, tz = (d.nz + ZC - 1) / ZC;
   const int nt = tx * ty * tz;
-  const int nw = nwave > 0 ? (d.nv + NT - 1) / NT : 0;
+  fa.kin_blocks = wave ? (d.nv + NT - 1) / NT : 0;
+  fa.fin_blocks = d.finish_pending ? (d.nv + NT - 1) / NT + 1 : 0;
+  const int nw = fa.kin_blocks + fa.fin_blocks;
   const int nbc = xcd_grid(boundary_cell_blocks(d, NT) + nw) - nw;
   const unsigned grid = (unsigned)(nbc + nw + xcd_grid(nt));
   k_step3d<S, PERIODIC, RING, TX, TY, ZC, NT><<<grid, NT, 0, d.stream>>>(i, o, tx, ty, nt, nbc, nw, la, fa);
 }
 
 template <typename S, bool PERIODIC>
-static void step3d_variant(Domain& d, const S* i, S* o, const LinkArgs& la, const FinalArgs& fa, int nwave) {
+static void step3d_variant(Domain& d, const S* i, S* o, const LinkArgs& la, const FinalArgs& fa, bool wave) {
   switch (d.tile3) {
-    case 1: step3d_shape<S, PERIODIC, 0, 16, 4, 4, 256>(d, i, o, la, fa, nwave); break;
-    case 2: step3d_shape<S, PERIODIC, 1, 16, 8, 32, 128>(d, i, o, la, fa, nwave); break;
-    case 3: step3d_shape<S, PERIODIC, 1, 16, 8, 16, 128>(d, i, o, la, fa, nwave); break;
-    case 4: step3d_shape<S, PERIODIC, 1, 16, 8, 8, 128>(d, i, o, la, fa, nwave); break;
-    case 5: step3d_shape<S, PERIODIC, 1, 16, 16, 32, 256>(d, i, o, la, fa, nwave); break;
-    case 6: step3d_shape<S, PERIODIC, 1, 16, 16, 16, 256>(d, i, o, la, fa, nwave); break;
-    default: step3d_shape<S, PERIODIC, 0, 8, 8, 4, 256>(d, i, o, la, fa, nwave); break;
+    case 1: step3d_shape<S, PERIODIC, 0, 16, 4, 4, 256>(d, i, o, la, fa, wave); break;
+    case 2: step3d_shape<S, PERIODIC, 1, 16, 8, 32, 128>(d, i, o, la, fa, wave); break;
+    case 3: step3d_shape<S, PERIODIC, 1, 16, 8, 16, 128>(d, i, o, la, fa, wave); break;
+    case 4: step3d_shape<S, PERIODIC, 1, 16, 8, 8, 128>(d, i, o, la, fa, wave); break;
+    case 5: step3d_shape<S, PERIODIC, 1, 16, 16, 32, 256>(d, i, o, la, fa, wave); break;
+    case 6: step3d_shape<S, PERIODIC, 1, 16, 16, 16, 256>(d, i, o, la, fa, wave); break;
+    default: step3d_shape<S, PERIODIC, 0, 8, 8, 4, 256>(d, i, o, la, fa, wave); break;
   }
 }
 
 template <typename S>
-static int step3d_typed(Domain& d, const void* in, void* out, const LinkArgs& la, const FinalArgs& fa, int nwave) {
-  if (d.options & FF_OPT_PERIODIC) step3d_variant<S, true>(d, (const S*)in, (S*)out, la, fa, nwave);
-  else step3d_variant<S, false>(d, (const S*)in, (S*)out, la, fa, nwave);
+static int step3d_typed(Domain& d, const void* in, void* out, const LinkArgs& la, const FinalArgs& fa, bool wave) {
+  if (d.options & FF_OPT_PERIODIC) step3d_variant<S, true>(d, (const S*)in, (S*)out, la, fa, wave);
+  else step3d_variant<S, false>(d, (const S*)in, (S*)out, la, fa, wave);
   FF_HIP(hipGetLastError());
   return FF_OK;
 }
 
-int launch_step_stream3d(Domain& d, const void* in, void* out, const LinkArgs& la, const FinalArgs& fa, int nwave) {
+int launch_step_stream3d(Domain& d, const void* in, void* out, const LinkArgs& la, FinalArgs fa, bool wave) {
   switch (d.storage) {
-    case FF_STORAGE_F32: return step3d_typed<float>(d, in, out, la, fa, nwave);
-    case FF_STORAGE_F16: return step3d_typed<__half>(d, in, out, la, fa, nwave);
-    default: return step3d_typed<__hip_bfloat16>(d, in, out, la, fa, nwave);
+    case FF_STORAGE_F32: return step3d_typed<float>(d, in, out, la, fa, wave);
+    case FF_STORAGE_F16: return step3d_typed<__half>(d, in, out, la, fa, wave);
+    default: return step3d_typed<__hip_bfloat16>(d, in, out, la, fa, wave);
   }
 }
 

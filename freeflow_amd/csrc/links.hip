@@ -43,17 +43,18 @@ namespace ff {
 // Links of the current step under the current tag and vertex buffer.
 template <int DIM>
 __global__ void __launch_bounds__(256) k_links_build(LinkArgs la) {
-  build_links_block<DIM>(la, blockIdx.x, la.st->tag, la.v[la.st->gen & 1]);
+  build_links_block<DIM>(la, blockIdx.x, step_tag(la), la.v[step_gen(la) & 1]);
 }
 
-// Invalidate links detected ahead for a boundary that has since moved: open a
-// fresh tag (entries under older tags read as absent).
+// Batched domains: invalidate links detected ahead for a boundary that has
+// since moved by opening a fresh tag (entries under older tags read as
+// absent).  A domain stepped on its own has its tag advanced by the host.
 __global__ void k_tag_bump(StepState* st) {
   if (threadIdx.x == 0) st->tag = st->tag + 1;
 }
 
 int launch_detect(Domain& d, const LinkArgs& la, bool bump) {
-  if (bump) k_tag_bump<<<1, 64, 0, d.stream>>>(d.st);
+  if (bump && la.dev_state) k_tag_bump<<<1, 64, 0, d.stream>>>(d.st);
   const int K = d.q - 1;
   const long long pairs = (long long)d.ne * K;
   const unsigned g = (unsigned)((pairs + 15) / 16);
@@ -80,7 +81,7 @@ template <int DIM, int FB>
 __global__ void __launch_bounds__(FB) k_step_finish(LinkArgs la, FinalArgs fa, int nvb) {
   const int b = blockIdx.x;
   if (b < nvb) finish_vertex_body<DIM, FB>(la, fa, b);
-  else finish_last_body<DIM, FB>(la, fa);
+  else finish_last_body<DIM, FB>(la, fa, fa.fin_gen, false);
 }
 
 // Detection blocks of the element-sum launch (kSumBlock threads): one
@@ -121,14 +122,24 @@ int launch_step_sums(Domain& d, const LinkArgs& la, void* out, bool detect_next)
   return FF_OK;
 }
 
-int launch_step_finish(Domain& d, const LinkArgs& la, float* proxy_forces) {
-  const FinalArgs fa = make_final_args(d, nullptr, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], proxy_forces);
+int launch_step_finish(Domain& d, const LinkArgs& la, unsigned gen, float* proxy_forces) {
+  FinalArgs fa = make_final_args(d, nullptr, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], proxy_forces);
+  fa.fin_gen = gen;
   const int nvb = (d.nv + 255) / 256;
   const unsigned grid = (unsigned)(nvb + 1);
   if (d.dim == 2) k_step_finish<2, 256><<<grid, 256, 0, d.stream>>>(la, fa, nvb);
   else k_step_finish<3, 256><<<grid, 256, 0, d.stream>>>(la, fa, nvb);
   FF_HIP(hipGetLastError());
   return FF_OK;
+}
+
+// Run the finish phase of the last step if it was deferred (to the next
+// step's stream launch) and something is about to read its results.
+int flush_finish(Domain& d) {
+  if (!d.finish_pending) return FF_OK;
+  d.finish_pending = false;
+  return launch_step_finish(d, link_args(d, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], d.elems), d.finish_gen,
+                            nullptr);
 }
 
 FinalArgs make_final_args(Domain& d, void* out, float* vn0, float* vn1, float* un0, float* un1, float* proxy_forces) {
@@ -162,6 +173,9 @@ FinalArgs make_final_args(Domain& d, void* out, float* vn0, float* vn1, float* u
   const int fb = kSumBlock;
   fa.outlet_blocks = (d.n_outlets + fb - 1) / fb;
   fa.copy_blocks = (d.has_boundary && d.ne > 0) ? copyback_blocks(d, fb) : 0;
+  fa.kin_blocks = 0;
+  fa.fin_blocks = 0;
+  fa.fin_gen = 0;
   return fa;
 }
 

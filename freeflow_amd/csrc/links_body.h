@@ -56,19 +56,13 @@ __device__ __forceinline__ void bcells_body(const S* __restrict__ in, const Link
   constexpr int NW = NT / 64;
   __shared__ float sp[CPB][Q];
   __shared__ int s_slot[NT], s_cell[NT], s_wn[NW];
-  const unsigned gen = la.st->gen;
-  const unsigned tag = la.st->tag;
+  const unsigned gen = step_gen(la);
+  const unsigned tag = step_tag(la);
   const float* v = la.v[gen & 1];
   const float* u = la.u[gen & 1];
   const int lc = threadIdx.x / LPC, dir = threadIdx.x % LPC;
   const unsigned long long* cells = table_cells(la, tag);
   const unsigned long long* keys = table_keys<DIM>(la, tag);
-  if (blk == 0 && threadIdx.x == 0) {
-    // open the next step's tag: its links are detected in the element-sum
-    // launch into the other table
-    la.st->tag_next = tag + 1;
-    la.st->vnext = (gen + 1) & 1;
-  }
   for (int base = blk * NT; base < la.cap; base += nblk * NT) {
     // occupied slots of this chunk, in slot order
     const int s = base + (int)threadIdx.x;
@@ -235,6 +229,9 @@ struct FinalArgs {
   int n_outlets;
   int outlet_blocks;
   int copy_blocks;  // copy-back blocks of the sums launch
+  int kin_blocks;   // stream launch extras: kinematics blocks, then fin_blocks
+  int fin_blocks;   // of a deferred finish (vertex blocks + 1) of step fin_gen
+  unsigned fin_gen;
 };
 
 // Block size of the element-sum launch (2D and 3D) and elements per block
@@ -278,16 +275,17 @@ template <int DIM, int FB>
 __device__ __forceinline__ void wave_block_body(const LinkArgs& la, const FinalArgs& fa, int blk) {
   const int k = blk * FB + threadIdx.x;
   if (k < la.nv) {
-    const unsigned g = la.st->gen;
+    const unsigned g = step_gen(la);
     wave_vertex<DIM>(fa.rest, fa.s, fa.vbody, fa.wpar, k, (double)g * fa.dt, fa.vn[(g + 1) & 1], fa.un[(g + 1) & 1]);
   }
 }
 
-// Detection of the next step's links under tag_next (opened by the stream
-// launch) and the next step's vertex positions, into the other link table.
+// Detection of the next step's links (tag + 1, the next step's vertex
+// positions) into the other link table.
 template <int DIM>
 __device__ __forceinline__ void detect_next_body(const LinkArgs& la, int blk) {
-  build_links_block<DIM>(la, (long long)blk, la.st->tag_next, la.v[la.st->vnext & 1]);
+  const unsigned gen = step_gen(la);
+  build_links_block<DIM>(la, (long long)blk, step_tag(la) + 1u, la.v[(gen + 1u) & 1u]);
 }
 
 // Second launch of a boundary step, kSumBlock threads per block.  The first
@@ -306,8 +304,8 @@ __device__ __forceinline__ void elem_sum_wave(const LinkArgs& la, const FinalArg
   constexpr int K = DIM == 2 ? 8 : 26;
   constexpr int NT = DIM == 2 ? 1 : 3;
   constexpr int STRIDE = DIM * DIM + DIM + NT;
-  const unsigned gen = la.st->gen;
-  const unsigned tag = la.st->tag;
+  const unsigned gen = step_gen(la);
+  const unsigned tag = step_tag(la);
   const float* v = la.v[gen & 1];
   const int n[3] = {la.nx, la.ny, la.nz};
   int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
@@ -396,7 +394,7 @@ __device__ __forceinline__ void elem_sums_body(const LinkArgs& la, const FinalAr
     return;
   }
   if (blk >= neb + fa.outlet_blocks) {
-    const unsigned tag = la.st->tag;
+    const unsigned tag = step_tag(la);
     const int b = blk - neb - fa.outlet_blocks;
     const int nb = fa.copy_blocks;
     const unsigned long long* cells = table_cells(la, tag);
@@ -418,7 +416,7 @@ __device__ __forceinline__ void elem_sums_body(const LinkArgs& la, const FinalAr
     S* buf = (S*)out;
     const OutletCell c = fa.outlets[t];
     if (la.flags[c.nb] != kFluid) return;
-    const unsigned tag = la.st->tag;
+    const unsigned tag = step_tag(la);
     const int nslot = la.ne > 0 ? lt_find(table_cells(la, tag), la.cap, tag, c.nb) : -1;
     const float up = nslot >= 0 ? la.sb[(size_t)nslot * NM + c.comp]
                                 : Store<S>::template load<DIM>(buf + c.comp * la.plane + c.nb, c.comp);
@@ -435,7 +433,8 @@ __device__ __forceinline__ void elem_sums_body(const LinkArgs& la, const FinalAr
 
 // Vertex blocks sum their incident element shares in CSR order (the
 // per-vertex forces addForceAt accumulates); the last block folds element
-// sums into body wrenches with a fixed tree and advances gen and tag.
+// sums into body wrenches with a fixed tree (ring slot of step `gen`) and,
+// for batched domains, advances gen and tag.
 template <int DIM, int FB>
 __device__ __forceinline__ void finish_vertex_body(const LinkArgs& la, const FinalArgs& fa, int blk) {
   constexpr int NT = DIM == 2 ? 1 : 3;
@@ -459,12 +458,11 @@ __device__ __forceinline__ void finish_vertex_body(const LinkArgs& la, const Fin
 }
 
 template <int DIM, int FB>
-__device__ __forceinline__ void finish_last_body(const LinkArgs& la, const FinalArgs& fa) {
+__device__ __forceinline__ void finish_last_body(const LinkArgs& la, const FinalArgs& fa, unsigned gen, bool advance) {
   constexpr int NT = DIM == 2 ? 1 : 3;
   constexpr int STRIDE = DIM * DIM + DIM + NT;
   constexpr int NW = DIM + NT;
   __shared__ double s_red[NW][FB];
-  const unsigned gen = la.st->gen;
   for (int bd = 0; bd < fa.n_bodies; bd++) {
     double a[NW];
 #pragma unroll
@@ -491,10 +489,26 @@ __device__ __forceinline__ void finish_last_body(const LinkArgs& la, const Final
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
+  if (advance && threadIdx.x == 0) {  // batched domains: the device keeps gen and tag
     la.st->gen = gen + 1;
-    la.st->tag = la.st->tag_next;
+    la.st->tag = la.st->tag + 1;
   }
+}
+
+// Extra blocks of the stream launch of a boundary step: next-step
+// kinematics (kin_blocks), then, when the previous step's finish was
+// deferred to this launch, its vertex blocks and wrench block (fin_blocks).
+// The previous step's element sums are complete (earlier launch) and this
+// step's are not started (later launch).
+template <int DIM, int NT>
+__device__ __forceinline__ void stream_extra_body(const LinkArgs& la, const FinalArgs& fa, int blk) {
+  if (blk < fa.kin_blocks) {
+    wave_block_body<DIM, NT>(la, fa, blk);
+    return;
+  }
+  blk -= fa.kin_blocks;
+  if (blk < fa.fin_blocks - 1) finish_vertex_body<DIM, NT>(la, fa, blk);
+  else finish_last_body<DIM, NT>(la, fa, fa.fin_gen, false);
 }
 
 }  // namespace ff
