@@ -119,7 +119,7 @@ template <int DIM, typename S, int FB>
 __global__ void __launch_bounds__(FB) kb_sums(const DomDev* __restrict__ doms, const int2* __restrict__ map) {
   const int2 m = map[blockIdx.x];
   const DomDev* D = doms + m.x;
-  elem_sums_body<DIM, S, FB>(D->la, D->fa, m.y, D->mom[(*D->step & 1u) ^ 1u]);
+  elem_sums_body<DIM, S, FB, true>(D->la, D->fa, m.y, D->mom[(*D->step & 1u) ^ 1u]);
 }
 
 // Per domain: vertex blocks, then one block that folds the wrench, advances
@@ -286,7 +286,8 @@ static int build_tables(Batch& b, int tile) {
         for (int t = 0; t < nw; t++) wave.push_back(make_int2(i, t));
       }
       r.nvb = (d.nv + 255) / 256;
-      r.ndet = detect_blocks(d);
+      r.ndet = detect_blocks(d, kSumBlock, true);
+      r.fa.copy_blocks = copyback_blocks(d, kSumBlock, 4);  // fewer blocks: many domains share the launch
     } else {
       r.la.ne = 0;
       r.fa.copy_blocks = 0;

@@ -205,14 +205,85 @@ __device__ __forceinline__ void elem_window(const float* v, const uint32_t* e, i
 
 namespace ff {
 
-// Link detection for one block of (element, direction) pairs: 16 lanes per
-// pair walk the element's candidate window (the loops of
-// mrSolver2DComputeOccupancyFlagKernel_v3, FsiCoupling2D.cu:44-74, and its 3D
-// twin, FsiCoupling3D.cu:43-76) and offer depthIndexCompact keys for every
-// hit of an updated cell.  `block` indexes the pair groups; the links are
-// entered under `tag` for the vertex positions `v`.
+// Lattice vectors of D3Q27 packed two bits per direction (c + 1), for
+// direction indices only known at run time.
+constexpr unsigned long long d3_pack(int axis) {
+  unsigned long long p = 0;
+  for (int i = 0; i < 27; i++) {
+    const int c = axis == 0 ? D3::cx(i) : (axis == 1 ? D3::cy(i) : D3::cz(i));
+    p |= (unsigned long long)(c + 1) << (2 * i);
+  }
+  return p;
+}
+__device__ __forceinline__ int d3_c(unsigned long long packed, int i) { return (int)((packed >> (2 * i)) & 3ull) - 1; }
+
+// Lanes that detect one element's links in a batch (many domains: whole
+// elements per lane group need fewer blocks for the same work).
 template <int DIM>
+constexpr int kDetectLanes = DIM == 2 ? 64 : 256;
+
+// Link detection: a group of LPE lanes per element walks the
+// element's candidate window once for every lattice direction, one (window
+// cell, direction) task per lane (the loops of
+// mrSolver2DComputeOccupancyFlagKernel_v3, FsiCoupling2D.cu:44-74, and its 3D
+// twin, FsiCoupling3D.cu:43-76), and offers depthIndexCompact keys for every
+// hit of an updated cell.  `block` indexes groups of blockDim / LPE
+// elements; the links are entered under `tag` for the vertex positions `v`.
+template <int DIM, int LPE>
 __device__ __forceinline__ void build_links_block(const LinkArgs& la, long long block, unsigned tag, const float* v) {
+  const unsigned gen = tag;
+  constexpr int K = DIM == 2 ? 8 : 26;
+  const long long e = block * (long long)(blockDim.x / LPE) + threadIdx.x / LPE;
+  if (e >= (long long)la.ne) return;
+  const int elem = (int)e, lane = (int)(threadIdx.x % LPE);
+  const int n[3] = {la.nx, la.ny, la.nz};
+  int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  elem_window<DIM>(v, la.e, elem, la.cl, n, lo, hi);
+  const int wx = hi[0] - lo[0] + 1, wy = hi[1] - lo[1] + 1, wz = DIM == 3 ? hi[2] - lo[2] + 1 : 1;
+  if (wx <= 0 || wy <= 0 || wz <= 0) return;
+  const long long ntask = (long long)wx * wy * wz * K;
+  unsigned long long* cells = table_cells(la, gen);
+  unsigned long long* keys = table_keys<DIM>(la, gen);
+  for (long long t = lane; t < ntask; t += LPE) {
+    const long long k = t / K;
+    const int l = (int)(t - k * K) + 1;
+    const int i = lo[0] + (int)(k % wx);
+    const int j = lo[1] + (int)((k / wx) % wy);
+    const int z = DIM == 3 ? lo[2] + (int)(k / ((long long)wx * wy)) : 0;
+    int cx, cy, cz = 0;
+    if constexpr (DIM == 2) {
+      cx = D2::cx(l);
+      cy = D2::cy(l);
+    } else {
+      cx = d3_c(d3_pack(0), l);
+      cy = d3_c(d3_pack(1), l);
+      cz = d3_c(d3_pack(2), l);
+    }
+    const int cell = (z * la.ny + j) * la.nx + i;
+    Hit h;
+    bool hit = false;
+    if (is_updated(la.flags[cell])) {
+      if constexpr (DIM == 2) {
+        hit = hit_elem2(v, la.e, elem, (float)i * la.cl, (float)j * la.cl, ((float)i - (float)cx) * la.cl,
+                        ((float)j - (float)cy) * la.cl, h);
+      } else {
+        const float p1[3] = {(float)i * la.cl, (float)j * la.cl, (float)z * la.cl};
+        const float p2[3] = {((float)i - (float)cx) * la.cl, ((float)j - (float)cy) * la.cl,
+                             ((float)z - (float)cz) * la.cl};
+        hit = hit_elem3(v, la.e, elem, p1, p2, h);
+      }
+    }
+    const int slot = hit ? lt_claim(cells, la.cap, gen, cell, &la.st->overflow) : -1;
+    if (slot >= 0) lt_offer_key(keys, (size_t)slot * K + (l - 1), gen, depth_key(h.depth, elem));
+  }
+}
+
+// Link detection with one 16-lane group per (element, direction) pair, each
+// lane one window cell: the shortest latency for a domain stepped on its own
+// (the same predicates, hits and keys as build_links_block).  `block` indexes
+// groups of blockDim / 16 pairs.
+template <int DIM>
+__device__ __forceinline__ void build_links_pairs(const LinkArgs& la, long long block, unsigned tag, const float* v) {
   const unsigned gen = tag;
   constexpr int K = DIM == 2 ? 8 : 26;
   const int lane = threadIdx.x & 15;
@@ -226,9 +297,11 @@ __device__ __forceinline__ void build_links_block(const LinkArgs& la, long long 
   const int wx = hi[0] - lo[0] + 1, wy = hi[1] - lo[1] + 1, wz = DIM == 3 ? hi[2] - lo[2] + 1 : 1;
   if (wx <= 0 || wy <= 0 || wz <= 0) return;
   const long long ncell = (long long)wx * wy * wz;
-  const int cx = DIM == 2 ? D2::cx(l) : D3::cx(l);
-  const int cy = DIM == 2 ? D2::cy(l) : D3::cy(l);
-  const int cz = DIM == 2 ? 0 : D3::cz(l);
+  const int cx = DIM == 2 ? D2::cx(l) : d3_c(d3_pack(0), l);
+  const int cy = DIM == 2 ? D2::cy(l) : d3_c(d3_pack(1), l);
+  const int cz = DIM == 2 ? 0 : d3_c(d3_pack(2), l);
+  unsigned long long* cells = table_cells(la, gen);
+  unsigned long long* keys = table_keys<DIM>(la, gen);
   for (long long k = lane; k < ncell; k += 16) {
     const int i = lo[0] + (int)(k % wx);
     const int j = lo[1] + (int)((k / wx) % wy);
@@ -247,8 +320,8 @@ __device__ __forceinline__ void build_links_block(const LinkArgs& la, long long 
         hit = hit_elem3(v, la.e, elem, p1, p2, h);
       }
     }
-    const int slot = hit ? lt_claim(table_cells(la, gen), la.cap, gen, cell, &la.st->overflow) : -1;
-    if (slot >= 0) lt_offer_key(table_keys<DIM>(la, gen), (size_t)slot * K + (l - 1), gen, depth_key(h.depth, elem));
+    const int slot = hit ? lt_claim(cells, la.cap, gen, cell, &la.st->overflow) : -1;
+    if (slot >= 0) lt_offer_key(keys, (size_t)slot * K + (l - 1), gen, depth_key(h.depth, elem));
   }
 }
 

@@ -43,7 +43,7 @@ namespace ff {
 // Links of the current step under the current tag and vertex buffer.
 template <int DIM>
 __global__ void __launch_bounds__(256) k_links_build(LinkArgs la) {
-  build_links_block<DIM>(la, blockIdx.x, step_tag(la), la.v[step_gen(la) & 1]);
+  build_links_pairs<DIM>(la, blockIdx.x, step_tag(la), la.v[step_gen(la) & 1]);
 }
 
 // Batched domains: invalidate links detected ahead for a boundary that has
@@ -55,9 +55,7 @@ __global__ void k_tag_bump(StepState* st) {
 
 int launch_detect(Domain& d, const LinkArgs& la, bool bump) {
   if (bump && la.dev_state) k_tag_bump<<<1, 64, 0, d.stream>>>(d.st);
-  const int K = d.q - 1;
-  const long long pairs = (long long)d.ne * K;
-  const unsigned g = (unsigned)((pairs + 15) / 16);
+  const unsigned g = (unsigned)detect_blocks(d, 256, false);
   if (g > 0) {
     if (d.dim == 2) k_links_build<2><<<g, 256, 0, d.stream>>>(la);
     else k_links_build<3><<<g, 256, 0, d.stream>>>(la);
@@ -70,7 +68,7 @@ int launch_detect(Domain& d, const LinkArgs& la, bool bump) {
 // Element sums, outlets, copy-back and next-step detection.
 template <int DIM, typename S, int FB>
 __global__ void __launch_bounds__(FB) k_step_sums(LinkArgs la, FinalArgs fa, void* out) {
-  elem_sums_body<DIM, S, FB>(la, fa, blockIdx.x, out);
+  elem_sums_body<DIM, S, FB, false>(la, fa, blockIdx.x, out);
 }
 
 // Vertex blocks, then one last block for the body wrenches and the advance
@@ -84,18 +82,20 @@ __global__ void __launch_bounds__(FB) k_step_finish(LinkArgs la, FinalArgs fa, i
   else finish_last_body<DIM, FB>(la, fa, fa.fin_gen, false);
 }
 
-// Detection blocks of the element-sum launch (kSumBlock threads): one
-// (element, direction) pair per 16 threads.
-int detect_blocks(const Domain& d) {
-  const long long per = kSumBlock / 16;
-  return (int)(((long long)d.ne * (d.q - 1) + per - 1) / per);
+// Detection blocks of `nt` threads: batched, one element per kDetectLanes
+// lanes; otherwise one (element, direction) pair per 16 lanes.
+int detect_blocks(const Domain& d, int nt, bool batched) {
+  if (!batched) return (int)(((long long)d.ne * (d.q - 1) + nt / 16 - 1) / (nt / 16));
+  const long long per = std::max(1, nt / (d.dim == 2 ? kDetectLanes<2> : kDetectLanes<3>));
+  return (int)(((long long)d.ne + per - 1) / per);
 }
 
 // Copy-back blocks of the sums launch and boundary-cell blocks of the stream
 // launch: both scan the link table's slots (grid stride), so the block
 // counts follow its capacity.
-int copyback_blocks(const Domain& d, int fb) {
-  return (int)std::min<long long>(256, std::max<long long>(1, ((long long)d.lt_cap + fb - 1) / fb));
+int copyback_blocks(const Domain& d, int fb, int per_thread) {
+  const long long span = (long long)fb * per_thread;
+  return (int)std::min<long long>(256, std::max<long long>(1, ((long long)d.lt_cap + span - 1) / span));
 }
 
 int boundary_cell_blocks(const Domain& d, int nt) {
@@ -106,7 +106,7 @@ int launch_step_sums(Domain& d, const LinkArgs& la, void* out, bool detect_next)
   constexpr int FB = kSumBlock;
   const FinalArgs fa = make_final_args(d, out, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], nullptr);
   const unsigned grid =
-      (unsigned)(elem_blocks(d.ne) + fa.outlet_blocks + fa.copy_blocks + (detect_next ? detect_blocks(d) : 0));
+      (unsigned)(elem_blocks(d.ne) + fa.outlet_blocks + fa.copy_blocks + (detect_next ? detect_blocks(d, kSumBlock, false) : 0));
 #define FF_SUM(DIM, S) k_step_sums<DIM, S, FB><<<grid, FB, 0, d.stream>>>(la, fa, out)
   if (d.dim == 2) {
     if (d.storage == FF_STORAGE_F32) FF_SUM(2, float);
@@ -172,7 +172,7 @@ FinalArgs make_final_args(Domain& d, void* out, float* vn0, float* vn1, float* u
   fa.n_outlets = d.n_outlets;
   const int fb = kSumBlock;
   fa.outlet_blocks = (d.n_outlets + fb - 1) / fb;
-  fa.copy_blocks = (d.has_boundary && d.ne > 0) ? copyback_blocks(d, fb) : 0;
+  fa.copy_blocks = (d.has_boundary && d.ne > 0) ? copyback_blocks(d, fb, 1) : 0;
   fa.kin_blocks = 0;
   fa.fin_blocks = 0;
   fa.fin_gen = 0;

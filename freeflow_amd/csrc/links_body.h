@@ -244,8 +244,8 @@ __host__ __device__ inline int elem_blocks(int ne) { return (ne + kElemPerBlock 
 FinalArgs make_final_args(Domain& d, void* out, float* vn0, float* vn1, float* un0, float* un1, float* proxy_forces);
 // Host: grid sizes of the boundary-cell and copy-back parts of a step.
 int boundary_cell_blocks(const Domain& d, int nt);
-int copyback_blocks(const Domain& d, int fb);
-int detect_blocks(const Domain& d);
+int copyback_blocks(const Domain& d, int fb, int per_thread);
+int detect_blocks(const Domain& d, int nt, bool batched);
 
 // Travelling wave along the body axis, FP64:
 //   y = y_rest + A s sin(theta), v_y = -2 pi f A s cos(theta),
@@ -282,10 +282,13 @@ __device__ __forceinline__ void wave_block_body(const LinkArgs& la, const FinalA
 
 // Detection of the next step's links (tag + 1, the next step's vertex
 // positions) into the other link table.
-template <int DIM>
+template <int DIM, bool BATCHED>
 __device__ __forceinline__ void detect_next_body(const LinkArgs& la, int blk) {
   const unsigned gen = step_gen(la);
-  build_links_block<DIM>(la, (long long)blk, step_tag(la) + 1u, la.v[(gen + 1u) & 1u]);
+  if constexpr (BATCHED)
+    build_links_block<DIM, kDetectLanes<DIM>>(la, (long long)blk, step_tag(la) + 1u, la.v[(gen + 1u) & 1u]);
+  else
+    build_links_pairs<DIM>(la, (long long)blk, step_tag(la) + 1u, la.v[(gen + 1u) & 1u]);
 }
 
 // Second launch of a boundary step, kSumBlock threads per block.  The first
@@ -381,7 +384,7 @@ __device__ __forceinline__ void elem_sum_wave(const LinkArgs& la, const FinalArg
 }
 
 
-template <int DIM, typename S, int FB>
+template <int DIM, typename S, int FB, bool BATCHED>
 __device__ __forceinline__ void elem_sums_body(const LinkArgs& la, const FinalArgs& fa, int blk, void* out) {
   static_assert(FB == kSumBlock, "element sums run one element per wave of a kSumBlock block");
   constexpr int K = DIM == 2 ? 8 : 26;
@@ -390,7 +393,7 @@ __device__ __forceinline__ void elem_sums_body(const LinkArgs& la, const FinalAr
   constexpr int STRIDE = DIM * DIM + DIM + NT;
   const int neb = elem_blocks(la.ne);
   if (blk >= neb + fa.outlet_blocks + fa.copy_blocks) {
-    detect_next_body<DIM>(la, blk - neb - fa.outlet_blocks - fa.copy_blocks);
+    detect_next_body<DIM, BATCHED>(la, blk - neb - fa.outlet_blocks - fa.copy_blocks);
     return;
   }
   if (blk >= neb + fa.outlet_blocks) {
