@@ -29,7 +29,7 @@ FLAG = {
 
 EXPORTS = [
     "ff_domain_create", "ff_domain_destroy", "ff_set_flags", "ff_get_flags", "ff_init_state", "ff_set_moments",
-    "ff_get_moments", "ff_set_boundary", "ff_update_boundary", "ff_set_body_ref", "ff_set_wave", "ff_step",
+    "ff_get_moments", "ff_set_boundary", "ff_update_boundary", "ff_set_body_ref", "ff_set_wave", "ff_set_wave_params", "ff_step",
     "ff_step_proxy", "ff_get_vertex_forces", "ff_get_step_vertex_forces", "ff_get_body_wrench", "ff_set_wrench_history",
     "ff_get_wrench_history", "ff_get_boundary", "ff_mass", "ff_fill_solid", "ff_sync", "ff_time_bulk_kernel",
     "ff_time_steps", "ff_domain_footprint", "ff_set_tile", "ff_autotune", "ff_batch_create", "ff_batch_destroy", "ff_batch_add",
@@ -73,6 +73,7 @@ def lib():
         L.ff_update_boundary.argtypes = [vp, vp, vp]
         L.ff_set_body_ref.argtypes = [vp, vp]
         L.ff_set_wave.argtypes = [vp, vp, vp, vp]
+        L.ff_set_wave_params.argtypes = [vp, vp]
         L.ff_step.argtypes = [vp, C.c_int]
         L.ff_step_proxy.argtypes = [vp, vp, vp, vp, vp, C.c_int, C.c_int]
         L.ff_get_vertex_forces.argtypes = [vp, vp, C.c_int]
@@ -217,6 +218,11 @@ class Domain:
         s = np.ascontiguousarray(s, np.float32).ravel()
         p = np.ascontiguousarray(params, np.float64).ravel()
         _check(lib().ff_set_wave(self.h, _ptr(r), _ptr(s), _ptr(p)))
+
+    def set_wave_params(self, params):
+        """New parameters for the wave set by set_wave (no wait for the device)."""
+        p = np.ascontiguousarray(params, np.float64).ravel()
+        _check(lib().ff_set_wave_params(self.h, _ptr(p)))
 
     def step(self, n=1):
         _check(lib().ff_step(self.h, n))

@@ -113,72 +113,6 @@ static int upload_aos(Domain& d, const float* aos, int which) {
   return FF_OK;
 }
 
-// Reference initial state computed on the host (initialize_state_kernel_2d/3d,
-// LbmInitializer2D.cu:32-95 / LbmInitializer3D.cu:33-113), same arithmetic.
-static void init_state_host(const Domain& d, const float* bc, std::vector<float>& aos) {
-  aos.assign((size_t)d.n * d.nm, 0.0f);
-  static const float ex2[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
-  static const float ey2[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
-  static const float w2[9] = {4.0f / 9.0f, 1.0f / 9.0f, 1.0f / 9.0f, 1.0f / 9.0f, 1.0f / 9.0f,
-                              1.0f / 36.0f, 1.0f / 36.0f, 1.0f / 36.0f, 1.0f / 36.0f};
-  static const float ex3[27] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1};
-  static const float ey3[27] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1, 1, -1};
-  static const float ez3[27] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1, 1, -1, -1, 1, 1, -1, 1, -1};
-  const float cs2 = 1.0f / 3.0f;
-  for (long long c = 0; c < d.n; c++) {
-    const uint8_t fl = d.h_flags[c];
-    float ux = 0.0f, uy = 0.0f, uz = 0.0f;
-    const float rho = 1.0f;
-    float* m = &aos[(size_t)c * d.nm];
-    if (d.dim == 2) {
-      if (fl == kInletLeft) ux = bc[0];
-      else if (fl == kInletDown) uy = bc[1];
-      else if (fl == kInletRight) ux = -bc[2];
-      else if (fl == kInletUp) uy = -bc[3];
-      const float u_sq = ux * ux + uy * uy;
-      float feq[9];
-      for (int k = 0; k < 9; k++) {
-        const float cu = ex2[k] * ux + ey2[k] * uy;
-        feq[k] = w2[k] * rho * (1.0f + 3.0f * cu + 4.5f * cu * cu - 1.5f * u_sq);
-      }
-      const float inv = 1 / rho;
-      const float pixx = feq[1] + feq[3] + feq[5] + feq[6] + feq[7] + feq[8];
-      const float piyy = feq[2] + feq[4] + feq[5] + feq[6] + feq[7] + feq[8];
-      const float pixy = feq[5] - feq[6] + feq[7] - feq[8];
-      m[0] = rho; m[1] = ux; m[2] = uy;
-      m[3] = (float)((double)(pixx * inv) - 1.0 * (double)cs2);
-      m[4] = (float)((double)(piyy * inv) - 1.0 * (double)cs2);
-      m[5] = pixy * inv;
-    } else {
-      if (fl == kInletLeft) ux = bc[0];
-      else if (fl == kInletDown) uz = bc[1];
-      else if (fl == kInletRight) ux = -bc[2];
-      else if (fl == kInletUp) uz = -bc[3];
-      else if (fl == kInletFront) uy = bc[4];
-      else if (fl == kInletBack) uy = -bc[5];
-      const float u_sq = ux * ux + uy * uy + uz * uz;
-      float f[27];
-      for (int k = 0; k < 27; k++) {
-        const float cu = ex3[k] * ux + ey3[k] * uy + ez3[k] * uz;
-        const float w = k == 0 ? 8.0f / 27.0f : (k < 7 ? 2.0f / 27.0f : (k < 19 ? 1.0f / 54.0f : 1.0f / 216.0f));
-        f[k] = w * rho * (1.0f + 3.0f * cu + 4.5f * cu * cu - 1.5f * u_sq);
-      }
-      const float pixx = f[1] + f[2] + f[7] + f[8] + f[9] + f[10] + f[13] + f[14] + f[15] + f[16] + f[19] + f[20] + f[21] + f[22] + f[23] + f[24] + f[25] + f[26];
-      const float piyy = f[3] + f[4] + f[7] + f[8] + f[11] + f[12] + f[13] + f[14] + f[17] + f[18] + f[19] + f[20] + f[21] + f[22] + f[23] + f[24] + f[25] + f[26];
-      const float pizz = f[5] + f[6] + f[9] + f[10] + f[11] + f[12] + f[15] + f[16] + f[17] + f[18] + f[19] + f[20] + f[21] + f[22] + f[23] + f[24] + f[25] + f[26];
-      const float pixy = (f[7] + f[8] + f[19] + f[20] + f[21] + f[22]) - (f[13] + f[14] + f[23] + f[24] + f[25] + f[26]);
-      const float piyz = (f[11] + f[12] + f[19] + f[20] + f[25] + f[26]) - (f[17] + f[18] + f[21] + f[22] + f[23] + f[24]);
-      const float pixz = (f[9] + f[10] + f[19] + f[20] + f[23] + f[24]) - (f[15] + f[16] + f[21] + f[22] + f[25] + f[26]);
-      const float inv = 1 / rho;
-      m[0] = rho; m[1] = ux; m[2] = uy; m[3] = uz;
-      m[4] = (float)((double)(pixx * inv) - 1.0 * (double)cs2);
-      m[5] = (float)((double)(piyy * inv) - 1.0 * (double)cs2);
-      m[6] = (float)((double)(pizz * inv) - 1.0 * (double)cs2);
-      m[7] = pixy * inv; m[8] = piyz * inv; m[9] = pixz * inv;
-    }
-  }
-}
-
 static unsigned next_pow2(unsigned long long v) {
   unsigned p = 1024;
   while (p < v && p < (1u << 30)) p <<= 1;
@@ -456,12 +390,13 @@ int ff_init_state(ff_domain* dd, const float* bc, int n_bc) {
   float b[6] = {0, 0, 0, 0, 0, 0};
   const int need = d.dim == 2 ? 4 : 6;
   if (bc && n_bc == need) for (int k = 0; k < need; k++) b[k] = bc[k];
-  std::vector<float> aos;
-  init_state_host(d, b, aos);
   FF_HIP(hipStreamSynchronize(d.stream));
   d.cur = 0;
   batch_mark_dirty(d.batch);
-  return upload_aos(d, aos.data(), -1);
+  int rc = launch_init_state(d, b);
+  if (rc) return rc;
+  FF_HIP(hipStreamSynchronize(d.stream));
+  return FF_OK;
 }
 
 int ff_set_moments(ff_domain* dd, const float* aos, int which) {
@@ -658,6 +593,18 @@ int ff_set_wave(ff_domain* dd, const float* rest, const float* s, const double* 
   if ((rc = launch_wave_now(d))) return rc;
   FF_HIP(hipStreamSynchronize(d.stream));
   return FF_OK;
+}
+
+int ff_set_wave_params(ff_domain* dd, const double* params) {
+  FF_CHECK_DOMAIN(dd);
+  Domain& d = *dd;
+  if (!d.wave) return fail(FF_ERR_STATE, "no wave set (ff_set_wave)");
+  if (!params) return fail(FF_ERR_INVALID, "null wave parameters");
+  FF_HIP(hipSetDevice(d.device));
+  FF_HIP(hipMemcpyAsync(d.wave_params, params, (size_t)d.n_bodies * 4 * sizeof(double), hipMemcpyHostToDevice,
+                        d.stream));
+  links_moved(d);
+  return launch_wave_now(d);
 }
 
 int ff_step(ff_domain* dd, int nsteps) {

@@ -172,6 +172,7 @@ int launch_step_sums(Domain& d, const LinkArgs& la, void* out, bool detect_next)
 int launch_step_finish(Domain& d, const LinkArgs& la, unsigned gen, float* proxy_forces);
 int flush_finish(Domain& d);
 int launch_outlet(Domain& d, void* buf);
+int launch_init_state(Domain& d, const float* bc6);
 int launch_wave_now(Domain& d);
 int launch_step_vertex_forces(Domain& d, float* out_dev);
 int launch_convert_in(Domain& d, const float* aos_dev, void* buf);

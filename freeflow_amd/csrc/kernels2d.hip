@@ -170,6 +170,115 @@ __global__ void k_soa_to_aos(const S* __restrict__ soa, float* __restrict__ aos,
   for (int q = 0; q < NM; q++) aos[c * NM + q] = Store<S>::template load<DIM>(soa + q * plane + c, q);
 }
 
+// ---------------------------------------------------------------- initial state
+// initialize_state_kernel_2d/3d (LbmInitializer2D.cu:32-95,
+// LbmInitializer3D.cu:33-113): rho = 1, the inlet velocity of Inlet* cells,
+// Pi from the equilibrium populations; same arithmetic, on the device.
+struct InitBC {
+  float v[6];
+};
+
+template <int DIM>
+__device__ __forceinline__ void init_moments(uint8_t fl, const InitBC& bc, float* m) {
+  const float cs2 = 1.0f / 3.0f;
+  const float rho = 1.0f;
+  float ux = 0.0f, uy = 0.0f, uz = 0.0f;
+  if constexpr (DIM == 2) {
+    constexpr float ex2[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+    constexpr float ey2[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+    constexpr float w2[9] = {4.0f / 9.0f, 1.0f / 9.0f, 1.0f / 9.0f, 1.0f / 9.0f, 1.0f / 9.0f,
+                             1.0f / 36.0f, 1.0f / 36.0f, 1.0f / 36.0f, 1.0f / 36.0f};
+    if (fl == kInletLeft) ux = bc.v[0];
+    else if (fl == kInletDown) uy = bc.v[1];
+    else if (fl == kInletRight) ux = -bc.v[2];
+    else if (fl == kInletUp) uy = -bc.v[3];
+    const float u_sq = ux * ux + uy * uy;
+    float feq[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const float cu = ex2[k] * ux + ey2[k] * uy;
+      feq[k] = w2[k] * rho * (1.0f + 3.0f * cu + 4.5f * cu * cu - 1.5f * u_sq);
+    }
+    const float inv = 1 / rho;
+    const float pixx = feq[1] + feq[3] + feq[5] + feq[6] + feq[7] + feq[8];
+    const float piyy = feq[2] + feq[4] + feq[5] + feq[6] + feq[7] + feq[8];
+    const float pixy = feq[5] - feq[6] + feq[7] - feq[8];
+    m[0] = rho;
+    m[1] = ux;
+    m[2] = uy;
+    m[3] = (float)((double)(pixx * inv) - 1.0 * (double)cs2);
+    m[4] = (float)((double)(piyy * inv) - 1.0 * (double)cs2);
+    m[5] = pixy * inv;
+  } else {
+    if (fl == kInletLeft) ux = bc.v[0];
+    else if (fl == kInletDown) uz = bc.v[1];
+    else if (fl == kInletRight) ux = -bc.v[2];
+    else if (fl == kInletUp) uz = -bc.v[3];
+    else if (fl == kInletFront) uy = bc.v[4];
+    else if (fl == kInletBack) uy = -bc.v[5];
+    const float u_sq = ux * ux + uy * uy + uz * uz;
+    float f[27];
+#pragma unroll
+    for (int k = 0; k < 27; k++) {
+      const float cu = (float)D3::cx(k) * ux + (float)D3::cy(k) * uy + (float)D3::cz(k) * uz;
+      const float w = k == 0 ? 8.0f / 27.0f : (k < 7 ? 2.0f / 27.0f : (k < 19 ? 1.0f / 54.0f : 1.0f / 216.0f));
+      f[k] = w * rho * (1.0f + 3.0f * cu + 4.5f * cu * cu - 1.5f * u_sq);
+    }
+    const float pixx = f[1] + f[2] + f[7] + f[8] + f[9] + f[10] + f[13] + f[14] + f[15] + f[16] + f[19] + f[20] +
+                       f[21] + f[22] + f[23] + f[24] + f[25] + f[26];
+    const float piyy = f[3] + f[4] + f[7] + f[8] + f[11] + f[12] + f[13] + f[14] + f[17] + f[18] + f[19] + f[20] +
+                       f[21] + f[22] + f[23] + f[24] + f[25] + f[26];
+    const float pizz = f[5] + f[6] + f[9] + f[10] + f[11] + f[12] + f[15] + f[16] + f[17] + f[18] + f[19] + f[20] +
+                       f[21] + f[22] + f[23] + f[24] + f[25] + f[26];
+    const float pixy = (f[7] + f[8] + f[19] + f[20] + f[21] + f[22]) - (f[13] + f[14] + f[23] + f[24] + f[25] + f[26]);
+    const float piyz = (f[11] + f[12] + f[19] + f[20] + f[25] + f[26]) - (f[17] + f[18] + f[21] + f[22] + f[23] + f[24]);
+    const float pixz = (f[9] + f[10] + f[19] + f[20] + f[23] + f[24]) - (f[15] + f[16] + f[21] + f[22] + f[25] + f[26]);
+    const float inv = 1 / rho;
+    m[0] = rho;
+    m[1] = ux;
+    m[2] = uy;
+    m[3] = uz;
+    m[4] = (float)((double)(pixx * inv) - 1.0 * (double)cs2);
+    m[5] = (float)((double)(piyy * inv) - 1.0 * (double)cs2);
+    m[6] = (float)((double)(pizz * inv) - 1.0 * (double)cs2);
+    m[7] = pixy * inv;
+    m[8] = piyz * inv;
+    m[9] = pixz * inv;
+  }
+}
+
+template <typename S, int DIM>
+__global__ void k_init_state(const uint8_t* __restrict__ flags, InitBC bc, S* __restrict__ a, S* __restrict__ b,
+                             long long n, size_t plane) {
+  constexpr int NM = DIM == 2 ? 6 : 10;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  float m[NM];
+  init_moments<DIM>(flags[c], bc, m);
+#pragma unroll
+  for (int q = 0; q < NM; q++) {
+    Store<S>::template store<DIM>(a + q * plane + c, q, m[q]);
+    Store<S>::template store<DIM>(b + q * plane + c, q, m[q]);
+  }
+}
+
+// Both moment buffers of d from its flags and the inlet velocities bc[6].
+int launch_init_state(Domain& d, const float* bc6) {
+  InitBC bc;
+  for (int k = 0; k < 6; k++) bc.v[k] = bc6[k];
+  const int b = 256;
+  const unsigned g = (unsigned)((d.n + b - 1) / b);
+#define FF_INIT(S)                                                                                                \
+  if (d.dim == 2) k_init_state<S, 2><<<g, b, 0, d.stream>>>(d.flags, bc, (S*)d.mom[0], (S*)d.mom[1], d.n, d.plane); \
+  else k_init_state<S, 3><<<g, b, 0, d.stream>>>(d.flags, bc, (S*)d.mom[0], (S*)d.mom[1], d.n, d.plane);
+  if (d.storage == FF_STORAGE_F32) { FF_INIT(float) }
+  else if (d.storage == FF_STORAGE_F16) { FF_INIT(__half) }
+  else { FF_INIT(__hip_bfloat16) }
+#undef FF_INIT
+  FF_HIP(hipGetLastError());
+  return FF_OK;
+}
+
 int launch_convert_in(Domain& d, const float* aos, void* buf) {
   const int b = 256;
   const unsigned g = (unsigned)((d.n + b - 1) / b);

@@ -70,8 +70,7 @@ class Scene3D:
         return dict(dim=3, nx=self.nx, ny=self.ny, nz=self.nz, viscosity=self.viscosity)
 
     def setup(self, dom, history=0):
-        n = self.nx * self.ny * self.nz
-        dom.set_moments(wl.equilibrium_moments(np.zeros((n, 3))))
+        dom.init_state([0.0] * 6)  # rest fluid, computed on the device
         dom.set_boundary(self.v, np.zeros_like(self.v), self.f)
         dom.set_wave(self.v, self.s, self.params[None, :])
         if history:
@@ -103,8 +102,9 @@ def make_episodes(rate, duration, seed, frac2d=0.7, steps=500, action_every=10,
 
 
 class Server:
-    def __init__(self, storage="f16", budget_bytes=None, max_active=512, device=0):
+    def __init__(self, storage="f16", budget_bytes=None, max_active=512, device=0, keep_final=False):
         self.storage = storage
+        self.keep_final = keep_final  # read back each episode's final moments (ep.final_moments)
         self.budget = budget_bytes if budget_bytes is not None else 288 * 10 ** 9
         self.max_active = max_active
         self.batches = {2: capi.Batch(2, storage, stage_fp16=True, device=device),
@@ -146,17 +146,16 @@ class Server:
         k = ep.done_steps // ep.action_every
         A, phi = ep.actions[min(k, len(ep.actions) - 1)]
         sc = ep.scene
-        rest = sc.swimmer.rest if ep.kind == 2 else sc.v
-        s = sc.swimmer.s if ep.kind == 2 else sc.s
         p = (sc.swimmer.params if ep.kind == 2 else sc.params).copy()
         p[0] = p[0] * A
         p[2] = phi
-        ep.dom.set_wave(rest, s, p[None, :])
+        ep.dom.set_wave_params(p[None, :])
 
     def _retire(self, ep, now):
         ep.finished_at = now
         self.batches[ep.kind].remove(ep.dom)
-        ep.final_moments = ep.dom.moments(0)
+        if self.keep_final:
+            ep.final_moments = ep.dom.moments(0)
         ep.dom.close()
         ep.dom = None
         self.active.remove(ep)

@@ -144,7 +144,7 @@ class SwimmerScene2D:
     def setup(self, dom, history=0):
         """Flags, rest state, boundary and device-side wave kinematics on `dom`."""
         dom.set_flags(self.flags)
-        dom.set_moments(equilibrium_moments(np.zeros((self.nx * self.ny, 2))))
+        dom.init_state([0.0, 0.0, 0.0, 0.0])  # rest fluid, computed on the device
         v0, vel0 = self.swimmer.state(0.0)
         dom.set_boundary(v0, vel0, self.swimmer.elems)
         dom.set_wave(self.swimmer.rest, self.swimmer.s, self.swimmer.params[None, :])

@@ -124,6 +124,11 @@ int ff_set_body_ref(ff_domain* d, const double* ref);
    given per vertex and t = step * dt.  params holds 4 doubles per body:
    {A (m), f (Hz), phi (rad), L/lambda}.  Replaces ff_update_boundary. */
 int ff_set_wave(ff_domain* d, const float* rest_verts, const float* s_coord, const double* params);
+/* New wave parameters (n_bodies x 4 doubles, as ff_set_wave) for the wave set
+   by ff_set_wave, e.g. an RL action: ordered on the domain's stream without
+   waiting for it (the host array may be reused on return).  No counterpart
+   in the reference (its actions go through applyKinematicControl). */
+int ff_set_wave_params(ff_domain* d, const double* params);
 
 /* One or more fluid steps with the library-held boundary: the work of
    solveLbmAndFsiStep2D/3D (FsiCoupling2D.cu:358-432, FsiCoupling3D.cu:424-484):

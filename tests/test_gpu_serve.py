@@ -26,7 +26,7 @@ def small_episodes(seed, n=5, steps=40):
 def test_alone_equals_mix():
     eps = small_episodes(7)
     note(f"case serve mix {[(e.kind, e.nx, e.ny, e.nz) for e in eps]}")
-    srv = serve.Server(storage="f16")
+    srv = serve.Server(storage="f16", keep_final=True)
     summary = srv.run(eps, realtime=False)
     srv.close()
     assert summary["episodes_done"] == len(eps)
@@ -35,7 +35,7 @@ def test_alone_equals_mix():
         solo = serve.make_episodes(rate=1000.0, duration=1.0, seed=7, steps=40, action_every=10,
                                    sizes2d=((160, 240), (80, 120)), sizes3d=((32, 48), (32, 32), (24, 32)))
         alone = solo[target.eid]
-        s2 = serve.Server(storage="f16")
+        s2 = serve.Server(storage="f16", keep_final=True)
         s2.run([alone], realtime=False)
         s2.close()
         assert np.array_equal(alone.final_moments, target.final_moments), f"episode {target.eid}"
@@ -47,7 +47,7 @@ def test_alone_equals_mix():
 
 def test_admission_by_footprint():
     eps = small_episodes(9, n=4, steps=20)
-    srv = serve.Server(storage="f16")
+    srv = serve.Server(storage="f16", keep_final=True)
     fps = sorted(srv.footprint(e) for e in eps)
     srv.budget = fps[-1] + fps[-2]  # room for two of the largest at a time
     note(f"case admission budget {srv.budget}")
