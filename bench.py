@@ -252,6 +252,16 @@ def run_served(args):
     print(f"[bench] served: {len(eps)} episodes at {args.rate}/s over {args.duration} s", file=sys.stderr, flush=True)
     summ = srv.run(eps, realtime=True)
     srv.close()
+    extra = {}
+    if args.plain_steps > 0:
+        # the same arrivals served in the plain per-domain FP32 form
+        plain = serve.Server(storage="f32", batched=False)
+        print("[bench] served, plain per-domain FP32 form", file=sys.stderr, flush=True)
+        ps = plain.run(serve.make_episodes(args.rate, args.duration, seed=1234), realtime=True)
+        plain.close()
+        extra = dict(plain_fp32_mlups=round(ps["aggregate_mlups"], 2),
+                     plain_fp32_step_latency_ms={"p50": ps["p50_step_ms"], "p99": ps["p99_step_ms"]},
+                     speedup_vs_plain_fp32=round(summ["aggregate_mlups"] / max(ps["aggregate_mlups"], 1e-9), 3))
     out = base_line(summ["aggregate_mlups"], summ["iterations"], 0, summ["wall_s"] * 1e3 / max(summ["iterations"], 1),
                     "f32 compute, f16 storage",
                     {"workload": f"Poisson rollout episodes at {args.rate}/s for {args.duration} s: 70% 2D "
@@ -259,7 +269,7 @@ def run_served(args):
                                  "action every 10 steps", "rate": args.rate, "duration": args.duration,
                      "parallelism": "single-gpu"}, None, None,
                     step_latency_ms={"p50": summ["p50_step_ms"], "p99": summ["p99_step_ms"]},
-                    served=summ)
+                    served=summ, **extra)
     print(json.dumps(out), flush=True)
     return 0
 
@@ -291,7 +301,8 @@ def main():
     ap.add_argument("--storage", default="f16", help="config3: f32, f16 or bf16")
     ap.add_argument("--rate", type=float, default=50.0, help="served: episode arrivals per second")
     ap.add_argument("--duration", type=float, default=5.0, help="served: arrival window in seconds")
-    ap.add_argument("--plain-steps", type=int, default=200, help="batched: steps of the per-domain FP32 form")
+    ap.add_argument("--plain-steps", type=int, default=200,
+                    help="batched: steps of the per-domain FP32 form; served: 0 skips that form")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (the reference kernel that calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters), then exit")

@@ -47,6 +47,7 @@ struct DomDev {
   int nvb;           // vertex blocks of the finish phase
   int ndet;          // next-step detection blocks of the element-sum phase
   int boundary;
+  int* bflag;        // the batch's overflow flag (set when this domain's table overflowed)
 };
 
 struct Batch {
@@ -57,6 +58,7 @@ struct Batch {
   bool dirty = true;
   int tile = -1;        // fixed tile shape (index into kTiles2 / kTiles3), -1 = autotune
   int tuned_tile = -1;  // result of the last tile search
+  double tuned_cells = 0.0;  // member cells when the search ran (a change by 2x searches again)
   DomDev* d_doms = nullptr;
   size_t doms_cap = 0;
   int2* d_map = nullptr;
@@ -64,6 +66,15 @@ struct Batch {
   unsigned* d_steps = nullptr;
   size_t steps_cap = 0;
   int n_bc = 0, n_wave = 0, n_bulk = 0, n_es = 0, n_fin = 0;
+  int* d_flag = nullptr;  // set by a replay whose detection overflowed a member's link table
+  // pending detections (members whose links were not detected ahead): pinned
+  // staging, reused once the copy recorded by pend_ev has completed
+  int2* h_pend = nullptr;
+  size_t h_pend_cap = 0;
+  int2* d_pend = nullptr;
+  size_t d_pend_cap = 0;
+  hipEvent_t pend_ev = nullptr;
+  std::vector<int2> bumps, dets;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -132,8 +143,29 @@ __global__ void __launch_bounds__(FB) kb_finish(const DomDev* __restrict__ doms,
     finish_vertex_body<DIM, FB>(D->la, D->fa, m.y);
     return;
   }
-  if (D->boundary) finish_last_body<DIM, FB>(D->la, D->fa, D->la.st->gen, true);
+  if (D->boundary) {
+    finish_last_body<DIM, FB>(D->la, D->fa, D->la.st->gen, true);
+    if (threadIdx.x == 0 && D->la.st->overflow) *D->bflag = 1;
+  }
   if (threadIdx.x == 0) *D->step = *D->step + 1u;
+}
+
+// Pending detections: tag bumps of members whose earlier detection went
+// stale, then the detection blocks of every pending member (one launch for
+// the whole batch instead of one per domain).
+__global__ void kb_tag_bump(const DomDev* __restrict__ doms, const int2* __restrict__ list, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    StepState* st = doms[list[i].x].la.st;
+    st->tag = st->tag + 1u;
+  }
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(kSumBlock) kb_detect(const DomDev* __restrict__ doms, const int2* __restrict__ map) {
+  const int2 m = map[blockIdx.x];
+  const LinkArgs& la = doms[m.x].la;
+  detect_block<DIM>(la, (long long)m.y, step_tag(la), la.v[step_gen(la) & 1u]);
 }
 
 // ---------------------------------------------------------------- launch helpers
@@ -273,6 +305,7 @@ static int build_tables(Batch& b, int tile) {
     r.mom[1] = d.mom[d.cur ^ 1];
     r.step = b.d_steps + i;
     r.boundary = bnd ? 1 : 0;
+    r.bflag = b.d_flag;
     int tx, ty, ntl;
     tiles_of(b, d, tile, &tx, &ty, &ntl);
     r.tiles_x = tx;
@@ -328,6 +361,12 @@ static int build_tables(Batch& b, int tile) {
   return FF_OK;
 }
 
+static double member_cells(const Batch& b) {
+  double c = 0.0;
+  for (const Domain* d : b.mem) c += (double)d->nx * d->ny * d->nz;
+  return c;
+}
+
 // Time the stream-collide tiles of every member for each candidate shape
 // and keep the fastest.
 static int autotune(Batch& b) {
@@ -345,8 +384,10 @@ static int autotune(Batch& b) {
   for (int t = 0; t < ntiles && !rc; t++) {
     if ((rc = build_tables(b, t))) break;
     const int2* map = b.d_map + b.n_bc + b.n_wave;
-    for (int w = 0; w < 3; w++) launch_bulk_only(b, t, map, b.n_bulk);
-    constexpr int iters = 10;
+    // long launches need fewer repetitions for a stable time
+    const bool big = b.n_bulk > 20000;
+    const int warm = big ? 1 : 3, iters = big ? 3 : 10;
+    for (int w = 0; w < warm; w++) launch_bulk_only(b, t, map, b.n_bulk);
     FF_HIP(hipEventRecord(e0, b.stream));
     for (int k = 0; k < iters; k++) launch_bulk_only(b, t, map, b.n_bulk);
     FF_HIP(hipEventRecord(e1, b.stream));
@@ -362,6 +403,7 @@ static int autotune(Batch& b) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   b.tuned_tile = best_t;
+  b.tuned_cells = member_cells(b);
   return rc;
 }
 
@@ -379,7 +421,11 @@ static int rebuild(Batch& b) {
   FF_HIP(hipStreamSynchronize(b.stream));
   drop_graph(b);
   int rc;
-  if (b.tile < 0 && b.tuned_tile < 0 && (rc = autotune(b))) return rc;
+  // the best tile depends on how many blocks the launch holds: search again
+  // when the member cells changed by more than 2x since the last search
+  const double cells = member_cells(b);
+  const bool retune = b.tuned_tile < 0 || cells > 2.0 * b.tuned_cells || 2.0 * cells < b.tuned_cells;
+  if (b.tile < 0 && retune && (rc = autotune(b))) return rc;
   if ((rc = build_tables(b, current_tile(b)))) return rc;
   FF_HIP(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
   launch_step(b, current_tile(b));
@@ -417,19 +463,48 @@ static int detach(Batch& b, Domain& d) {
 int batch_detach(Domain& d) { return d.batch ? detach(*d.batch, d) : FF_OK; }
 
 // Members whose links were not detected ahead (new boundary, or a boundary
-// moved by the host since the last step) detect them before the replay.
+// moved by the host since the last step) detect them before the replay, all
+// in one launch.  The block maps index b.mem in the order of the last
+// build_tables (the caller rebuilds a dirty batch first).
 static int detect_pending(Batch& b) {
-  for (Domain* dp : b.mem) {
-    Domain& d = *dp;
+  b.bumps.clear();
+  b.dets.clear();
+  for (size_t i = 0; i < b.mem.size(); i++) {
+    Domain& d = *b.mem[i];
     if (!(d.has_boundary && d.ne > 0) || d.predetected) continue;
-    LinkArgs la = link_args(d, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], d.elems);
-    la.dev_state = 1;
-    int rc = launch_detect(d, la, d.stale);
-    if (rc) return rc;
-    if (d.stale) d.tag_host++;  // mirror of the device bump
+    if (d.stale) {
+      b.bumps.push_back(make_int2((int)i, 0));
+      d.tag_host++;  // mirror of the device bump
+    }
+    const int nd = detect_blocks(d, kSumBlock, true);
+    for (int t = 0; t < nd; t++) b.dets.push_back(make_int2((int)i, t));
     d.stale = false;
     d.predetected = true;
   }
+  const size_t nb = b.bumps.size(), nd = b.dets.size(), n = nb + nd;
+  if (n == 0) return FF_OK;
+  if (!b.pend_ev) FF_HIP(hipEventCreateWithFlags(&b.pend_ev, hipEventDisableTiming));
+  FF_HIP(hipEventSynchronize(b.pend_ev));  // the last copy out of h_pend has completed
+  if (n > b.h_pend_cap) {
+    if (b.h_pend) (void)hipHostFree(b.h_pend);
+    b.h_pend = nullptr;
+    b.h_pend_cap = 0;
+    const size_t cap = n + n / 2 + 256;
+    FF_HIP(hipHostMalloc((void**)&b.h_pend, cap * sizeof(int2), hipHostMallocDefault));
+    b.h_pend_cap = cap;
+  }
+  int rc = grow(&b.d_pend, &b.d_pend_cap, n);
+  if (rc) return rc;
+  std::memcpy(b.h_pend, b.bumps.data(), nb * sizeof(int2));
+  std::memcpy(b.h_pend + nb, b.dets.data(), nd * sizeof(int2));
+  FF_HIP(hipMemcpyAsync(b.d_pend, b.h_pend, n * sizeof(int2), hipMemcpyHostToDevice, b.stream));
+  FF_HIP(hipEventRecord(b.pend_ev, b.stream));
+  if (nb) kb_tag_bump<<<(unsigned)((nb + 255) / 256), 256, 0, b.stream>>>(b.d_doms, b.d_pend, (int)nb);
+  if (nd) {
+    if (b.dim == 2) kb_detect<2><<<(unsigned)nd, kSumBlock, 0, b.stream>>>(b.d_doms, b.d_pend + nb);
+    else kb_detect<3><<<(unsigned)nd, kSumBlock, 0, b.stream>>>(b.d_doms, b.d_pend + nb);
+  }
+  FF_HIP(hipGetLastError());
   return FF_OK;
 }
 
@@ -478,6 +553,14 @@ int ff_batch_create(int dim, int storage, unsigned options, int device, ff_batch
     delete b;
     return hip_fail(e, "hipStreamCreate");
   }
+  e = hipMalloc((void**)&b->d_flag, 256);
+  if (e == hipSuccess) e = hipMemset(b->d_flag, 0, 256);
+  if (e != hipSuccess) {
+    if (b->d_flag) (void)hipFree(b->d_flag);
+    (void)hipStreamDestroy(b->stream);
+    delete b;
+    return hip_fail(e, "batch flag");
+  }
   *out = b;
   return FF_OK;
 }
@@ -491,6 +574,10 @@ int ff_batch_destroy(ff_batch* b) {
   if (b->d_doms) (void)hipFree(b->d_doms);
   if (b->d_map) (void)hipFree(b->d_map);
   if (b->d_steps) (void)hipFree(b->d_steps);
+  if (b->d_flag) (void)hipFree(b->d_flag);
+  if (b->d_pend) (void)hipFree(b->d_pend);
+  if (b->h_pend) (void)hipHostFree(b->h_pend);
+  if (b->pend_ev) (void)hipEventDestroy(b->pend_ev);
   (void)hipStreamDestroy(b->stream);
   delete b;
   return FF_OK;
@@ -553,12 +640,9 @@ int ff_batch_sync(ff_batch* b) {
   FF_CHECK_BATCH(b);
   FF_HIP(hipSetDevice(b->device));
   FF_HIP(hipStreamSynchronize(b->stream));
-  for (Domain* d : b->mem) {
-    if (!d->has_boundary) continue;
-    StepState st;
-    FF_HIP(hipMemcpy(&st, d->st, sizeof(st), hipMemcpyDeviceToHost));
-    if (st.overflow) return fail(FF_ERR_CAPACITY, "boundary link table overflow in a batched domain");
-  }
+  int flag = 0;  // sticky, like a member's own overflow flag
+  FF_HIP(hipMemcpy(&flag, b->d_flag, sizeof(int), hipMemcpyDeviceToHost));
+  if (flag) return fail(FF_ERR_CAPACITY, "boundary link table overflow in a batched domain");
   return FF_OK;
 }
 

@@ -31,8 +31,6 @@ __device__ __forceinline__ unsigned mix32(unsigned h) {
   return h;
 }
 
-// Returns the slot of `cell` in this generation, inserting it if absent
-// (the first inserter appends the slot to `list`).  -1 on a full table.
 // Generation and tag of the step being run (see LinkArgs).
 __device__ __forceinline__ unsigned step_gen(const LinkArgs& la) { return la.dev_state ? la.st->gen : la.gen; }
 __device__ __forceinline__ unsigned step_tag(const LinkArgs& la) { return la.dev_state ? la.st->tag : la.tag; }
@@ -136,37 +134,61 @@ __device__ __forceinline__ bool hit_elem2(const float* v, const uint32_t* e, int
   return true;
 }
 
-// line_triangle_intersect with the NumericUtils 1e-6 tolerances;
-// glm::normalize = v * (1 / sqrt(dot(v, v))).
-__device__ __forceinline__ bool hit_elem3(const float* v, const uint32_t* e, int elem, const float* p1, const float* p2,
-                                          Hit& h) {
+// line_triangle_intersect (:306-341) with the NumericUtils 1e-6 tolerances;
+// glm::normalize = v * (1 / sqrt(dot(v, v))).  The triangle's own quantities
+// (edges, unit normal, Gram terms) are formed once per element.
+struct Tri3 {
+  float v0[3], e1[3], e2[3], n[3];
+  float d00, d01, d11, den;
+};
+
+__device__ __forceinline__ Tri3 tri3(const float* v, const uint32_t* e, int elem) {
   float tv[3][3];
 #pragma unroll
   for (int j = 0; j < 3; j++)
 #pragma unroll
     for (int k = 0; k < 3; k++) tv[j][k] = v[e[3 * elem + j] * 3 + k];
-  const float e1[3] = {tv[1][0] - tv[0][0], tv[1][1] - tv[0][1], tv[1][2] - tv[0][2]};
-  const float e2[3] = {tv[2][0] - tv[0][0], tv[2][1] - tv[0][1], tv[2][2] - tv[0][2]};
-  const float c[3] = {e1[1] * e2[2] - e2[1] * e1[2], e1[2] * e2[0] - e2[2] * e1[0], e1[0] * e2[1] - e2[0] * e1[1]};
+  Tri3 T;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    T.v0[k] = tv[0][k];
+    T.e1[k] = tv[1][k] - tv[0][k];
+    T.e2[k] = tv[2][k] - tv[0][k];
+  }
+  const float c[3] = {T.e1[1] * T.e2[2] - T.e2[1] * T.e1[2], T.e1[2] * T.e2[0] - T.e2[2] * T.e1[0],
+                      T.e1[0] * T.e2[1] - T.e2[0] * T.e1[1]};
   const float cs = 1.0f / sqrtf(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
-  const float n[3] = {c[0] * cs, c[1] * cs, c[2] * cs};
-  const float dd[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+#pragma unroll
+  for (int k = 0; k < 3; k++) T.n[k] = c[k] * cs;
+  T.d00 = T.e1[0] * T.e1[0] + T.e1[1] * T.e1[1] + T.e1[2] * T.e1[2];
+  T.d01 = T.e1[0] * T.e2[0] + T.e1[1] * T.e2[1] + T.e1[2] * T.e2[2];
+  T.d11 = T.e2[0] * T.e2[0] + T.e2[1] * T.e2[1] + T.e2[2] * T.e2[2];
+  T.den = T.d00 * T.d11 - T.d01 * T.d01;
+  return T;
+}
+
+// The plane parameter of the link p1 -> p1 + dd is num / den with
+// num = n . (v0 - p1) and den = n . dd.
+__device__ __forceinline__ float tri3_num(const Tri3& T, const float* p1) {
+  const float a[3] = {T.v0[0] - p1[0], T.v0[1] - p1[1], T.v0[2] - p1[2]};
+  return T.n[0] * a[0] + T.n[1] * a[1] + T.n[2] * a[2];
+}
+__device__ __forceinline__ float tri3_den(const Tri3& T, const float* dd) {
+  return T.n[0] * dd[0] + T.n[1] * dd[1] + T.n[2] * dd[2];
+}
+
+// The rest of the test for a link whose plane parameter t passed 0 <= t <= 1
+// (or is NaN): parallel-link rejection, then the barycentric inside test.
+__device__ __forceinline__ bool tri3_finish(const Tri3& T, const float* p1, const float* dd, float t, Hit& h) {
   const float ds = 1.0f / sqrtf(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]);
   const float d[3] = {dd[0] * ds, dd[1] * ds, dd[2] * ds};
-  if (fabsf(n[0] * d[0] + n[1] * d[1] + n[2] * d[2]) <= 1e-6f) return false;
-  const float a[3] = {tv[0][0] - p1[0], tv[0][1] - p1[1], tv[0][2] - p1[2]};
-  const float t = (n[0] * a[0] + n[1] * a[1] + n[2] * a[2]) / (n[0] * dd[0] + n[1] * dd[1] + n[2] * dd[2]);
-  if (t < 0.0f || t > 1.0f) return false;
+  if (fabsf(T.n[0] * d[0] + T.n[1] * d[1] + T.n[2] * d[2]) <= 1e-6f) return false;
   const float ip[3] = {p1[0] + t * dd[0], p1[1] + t * dd[1], p1[2] + t * dd[2]};
-  const float w[3] = {ip[0] - tv[0][0], ip[1] - tv[0][1], ip[2] - tv[0][2]};
-  const float d00 = e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2];
-  const float d01 = e1[0] * e2[0] + e1[1] * e2[1] + e1[2] * e2[2];
-  const float d11 = e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2];
-  const float d20 = w[0] * e1[0] + w[1] * e1[1] + w[2] * e1[2];
-  const float d21 = w[0] * e2[0] + w[1] * e2[1] + w[2] * e2[2];
-  const float den = d00 * d11 - d01 * d01;
-  const float u0 = (d11 * d20 - d01 * d21) / den;
-  const float v0 = (d00 * d21 - d01 * d20) / den;
+  const float w[3] = {ip[0] - T.v0[0], ip[1] - T.v0[1], ip[2] - T.v0[2]};
+  const float d20 = w[0] * T.e1[0] + w[1] * T.e1[1] + w[2] * T.e1[2];
+  const float d21 = w[0] * T.e2[0] + w[1] * T.e2[1] + w[2] * T.e2[2];
+  const float u0 = (T.d11 * d20 - T.d01 * d21) / T.den;
+  const float v0 = (T.d00 * d21 - T.d01 * d20) / T.den;
   if (u0 >= 0.0f - 1e-6f && v0 >= 0.0f - 1e-6f && u0 + v0 <= 1.0f + 1e-6f) {
     h.depth = t;
     h.b[0] = 1.0f - u0 - v0;
@@ -178,6 +200,20 @@ __device__ __forceinline__ bool hit_elem3(const float* v, const uint32_t* e, int
     return true;
   }
   return false;
+}
+
+// Link p1 -> p2 against a triangle: every test of the reference, the plane
+// parameter first (a conjunction of the same predicates on the same values).
+__device__ __forceinline__ bool hit_tri3(const Tri3& T, const float* p1, const float* p2, Hit& h) {
+  const float dd[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+  const float t = tri3_num(T, p1) / tri3_den(T, dd);
+  if (t < 0.0f || t > 1.0f) return false;
+  return tri3_finish(T, p1, dd, t, h);
+}
+
+__device__ __forceinline__ bool hit_elem3(const float* v, const uint32_t* e, int elem, const float* p1, const float* p2,
+                                          Hit& h) {
+  return hit_tri3(tri3(v, e, elem), p1, p2, h);
 }
 
 // Candidate cells of an element: getBoundingBox (max starts at FLT_MIN) / dx,
@@ -302,6 +338,8 @@ __device__ __forceinline__ void build_links_pairs(const LinkArgs& la, long long 
   const int cz = DIM == 2 ? 0 : d3_c(d3_pack(2), l);
   unsigned long long* cells = table_cells(la, gen);
   unsigned long long* keys = table_keys<DIM>(la, gen);
+  Tri3 T;
+  if constexpr (DIM == 3) T = tri3(v, la.e, elem);
   for (long long k = lane; k < ncell; k += 16) {
     const int i = lo[0] + (int)(k % wx);
     const int j = lo[1] + (int)((k / wx) % wy);
@@ -317,12 +355,118 @@ __device__ __forceinline__ void build_links_pairs(const LinkArgs& la, long long 
         const float p1[3] = {(float)i * la.cl, (float)j * la.cl, (float)z * la.cl};
         const float p2[3] = {((float)i - (float)cx) * la.cl, ((float)j - (float)cy) * la.cl,
                              ((float)z - (float)cz) * la.cl};
-        hit = hit_elem3(v, la.e, elem, p1, p2, h);
+        hit = hit_tri3(T, p1, p2, h);
       }
     }
     const int slot = hit ? lt_claim(cells, la.cap, gen, cell, &la.st->overflow) : -1;
     if (slot >= 0) lt_offer_key(keys, (size_t)slot * K + (l - 1), gen, depth_key(h.depth, elem));
   }
+}
+
+// Tasks per pass of build_links_block3 (LDS: one int and one float each).
+constexpr int kDet3Chunk = 1024;
+
+// 3D link detection with one 256-thread block per element, in passes of
+// kDet3Chunk (window cell, direction) tasks: every task first forms the
+// link's plane parameter t (cells farther than 2 cl from the triangle's
+// plane skip the division: no link of length <= sqrt(3) cl reaches it, with a
+// margin above the rounding of the link ends for grids up to millions of
+// cells per axis); the tasks with 0 <= t <= 1 (or NaN) are gathered in LDS
+// and finished densely (parallel test, barycentrics, table entry).  Same
+// predicates, hits and keys as build_links_pairs.
+__device__ __forceinline__ void build_links_block3(const LinkArgs& la, long long block, unsigned tag, const float* v) {
+  constexpr int K = 26;
+  constexpr int NT = 256;
+  __shared__ int s_n;
+  __shared__ int s_task[kDet3Chunk];
+  __shared__ float s_t[kDet3Chunk];
+  if (block >= (long long)la.ne) return;  // block-uniform
+  const unsigned gen = tag;
+  const int elem = (int)block;
+  const int n[3] = {la.nx, la.ny, la.nz};
+  int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  elem_window<3>(v, la.e, elem, la.cl, n, lo, hi);
+  const int wx = hi[0] - lo[0] + 1, wy = hi[1] - lo[1] + 1, wz = hi[2] - lo[2] + 1;
+  if (wx <= 0 || wy <= 0 || wz <= 0) return;  // block-uniform
+  const long long ntask = (long long)wx * wy * wz * K;
+  const Tri3 T = tri3(v, la.e, elem);
+  const float far = 2.0f * la.cl;
+  unsigned long long* cells = table_cells(la, gen);
+  unsigned long long* keys = table_keys<3>(la, gen);
+  const int lane = (int)(threadIdx.x & 63u);
+  for (long long t0 = 0; t0 < ntask; t0 += kDet3Chunk) {
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    for (int r = 0; r < kDet3Chunk / NT; r++) {
+      const int lt = r * NT + (int)threadIdx.x;
+      const long long t = t0 + lt;
+      bool keep = false;
+      float tt = 0.0f;
+      if (t < ntask) {
+        const long long k = t / K;
+        const int l = (int)(t - k * K) + 1;
+        const int i = lo[0] + (int)(k % wx);
+        const int j = lo[1] + (int)((k / wx) % wy);
+        const int z = lo[2] + (int)(k / ((long long)wx * wy));
+        if (is_updated(la.flags[(z * la.ny + j) * la.nx + i])) {
+          const float p1[3] = {(float)i * la.cl, (float)j * la.cl, (float)z * la.cl};
+          const float num = tri3_num(T, p1);
+          if (fabsf(num) <= far) {
+            const float p2[3] = {((float)i - (float)d3_c(d3_pack(0), l)) * la.cl,
+                                 ((float)j - (float)d3_c(d3_pack(1), l)) * la.cl,
+                                 ((float)z - (float)d3_c(d3_pack(2), l)) * la.cl};
+            const float dd[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+            tt = num / tri3_den(T, dd);
+            keep = !(tt < 0.0f || tt > 1.0f);
+          }
+        }
+      }
+      const unsigned long long m = __ballot(keep);
+      if (m) {
+        int base = 0;
+        if (lane == 0) base = atomicAdd(&s_n, __popcll(m));
+        base = __shfl(base, 0, 64);
+        if (keep) {
+          const int idx = base + __popcll(m & ((1ull << lane) - 1ull));
+          s_task[idx] = lt;
+          s_t[idx] = tt;
+        }
+      }
+    }
+    __syncthreads();
+    const int ns = s_n;
+    for (int q = (int)threadIdx.x; q < ns; q += NT) {
+      const long long t = t0 + s_task[q];
+      const long long k = t / K;
+      const int l = (int)(t - k * K) + 1;
+      const int i = lo[0] + (int)(k % wx);
+      const int j = lo[1] + (int)((k / wx) % wy);
+      const int z = lo[2] + (int)(k / ((long long)wx * wy));
+      const float p1[3] = {(float)i * la.cl, (float)j * la.cl, (float)z * la.cl};
+      const float p2[3] = {((float)i - (float)d3_c(d3_pack(0), l)) * la.cl,
+                           ((float)j - (float)d3_c(d3_pack(1), l)) * la.cl,
+                           ((float)z - (float)d3_c(d3_pack(2), l)) * la.cl};
+      const float dd[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+      Hit h;
+      if (tri3_finish(T, p1, dd, s_t[q], h)) {
+        const int cell = (z * la.ny + j) * la.nx + i;
+        const int slot = lt_claim(cells, la.cap, gen, cell, &la.st->overflow);
+        if (slot >= 0) lt_offer_key(keys, (size_t)slot * K + (l - 1), gen, depth_key(h.depth, elem));
+      }
+    }
+    __syncthreads();  // the next pass resets s_n and rewrites the lists
+  }
+}
+
+// Detection block of a batch (blockDim = kDetectLanes<3> = 256): 2D packs
+// blockDim / 64 elements per block, 3D takes one element per block.
+static_assert(kDetectLanes<3> == 256, "build_links_block3 assumes 256-thread blocks");
+template <int DIM>
+__device__ __forceinline__ void detect_block(const LinkArgs& la, long long block, unsigned tag, const float* v) {
+  if constexpr (DIM == 2)
+    build_links_block<2, kDetectLanes<2>>(la, block, tag, v);
+  else
+    build_links_block3(la, block, tag, v);
 }
 
 }  // namespace ff

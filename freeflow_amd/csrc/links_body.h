@@ -286,7 +286,7 @@ template <int DIM, bool BATCHED>
 __device__ __forceinline__ void detect_next_body(const LinkArgs& la, int blk) {
   const unsigned gen = step_gen(la);
   if constexpr (BATCHED)
-    build_links_block<DIM, kDetectLanes<DIM>>(la, (long long)blk, step_tag(la) + 1u, la.v[(gen + 1u) & 1u]);
+    detect_block<DIM>(la, (long long)blk, step_tag(la) + 1u, la.v[(gen + 1u) & 1u]);
   else
     build_links_pairs<DIM>(la, (long long)blk, step_tag(la) + 1u, la.v[(gen + 1u) & 1u]);
 }

@@ -102,9 +102,12 @@ def make_episodes(rate, duration, seed, frac2d=0.7, steps=500, action_every=10,
 
 
 class Server:
-    def __init__(self, storage="f16", budget_bytes=None, max_active=512, device=0, keep_final=False):
+    def __init__(self, storage="f16", budget_bytes=None, max_active=512, device=0, keep_final=False, batched=True):
         self.storage = storage
         self.keep_final = keep_final  # read back each episode's final moments (ep.final_moments)
+        # batched=False: the plain per-domain form, every episode stepped by
+        # its own launches on its own stream (the same admission and actions)
+        self.batched = batched
         self.budget = budget_bytes if budget_bytes is not None else 288 * 10 ** 9
         self.max_active = max_active
         self.batches = {2: capi.Batch(2, storage, stage_fp16=True, device=device),
@@ -138,7 +141,8 @@ class Server:
         sc.setup(d, history=ep.action_every)
         ep.dom = d
         ep.admitted_at = now
-        self.batches[ep.kind].add(d)
+        if self.batched:
+            self.batches[ep.kind].add(d)
         self.active.append(ep)
         self.used += ep.footprint
 
@@ -153,7 +157,8 @@ class Server:
 
     def _retire(self, ep, now):
         ep.finished_at = now
-        self.batches[ep.kind].remove(ep.dom)
+        if self.batched:
+            self.batches[ep.kind].remove(ep.dom)
         if self.keep_final:
             ep.final_moments = ep.dom.moments(0)
         ep.dom.close()
@@ -169,12 +174,18 @@ class Server:
         for ep in self.active:
             if ep.done_steps % ep.action_every == 0:
                 self._act(ep)
-        for k in (2, 3):
-            if len(self.batches[k].members):
-                self.batches[k].step(1)
-        for k in (2, 3):
-            if len(self.batches[k].members):
-                self.batches[k].sync()
+        if self.batched:
+            for k in (2, 3):
+                if len(self.batches[k].members):
+                    self.batches[k].step(1)
+            for k in (2, 3):
+                if len(self.batches[k].members):
+                    self.batches[k].sync()
+        else:
+            for ep in self.active:
+                ep.dom.step(1)
+            for ep in self.active:
+                ep.dom.sync()
         for ep in self.active:
             ep.done_steps += 1
             self.cell_steps += ep.cells

@@ -57,3 +57,23 @@ def test_admission_by_footprint():
     assert summary["peak_bytes"] <= srv.budget
     assert summary["peak_active"] < len(eps)
     assert summary["p99_step_ms"] >= summary["p50_step_ms"] > 0
+
+
+def test_batched_equals_plain_form():
+    """Served in one batch per kind or every episode on its own launches: the
+    same moments and wrench histories, bit for bit (actions re-detect links in
+    one batched launch in the first form, per domain in the second)."""
+    res = {}
+    for batched in (True, False):
+        eps = small_episodes(11, n=4, steps=30)
+        note(f"case serve batched={batched} {[(e.kind, e.nx, e.ny, e.nz) for e in eps]}")
+        srv = serve.Server(storage="f16", keep_final=True, batched=batched)
+        summary = srv.run(eps, realtime=False)
+        srv.close()
+        assert summary["episodes_done"] == len(eps)
+        res[batched] = eps
+    for a, b in zip(res[True], res[False]):
+        assert np.array_equal(a.final_moments, b.final_moments), f"episode {a.eid} kind {a.kind}"
+        assert len(a.wrench) == len(b.wrench) == 3
+        for wa, wb in zip(a.wrench, b.wrench):
+            assert np.array_equal(wa, wb)
