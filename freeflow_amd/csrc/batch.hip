@@ -312,6 +312,7 @@ static int build_tables(Batch& b, int tile) {
     r.tiles_y = ty;
     for (int t = 0; t < ntl; t++) bulk.push_back(make_int2(i, t));
     if (bnd) {
+      r.la.bc_span = nt;  // whole blocks per pass: many domains fill the chip
       r.nbc = boundary_cell_blocks(d, nt);
       for (int t = 0; t < r.nbc; t++) bc.push_back(make_int2(i, t));
       if (d.wave) {

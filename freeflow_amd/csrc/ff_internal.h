@@ -65,6 +65,7 @@ struct LinkArgs {
   float* lh;                  // cap * (Q-1) * DIM  hit point
   float* sb;                  // cap * NM  updated moments of each boundary cell (copied back after the step)
   int cap;
+  int bc_span;  // link-table slots a boundary-cell block scans per pass (<= its block size)
   StepState* st;
   // Step generation (vertex-buffer parity gen & 1, wrench ring slot) and
   // link-table tag of the step: passed by value for a domain stepped on its
@@ -174,6 +175,7 @@ int flush_finish(Domain& d);
 int launch_outlet(Domain& d, void* buf);
 int launch_init_state(Domain& d, const float* bc6);
 int launch_wave_now(Domain& d);
+int bc_span(const Domain& d);  // boundary-cell scan span of a domain stepped on its own
 int launch_step_vertex_forces(Domain& d, float* out_dev);
 int launch_convert_in(Domain& d, const float* aos_dev, void* buf);
 int launch_convert_out(Domain& d, const void* buf, float* aos_dev);

@@ -116,7 +116,7 @@ int launch_step_stream(Domain& d, const void* in, void* out, const LinkArgs& la,
   if (d.dim == 3) return launch_step_stream3d(d, in, out, la, fa, wave);
   fa.kin_blocks = wave ? (d.nv + 255) / 256 : 0;
   fa.fin_blocks = d.finish_pending ? (d.nv + 255) / 256 + 1 : 0;
-  const StreamExtra x{&la, &fa, boundary_cell_blocks(d, 256), fa.kin_blocks + fa.fin_blocks};
+  const StreamExtra x{&la, &fa, boundary_cell_blocks(d, la.bc_span), fa.kin_blocks + fa.fin_blocks};
   return bulk2d_dispatch(d, in, out, &x);
 }
 

@@ -79,7 +79,7 @@ static void step3d_shape(Domain& d, const S* i, S* o, const LinkArgs& la, FinalA
   fa.kin_blocks = wave ? (d.nv + NT - 1) / NT : 0;
   fa.fin_blocks = d.finish_pending ? (d.nv + NT - 1) / NT + 1 : 0;
   const int nw = fa.kin_blocks + fa.fin_blocks;
-  const int nbc = xcd_grid(boundary_cell_blocks(d, NT) + nw) - nw;
+  const int nbc = xcd_grid(boundary_cell_blocks(d, la.bc_span) + nw) - nw;
   const unsigned grid = (unsigned)(nbc + nw + xcd_grid(nt));
   k_step3d<S, PERIODIC, RING, TX, TY, ZC, NT><<<grid, NT, 0, d.stream>>>(i, o, tx, ty, nt, nbc, nw, la, fa);
 }

@@ -98,8 +98,18 @@ int copyback_blocks(const Domain& d, int fb, int per_thread) {
   return (int)std::min<long long>(256, std::max<long long>(1, ((long long)d.lt_cap + span - 1) / span));
 }
 
-int boundary_cell_blocks(const Domain& d, int nt) {
-  return (int)std::min<long long>(512, std::max<long long>(1, ((long long)d.lt_cap + nt - 1) / nt));
+// A domain stepped on its own spreads its boundary cells over about 256
+// blocks (one per CU, few cells each: the launch's latency is the slowest
+// block's), scanning 64..(smallest block size) slots per pass.
+int bc_span(const Domain& d) {
+  const int nt_min = d.dim == 2 ? 256 : 128;
+  int s = 64;
+  while (s < nt_min && (long long)s * 256 < d.lt_cap) s *= 2;
+  return s;
+}
+
+int boundary_cell_blocks(const Domain& d, int span) {
+  return (int)std::min<long long>(512, std::max<long long>(1, ((long long)d.lt_cap + span - 1) / span));
 }
 
 int launch_step_sums(Domain& d, const LinkArgs& la, void* out, bool detect_next) {

@@ -44,8 +44,9 @@ __device__ __forceinline__ void cell_pops(const S* in, size_t plane, size_t c, f
 // The result is written to the cell's side-buffer row (la.sb), not to the
 // output moments: the stream-collide tiles of the same launch also write the
 // cell, and the element-sum launch copies the side buffer over them.
-// Each block scans NT link-table slots at a time (grid stride nblk * NT),
-// compacts the occupied ones in LDS and updates those cells CPB at a time.
+// Each block scans la.bc_span (<= NT) link-table slots at a time (grid
+// stride nblk * bc_span), compacts the occupied ones in LDS and updates those
+// cells CPB at a time.
 template <int DIM, typename S, bool STAGE, bool PERIODIC, int NT>
 __device__ __forceinline__ void bcells_body(const S* __restrict__ in, const LinkArgs& la, int blk, int nblk) {
   constexpr int Q = DIM == 2 ? 9 : 27;
@@ -63,11 +64,13 @@ __device__ __forceinline__ void bcells_body(const S* __restrict__ in, const Link
   const int lc = threadIdx.x / LPC, dir = threadIdx.x % LPC;
   const unsigned long long* cells = table_cells(la, tag);
   const unsigned long long* keys = table_keys<DIM>(la, tag);
-  for (int base = blk * NT; base < la.cap; base += nblk * NT) {
+  const int span = la.bc_span;
+  for (int base = blk * span; base < la.cap; base += nblk * span) {
     // occupied slots of this chunk, in slot order
     const int s = base + (int)threadIdx.x;
-    const unsigned long long ent = s < la.cap ? cells[s] : 0ull;
-    const bool occ = s < la.cap && (unsigned)(ent >> 32) == tag;
+    const bool scan = (int)threadIdx.x < span && s < la.cap;
+    const unsigned long long ent = scan ? cells[s] : 0ull;
+    const bool occ = scan && (unsigned)(ent >> 32) == tag;
     const unsigned long long m = __ballot(occ);
     const int w = (int)(threadIdx.x >> 6);
     if ((threadIdx.x & 63u) == 0) s_wn[w] = (int)__popcll(m);
@@ -243,7 +246,7 @@ __host__ __device__ inline int elem_blocks(int ne) { return (ne + kElemPerBlock 
 // kinematics into vn/un, per-vertex forces optionally added to proxy_forces).
 FinalArgs make_final_args(Domain& d, void* out, float* vn0, float* vn1, float* un0, float* un1, float* proxy_forces);
 // Host: grid sizes of the boundary-cell and copy-back parts of a step.
-int boundary_cell_blocks(const Domain& d, int nt);
+int boundary_cell_blocks(const Domain& d, int span);
 int copyback_blocks(const Domain& d, int fb, int per_thread);
 int detect_blocks(const Domain& d, int nt, bool batched);
 
