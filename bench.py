@@ -189,6 +189,8 @@ def run_batched(args, capi, wl):
         sc.setup(d)
         b.add(d)
         doms.append(d)
+    if args.tile >= 0:
+        b.set_tile(args.tile)
     b.step(W)
     b.sync()
     print(f"[bench] batched FP16 form: warmup {W} done; timing {K} steps", file=sys.stderr, flush=True)

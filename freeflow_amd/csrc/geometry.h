@@ -116,10 +116,9 @@ struct Hit {
 };
 
 // intersectAtIndex<2>: barycentrics from glm::distance (:220-221).
-__device__ __forceinline__ bool hit_elem2(const float* v, const uint32_t* e, int elem, float p1x, float p1y, float p2x,
-                                          float p2y, Hit& h) {
-  const uint32_t s = e[2 * elem], t = e[2 * elem + 1];
-  const float p3x = v[2 * s], p3y = v[2 * s + 1], p4x = v[2 * t], p4y = v[2 * t + 1];
+// Segment endpoints (p3, p4) given by value.
+__device__ __forceinline__ bool hit_seg2(float p3x, float p3y, float p4x, float p4y, float p1x, float p1y, float p2x,
+                                         float p2y, Hit& h) {
   float xo = 0.f, yo = 0.f;
   const float d = seg_link(p1x, p1y, p2x, p2y, p3x, p3y, p4x, p4y, xo, yo);
   if (!(d > 0.0f && d < 1.0f)) return false;
@@ -132,6 +131,12 @@ __device__ __forceinline__ bool hit_elem2(const float* v, const uint32_t* e, int
   h.b[1] = 1.0f - h.b[0];
   h.b[2] = 0.0f;
   return true;
+}
+
+__device__ __forceinline__ bool hit_elem2(const float* v, const uint32_t* e, int elem, float p1x, float p1y, float p2x,
+                                          float p2y, Hit& h) {
+  const uint32_t s = e[2 * elem], t = e[2 * elem + 1];
+  return hit_seg2(v[2 * s], v[2 * s + 1], v[2 * t], v[2 * t + 1], p1x, p1y, p2x, p2y, h);
 }
 
 // line_triangle_intersect (:306-341) with the NumericUtils 1e-6 tolerances;
@@ -258,57 +263,39 @@ __device__ __forceinline__ int d3_c(unsigned long long packed, int i) { return (
 template <int DIM>
 constexpr int kDetectLanes = DIM == 2 ? 64 : 256;
 
-// Link detection: a group of LPE lanes per element walks the
-// element's candidate window once for every lattice direction, one (window
-// cell, direction) task per lane (the loops of
-// mrSolver2DComputeOccupancyFlagKernel_v3, FsiCoupling2D.cu:44-74, and its 3D
-// twin, FsiCoupling3D.cu:43-76), and offers depthIndexCompact keys for every
-// hit of an updated cell.  `block` indexes groups of blockDim / LPE
-// elements; the links are entered under `tag` for the vertex positions `v`.
-template <int DIM, int LPE>
-__device__ __forceinline__ void build_links_block(const LinkArgs& la, long long block, unsigned tag, const float* v) {
+// 2D link detection: a group of LPE lanes per element walks the element's
+// candidate window once for every lattice direction, one (window cell,
+// direction) task per lane (the loops of mrSolver2DComputeOccupancyFlagKernel_v3,
+// FsiCoupling2D.cu:44-74), and offers depthIndexCompact keys for every hit of
+// an updated cell.  `block` indexes groups of blockDim / LPE elements; the
+// links are entered under `tag` for the vertex positions `v`.
+template <int LPE>
+__device__ __forceinline__ void build_links_block2(const LinkArgs& la, long long block, unsigned tag, const float* v) {
   const unsigned gen = tag;
-  constexpr int K = DIM == 2 ? 8 : 26;
+  constexpr int K = 8;
   const long long e = block * (long long)(blockDim.x / LPE) + threadIdx.x / LPE;
   if (e >= (long long)la.ne) return;
   const int elem = (int)e, lane = (int)(threadIdx.x % LPE);
   const int n[3] = {la.nx, la.ny, la.nz};
   int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-  elem_window<DIM>(v, la.e, elem, la.cl, n, lo, hi);
-  const int wx = hi[0] - lo[0] + 1, wy = hi[1] - lo[1] + 1, wz = DIM == 3 ? hi[2] - lo[2] + 1 : 1;
-  if (wx <= 0 || wy <= 0 || wz <= 0) return;
-  const long long ntask = (long long)wx * wy * wz * K;
+  elem_window<2>(v, la.e, elem, la.cl, n, lo, hi);
+  const int wx = hi[0] - lo[0] + 1, wy = hi[1] - lo[1] + 1;
+  if (wx <= 0 || wy <= 0) return;
+  const int ntask = wx * wy * K;
+  const uint32_t sa = la.e[2 * elem], sb = la.e[2 * elem + 1];
+  const float p3x = v[2 * sa], p3y = v[2 * sa + 1], p4x = v[2 * sb], p4y = v[2 * sb + 1];
   unsigned long long* cells = table_cells(la, gen);
-  unsigned long long* keys = table_keys<DIM>(la, gen);
-  for (long long t = lane; t < ntask; t += LPE) {
-    const long long k = t / K;
-    const int l = (int)(t - k * K) + 1;
-    const int i = lo[0] + (int)(k % wx);
-    const int j = lo[1] + (int)((k / wx) % wy);
-    const int z = DIM == 3 ? lo[2] + (int)(k / ((long long)wx * wy)) : 0;
-    int cx, cy, cz = 0;
-    if constexpr (DIM == 2) {
-      cx = D2::cx(l);
-      cy = D2::cy(l);
-    } else {
-      cx = d3_c(d3_pack(0), l);
-      cy = d3_c(d3_pack(1), l);
-      cz = d3_c(d3_pack(2), l);
-    }
-    const int cell = (z * la.ny + j) * la.nx + i;
+  unsigned long long* keys = table_keys<2>(la, gen);
+  for (int t = lane; t < ntask; t += LPE) {
+    const int k = t / K;
+    const int l = t - k * K + 1;
+    const int i = lo[0] + k % wx, j = lo[1] + k / wx;
+    const int cell = j * la.nx + i;
     Hit h;
     bool hit = false;
-    if (is_updated(la.flags[cell])) {
-      if constexpr (DIM == 2) {
-        hit = hit_elem2(v, la.e, elem, (float)i * la.cl, (float)j * la.cl, ((float)i - (float)cx) * la.cl,
-                        ((float)j - (float)cy) * la.cl, h);
-      } else {
-        const float p1[3] = {(float)i * la.cl, (float)j * la.cl, (float)z * la.cl};
-        const float p2[3] = {((float)i - (float)cx) * la.cl, ((float)j - (float)cy) * la.cl,
-                             ((float)z - (float)cz) * la.cl};
-        hit = hit_elem3(v, la.e, elem, p1, p2, h);
-      }
-    }
+    if (is_updated(la.flags[cell]))
+      hit = hit_seg2(p3x, p3y, p4x, p4y, (float)i * la.cl, (float)j * la.cl, ((float)i - (float)D2::cx(l)) * la.cl,
+                     ((float)j - (float)D2::cy(l)) * la.cl, h);
     const int slot = hit ? lt_claim(cells, la.cap, gen, cell, &la.st->overflow) : -1;
     if (slot >= 0) lt_offer_key(keys, (size_t)slot * K + (l - 1), gen, depth_key(h.depth, elem));
   }
@@ -316,7 +303,8 @@ __device__ __forceinline__ void build_links_block(const LinkArgs& la, long long 
 
 // Link detection with one 16-lane group per (element, direction) pair, each
 // lane one window cell: the shortest latency for a domain stepped on its own
-// (the same predicates, hits and keys as build_links_block).  `block` indexes
+// (the same predicates, hits and keys as build_links_block2 and
+// build_links_block3).  `block` indexes
 // groups of blockDim / 16 pairs.
 template <int DIM>
 __device__ __forceinline__ void build_links_pairs(const LinkArgs& la, long long block, unsigned tag, const float* v) {
@@ -332,25 +320,34 @@ __device__ __forceinline__ void build_links_pairs(const LinkArgs& la, long long 
   elem_window<DIM>(v, la.e, elem, la.cl, n, lo, hi);
   const int wx = hi[0] - lo[0] + 1, wy = hi[1] - lo[1] + 1, wz = DIM == 3 ? hi[2] - lo[2] + 1 : 1;
   if (wx <= 0 || wy <= 0 || wz <= 0) return;
-  const long long ncell = (long long)wx * wy * wz;
+  const int ncell = wx * wy * wz;
   const int cx = DIM == 2 ? D2::cx(l) : d3_c(d3_pack(0), l);
   const int cy = DIM == 2 ? D2::cy(l) : d3_c(d3_pack(1), l);
   const int cz = DIM == 2 ? 0 : d3_c(d3_pack(2), l);
   unsigned long long* cells = table_cells(la, gen);
   unsigned long long* keys = table_keys<DIM>(la, gen);
   Tri3 T;
-  if constexpr (DIM == 3) T = tri3(v, la.e, elem);
-  for (long long k = lane; k < ncell; k += 16) {
-    const int i = lo[0] + (int)(k % wx);
-    const int j = lo[1] + (int)((k / wx) % wy);
-    const int z = DIM == 3 ? lo[2] + (int)(k / ((long long)wx * wy)) : 0;
+  float p3x = 0.f, p3y = 0.f, p4x = 0.f, p4y = 0.f;
+  if constexpr (DIM == 3) {
+    T = tri3(v, la.e, elem);
+  } else {
+    const uint32_t sa = la.e[2 * elem], sb = la.e[2 * elem + 1];
+    p3x = v[2 * sa];
+    p3y = v[2 * sa + 1];
+    p4x = v[2 * sb];
+    p4y = v[2 * sb + 1];
+  }
+  for (int k = lane; k < ncell; k += 16) {
+    const int i = lo[0] + k % wx;
+    const int j = lo[1] + (k / wx) % wy;
+    const int z = DIM == 3 ? lo[2] + k / (wx * wy) : 0;
     const int cell = (z * la.ny + j) * la.nx + i;
     Hit h;
     bool hit = false;
     if (is_updated(la.flags[cell])) {
       if constexpr (DIM == 2) {
-        hit = hit_elem2(v, la.e, elem, (float)i * la.cl, (float)j * la.cl, ((float)i - (float)cx) * la.cl,
-                        ((float)j - (float)cy) * la.cl, h);
+        hit = hit_seg2(p3x, p3y, p4x, p4y, (float)i * la.cl, (float)j * la.cl, ((float)i - (float)cx) * la.cl,
+                       ((float)j - (float)cy) * la.cl, h);
       } else {
         const float p1[3] = {(float)i * la.cl, (float)j * la.cl, (float)z * la.cl};
         const float p2[3] = {((float)i - (float)cx) * la.cl, ((float)j - (float)cy) * la.cl,
@@ -363,23 +360,40 @@ __device__ __forceinline__ void build_links_pairs(const LinkArgs& la, long long 
   }
 }
 
-// Tasks per pass of build_links_block3 (LDS: one int and one float each).
-constexpr int kDet3Chunk = 1024;
+// Survivor capacity of build_links_block3's LDS lists (an int and a byte
+// each).
+constexpr int kDet3Cap = 2048;
 
-// 3D link detection with one 256-thread block per element, in passes of
-// kDet3Chunk (window cell, direction) tasks: every task first forms the
-// link's plane parameter t (cells farther than 2 cl from the triangle's
-// plane skip the division: no link of length <= sqrt(3) cl reaches it, with a
-// margin above the rounding of the link ends for grids up to millions of
-// cells per axis); the tasks with 0 <= t <= 1 (or NaN) are gathered in LDS
-// and finished densely (parallel test, barycentrics, table entry).  Same
-// predicates, hits and keys as build_links_pairs.
+// Superset test, without a division, of "num / den rounds into [0, 1] or is
+// NaN" (the plane-parameter test of line_triangle_intersect): num = 0 gives
+// +-0 or NaN; a zero den otherwise gives +-inf; equal signs need
+// |num| <= |den| (1 + 2^-24) for the quotient to round to at most 1, which
+// fl(|den| * 1.0001) bounds from above (rounding is monotone); opposite signs
+// round to -0 only when the quotient underflows, below 1e-30 by far.
+__device__ __forceinline__ bool tri3_t_may_pass(float num, float den) {
+  if (num == 0.0f) return true;
+  if (den == 0.0f) return false;
+  const float an = fabsf(num), ad = fabsf(den);
+  return (num > 0.0f) == (den > 0.0f) ? an <= ad * 1.0001f : an < ad * 1e-30f;
+}
+
+// 3D link detection with one 256-thread block per element, cell-major: each
+// thread takes one cell of the element's window per pass (window coordinates
+// formed once per cell), forms num = n . (v0 - p1), and only cells within
+// 2 cl of the triangle's plane go on (no link of length <= sqrt(3) cl from a
+// farther cell reaches it; the margin is far above the rounding of the link
+// ends for grids up to millions of cells per axis).  For each of the 26
+// directions (unrolled, lattice vectors as constants) links whose plane
+// parameter t = num / (n . dd) may lie in [0, 1] (tri3_t_may_pass, no
+// division) are gathered in LDS; once the lists may fill, and at the end, the
+// block finishes them (t itself, parallel test, barycentrics, table entry).
+// Same predicates, hits and keys as build_links_pairs.
 __device__ __forceinline__ void build_links_block3(const LinkArgs& la, long long block, unsigned tag, const float* v) {
   constexpr int K = 26;
   constexpr int NT = 256;
   __shared__ int s_n;
-  __shared__ int s_task[kDet3Chunk];
-  __shared__ float s_t[kDet3Chunk];
+  __shared__ int s_cell[kDet3Cap];
+  __shared__ unsigned char s_dir[kDet3Cap];
   if (block >= (long long)la.ne) return;  // block-uniform
   const unsigned gen = tag;
   const int elem = (int)block;
@@ -388,38 +402,60 @@ __device__ __forceinline__ void build_links_block3(const LinkArgs& la, long long
   elem_window<3>(v, la.e, elem, la.cl, n, lo, hi);
   const int wx = hi[0] - lo[0] + 1, wy = hi[1] - lo[1] + 1, wz = hi[2] - lo[2] + 1;
   if (wx <= 0 || wy <= 0 || wz <= 0) return;  // block-uniform
-  const long long ntask = (long long)wx * wy * wz * K;
+  const int ncell = wx * wy * wz;
   const Tri3 T = tri3(v, la.e, elem);
   const float far = 2.0f * la.cl;
   unsigned long long* cells = table_cells(la, gen);
   unsigned long long* keys = table_keys<3>(la, gen);
   const int lane = (int)(threadIdx.x & 63u);
-  for (long long t0 = 0; t0 < ntask; t0 += kDet3Chunk) {
+  // finishes the ns gathered links; called block-uniformly, with ns read by
+  // every thread between two barriers
+  auto flush = [&](int ns) {
+    for (int q = (int)threadIdx.x; q < ns; q += NT) {
+      const int k = s_cell[q], l = s_dir[q];
+      const int i = lo[0] + k % wx, j = lo[1] + (k / wx) % wy, z = lo[2] + k / (wx * wy);
+      const float p1[3] = {(float)i * la.cl, (float)j * la.cl, (float)z * la.cl};
+      const float p2[3] = {((float)i - (float)d3_c(d3_pack(0), l)) * la.cl,
+                           ((float)j - (float)d3_c(d3_pack(1), l)) * la.cl,
+                           ((float)z - (float)d3_c(d3_pack(2), l)) * la.cl};
+      const float dd[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+      const float t = tri3_num(T, p1) / tri3_den(T, dd);
+      Hit h;
+      if (!(t < 0.0f || t > 1.0f) && tri3_finish(T, p1, dd, t, h)) {
+        const int slot = lt_claim(cells, la.cap, gen, (z * la.ny + j) * la.nx + i, &la.st->overflow);
+        if (slot >= 0) lt_offer_key(keys, (size_t)slot * K + (l - 1), gen, depth_key(h.depth, elem));
+      }
+    }
+    __syncthreads();
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
-    for (int r = 0; r < kDet3Chunk / NT; r++) {
-      const int lt = r * NT + (int)threadIdx.x;
-      const long long t = t0 + lt;
+  };
+  if (threadIdx.x == 0) s_n = 0;
+  __syncthreads();
+  for (int c0 = 0; c0 < ncell; c0 += NT) {
+    const int k = c0 + (int)threadIdx.x;
+    bool near = false;
+    float fi = 0.0f, fj = 0.0f, fz = 0.0f, num = 0.0f;
+    if (k < ncell) {
+      const int i = lo[0] + k % wx, j = lo[1] + (k / wx) % wy, z = lo[2] + k / (wx * wy);
+      if (is_updated(la.flags[(z * la.ny + j) * la.nx + i])) {
+        fi = (float)i;
+        fj = (float)j;
+        fz = (float)z;
+        const float p1[3] = {fi * la.cl, fj * la.cl, fz * la.cl};
+        num = tri3_num(T, p1);
+        near = fabsf(num) <= far;
+      }
+    }
+    const float p1[3] = {fi * la.cl, fj * la.cl, fz * la.cl};
+#pragma unroll
+    for (int l = 1; l <= K; l++) {
       bool keep = false;
-      float tt = 0.0f;
-      if (t < ntask) {
-        const long long k = t / K;
-        const int l = (int)(t - k * K) + 1;
-        const int i = lo[0] + (int)(k % wx);
-        const int j = lo[1] + (int)((k / wx) % wy);
-        const int z = lo[2] + (int)(k / ((long long)wx * wy));
-        if (is_updated(la.flags[(z * la.ny + j) * la.nx + i])) {
-          const float p1[3] = {(float)i * la.cl, (float)j * la.cl, (float)z * la.cl};
-          const float num = tri3_num(T, p1);
-          if (fabsf(num) <= far) {
-            const float p2[3] = {((float)i - (float)d3_c(d3_pack(0), l)) * la.cl,
-                                 ((float)j - (float)d3_c(d3_pack(1), l)) * la.cl,
-                                 ((float)z - (float)d3_c(d3_pack(2), l)) * la.cl};
-            const float dd[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
-            tt = num / tri3_den(T, dd);
-            keep = !(tt < 0.0f || tt > 1.0f);
-          }
-        }
+      if (near) {
+        const float p2[3] = {(fi - (float)d3_c(d3_pack(0), l)) * la.cl, (fj - (float)d3_c(d3_pack(1), l)) * la.cl,
+                             (fz - (float)d3_c(d3_pack(2), l)) * la.cl};
+        const float dd[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+        keep = tri3_t_may_pass(num, tri3_den(T, dd));
       }
       const unsigned long long m = __ballot(keep);
       if (m) {
@@ -428,34 +464,23 @@ __device__ __forceinline__ void build_links_block3(const LinkArgs& la, long long
         base = __shfl(base, 0, 64);
         if (keep) {
           const int idx = base + __popcll(m & ((1ull << lane) - 1ull));
-          s_task[idx] = lt;
-          s_t[idx] = tt;
+          s_cell[idx] = k;
+          s_dir[idx] = (unsigned char)l;
         }
       }
-    }
-    __syncthreads();
-    const int ns = s_n;
-    for (int q = (int)threadIdx.x; q < ns; q += NT) {
-      const long long t = t0 + s_task[q];
-      const long long k = t / K;
-      const int l = (int)(t - k * K) + 1;
-      const int i = lo[0] + (int)(k % wx);
-      const int j = lo[1] + (int)((k / wx) % wy);
-      const int z = lo[2] + (int)(k / ((long long)wx * wy));
-      const float p1[3] = {(float)i * la.cl, (float)j * la.cl, (float)z * la.cl};
-      const float p2[3] = {((float)i - (float)d3_c(d3_pack(0), l)) * la.cl,
-                           ((float)j - (float)d3_c(d3_pack(1), l)) * la.cl,
-                           ((float)z - (float)d3_c(d3_pack(2), l)) * la.cl};
-      const float dd[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
-      Hit h;
-      if (tri3_finish(T, p1, dd, s_t[q], h)) {
-        const int cell = (z * la.ny + j) * la.nx + i;
-        const int slot = lt_claim(cells, la.cap, gen, cell, &la.st->overflow);
-        if (slot >= 0) lt_offer_key(keys, (size_t)slot * K + (l - 1), gen, depth_key(h.depth, elem));
+      // at most 4 * NT entries join between checks: finish the lists while
+      // that still fits
+      if (l % 4 == 0 || l == K) {
+        __syncthreads();
+        const int ns = s_n;
+        __syncthreads();  // read by all before any wave adds again
+        if (ns > kDet3Cap - 4 * NT) flush(ns);
       }
     }
-    __syncthreads();  // the next pass resets s_n and rewrites the lists
   }
+  __syncthreads();
+  const int ns = s_n;
+  if (ns > 0) flush(ns);
 }
 
 // Detection block of a batch (blockDim = kDetectLanes<3> = 256): 2D packs
@@ -464,7 +489,7 @@ static_assert(kDetectLanes<3> == 256, "build_links_block3 assumes 256-thread blo
 template <int DIM>
 __device__ __forceinline__ void detect_block(const LinkArgs& la, long long block, unsigned tag, const float* v) {
   if constexpr (DIM == 2)
-    build_links_block<2, kDetectLanes<2>>(la, block, tag, v);
+    build_links_block2<kDetectLanes<2>>(la, block, tag, v);
   else
     build_links_block3(la, block, tag, v);
 }
