@@ -77,7 +77,13 @@ struct Batch {
   std::vector<int2> bumps, dets;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // kGraphSteps steps in one graph: long runs pay one graph launch per
+  // kGraphSteps steps
+  hipGraph_t graph_n = nullptr;
+  hipGraphExec_t exec_n = nullptr;
 };
+
+constexpr int kGraphSteps = 8;
 
 // ---------------------------------------------------------------- kernels
 // Stream launch: [boundary-cell blocks of every domain][wave blocks][tiles].
@@ -411,11 +417,46 @@ static int autotune(Batch& b) {
 static void drop_graph(Batch& b) {
   if (b.exec) (void)hipGraphExecDestroy(b.exec);
   if (b.graph) (void)hipGraphDestroy(b.graph);
+  if (b.exec_n) (void)hipGraphExecDestroy(b.exec_n);
+  if (b.graph_n) (void)hipGraphDestroy(b.graph_n);
   b.exec = nullptr;
   b.graph = nullptr;
+  b.exec_n = nullptr;
+  b.graph_n = nullptr;
 }
 
 static int current_tile(const Batch& b) { return b.tile >= 0 ? b.tile : std::max(b.tuned_tile, 0); }
+
+// Captures `steps` batched steps into one instantiated graph.
+static int capture_steps(Batch& b, int steps, hipGraph_t* graph, hipGraphExec_t* exec) {
+  FF_HIP(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
+  for (int k = 0; k < steps; k++) launch_step(b, current_tile(b));
+  hipError_t le = hipGetLastError();
+  hipGraph_t g = nullptr;
+  hipError_t ce = hipStreamEndCapture(b.stream, &g);
+  if (le != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return hip_fail(le, "batched step launch");
+  }
+  if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture");
+  *graph = g;
+  FF_HIP(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+  return FF_OK;
+}
+
+// Replays `nsteps` steps: kGraphSteps at a time (that graph is captured on
+// the first long run after a rebuild), then one at a time.
+static int replay(Batch& b, int nsteps) {
+  int s = 0;
+  if (nsteps >= kGraphSteps && !b.exec_n) {
+    const int rc = capture_steps(b, kGraphSteps, &b.graph_n, &b.exec_n);
+    if (rc) return rc;
+  }
+  for (; s + kGraphSteps <= nsteps; s += kGraphSteps) FF_HIP(hipGraphLaunch(b.exec_n, b.stream));
+  for (; s < nsteps; s++) FF_HIP(hipGraphLaunch(b.exec, b.stream));
+  return FF_OK;
+}
+
 
 static int rebuild(Batch& b) {
   FF_HIP(hipSetDevice(b.device));
@@ -428,18 +469,7 @@ static int rebuild(Batch& b) {
   const bool retune = b.tuned_tile < 0 || cells > 2.0 * b.tuned_cells || 2.0 * cells < b.tuned_cells;
   if (b.tile < 0 && retune && (rc = autotune(b))) return rc;
   if ((rc = build_tables(b, current_tile(b)))) return rc;
-  FF_HIP(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
-  launch_step(b, current_tile(b));
-  hipError_t le = hipGetLastError();
-  hipGraph_t g = nullptr;
-  hipError_t ce = hipStreamEndCapture(b.stream, &g);
-  if (le != hipSuccess) {
-    if (g) (void)hipGraphDestroy(g);
-    return hip_fail(le, "batched step launch");
-  }
-  if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture");
-  b.graph = g;
-  FF_HIP(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
+  if ((rc = capture_steps(b, 1, &b.graph, &b.exec))) return rc;
   b.dirty = false;
   return FF_OK;
 }
@@ -632,7 +662,7 @@ int ff_batch_step(ff_batch* b, int nsteps) {
   int rc;
   if (b->dirty && (rc = rebuild(*b))) return rc;
   if ((rc = detect_pending(*b))) return rc;
-  for (int s = 0; s < nsteps; s++) FF_HIP(hipGraphLaunch(b->exec, b->stream));
+  if ((rc = replay(*b, nsteps))) return rc;
   advance_mirrors(*b, nsteps);
   return FF_OK;
 }
@@ -680,8 +710,7 @@ int ff_batch_time_steps(ff_batch* b, int nsteps, double* total_ms) {
   FF_HIP(hipEventCreate(&e1));
   if (!b->mem.empty() && (rc = detect_pending(*b))) return rc;
   FF_HIP(hipEventRecord(e0, b->stream));
-  if (!b->mem.empty())
-    for (int s = 0; s < nsteps; s++) FF_HIP(hipGraphLaunch(b->exec, b->stream));
+  if (!b->mem.empty() && (rc = replay(*b, nsteps))) return rc;
   FF_HIP(hipEventRecord(e1, b->stream));
   FF_HIP(hipEventSynchronize(e1));
   float ms = 0.f;

@@ -76,6 +76,32 @@ def test_batch_equals_individual_steps_2d():
         mixed[0].step(1)
 
 
+def test_graph_replays_between_eager_work():
+    """Runs of 8 or more steps replay an 8-step graph: replay it several
+    times with eager work (host reads, single-step replays, a wave change)
+    between the replays, checking every stop against the domains alone."""
+    sc = scenes(21, n=2)
+    note(f"case batch replays sizes {[(s.nx, s.ny) for s in sc]}")
+    alone = [make(s) for s in sc]
+    mixed = [make(s) for s in sc]
+    b = capi.Batch(2, "f32", stage_fp16=True)
+    for d in mixed:
+        b.add(d)
+    for k, n in enumerate((8, 3, 17, 8, 1)):
+        if k == 3:  # an action between replays
+            for d in alone + mixed:
+                p = sc[0].swimmer.params.copy() if d in (alone[0], mixed[0]) else sc[1].swimmer.params.copy()
+                p[0] *= 0.5
+                d.set_wave_params(p[None, :])
+        for d in alone:
+            d.step(n)
+        b.step(n)
+        b.sync()
+        for i, (a, m) in enumerate(zip(alone, mixed)):
+            same(a, m, f"stop {k} domain {i}")
+            assert np.array_equal(a.body_wrench(), m.body_wrench())
+
+
 @pytest.mark.parametrize("tile", [1, 2, 3, 4])
 def test_tile_shapes_give_identical_results(tile):
     sc = scenes(21, n=2)
