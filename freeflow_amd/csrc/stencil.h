@@ -245,27 +245,32 @@ __device__ __forceinline__ void bulk3d_ring(const S* __restrict__ in, S* __restr
   float mreg[PER][10];
   uint8_t freg[PER];
   float hold[PER][18];  // c_z = +1 (0..8) and c_z = 0 (9..17) of the plane in flight
+  // in-plane offsets of this thread's halo cells, the same on every plane:
+  // the pull source (clamped or periodic) and the flag (always wrapped)
+  int off_src[PER], off_flag[PER];
+#pragma unroll
+  for (int r = 0; r < PER; r++) {
+    const int k = threadIdx.x + r * NT;
+    const int gx = x0 + k % HX - 1, gy = y0 + k / HX - 1;
+    const int wx = ((gx % nx) + nx) % nx, wy = ((gy % ny) + ny) % ny;
+    const int sx = PERIODIC ? wx : min(nx - 1, max(0, gx));
+    const int sy = PERIODIC ? wy : min(ny - 1, max(0, gy));
+    off_src[r] = sy * nx + sx;
+    off_flag[r] = wy * nx + wx;
+  }
   // fetch the moments and flags of plane gz (may be -1 or nz: halo) into registers
   auto fetch = [&](int gz) {
     const int wz = ((gz % nz) + nz) % nz;
     const int sz = PERIODIC ? wz : min(nz - 1, max(0, gz));
+    const size_t zs = (size_t)sz * ny * nx, zf = (size_t)wz * ny * nx;
 #pragma unroll
     for (int r = 0; r < PER; r++) {
       const int k = threadIdx.x + r * NT;
       if (k < HA) {
-        const int gx = x0 + k % HX - 1, gy = y0 + k / HX - 1;
-        const int wx = ((gx % nx) + nx) % nx, wy = ((gy % ny) + ny) % ny;
-        int sx, sy;
-        if (PERIODIC) {
-          sx = wx; sy = wy;
-        } else {
-          sx = min(nx - 1, max(0, gx));
-          sy = min(ny - 1, max(0, gy));
-        }
-        const size_t c = ((size_t)sz * ny + sy) * nx + sx;
+        const size_t c = zs + off_src[r];
 #pragma unroll
         for (int q = 0; q < 10; q++) mreg[r][q] = Store<S>::template load<3>(in + q * plane + c, q);
-        freg[r] = flags[((size_t)wz * ny + wy) * nx + wx];
+        freg[r] = flags[zf + off_flag[r]];
       }
     }
   };
