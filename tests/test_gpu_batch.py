@@ -188,3 +188,38 @@ def test_batch_3d_sphere_and_periodic():
     same(alone[1], mixed_c, "3D sphere")
     assert np.array_equal(alone[1].body_wrench(), mixed_c.body_wrench())
     assert np.abs(alone[1].body_wrench()).max() > 0
+
+
+@pytest.mark.parametrize("batched", [False, True])
+def test_link_table_overflow_is_reported(batched):
+    """A boundary moved far beyond the shape its link table was sized for
+    fills the table: the step reports FF_ERR_CAPACITY (alone through ff_sync,
+    batched through ff_batch_sync) instead of dropping links silently."""
+    nx, ny, n = 2000, 64, 20
+    e = np.stack([np.arange(n), np.arange(1, n + 1)], 1)
+    small = np.stack([600.0 + 0.04 * np.arange(n + 1), np.full(n + 1, 32.3)], 1)  # inside one cell
+    big = np.stack([np.linspace(10.0, 1990.0, n + 1), np.full(n + 1, 32.3)], 1)  # ~4000 link cells
+    note(f"case overflow batched={batched}")
+    d = capi.Domain(2, nx, ny, viscosity=0.01)
+    d.init_state([0.0] * 4)
+    d.set_boundary(small, np.zeros_like(small), e)
+    b = None
+    if batched:
+        b = capi.Batch(2, "f32", stage_fp16=True)
+        b.add(d)
+
+    def run(k):
+        if batched:
+            b.step(k)
+            b.sync()
+        else:
+            d.step(k)
+            d.sync()
+
+    run(2)  # the shape it was sized for fits
+    d.update_boundary(big, np.zeros_like(big))
+    with pytest.raises(capi.FreeFlowError, match="error 4:"):
+        run(1)
+    if b is not None:
+        b.close()
+    d.close()
