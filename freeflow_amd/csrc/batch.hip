@@ -68,7 +68,8 @@ struct Batch {
   int n_bc = 0, n_wave = 0, n_bulk = 0, n_es = 0, n_fin = 0;
   int* d_flag = nullptr;  // set by a replay whose detection overflowed a member's link table
   // pending detections (members whose links were not detected ahead): pinned
-  // staging, reused once the copy recorded by pend_ev has completed
+  // staging and its device copy, reused or regrown only once the copy and the
+  // launches that read it (pend_ev) have completed
   int2* h_pend = nullptr;
   size_t h_pend_cap = 0;
   int2* d_pend = nullptr;
@@ -515,7 +516,7 @@ static int detect_pending(Batch& b) {
   const size_t nb = b.bumps.size(), nd = b.dets.size(), n = nb + nd;
   if (n == 0) return FF_OK;
   if (!b.pend_ev) FF_HIP(hipEventCreateWithFlags(&b.pend_ev, hipEventDisableTiming));
-  FF_HIP(hipEventSynchronize(b.pend_ev));  // the last copy out of h_pend has completed
+  FF_HIP(hipEventSynchronize(b.pend_ev));  // the last use of h_pend and d_pend has completed
   if (n > b.h_pend_cap) {
     if (b.h_pend) (void)hipHostFree(b.h_pend);
     b.h_pend = nullptr;
@@ -529,13 +530,13 @@ static int detect_pending(Batch& b) {
   std::memcpy(b.h_pend, b.bumps.data(), nb * sizeof(int2));
   std::memcpy(b.h_pend + nb, b.dets.data(), nd * sizeof(int2));
   FF_HIP(hipMemcpyAsync(b.d_pend, b.h_pend, n * sizeof(int2), hipMemcpyHostToDevice, b.stream));
-  FF_HIP(hipEventRecord(b.pend_ev, b.stream));
   if (nb) kb_tag_bump<<<(unsigned)((nb + 255) / 256), 256, 0, b.stream>>>(b.d_doms, b.d_pend, (int)nb);
   if (nd) {
     if (b.dim == 2) kb_detect<2><<<(unsigned)nd, kSumBlock, 0, b.stream>>>(b.d_doms, b.d_pend + nb);
     else kb_detect<3><<<(unsigned)nd, kSumBlock, 0, b.stream>>>(b.d_doms, b.d_pend + nb);
   }
   FF_HIP(hipGetLastError());
+  FF_HIP(hipEventRecord(b.pend_ev, b.stream));
   return FF_OK;
 }
 

@@ -261,7 +261,7 @@ __device__ __forceinline__ int d3_c(unsigned long long packed, int i) { return (
 // Lanes that detect one element's links in a batch (many domains: whole
 // elements per lane group need fewer blocks for the same work).
 template <int DIM>
-constexpr int kDetectLanes = DIM == 2 ? 64 : 256;
+constexpr int kDetectLanes = DIM == 2 ? 32 : 256;
 
 // 2D link detection: a group of LPE lanes per element walks the element's
 // candidate window once for every lattice direction, one (window cell,
@@ -484,7 +484,9 @@ __device__ __forceinline__ void build_links_block3(const LinkArgs& la, long long
 }
 
 // Detection block of a batch (blockDim = kDetectLanes<3> = 256): 2D packs
-// blockDim / 64 elements per block, 3D takes one element per block.
+// blockDim / kDetectLanes<2> elements per block (32 lanes per segment: about
+// 5 tasks each, fewer waves in flight for the same work than 64), 3D takes one
+// element per block.
 static_assert(kDetectLanes<3> == 256, "build_links_block3 assumes 256-thread blocks");
 template <int DIM>
 __device__ __forceinline__ void detect_block(const LinkArgs& la, long long block, unsigned tag, const float* v) {
