@@ -82,6 +82,7 @@ struct Batch {
   // kGraphSteps steps
   hipGraph_t graph_n = nullptr;
   hipGraphExec_t exec_n = nullptr;
+  hipEvent_t tev0 = nullptr, tev1 = nullptr;  // set only while ff_batch_time_bulk times a launch
 };
 
 constexpr int kGraphSteps = 8;
@@ -185,7 +186,8 @@ static int stream_threads(int dim, int tile) {
 
 template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT>
 static void step2d_batch_shape(Batch& b, const int2* map, int nblocks, int n_bc, int n_wave) {
-  kb_step2d<S, STAGE, PERIODIC, TX, TY, CPT><<<nblocks, 256, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave);
+  launch_timed(kb_step2d<S, STAGE, PERIODIC, TX, TY, CPT>, dim3(nblocks), dim3(256), b.stream, b.tev0, b.tev1,
+               (const DomDev*)b.d_doms, map, n_bc, n_wave);
 }
 
 template <typename S, bool STAGE, bool PERIODIC>
@@ -199,16 +201,22 @@ static void step2d_batch_variant(Batch& b, int tile, const int2* map, int nblock
   }
 }
 
+template <typename S, bool PERIODIC, int RING, int TX, int TY, int ZC, int NT>
+static void step3d_batch_shape(Batch& b, const int2* map, int nblocks, int n_bc, int n_wave) {
+  launch_timed(kb_step3d<S, PERIODIC, RING, TX, TY, ZC, NT>, dim3(nblocks), dim3(NT), b.stream, b.tev0, b.tev1,
+               (const DomDev*)b.d_doms, map, n_bc, n_wave);
+}
+
 template <typename S, bool PERIODIC>
 static void step3d_batch_variant(Batch& b, int tile, const int2* map, int nblocks, int n_bc, int n_wave) {
   switch (tile) {
-    case 1: kb_step3d<S, PERIODIC, 0, 16, 4, 4, 256><<<nblocks, 256, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
-    case 2: kb_step3d<S, PERIODIC, 1, 16, 8, 32, 128><<<nblocks, 128, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
-    case 3: kb_step3d<S, PERIODIC, 1, 16, 8, 16, 128><<<nblocks, 128, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
-    case 4: kb_step3d<S, PERIODIC, 1, 16, 8, 8, 128><<<nblocks, 128, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
-    case 5: kb_step3d<S, PERIODIC, 1, 16, 16, 32, 256><<<nblocks, 256, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
-    case 6: kb_step3d<S, PERIODIC, 1, 16, 16, 16, 256><<<nblocks, 256, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
-    default: kb_step3d<S, PERIODIC, 0, 8, 8, 4, 256><<<nblocks, 256, 0, b.stream>>>(b.d_doms, map, n_bc, n_wave); break;
+    case 1: step3d_batch_shape<S, PERIODIC, 0, 16, 4, 4, 256>(b, map, nblocks, n_bc, n_wave); break;
+    case 2: step3d_batch_shape<S, PERIODIC, 1, 16, 8, 32, 128>(b, map, nblocks, n_bc, n_wave); break;
+    case 3: step3d_batch_shape<S, PERIODIC, 1, 16, 8, 16, 128>(b, map, nblocks, n_bc, n_wave); break;
+    case 4: step3d_batch_shape<S, PERIODIC, 1, 16, 8, 8, 128>(b, map, nblocks, n_bc, n_wave); break;
+    case 5: step3d_batch_shape<S, PERIODIC, 1, 16, 16, 32, 256>(b, map, nblocks, n_bc, n_wave); break;
+    case 6: step3d_batch_shape<S, PERIODIC, 1, 16, 16, 16, 256>(b, map, nblocks, n_bc, n_wave); break;
+    default: step3d_batch_shape<S, PERIODIC, 0, 8, 8, 4, 256>(b, map, nblocks, n_bc, n_wave); break;
   }
 }
 
@@ -737,10 +745,11 @@ int ff_batch_time_bulk(ff_batch* b, int iters, double* avg_ms) {
   std::vector<hipEvent_t> ev(2 * iters);
   for (auto& e : ev) FF_HIP(hipEventCreate(&e));
   for (int k = 0; k < iters; k++) {
-    FF_HIP(hipEventRecord(ev[2 * k], b->stream));
+    b->tev0 = ev[2 * k];  // the launch's own start and end (launch_timed)
+    b->tev1 = ev[2 * k + 1];
     launch_bulk_only(*b, tile, map, b->n_bulk);
-    FF_HIP(hipEventRecord(ev[2 * k + 1], b->stream));
   }
+  b->tev0 = b->tev1 = nullptr;
   FF_HIP(hipGetLastError());
   FF_HIP(hipStreamSynchronize(b->stream));
   double tot = 0.0;

@@ -830,11 +830,13 @@ int ff_time_bulk_kernel(ff_domain* dd, int iters, double* avg_ms) {
   const LinkArgs la = links ? link_args(d, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], d.elems) : LinkArgs{};
   int rc = FF_OK;
   for (int k = 0; k < iters && !rc; k++) {
-    FF_HIP(hipEventRecord(ev[2 * k], d.stream));
+    d.tev0 = ev[2 * k];  // the launch's own start and end (launch_timed)
+    d.tev1 = ev[2 * k + 1];
     if (links) rc = launch_step_stream(d, in, out, la, d.wave);
     else rc = d.dim == 2 ? launch_bulk2d(d, in, out) : launch_bulk3d(d, in, out);
-    FF_HIP(hipEventRecord(ev[2 * k + 1], d.stream));
   }
+  d.tev0 = d.tev1 = nullptr;
+  FF_HIP(hipGetLastError());
   FF_HIP(hipStreamSynchronize(d.stream));
   double tot = 0.0;
   for (int k = 0; k < iters; k++) {

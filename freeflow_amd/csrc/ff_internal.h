@@ -1,6 +1,7 @@
 // ff_internal.h -- domain state and helpers shared by the HIP translation units.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -149,6 +150,8 @@ struct Domain {
   // domain's own stream is parked in `own_stream`
   Batch* batch = nullptr;
   hipStream_t own_stream = nullptr;
+  // set only while ff_time_bulk_kernel times a launch (see launch_timed)
+  hipEvent_t tev0 = nullptr, tev1 = nullptr;
 };
 
 // Batch bookkeeping used by the per-domain entry points (batch.hip).
@@ -174,6 +177,19 @@ int launch_step_finish(Domain& d, const LinkArgs& la, unsigned gen, float* proxy
 int flush_finish(Domain& d);
 int launch_outlet(Domain& d, void* buf);
 int launch_init_state(Domain& d, const float* bc6);
+// Launch of `kern` on `stream`.  With timing events (ev0 non-null) the
+// events take the kernel's own start and end (hipExtLaunchKernel), so a timed
+// launch measures the kernel alone, as a kernel trace does, without the
+// event packets around it.
+template <typename... P, typename... A>
+inline void launch_timed(void (*kern)(P...), dim3 grid, dim3 block, hipStream_t stream, hipEvent_t ev0,
+                         hipEvent_t ev1, A... args) {
+  if (ev0)
+    hipExtLaunchKernelGGL(kern, grid, block, 0, stream, ev0, ev1, 0, (P)args...);
+  else
+    hipLaunchKernelGGL(kern, grid, block, 0, stream, (P)args...);
+}
+
 int launch_wave_now(Domain& d);
 int bc_span(const Domain& d);  // boundary-cell scan span of a domain stepped on its own
 int launch_step_vertex_forces(Domain& d, float* out_dev);

@@ -64,12 +64,13 @@ static void bulk2d_shape(Domain& d, const S* i, S* o, const StreamExtra* x) {
   const int tx = (d.nx + TX - 1) / TX, ty = (d.ny + TY * CPT - 1) / (TY * CPT);
   const int nt = tx * ty;
   if (!x) {
-    k_bulk2d<S, STAGE, PERIODIC, TX, TY, CPT><<<xcd_grid(nt), 256, 0, d.stream>>>(i, o, d.flags, d.nx, d.ny, d.plane,
-                                                                                  d.omega, tx, nt);
+    launch_timed(k_bulk2d<S, STAGE, PERIODIC, TX, TY, CPT>, dim3(xcd_grid(nt)), dim3(256), d.stream, d.tev0, d.tev1, i,
+                 o, d.flags, d.nx, d.ny, d.plane, d.omega, tx, nt);
   } else {
     const int nbc = xcd_grid(x->nbc + x->nwave) - x->nwave;  // extra boundary-cell blocks share the grid stride
     const unsigned grid = (unsigned)(nbc + x->nwave + xcd_grid(nt));
-    k_step2d<S, STAGE, PERIODIC, TX, TY, CPT><<<grid, 256, 0, d.stream>>>(i, o, tx, nt, nbc, x->nwave, *x->la, *x->fa);
+    launch_timed(k_step2d<S, STAGE, PERIODIC, TX, TY, CPT>, dim3(grid), dim3(256), d.stream, d.tev0, d.tev1, i, o, tx, nt,
+                 nbc, x->nwave, *x->la, *x->fa);
   }
 }
 

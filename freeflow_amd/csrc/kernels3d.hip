@@ -36,8 +36,8 @@ template <typename S, bool PERIODIC, int RING, int TX, int TY, int ZC, int NT>
 static void bulk3d_shape(Domain& d, const S* i, S* o) {
   const int tx = (d.nx + TX - 1) / TX, ty = (d.ny + TY - 1) / TY, tz = (d.nz + ZC - 1) / ZC;
   const int nt = tx * ty * tz;
-  k_bulk3d<S, PERIODIC, RING, TX, TY, ZC, NT><<<xcd_grid(nt), NT, 0, d.stream>>>(i, o, d.flags, d.nx, d.ny, d.nz,
-                                                                                 d.plane, d.omega, tx, ty, nt);
+  launch_timed(k_bulk3d<S, PERIODIC, RING, TX, TY, ZC, NT>, dim3(xcd_grid(nt)), dim3(NT), d.stream, d.tev0, d.tev1, i, o,
+               d.flags, d.nx, d.ny, d.nz, d.plane, d.omega, tx, ty, nt);
 }
 
 template <typename S, bool PERIODIC>
@@ -81,7 +81,8 @@ static void step3d_shape(Domain& d, const S* i, S* o, const LinkArgs& la, FinalA
   const int nw = fa.kin_blocks + fa.fin_blocks;
   const int nbc = xcd_grid(boundary_cell_blocks(d, la.bc_span) + nw) - nw;
   const unsigned grid = (unsigned)(nbc + nw + xcd_grid(nt));
-  k_step3d<S, PERIODIC, RING, TX, TY, ZC, NT><<<grid, NT, 0, d.stream>>>(i, o, tx, ty, nt, nbc, nw, la, fa);
+  launch_timed(k_step3d<S, PERIODIC, RING, TX, TY, ZC, NT>, dim3(grid), dim3(NT), d.stream, d.tev0, d.tev1, i, o, tx, ty,
+               nt, nbc, nw, la, fa);
 }
 
 template <typename S, bool PERIODIC>
