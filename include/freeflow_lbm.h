@@ -172,9 +172,11 @@ int ff_mass(ff_domain* d, double* out);
 int ff_fill_solid(ff_domain* d);
 
 int ff_sync(ff_domain* d);
-/* Average duration (ms) of the stream-collide kernel over `iters` launches on
-   the domain's stream, timed with HIP events around each launch; does not
-   advance the simulation state. */
+/* Average duration (ms) of the step's stream launch (the stream-collide
+   kernel; with a boundary, the fused boundary-cell and kinematics blocks too)
+   over `iters` launches on the domain's stream, each timed by HIP events
+   attached to the launch itself (hipExtLaunchKernel: the kernel's own start
+   and end); does not advance the simulation state. */
 int ff_time_bulk_kernel(ff_domain* d, int iters, double* avg_ms);
 /* Wall time (ms) of `nsteps` ff_step-equivalent steps measured with HIP
    events on the domain stream (state advances). */
@@ -192,16 +194,17 @@ int ff_domain_footprint(const ff_domain_desc* desc, int nv, int ne, const float*
    planes.  0 is the default; results do not depend on it.
    No counterpart in the reference (its tile is fixed, CudaCommon.cuh:44-51). */
 int ff_set_tile(ff_domain* d, int tile);
-/* Time every tile shape on this domain (the state is left unchanged) and
-   keep the fastest; writes its index to *tile. */
+/* Time every tile shape on this domain (three interleaved rounds, each
+   shape's best round; the state is left unchanged) and keep the fastest;
+   writes its index to *tile. */
 int ff_autotune(ff_domain* d, int* tile);
 
 /* ---- batches: many domains of one kind stepped together ----------------
    No counterpart in the reference, which steps one flow field per Simulator
    (src/core/Simulator2D.cpp:88-130).  A batch holds domains with the same
    dim, storage, options and device.  One ff_batch_step step of every member
-   is one replay of a captured HIP graph; results of each member equal those
-   of ff_step on it alone.  While a domain is in a batch, its own ff_step /
+   is one replay of a captured HIP graph (runs of 8 or more steps replay an
+   8-step graph); results of each member equal those of ff_step on it alone.  While a domain is in a batch, its own ff_step /
    ff_step_proxy refuse to run; every other per-domain call (getters,
    ff_set_wave, ff_update_boundary, ...) keeps working and is ordered with
    the batch's steps.  Changing a member's buffers (ff_set_boundary,
@@ -224,9 +227,9 @@ int ff_batch_set_tile(ff_batch* b, int tile);
 int ff_batch_get_tile(ff_batch* b, int* tile, int* tile_x, int* tile_y);
 /* Run nsteps steps and return their GPU time in ms (HIP events). */
 int ff_batch_time_steps(ff_batch* b, int nsteps, double* total_ms);
-/* Average duration (ms, HIP events) of the batched stream-collide kernel
-   over `iters` launches, without the link-detection blocks; the state is
-   left unchanged. */
+/* Average duration (ms, HIP events attached to each launch) of the batched
+   stream-collide kernel over `iters` launches, tiles only (no boundary-cell
+   or kinematics blocks); the state is left unchanged. */
 int ff_batch_time_bulk(ff_batch* b, int iters, double* avg_ms);
 
 const char* ff_last_error(void);
