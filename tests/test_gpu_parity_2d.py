@@ -59,6 +59,20 @@ def test_vortex_street_bitexact(stage, periodic):
     assert_same(g, o, 1)
 
 
+@pytest.mark.parametrize("tile,periodic", [(0, True), (4, True), (4, False)])
+def test_grid_smaller_than_tile_bitexact(tile, periodic):
+    """7x3 cells: every tile is partial and the periodic halo wraps."""
+    nx, ny = 7, 3
+    note(f"case 2d tiny tile {tile} periodic {periodic}")
+    rng = np.random.default_rng(3)
+    m = wl.equilibrium_moments(0.02 * rng.standard_normal((nx * ny, 2)))
+    g, o = pair(nx, ny, np.ones(nx * ny, np.uint8), m, periodic=periodic)
+    g.set_tile(tile)
+    g.step(10)
+    o.step(10)
+    assert_same(g, o)
+
+
 def test_reference_initializer_bitexact():
     nx, ny = 64, 40
     note("case init_state")

@@ -126,6 +126,21 @@ def test_fp16_storage_3d():
     compare(g, o, tol=1e-5)
 
 
+@pytest.mark.parametrize("tile", [0, 2, 5])
+def test_grid_smaller_than_tile(tile):
+    """6x5x3 periodic cells: every box or column is partial and the halo
+    wraps around the whole grid."""
+    nx, ny, nz = 6, 5, 3
+    note(f"case 3d tiny tile {tile}")
+    m = wl.taylor_green_ring(nx, ny, nz, amp=0.04, ring_r=1.5, seed=17)
+    g, o = pair3(nx, ny, nz, np.ones(nx * ny * nz, np.uint8), m, periodic=True)
+    g.set_tile(tile)
+    g.step(10)
+    o.step(10)
+    rel, same = compare(g, o, 0)
+    assert same == 1.0
+
+
 @pytest.mark.parametrize("tile", [1, 2, 3, 4, 5, 6])
 def test_tile_variants_match_oracle(tile):
     # walls on every side (3D flag bounce-back), inlet/outlet, nz = 40 so the
