@@ -1,6 +1,6 @@
 """Benchmark of the HOME-LBM fluid step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config2]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload swimmer2d]
 
 Default workload (the configuration the headline metric is quoted on): 2D FSI,
 plain FP32, D2Q9 1200x400, dx=0.005, dt=0.0025, an elliptical swimmer
@@ -22,7 +22,7 @@ Other workloads (one JSON line each, same keys):
                        200-800 segments) in one batch, FP16 storage, one graph
                        replay per step; also times the plain per-domain FP32
                        form of the same domains and reports the ratio.
-  --workload config3   3D D3Q27 256x128x128 Taylor-Green + vortex ring,
+  --workload tg3d   3D D3Q27 256x128x128 Taylor-Green + vortex ring,
                        periodic, storage from --storage (f32, f16, bf16).
   --workload served    Poisson-arriving 2D/3D rollout episodes at --rate per s
                        for --duration s through the continuous-batching server.
@@ -51,13 +51,13 @@ def bytes_per_lup(dim, storage_bytes):
     return 2 * nm * storage_bytes + 1
 
 
-def setup_config2(capi, wl):
-    c = wl.CONFIG2
+def setup_swimmer2d(capi, wl):
+    c = wl.SWIMMER2D
     nx, ny = c["nx"], c["ny"]
     flags = wl.outlet_box_flags(nx, ny)
     m = wl.equilibrium_moments(np.zeros((nx * ny, 2)))
-    sw = wl.config2_swimmer()
-    d = capi.Domain(**wl.config2_domain_args(), storage="f32", stage_fp16=True)
+    sw = wl.swimmer2d_swimmer()
+    d = capi.Domain(**wl.swimmer2d_domain_args(), storage="f32", stage_fp16=True)
     d.set_flags(flags)
     d.set_moments(m)
     v0, vel0 = sw.state(0.0)
@@ -66,14 +66,14 @@ def setup_config2(capi, wl):
     return d, (flags, m, sw)
 
 
-def cpu_baseline_config2(state, budget_s=12.0):
+def cpu_baseline_swimmer2d(state, budget_s=12.0):
     """Time the CPU oracle on the same workload for ~budget_s seconds."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol
     from freeflow_amd import capi, workloads as wl
 
     flags, m, sw = state
-    c = wl.CONFIG2
+    c = wl.SWIMMER2D
     nx, ny = c["nx"], c["ny"]
     threads = ol.set_threads(0)
     o = ol.OracleDomain(2, nx, ny, fp64=False, stage_fp16=True, nu=c["viscosity"], dx=c["dx"], dt=c["dt"],
@@ -152,7 +152,7 @@ def base_line(value, K, W, ms_per_step, dtype, config, roof, cpu=None, **extra):
 
 
 def run_batched(args, capi, wl):
-    c = wl.CONFIG4
+    c = wl.BATCH64
     scenes = wl.batched_scenes(c["n_domains"], c["seed"])
     cells = sum(s.nx * s.ny for s in scenes)
     K = args.steps if args.steps != 4000 else c["steps"]
@@ -216,8 +216,8 @@ def run_batched(args, capi, wl):
     return 0
 
 
-def run_config3(args, capi, wl):
-    c = wl.CONFIG3
+def run_tg3d(args, capi, wl):
+    c = wl.TG3D
     nx, ny, nz = c["nx"], c["ny"], c["nz"]
     cells = nx * ny * nz
     st = args.storage
@@ -229,7 +229,7 @@ def run_config3(args, capi, wl):
     m0 = d.mass()
     d.step(W)
     d.sync()
-    print(f"[bench] config3 {st}: timing {K} steps", file=sys.stderr, flush=True)
+    print(f"[bench] tg3d {st}: timing {K} steps", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     d.step(K)
     d.sync()
@@ -295,12 +295,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--workload", default="config2")
+    ap.add_argument("--workload", default="swimmer2d")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tile shape instead of timing the candidates")
     ap.add_argument("--tile", type=int, default=-1, help="use this tile index instead of timing the candidates")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
-    ap.add_argument("--storage", default="f16", help="config3: f32, f16 or bf16")
+    ap.add_argument("--storage", default="f16", help="tg3d: f32, f16 or bf16")
     ap.add_argument("--rate", type=float, default=50.0, help="served: episode arrivals per second")
     ap.add_argument("--duration", type=float, default=5.0, help="served: arrival window in seconds")
     ap.add_argument("--plain-steps", type=int, default=200,
@@ -320,16 +320,16 @@ def main():
 
     if args.workload == "batched":
         return run_batched(args, capi, wl)
-    if args.workload == "config3":
-        return run_config3(args, capi, wl)
+    if args.workload == "tg3d":
+        return run_tg3d(args, capi, wl)
     if args.workload == "served":
         return run_served(args)
-    if args.workload != "config2":
+    if args.workload != "swimmer2d":
         print(f"unknown workload {args.workload}", file=sys.stderr)
         return 2
-    d, state = setup_config2(capi, wl)
+    d, state = setup_swimmer2d(capi, wl)
     tile = pick_tile(d, args)
-    c = wl.CONFIG2
+    c = wl.SWIMMER2D
     cells = c["nx"] * c["ny"]
     K, W = args.steps, args.warmup
     d.set_wrench_history(K)
@@ -363,7 +363,7 @@ def main():
             "kernel": kname + " (stream-collide tiles + boundary cells + next-step kinematics)",
             "avg_kernel_us": round(avg_ms * 1e3, 3),
             "bytes_per_lup": bpl, "cells_per_launch": cells}
-    cpu = None if args.no_cpu else cpu_baseline_config2(state, args.cpu_budget)
+    cpu = None if args.no_cpu else cpu_baseline_swimmer2d(state, args.cpu_budget)
     out = {
         "metric": METRIC, "value": round(mlups, 2), "unit": "MLUPS", "n_gpus": 1, "steps": K, "warmup": W,
         "ms_per_step": round(ms_per_step, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,

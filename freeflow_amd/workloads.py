@@ -125,7 +125,7 @@ def icosphere(center, r, subdiv=1):
 
 class SwimmerScene2D:
     """One 2D swimmer domain: outlet box, equilibrium at rest, elliptical
-    swimmer with travelling-wave kinematics (the layout of config 2 scaled to
+    swimmer with travelling-wave kinematics (the layout of the 1200x400 swimmer workload scaled to
     the domain)."""
 
     def __init__(self, nx, ny, n_seg, rng, dx=0.005, dt=0.0025, viscosity=0.005, rho=10.0):
@@ -153,7 +153,7 @@ class SwimmerScene2D:
 
 
 def batched_scenes(n, seed, nx_range=(256, 1200), ny_range=(128, 400), seg_range=(200, 800)):
-    """`n` uneven swimmer domains, sizes and swimmers from one seed (config 4)."""
+    """`n` uneven swimmer domains, sizes and swimmers from one seed."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(n):
@@ -166,19 +166,19 @@ def batched_scenes(n, seed, nx_range=(256, 1200), ny_range=(128, 400), seg_range
 
 # ---------------------------------------------------------------- configurations
 CONFIG1 = dict(nx=400, ny=200, viscosity=0.005, u_in=0.05, r=20.0, cx=100.0, cy=100.0, n_poly=128, seed=1234)
-CONFIG2 = dict(nx=1200, ny=400, dx=0.005, dt=0.0025, viscosity=0.005, rho=10.0, n_seg=600,
+SWIMMER2D = dict(nx=1200, ny=400, dx=0.005, dt=0.0025, viscosity=0.005, rho=10.0, n_seg=600,
                cx=1.0, cy=1.0, a=0.3, b=0.08, seed=1234)
-CONFIG3 = dict(nx=256, ny=128, nz=128, viscosity=0.01, amp=0.04, ring_r=20.0, seed=1234)
-CONFIG4 = dict(n_domains=64, seed=1234, steps=1000, storage="f16")
+TG3D = dict(nx=256, ny=128, nz=128, viscosity=0.01, amp=0.04, ring_r=20.0, seed=1234)
+BATCH64 = dict(n_domains=64, seed=1234, steps=1000, storage="f16")
 
 
-def config2_domain_args():
-    c = CONFIG2
+def swimmer2d_domain_args():
+    c = SWIMMER2D
     return dict(dim=2, nx=c["nx"], ny=c["ny"], viscosity=c["viscosity"], dx=c["dx"], dt=c["dt"], fluid_density=c["rho"])
 
 
-def config2_swimmer(seed=None):
-    c = CONFIG2
+def swimmer2d_swimmer(seed=None):
+    c = SWIMMER2D
     rng = np.random.default_rng(c["seed"] if seed is None else seed)
     return random_swimmer(rng, c["n_seg"], c["cx"], c["cy"], c["a"], c["b"])
 

@@ -53,7 +53,7 @@ def test_footprint_without_gpu():
     base16 = capi.footprint(2, nx, ny, storage="f16")
     assert base32 - base16 == 2 * plane * 6 * 2  # two ping-pong buffers, 6 planes, 2 bytes saved each
     assert base32 == 2 * plane * 6 * 4 + nx * ny + 8192
-    sw = wl.config2_swimmer()
+    sw = wl.swimmer2d_swimmer()
     with_b = capi.footprint(2, nx, ny, storage="f32", verts=sw.rest, elems=sw.elems, dx=0.005)
     assert with_b > base32
     p3 = (96 ** 3 + 255) // 256 * 256

@@ -31,7 +31,7 @@ def note(msg):
 
 
 # ------------------------------------------------------------------ config
-def test_config2d_fields(tmp_path):
+def test_json_config_2d_fields(tmp_path):
     mesh = tmp_path / "d.mesh"
     v, t = meshes.disk(12)
     meshes.write_mesh(mesh, 2, v, tris=t)

@@ -103,7 +103,7 @@ def test_16bit_storage_bitexact(storage):
 
 def test_config1_cylinder_links_bitexact():
     # 400x200 vortex street, static 128-gon cylinder r=20 at (100,100), through
-    # the moving-wall link path with zero wall velocity (config 1).
+    # the moving-wall link path with zero wall velocity (the cylinder case).
     c = wl.CONFIG1
     nx, ny = c["nx"], c["ny"]
     note(f"case config1 cylinder {nx}x{ny}")
@@ -229,19 +229,19 @@ def test_mass_fp64():
     assert abs(g.mass() - ref) < 1e-10 * ref
 
 
-def test_config2_full_size_parity_and_determinism():
-    # BASELINE config 2 at full size (1200x400, ~600-segment swimmer, wave
+def test_swimmer2d_1200x400_parity_and_determinism():
+    # the 1200x400 swimmer workload at full size (~600-segment swimmer, wave
     # kinematics on device): 20 steps against the oracle, then a repeat run
     # must be bit-identical (moments and per-step wrench).
-    c = wl.CONFIG2
+    c = wl.SWIMMER2D
     nx, ny = c["nx"], c["ny"]
-    note(f"case config2 full size {nx}x{ny}")
+    note(f"case swimmer2d full size {nx}x{ny}")
     flags = wl.outlet_box_flags(nx, ny)
     m = wl.equilibrium_moments(np.zeros((nx * ny, 2)))
-    sw = wl.config2_swimmer()
+    sw = wl.swimmer2d_swimmer()
     runs = []
     for rep in range(2):
-        g = capi.Domain(**wl.config2_domain_args())
+        g = capi.Domain(**wl.swimmer2d_domain_args())
         g.set_flags(flags)
         g.set_moments(m)
         v0, vel0 = sw.state(0.0)
