@@ -2,16 +2,20 @@
 //
 // A batch holds domains of one kind (dimension, storage format, options).
 // One step of every member is one replay of a HIP graph with three kernel
-// nodes, the batched forms of the per-domain step (kernels2d/3d.hip,
-// links.hip):
+// nodes (runs of 8+ steps replay an 8-step graph), the batched forms of the
+// per-domain step (kernels2d/3d.hip, links.hip):
 //
-//   kb_step      boundary cells (into side buffers), next-step wave
+//   kb_step      boundary cells (into side-buffer rows), next-step wave
 //                kinematics and the stream-collide tiles of every domain
-//   kb_sums      per-element FP64 force sums, outlet cells and the
-//                boundary-cell copy-back of every domain
-//   kb_finish    per-vertex forces and the next step's link detection; per
-//                domain one block for the body wrench, the link-table tag
-//                and the domain's step counter
+//   kb_sums      per-element FP64 force sums, outlet cells, the
+//                boundary-cell copy-back and the next step's link detection
+//                of every domain
+//   kb_finish    per-vertex forces; per domain one block for the body
+//                wrench, the generation/tag advance and the step counter
+//
+// Members whose links were not detected ahead (new boundary, host update,
+// new wave parameters) are detected before the replay by kb_detect, one
+// launch for all of them.
 //
 // Each block finds its domain and its block index within that domain in a
 // block map (int2 {domain, local block}) built on the host; per-domain

@@ -249,7 +249,7 @@ static int one_step(Domain& d, const float* pv, const float* pu, const uint32_t*
   if (!d.predetected && d.stale) d.tag_host++;
   const LinkArgs la = proxy ? link_args(d, pv, pv, pu, pu, pe) : link_args(d, d.vbuf[0], d.vbuf[1], d.ubuf[0], d.ubuf[1], d.elems);
   if (!d.predetected) {
-    if ((rc = launch_detect(d, la, false))) return rc;
+    if ((rc = launch_detect(d, la))) return rc;
     d.stale = false;
   }
   // the stream launch also runs the previous step's deferred finish
@@ -555,7 +555,6 @@ int ff_set_boundary(ff_domain* dd, int nv, const float* verts, const float* vels
   // link table capacity: candidate cells of every element's box, with margin
   const unsigned long long est = link_estimate(D, d.dx, ne, verts, elems);
   d.lt_cap = (int)next_pow2(est * 2 + 1024);
-  d.lt_est = est;
   const size_t K = (size_t)(d.q - 1);
   const size_t cap = (size_t)d.lt_cap;
   // two tables (cells and keys), alternating by tag parity

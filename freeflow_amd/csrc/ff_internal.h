@@ -145,7 +145,6 @@ struct Domain {
   // scratch
   double* red = nullptr;          // reduction scratch
   bool has_boundary = false;
-  unsigned long long lt_est = 0;  // candidate link cells estimated at ff_set_boundary
   // batch membership: while set, `stream` is the batch's stream and the
   // domain's own stream is parked in `own_stream`
   Batch* batch = nullptr;
@@ -170,7 +169,7 @@ int launch_bulk3d(Domain& d, const void* in, void* out);
 // boundary step: [detect] -> stream (tiles + boundary cells + next kinematics)
 // -> element sums + outlets + boundary-cell copy-back + next step's detection
 // -> finish (vertex forces, wrench)
-int launch_detect(Domain& d, const LinkArgs& la, bool bump);
+int launch_detect(Domain& d, const LinkArgs& la);
 int launch_step_stream(Domain& d, const void* in, void* out, const LinkArgs& la, bool wave);
 int launch_step_sums(Domain& d, const LinkArgs& la, void* out, bool detect_next);
 int launch_step_finish(Domain& d, const LinkArgs& la, unsigned gen, float* proxy_forces);

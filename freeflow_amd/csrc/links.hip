@@ -14,22 +14,25 @@
 // Forces: the reference scatters every link force to the element's vertices
 // with float atomicAdd (SolidGeometryProxy_Device.cuh:184-194, 401-421), whose
 // order is nondeterministic.  Here each link force is stored with its link;
-// each element then sums its own links in a fixed order (one block per
-// element), and one block sums elements into bodies and vertices in CSR
-// order, all in FP64.  Results do not depend on scheduling.
+// each element then sums its own links in a fixed order (one wave per
+// element), and the finish phase sums elements into vertices (CSR order) and
+// bodies, all in FP64.  Results do not depend on scheduling.
 //
-// A step with a boundary is three launches (four when the links must be
+// A step with a boundary is two launches (three when the links must be
 // detected first):
-//   [detect]  links of this step under the current tag (first step, or after
-//             the boundary moved by a host update)
-//   stream    stream-collide tiles + boundary cells (into a side buffer) +
-//             next-step wave kinematics (kernels2d.hip / kernels3d.hip)
-//   sums      per-element force sums, outlet cells, boundary-cell copy-back
-//   finish    per-vertex forces, body wrench, and the detection of the next
-//             step's links (positions of the next step are already known)
-// Per-step values (gen, tags, counters, vertex-buffer parity) live in
-// StepState on the device, so the launches take identical arguments every
-// step.
+//   [detect]  links of this step under the current tag (first step, after a
+//             host update of the boundary, proxy steps)
+//   stream    boundary cells (into side-buffer rows) + next-step wave
+//             kinematics + the previous step's deferred finish (vertex
+//             forces, body wrench) + stream-collide tiles (kernels2d.hip /
+//             kernels3d.hip)
+//   sums      per-element force sums, outlet cells, boundary-cell copy-back,
+//             and detection of the next step's links under the next tag (the
+//             next step's positions are already known)
+// k_step_finish runs the finish on its own only before a reader needs it or
+// for proxy steps.  A domain stepped on its own passes generation and tag by
+// value (the host keeps them); batch members keep them in StepState on the
+// device so that graph replays take identical arguments every step.
 
 #include <algorithm>
 
@@ -46,15 +49,9 @@ __global__ void __launch_bounds__(256) k_links_build(LinkArgs la) {
   build_links_pairs<DIM>(la, blockIdx.x, step_tag(la), la.v[step_gen(la) & 1]);
 }
 
-// Batched domains: invalidate links detected ahead for a boundary that has
-// since moved by opening a fresh tag (entries under older tags read as
-// absent).  A domain stepped on its own has its tag advanced by the host.
-__global__ void k_tag_bump(StepState* st) {
-  if (threadIdx.x == 0) st->tag = st->tag + 1;
-}
-
-int launch_detect(Domain& d, const LinkArgs& la, bool bump) {
-  if (bump && la.dev_state) k_tag_bump<<<1, 64, 0, d.stream>>>(d.st);
+// Links of the current step of a domain stepped on its own (a batch detects
+// its pending members with kb_detect).
+int launch_detect(Domain& d, const LinkArgs& la) {
   const unsigned g = (unsigned)detect_blocks(d, 256, false);
   if (g > 0) {
     if (d.dim == 2) k_links_build<2><<<g, 256, 0, d.stream>>>(la);
