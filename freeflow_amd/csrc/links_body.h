@@ -334,7 +334,9 @@ __device__ __forceinline__ void elem_sum_wave(const LinkArgs& la, const FinalArg
       const int kk = c0 + lane;
       const int i = lo[0] + kk % wx, j = lo[1] + (kk / wx) % wy, z = DIM == 3 ? lo[2] + kk / (wx * wy) : 0;
       const int cell = (z * la.ny + j) * la.nx + i;
-      sl[lane] = is_updated(la.flags[cell]) ? lt_find(cells, la.cap, tag, cell) : -1;
+      // the flag and the table probe are independent loads: issue both
+      const int found = lt_find(cells, la.cap, tag, cell);
+      sl[lane] = is_updated(la.flags[cell]) ? found : -1;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -344,15 +346,21 @@ __device__ __forceinline__ void elem_sum_wave(const LinkArgs& la, const FinalArg
       const int slot = sl[k];
       if (slot < 0) continue;
       const size_t li = (size_t)slot * K + l;
+      // every load of the link is issued before the tests that select it
+      // (all lie inside the slot's rows), so they overlap
       const uint32_t key = lt_key(keys, li, tag);
-      if (key == 0xFFFFFFFFu || (key & 0xFFFFu) != etag) continue;
-      if (la.lb[li * DIM] == kLinkUnused) continue;
-      float F[DIM];
+      float F[DIM], B[DIM], H[DIM];
 #pragma unroll
-      for (int c = 0; c < DIM; c++) F[c] = la.lf[li * DIM + c];
+      for (int c = 0; c < DIM; c++) {
+        F[c] = la.lf[li * DIM + c];
+        B[c] = la.lb[li * DIM + c];
+        H[c] = la.lh[li * DIM + c];
+      }
+      if (key == 0xFFFFFFFFu || (key & 0xFFFFu) != etag) continue;
+      if (B[0] == kLinkUnused) continue;
 #pragma unroll
       for (int jv = 0; jv < DIM; jv++) {
-        const float bj = la.lb[li * DIM + jv];
+        const float bj = B[jv];
 #pragma unroll
         for (int c = 0; c < DIM; c++) acc[jv * DIM + c] += (double)(F[c] * bj);  // addForceAt products
       }
@@ -360,7 +368,7 @@ __device__ __forceinline__ void elem_sum_wave(const LinkArgs& la, const FinalArg
       for (int c = 0; c < DIM; c++) acc[DIM * DIM + c] += (double)F[c];
       double r[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-      for (int c = 0; c < DIM; c++) r[c] = (double)la.lh[li * DIM + c] - ref[c];
+      for (int c = 0; c < DIM; c++) r[c] = (double)H[c] - ref[c];
       if constexpr (DIM == 2) {
         acc[DIM * DIM + DIM] += r[0] * (double)F[1] - r[1] * (double)F[0];
       } else {
