@@ -87,7 +87,18 @@ struct Batch {
   hipGraph_t graph_n = nullptr;
   hipGraphExec_t exec_n = nullptr;
   hipEvent_t tev0 = nullptr, tev1 = nullptr;  // set only while ff_batch_time_bulk times a launch
+  // pinned staging of the DomDev records and block maps (reused once the
+  // copies recorded by tab_ev have completed)
+  unsigned char* h_tab = nullptr;
+  size_t h_tab_cap = 0;
+  hipEvent_t tab_ev = nullptr;
+  int direct = 0;  // steps launched without a graph since the last rebuild
 };
+
+// A rebuilt batch launches its first kDirectSteps steps directly and
+// captures a graph only after that: a membership that changes every few
+// steps (a server admitting and retiring episodes) skips the capture.
+constexpr int kDirectSteps = 4;
 
 constexpr int kGraphSteps = 8;
 
@@ -371,13 +382,29 @@ static int build_tables(Batch& b, int tile) {
   std::vector<int2> map;
   map.reserve(bc.size() + wave.size() + bulk.size() + es.size() + fin.size());
   for (auto* v : {&bc, &wave, &bulk, &es, &fin}) map.insert(map.end(), v->begin(), v->end());
+  if ((size_t)std::max(n, 1) > b.doms_cap || std::max<size_t>(map.size(), 1) > b.map_cap)
+    FF_HIP(hipStreamSynchronize(b.stream));  // the old tables may still be read
   if ((rc = grow(&b.d_doms, &b.doms_cap, (size_t)std::max(n, 1)))) return rc;
   if ((rc = grow(&b.d_map, &b.map_cap, std::max<size_t>(map.size(), 1)))) return rc;
-  if (n) FF_HIP(hipMemcpyAsync(b.d_doms, doms.data(), n * sizeof(DomDev), hipMemcpyHostToDevice, b.stream));
-  if (!map.empty())
-    FF_HIP(hipMemcpyAsync(b.d_map, map.data(), map.size() * sizeof(int2), hipMemcpyHostToDevice, b.stream));
+  // stage both tables in pinned memory and copy them on the batch stream
+  // (ordered before every later launch, no host wait)
+  const size_t nd = (size_t)n * sizeof(DomDev), nm = map.size() * sizeof(int2);
+  if (!b.tab_ev) FF_HIP(hipEventCreateWithFlags(&b.tab_ev, hipEventDisableTiming));
+  FF_HIP(hipEventSynchronize(b.tab_ev));  // the last copies out of h_tab have completed
+  if (nd + nm > b.h_tab_cap) {
+    if (b.h_tab) (void)hipHostFree(b.h_tab);
+    b.h_tab = nullptr;
+    b.h_tab_cap = 0;
+    const size_t cap = nd + nm + (nd + nm) / 2 + 4096;
+    FF_HIP(hipHostMalloc((void**)&b.h_tab, cap, hipHostMallocDefault));
+    b.h_tab_cap = cap;
+  }
+  if (nd) std::memcpy(b.h_tab, doms.data(), nd);
+  if (nm) std::memcpy(b.h_tab + nd, map.data(), nm);
+  if (nd) FF_HIP(hipMemcpyAsync(b.d_doms, b.h_tab, nd, hipMemcpyHostToDevice, b.stream));
+  if (nm) FF_HIP(hipMemcpyAsync(b.d_map, b.h_tab + nd, nm, hipMemcpyHostToDevice, b.stream));
   FF_HIP(hipMemsetAsync(b.d_steps, 0, std::max(n, 1) * sizeof(unsigned), b.stream));
-  FF_HIP(hipStreamSynchronize(b.stream));
+  FF_HIP(hipEventRecord(b.tab_ev, b.stream));
   return FF_OK;
 }
 
@@ -457,10 +484,22 @@ static int capture_steps(Batch& b, int steps, hipGraph_t* graph, hipGraphExec_t*
   return FF_OK;
 }
 
-// Replays `nsteps` steps: kGraphSteps at a time (that graph is captured on
-// the first long run after a rebuild), then one at a time.
+// Runs `nsteps` steps: directly while the batch has taken fewer than
+// kDirectSteps steps since its rebuild, otherwise from graphs captured on
+// demand: kGraphSteps at a time (captured on the first long run), then one
+// at a time.
 static int replay(Batch& b, int nsteps) {
   int s = 0;
+  if (!b.exec) {
+    if (b.direct + nsteps <= kDirectSteps) {
+      for (; s < nsteps; s++) launch_step(b, current_tile(b));
+      FF_HIP(hipGetLastError());
+      b.direct += nsteps;
+      return FF_OK;
+    }
+    const int rc = capture_steps(b, 1, &b.graph, &b.exec);
+    if (rc) return rc;
+  }
   if (nsteps >= kGraphSteps && !b.exec_n) {
     const int rc = capture_steps(b, kGraphSteps, &b.graph_n, &b.exec_n);
     if (rc) return rc;
@@ -482,7 +521,7 @@ static int rebuild(Batch& b) {
   const bool retune = b.tuned_tile < 0 || cells > 2.0 * b.tuned_cells || 2.0 * cells < b.tuned_cells;
   if (b.tile < 0 && retune && (rc = autotune(b))) return rc;
   if ((rc = build_tables(b, current_tile(b)))) return rc;
-  if ((rc = capture_steps(b, 1, &b.graph, &b.exec))) return rc;
+  b.direct = 0;  // the graphs are captured on demand (replay)
   b.dirty = false;
   return FF_OK;
 }
@@ -622,6 +661,8 @@ int ff_batch_destroy(ff_batch* b) {
   if (b->d_pend) (void)hipFree(b->d_pend);
   if (b->h_pend) (void)hipHostFree(b->h_pend);
   if (b->pend_ev) (void)hipEventDestroy(b->pend_ev);
+  if (b->h_tab) (void)hipHostFree(b->h_tab);
+  if (b->tab_ev) (void)hipEventDestroy(b->tab_ev);
   (void)hipStreamDestroy(b->stream);
   delete b;
   return FF_OK;

@@ -204,7 +204,8 @@ int ff_autotune(ff_domain* d, int* tile);
    (src/core/Simulator2D.cpp:88-130).  A batch holds domains with the same
    dim, storage, options and device.  One ff_batch_step step of every member
    is one replay of a captured HIP graph (runs of 8 or more steps replay an
-   8-step graph); results of each member equal those of ff_step on it alone.  While a domain is in a batch, its own ff_step /
+   8-step graph; the first 4 steps after a membership change are launched
+   directly); results of each member equal those of ff_step on it alone.  While a domain is in a batch, its own ff_step /
    ff_step_proxy refuse to run; every other per-domain call (getters,
    ff_set_wave, ff_update_boundary, ...) keeps working and is ordered with
    the batch's steps.  Changing a member's buffers (ff_set_boundary,
