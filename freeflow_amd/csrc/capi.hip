@@ -41,13 +41,37 @@ static void dfree(T*& p) {
   p = nullptr;
 }
 
+// Arrays carved out of one device allocation (one hipMalloc and one hipFree
+// for a group of arrays that live and die together), each 256-byte aligned.
+struct Carve {
+  std::vector<std::pair<void**, size_t>> items;  // (pointer to set, byte offset)
+  size_t total = 0;
+  template <class T>
+  size_t add(T** p, size_t count) {
+    const size_t off = total;
+    items.push_back({(void**)p, off});
+    total += (std::max<size_t>(count, 1) * sizeof(T) + 255) / 256 * 256;
+    return off;
+  }
+  int commit(void** base) {
+    *base = nullptr;
+    hipError_t e = hipMalloc(base, total);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    for (auto& it : items) *it.first = (unsigned char*)*base + it.second;
+    return FF_OK;
+  }
+};
+
 static void free_boundary(Domain& d) {
-  for (int b = 0; b < 2; b++) { dfree(d.vbuf[b]); dfree(d.ubuf[b]); }
-  dfree(d.elems); dfree(d.elem_body); dfree(d.vforce);
-  dfree(d.body_elem_ptr); dfree(d.body_elem_idx); dfree(d.vert_inc_ptr); dfree(d.vert_inc_idx);
-  dfree(d.body_ref); dfree(d.elem_acc); dfree(d.wrench); dfree(d.wrench_hist);
-  dfree(d.lt_cells); dfree(d.lt_keys); dfree(d.lk_f); dfree(d.lk_b); dfree(d.lk_h); dfree(d.lk_sb);
-  dfree(d.st); dfree(d.rest_verts); dfree(d.s_coord); dfree(d.wave_params); dfree(d.vert_body);
+  // the arrays set up by ff_set_boundary share one allocation
+  dfree(d.bnd_mem);
+  for (int b = 0; b < 2; b++) d.vbuf[b] = d.ubuf[b] = nullptr;
+  d.elems = nullptr; d.elem_body = nullptr; d.vforce = nullptr;
+  d.body_elem_ptr = d.body_elem_idx = d.vert_inc_ptr = d.vert_inc_idx = nullptr;
+  d.body_ref = d.elem_acc = d.wrench = nullptr;
+  d.lt_cells = d.lt_keys = nullptr; d.lk_f = d.lk_b = d.lk_h = d.lk_sb = nullptr;
+  d.st = nullptr; d.vert_body = nullptr;
+  dfree(d.wrench_hist); dfree(d.rest_verts); dfree(d.s_coord); dfree(d.wave_params);
   d.nv = d.ne = d.n_bodies = 0;
   d.gen_host = 1;
   d.tag_host = 1;
@@ -356,16 +380,23 @@ int ff_domain_create(const ff_domain_desc* desc, ff_domain** out) {
   hipError_t e = hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking);
   if (e != hipSuccess) { delete dd; return hip_fail(e, "hipStreamCreate"); }
   const size_t bytes = d.plane * d.nm * storage_bytes(d.storage);
-  int rc = FF_OK;
-  for (int b = 0; b < 2 && !rc; b++) {
-    e = hipMalloc(&d.mom[b], bytes);
-    if (e != hipSuccess) rc = hip_fail(e, "hipMalloc(moments)");
-    else if ((e = hipMemset(d.mom[b], 0, bytes)) != hipSuccess) rc = hip_fail(e, "hipMemset");
+  // one allocation: both moment buffers (cleared), the flags (Fluid) and the
+  // reduction scratch
+  Carve cv;
+  unsigned char* mom_bytes[2] = {nullptr, nullptr};
+  cv.add(&mom_bytes[0], bytes);
+  cv.add(&mom_bytes[1], bytes);
+  const size_t flags_off = cv.add(&d.flags, (size_t)n);
+  cv.add(&d.red, 1024);
+  int rc = cv.commit(&d.core_mem);
+  if (!rc) {
+    d.mom[0] = mom_bytes[0];
+    d.mom[1] = mom_bytes[1];
+    unsigned char* base = (unsigned char*)d.core_mem;
+    if ((e = hipMemsetAsync(base, 0, flags_off, d.stream)) != hipSuccess) rc = hip_fail(e, "hipMemset(moments)");
+    else if ((e = hipMemsetAsync(d.flags, kFluid, (size_t)n, d.stream)) != hipSuccess) rc = hip_fail(e, "hipMemset(flags)");
+    else if ((e = hipStreamSynchronize(d.stream)) != hipSuccess) rc = hip_fail(e, "hipStreamSynchronize");
   }
-  if (!rc) rc = dalloc(&d.flags, (size_t)n);
-  if (!rc && (e = hipMemset(d.flags, kFluid, (size_t)n)) != hipSuccess) rc = hip_fail(e, "hipMemset(flags)");
-  if (!rc) rc = dalloc(&d.red, 1024);
-  if (!rc && (e = hipDeviceSynchronize()) != hipSuccess) rc = hip_fail(e, "hipDeviceSynchronize");
   if (rc) { ff_domain_destroy(dd); return rc; }
   *out = dd;
   return FF_OK;
@@ -378,10 +409,11 @@ int ff_domain_destroy(ff_domain* dd) {
   (void)batch_detach(d);
   if (d.stream) (void)hipStreamSynchronize(d.stream);
   free_boundary(d);
-  for (int b = 0; b < 2; b++) if (d.mom[b]) (void)hipFree(d.mom[b]);
-  dfree(d.flags);
+  dfree(d.core_mem);  // moments, flags and reduction scratch
+  d.mom[0] = d.mom[1] = nullptr;
+  d.flags = nullptr;
+  d.red = nullptr;
   dfree(d.outlets);
-  dfree(d.red);
   if (d.stream) (void)hipStreamDestroy(d.stream);
   delete dd;
   return FF_OK;
@@ -490,23 +522,7 @@ int ff_set_boundary(ff_domain* dd, int nv, const float* verts, const float* vels
   d.ne = ne;
   d.n_bodies = n_bodies;
   int rc = FF_OK;
-  const size_t vcount = (size_t)std::max(nv, 1) * D;
-  for (int b = 0; b < 2; b++)
-    if ((rc = dalloc(&d.vbuf[b], vcount)) || (rc = dalloc(&d.ubuf[b], vcount))) return rc;
-  if ((rc = dalloc(&d.elems, (size_t)std::max(ne, 1) * D)) || (rc = dalloc(&d.elem_body, std::max(ne, 1))) ||
-      (rc = dalloc(&d.vforce, vcount)))
-    return rc;
-  if (nv) {
-    for (int b = 0; b < 2; b++) {
-      FF_HIP(hipMemcpy(d.vbuf[b], verts, (size_t)nv * D * sizeof(float), hipMemcpyHostToDevice));
-      FF_HIP(hipMemcpy(d.ubuf[b], vels, (size_t)nv * D * sizeof(float), hipMemcpyHostToDevice));
-    }
-  }
-  if (ne) {
-    FF_HIP(hipMemcpy(d.elems, elems, (size_t)ne * D * sizeof(uint32_t), hipMemcpyHostToDevice));
-    FF_HIP(hipMemcpy(d.elem_body, eb.data(), (size_t)ne * sizeof(int), hipMemcpyHostToDevice));
-  }
-  FF_HIP(hipMemset(d.vforce, 0, vcount * sizeof(float)));
+  // host-side tables first, then one allocation, one upload and one clear
   // CSR body -> elements and vertex -> (element, end), in index order
   std::vector<int> bptr(n_bodies + 1, 0), bidx(ne);
   for (int k = 0; k < ne; k++) bptr[eb[k] + 1]++;
@@ -523,10 +539,10 @@ int ff_set_boundary(ff_domain* dd, int nv, const float* verts, const float* vels
     for (int k = 0; k < ne; k++)
       for (int j = 0; j < D; j++) vinc[fill[elems[(size_t)k * D + j]]++] = k * 4 + j;
   }
-  // body reference: centroid of the body's vertices
+  // body reference: centroid of the body's vertices; vertex -> body
   std::vector<double> ref((size_t)n_bodies * D, 0.0);
+  std::vector<int> vb(nv, -1);
   {
-    std::vector<int> vb(nv, -1);
     for (int k = 0; k < ne; k++)
       for (int j = 0; j < D; j++) if (vb[elems[(size_t)k * D + j]] < 0) vb[elems[(size_t)k * D + j]] = eb[k];
     std::vector<int> cnt(n_bodies, 0);
@@ -537,42 +553,71 @@ int ff_set_boundary(ff_domain* dd, int nv, const float* verts, const float* vels
       }
     for (int b = 0; b < n_bodies; b++)
       if (cnt[b]) for (int c = 0; c < D; c++) ref[(size_t)b * D + c] /= cnt[b];
-    if ((rc = dalloc(&d.vert_body, std::max(nv, 1)))) return rc;
     for (int v = 0; v < nv; v++) if (vb[v] < 0) vb[v] = 0;
-    if (nv) FF_HIP(hipMemcpy(d.vert_body, vb.data(), nv * sizeof(int), hipMemcpyHostToDevice));
   }
-  if ((rc = dalloc(&d.body_elem_ptr, bptr.size())) || (rc = dalloc(&d.body_elem_idx, std::max(ne, 1))) ||
-      (rc = dalloc(&d.vert_inc_ptr, vptr.size())) || (rc = dalloc(&d.vert_inc_idx, std::max<size_t>(vinc.size(), 1))) ||
-      (rc = dalloc(&d.body_ref, ref.size())) || (rc = dalloc(&d.elem_acc, (size_t)std::max(ne, 1) * elem_stride(D))) ||
-      (rc = dalloc(&d.wrench, (size_t)n_bodies * wrench_width(D))))
-    return rc;
-  FF_HIP(hipMemcpy(d.body_elem_ptr, bptr.data(), bptr.size() * sizeof(int), hipMemcpyHostToDevice));
-  if (ne) FF_HIP(hipMemcpy(d.body_elem_idx, bidx.data(), bidx.size() * sizeof(int), hipMemcpyHostToDevice));
-  FF_HIP(hipMemcpy(d.vert_inc_ptr, vptr.data(), vptr.size() * sizeof(int), hipMemcpyHostToDevice));
-  if (!vinc.empty()) FF_HIP(hipMemcpy(d.vert_inc_idx, vinc.data(), vinc.size() * sizeof(int), hipMemcpyHostToDevice));
-  FF_HIP(hipMemcpy(d.body_ref, ref.data(), ref.size() * sizeof(double), hipMemcpyHostToDevice));
-  FF_HIP(hipMemset(d.wrench, 0, (size_t)n_bodies * wrench_width(D) * sizeof(double)));
-  // link table capacity: candidate cells of every element's box, with margin
+  // link table capacity: distinct candidate cells near the mesh, with margin
   const unsigned long long est = link_estimate(D, d.dx, ne, verts, elems);
   d.lt_cap = (int)next_pow2(est * 2 + 1024);
   const size_t K = (size_t)(d.q - 1);
   const size_t cap = (size_t)d.lt_cap;
-  // two tables (cells and keys), alternating by tag parity
-  if ((rc = dalloc(&d.lt_cells, 2 * cap)) || (rc = dalloc(&d.lt_keys, 2 * cap * K)) || (rc = dalloc(&d.lk_f, cap * K * D)) ||
-      (rc = dalloc(&d.lk_b, cap * K * D)) || (rc = dalloc(&d.lk_h, cap * K * D)) ||
-      (rc = dalloc(&d.lk_sb, cap * d.nm)) || (rc = dalloc(&d.st, 1)))
-    return rc;
-  // generation 0 marks every entry empty; steps run from generation 1
-  FF_HIP(hipMemset(d.lt_cells, 0, 2 * cap * sizeof(unsigned long long)));
-  FF_HIP(hipMemset(d.lt_keys, 0, 2 * cap * K * sizeof(unsigned long long)));
+  const size_t vcount = (size_t)std::max(nv, 1) * D;
+  Carve cv;
+  // region 1: initialised from the host
+  size_t o_v[2], o_u[2];
+  for (int b = 0; b < 2; b++) {
+    o_v[b] = cv.add(&d.vbuf[b], vcount);
+    o_u[b] = cv.add(&d.ubuf[b], vcount);
+  }
+  const size_t o_el = cv.add(&d.elems, (size_t)std::max(ne, 1) * D);
+  const size_t o_eb = cv.add(&d.elem_body, (size_t)std::max(ne, 1));
+  const size_t o_vb = cv.add(&d.vert_body, (size_t)std::max(nv, 1));
+  const size_t o_bp = cv.add(&d.body_elem_ptr, bptr.size());
+  const size_t o_bi = cv.add(&d.body_elem_idx, (size_t)std::max(ne, 1));
+  const size_t o_vp = cv.add(&d.vert_inc_ptr, vptr.size());
+  const size_t o_vi = cv.add(&d.vert_inc_idx, std::max<size_t>(vinc.size(), 1));
+  const size_t o_rf = cv.add(&d.body_ref, ref.size());
+  const size_t o_st = cv.add(&d.st, 1);
+  const size_t init_end = cv.total;
+  // region 2: cleared (generation 0 marks every table entry empty; steps run
+  // from generation 1)
+  cv.add(&d.vforce, vcount);
+  cv.add(&d.wrench, (size_t)n_bodies * wrench_width(D));
+  cv.add(&d.lt_cells, 2 * cap);  // two tables (cells and keys), alternating by tag parity
+  cv.add(&d.lt_keys, 2 * cap * K);
+  const size_t zero_end = cv.total;
+  // region 3: written by the kernels before any read
+  cv.add(&d.elem_acc, (size_t)std::max(ne, 1) * elem_stride(D));
+  cv.add(&d.lk_f, cap * K * D);
+  cv.add(&d.lk_b, cap * K * D);
+  cv.add(&d.lk_h, cap * K * D);
+  cv.add(&d.lk_sb, cap * d.nm);
+  if ((rc = cv.commit(&d.bnd_mem))) return rc;
+  std::vector<unsigned char> host(init_end, 0);
+  auto put = [&](size_t off, const void* src, size_t bytes) {
+    if (bytes) std::memcpy(host.data() + off, src, bytes);
+  };
+  for (int b = 0; b < 2; b++) {
+    put(o_v[b], verts, (size_t)nv * D * sizeof(float));
+    put(o_u[b], vels, (size_t)nv * D * sizeof(float));
+  }
+  put(o_el, elems, (size_t)ne * D * sizeof(uint32_t));
+  put(o_eb, eb.data(), (size_t)ne * sizeof(int));
+  put(o_vb, vb.data(), (size_t)nv * sizeof(int));
+  put(o_bp, bptr.data(), bptr.size() * sizeof(int));
+  put(o_bi, bidx.data(), bidx.size() * sizeof(int));
+  put(o_vp, vptr.data(), vptr.size() * sizeof(int));
+  put(o_vi, vinc.data(), vinc.size() * sizeof(int));
+  put(o_rf, ref.data(), ref.size() * sizeof(double));
   StepState st0{};
   st0.gen = 1;
   st0.tag = 1;
-  FF_HIP(hipMemcpy(d.st, &st0, sizeof(st0), hipMemcpyHostToDevice));
+  put(o_st, &st0, sizeof(st0));
+  unsigned char* base = (unsigned char*)d.bnd_mem;
+  FF_HIP(hipMemcpyAsync(base, host.data(), init_end, hipMemcpyHostToDevice, d.stream));
+  FF_HIP(hipMemsetAsync(base + init_end, 0, zero_end - init_end, d.stream));
+  FF_HIP(hipStreamSynchronize(d.stream));
   d.gen_host = 1;
   d.tag_host = 1;
-  // the memsets above run on the null stream; the domain stream is non-blocking
-  FF_HIP(hipDeviceSynchronize());
   d.has_boundary = true;
   return FF_OK;
 }

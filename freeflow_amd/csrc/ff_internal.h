@@ -104,6 +104,8 @@ struct Domain {
   int n_outlets = 0;
   // boundary geometry (device, interleaved like the reference proxy)
   int nv = 0, ne = 0, n_bodies = 0;
+  void* bnd_mem = nullptr;   // one allocation holding the ff_set_boundary arrays below
+  void* core_mem = nullptr;  // one allocation holding the moments, flags and reduction scratch
   float* vbuf[2] = {nullptr, nullptr};  // positions by generation parity
   float* ubuf[2] = {nullptr, nullptr};  // velocities by generation parity
   uint32_t* elems = nullptr;
