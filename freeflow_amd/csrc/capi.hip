@@ -71,7 +71,8 @@ static void free_boundary(Domain& d) {
   d.body_ref = d.elem_acc = d.wrench = nullptr;
   d.lt_cells = d.lt_keys = nullptr; d.lk_f = d.lk_b = d.lk_h = d.lk_sb = nullptr;
   d.st = nullptr; d.vert_body = nullptr;
-  dfree(d.wrench_hist); dfree(d.rest_verts); dfree(d.s_coord); dfree(d.wave_params);
+  d.rest_verts = nullptr; d.s_coord = nullptr; d.wave_params = nullptr;
+  dfree(d.wrench_hist);
   d.nv = d.ne = d.n_bodies = 0;
   d.gen_host = 1;
   d.tag_host = 1;
@@ -207,6 +208,8 @@ static unsigned long long footprint(int dim, long long n, int storage, int nv, i
     b += 5ull * nv * D * 4 + (unsigned long long)ne * (D * 4 + 4 + 4 + D * 4 + (D * D + D + 3) * 8);
     b += (unsigned long long)(nv + 1) * 4 + 64;
     b += cap * (2 * 8 + 2 * K * 8 + 3 * K * D * 4 + nm * 4) + 64;
+    b += (unsigned long long)nv * (D + 1) * 4 + 32;  // wave inputs (rest positions, arc length, parameters)
+    b += 26ull * 256;                                // alignment of the arrays within the allocation
   }
   return b;
 }
@@ -591,6 +594,10 @@ int ff_set_boundary(ff_domain* dd, int nv, const float* verts, const float* vels
   cv.add(&d.lk_b, cap * K * D);
   cv.add(&d.lk_h, cap * K * D);
   cv.add(&d.lk_sb, cap * d.nm);
+  // wave kinematics inputs (ff_set_wave fills them when it is used)
+  cv.add(&d.rest_verts, vcount);
+  cv.add(&d.s_coord, (size_t)std::max(nv, 1));
+  cv.add(&d.wave_params, (size_t)n_bodies * 4);
   if ((rc = cv.commit(&d.bnd_mem))) return rc;
   std::vector<unsigned char> host(init_end, 0);
   auto put = [&](size_t off, const void* src, size_t bytes) {
@@ -659,12 +666,11 @@ int ff_set_wave(ff_domain* dd, const float* rest, const float* s, const double* 
   FF_HIP(hipSetDevice(d.device));
   FF_HIP(hipStreamSynchronize(d.stream));
   int rc;
-  if (!d.rest_verts && (rc = dalloc(&d.rest_verts, (size_t)std::max(d.nv, 1) * d.dim))) return rc;
-  if (!d.s_coord && (rc = dalloc(&d.s_coord, (size_t)std::max(d.nv, 1)))) return rc;
-  if (!d.wave_params && (rc = dalloc(&d.wave_params, (size_t)d.n_bodies * 4))) return rc;
-  FF_HIP(hipMemcpy(d.rest_verts, rest, (size_t)d.nv * d.dim * sizeof(float), hipMemcpyHostToDevice));
-  FF_HIP(hipMemcpy(d.s_coord, s, (size_t)d.nv * sizeof(float), hipMemcpyHostToDevice));
-  FF_HIP(hipMemcpy(d.wave_params, params, (size_t)d.n_bodies * 4 * sizeof(double), hipMemcpyHostToDevice));
+  // the arrays live in the boundary allocation (ff_set_boundary)
+  FF_HIP(hipMemcpyAsync(d.rest_verts, rest, (size_t)d.nv * d.dim * sizeof(float), hipMemcpyHostToDevice, d.stream));
+  FF_HIP(hipMemcpyAsync(d.s_coord, s, (size_t)d.nv * sizeof(float), hipMemcpyHostToDevice, d.stream));
+  FF_HIP(hipMemcpyAsync(d.wave_params, params, (size_t)d.n_bodies * 4 * sizeof(double), hipMemcpyHostToDevice,
+                        d.stream));
   if (!d.wave) batch_mark_dirty(d.batch);
   d.wave = true;
   links_moved(d);
