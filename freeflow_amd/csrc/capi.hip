@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -39,6 +40,39 @@ template <class T>
 static void dfree(T*& p) {
   if (p) (void)hipFree((void*)p);
   p = nullptr;
+}
+
+// Streams of destroyed domains, kept per device for the next domains created
+// there (a server creates and destroys a domain per episode, and creating a
+// HIP stream costs more than a step of a small domain).  A stream is idle
+// when it is returned.
+static std::mutex g_stream_mu;
+static std::vector<std::pair<int, hipStream_t>> g_streams;
+constexpr size_t kStreamPool = 64;
+
+static hipError_t acquire_stream(int device, hipStream_t* s) {
+  {
+    std::lock_guard<std::mutex> lock(g_stream_mu);
+    for (size_t i = g_streams.size(); i-- > 0;)
+      if (g_streams[i].first == device) {
+        *s = g_streams[i].second;
+        g_streams.erase(g_streams.begin() + (long)i);
+        return hipSuccess;
+      }
+  }
+  return hipStreamCreateWithFlags(s, hipStreamNonBlocking);
+}
+
+static void release_stream(int device, hipStream_t s) {
+  if (!s) return;
+  if (hipStreamSynchronize(s) == hipSuccess) {
+    std::lock_guard<std::mutex> lock(g_stream_mu);
+    if (g_streams.size() < kStreamPool) {
+      g_streams.push_back({device, s});
+      return;
+    }
+  }
+  (void)hipStreamDestroy(s);
 }
 
 // Arrays carved out of one device allocation (one hipMalloc and one hipFree
@@ -380,7 +414,7 @@ int ff_domain_create(const ff_domain_desc* desc, ff_domain** out) {
   d.cf = (float)((double)cm * d.dx / (d.dt * d.dt));
   d.plane = plane_stride(n);
   d.h_flags.assign((size_t)n, kFluid);
-  hipError_t e = hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking);
+  hipError_t e = acquire_stream(d.device, &d.stream);
   if (e != hipSuccess) { delete dd; return hip_fail(e, "hipStreamCreate"); }
   const size_t bytes = d.plane * d.nm * storage_bytes(d.storage);
   // one allocation: both moment buffers (cleared), the flags (Fluid) and the
@@ -417,7 +451,7 @@ int ff_domain_destroy(ff_domain* dd) {
   d.flags = nullptr;
   d.red = nullptr;
   dfree(d.outlets);
-  if (d.stream) (void)hipStreamDestroy(d.stream);
+  release_stream(d.device, d.stream);
   delete dd;
   return FF_OK;
 }
