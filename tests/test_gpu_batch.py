@@ -223,3 +223,32 @@ def test_link_table_overflow_is_reported(batched):
     if b is not None:
         b.close()
     d.close()
+
+
+def test_boundary_replaced_equals_set_once():
+    """ff_set_boundary on a domain that already has a boundary, a wave and a
+    wrench history frees and rebuilds all of them: stepping it afterwards
+    gives the same moments and wrench history as a domain set up once."""
+    sc_other, sc = scenes(31, n=2)
+    note(f"case boundary replaced {sc.nx}x{sc.ny}")
+    d_once = capi.Domain(**sc.domain_args(), storage="f32", stage_fp16=True)
+    sc.setup(d_once, history=16)
+    d_twice = capi.Domain(**sc.domain_args(), storage="f32", stage_fp16=True)
+    d_twice.set_flags(sc.flags)
+    d_twice.init_state([0.0] * 4)
+    # a first boundary (another swimmer, scaled into this grid) with wave and history
+    v1, _ = sc_other.swimmer.state(0.0)
+    v1 = v1 * (min(sc.nx, sc.ny) / max(sc_other.nx, sc_other.ny))
+    d_twice.set_boundary(v1, np.zeros_like(v1), sc_other.swimmer.elems)
+    d_twice.set_wave(sc_other.swimmer.rest, sc_other.swimmer.s, sc_other.swimmer.params[None, :])
+    d_twice.set_wrench_history(8)
+    # replaced before any step: the same setup as d_once
+    v0, vel0 = sc.swimmer.state(0.0)
+    d_twice.set_boundary(v0, vel0, sc.swimmer.elems)
+    d_twice.set_wave(sc.swimmer.rest, sc.swimmer.s, sc.swimmer.params[None, :])
+    d_twice.set_wrench_history(16)
+    d_once.step(12)
+    d_twice.step(12)
+    same(d_once, d_twice, "replaced boundary")
+    assert np.array_equal(d_once.wrench_history(12), d_twice.wrench_history(12))
+    assert np.array_equal(d_once.vertex_forces(), d_twice.vertex_forces())
