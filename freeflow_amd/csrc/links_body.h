@@ -341,40 +341,54 @@ __device__ __forceinline__ void elem_sum_wave(const LinkArgs& la, const FinalArg
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (int task = lane; task < K * chunk; task += 64) {
-      const int k = task % chunk, l = task / chunk;
-      const int slot = sl[k];
-      if (slot < 0) continue;
-      const size_t li = (size_t)slot * K + l;
-      // every load of the link is issued before the tests that select it
-      // (all lie inside the slot's rows), so they overlap
-      const uint32_t key = lt_key(keys, li, tag);
-      float F[DIM], B[DIM], H[DIM];
+    // a lane's tasks two at a time: the loads of both are issued before
+    // either is used (they overlap), then both are added in task order, so
+    // the FP64 sums are those of a one-at-a-time loop
+    constexpr int U = 2;
+    for (int t0 = lane; t0 < K * chunk; t0 += 64 * U) {
+      bool use[U];
+      uint32_t key[U];
+      float F[U][DIM], B[U][DIM], H[U][DIM];
 #pragma unroll
-      for (int c = 0; c < DIM; c++) {
-        F[c] = la.lf[li * DIM + c];
-        B[c] = la.lb[li * DIM + c];
-        H[c] = la.lh[li * DIM + c];
+      for (int u = 0; u < U; u++) {
+        const int task = t0 + 64 * u;
+        const int k = task % chunk, l = task / chunk;
+        const int slot = task < K * chunk ? sl[k] : -1;
+        use[u] = slot >= 0;
+        key[u] = 0xFFFFFFFFu;
+        if (use[u]) {  // every load of the link lies inside the slot's rows
+          const size_t li = (size_t)slot * K + l;
+          key[u] = lt_key(keys, li, tag);
+#pragma unroll
+          for (int c = 0; c < DIM; c++) {
+            F[u][c] = la.lf[li * DIM + c];
+            B[u][c] = la.lb[li * DIM + c];
+            H[u][c] = la.lh[li * DIM + c];
+          }
+        }
       }
-      if (key == 0xFFFFFFFFu || (key & 0xFFFFu) != etag) continue;
-      if (B[0] == kLinkUnused) continue;
 #pragma unroll
-      for (int jv = 0; jv < DIM; jv++) {
-        const float bj = B[jv];
+      for (int u = 0; u < U; u++) {
+        if (!use[u] || key[u] == 0xFFFFFFFFu || (key[u] & 0xFFFFu) != etag) continue;
+        if (B[u][0] == kLinkUnused) continue;
 #pragma unroll
-        for (int c = 0; c < DIM; c++) acc[jv * DIM + c] += (double)(F[c] * bj);  // addForceAt products
-      }
+        for (int jv = 0; jv < DIM; jv++) {
+          const float bj = B[u][jv];
 #pragma unroll
-      for (int c = 0; c < DIM; c++) acc[DIM * DIM + c] += (double)F[c];
-      double r[3] = {0.0, 0.0, 0.0};
+          for (int c = 0; c < DIM; c++) acc[jv * DIM + c] += (double)(F[u][c] * bj);  // addForceAt products
+        }
 #pragma unroll
-      for (int c = 0; c < DIM; c++) r[c] = (double)H[c] - ref[c];
-      if constexpr (DIM == 2) {
-        acc[DIM * DIM + DIM] += r[0] * (double)F[1] - r[1] * (double)F[0];
-      } else {
-        acc[DIM * DIM + DIM + 0] += r[1] * (double)F[2] - r[2] * (double)F[1];
-        acc[DIM * DIM + DIM + 1] += r[2] * (double)F[0] - r[0] * (double)F[2];
-        acc[DIM * DIM + DIM + 2] += r[0] * (double)F[1] - r[1] * (double)F[0];
+        for (int c = 0; c < DIM; c++) acc[DIM * DIM + c] += (double)F[u][c];
+        double r[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int c = 0; c < DIM; c++) r[c] = (double)H[u][c] - ref[c];
+        if constexpr (DIM == 2) {
+          acc[DIM * DIM + DIM] += r[0] * (double)F[u][1] - r[1] * (double)F[u][0];
+        } else {
+          acc[DIM * DIM + DIM + 0] += r[1] * (double)F[u][2] - r[2] * (double)F[u][1];
+          acc[DIM * DIM + DIM + 1] += r[2] * (double)F[u][0] - r[0] * (double)F[u][2];
+          acc[DIM * DIM + DIM + 2] += r[0] * (double)F[u][1] - r[1] * (double)F[u][0];
+        }
       }
     }
     // the next chunk overwrites sl[] only after every lane's reads above
