@@ -341,13 +341,18 @@ __device__ __forceinline__ void elem_sum_wave(const LinkArgs& la, const FinalArg
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // a lane's tasks two at a time: the loads of both are issued before
-    // either is used (they overlap), then both are added in task order, so
-    // the FP64 sums are those of a one-at-a-time loop
-    constexpr int U = 2;
+    // 2D: a lane's tasks two at a time, every load of both issued before
+    // any is used (they overlap), then both added in task order, so the
+    // FP64 sums are those of a one-at-a-time loop.  3D: one task at a time,
+    // and a link's force and geometry are loaded only once its key names
+    // this element (a 3D window's cells carry mostly other elements' links;
+    // loading them all doubled the batched 3D sums launch).
+    constexpr int U = DIM == 2 ? 2 : 1;
+    constexpr bool EARLY = DIM == 2;
     for (int t0 = lane; t0 < K * chunk; t0 += 64 * U) {
       bool use[U];
       uint32_t key[U];
+      size_t lis[U];
       float F[U][DIM], B[U][DIM], H[U][DIM];
 #pragma unroll
       for (int u = 0; u < U; u++) {
@@ -356,9 +361,27 @@ __device__ __forceinline__ void elem_sum_wave(const LinkArgs& la, const FinalArg
         const int slot = task < K * chunk ? sl[k] : -1;
         use[u] = slot >= 0;
         key[u] = 0xFFFFFFFFu;
+        lis[u] = 0;
         if (use[u]) {  // every load of the link lies inside the slot's rows
           const size_t li = (size_t)slot * K + l;
+          lis[u] = li;
           key[u] = lt_key(keys, li, tag);
+          if constexpr (EARLY) {
+#pragma unroll
+            for (int c = 0; c < DIM; c++) {
+              F[u][c] = la.lf[li * DIM + c];
+              B[u][c] = la.lb[li * DIM + c];
+              H[u][c] = la.lh[li * DIM + c];
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (!use[u] || key[u] == 0xFFFFFFFFu || (key[u] & 0xFFFFu) != etag) continue;
+        if constexpr (!EARLY) {
+          const size_t li = lis[u];
+          if (la.lb[li * DIM] == kLinkUnused) continue;
 #pragma unroll
           for (int c = 0; c < DIM; c++) {
             F[u][c] = la.lf[li * DIM + c];
@@ -366,10 +389,6 @@ __device__ __forceinline__ void elem_sum_wave(const LinkArgs& la, const FinalArg
             H[u][c] = la.lh[li * DIM + c];
           }
         }
-      }
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        if (!use[u] || key[u] == 0xFFFFFFFFu || (key[u] & 0xFFFFu) != etag) continue;
         if (B[u][0] == kLinkUnused) continue;
 #pragma unroll
         for (int jv = 0; jv < DIM; jv++) {
