@@ -305,7 +305,9 @@ __device__ __forceinline__ void detect_next_body(const LinkArgs& la, int blk) {
 // Sum of one element's links by one wave (lane = lane index, sl = 64 ints of
 // LDS for the wave): window cells are resolved 64 at a time, then every lane
 // takes (cell, direction) tasks; a fixed shuffle tree folds the FP64 sums.
-template <int DIM>
+// PAIR: a lane's 2D tasks two at a time (a domain stepped on its own, where
+// the launch's latency is the slowest wave's); the sums do not depend on it.
+template <int DIM, bool PAIR>
 __device__ __forceinline__ void elem_sum_wave(const LinkArgs& la, const FinalArgs& fa, int elem, int lane, int* sl) {
   constexpr int K = DIM == 2 ? 8 : 26;
   constexpr int NT = DIM == 2 ? 1 : 3;
@@ -341,13 +343,13 @@ __device__ __forceinline__ void elem_sum_wave(const LinkArgs& la, const FinalArg
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // 2D: a lane's tasks two at a time, every load of both issued before
-    // any is used (they overlap), then both added in task order, so the
-    // FP64 sums are those of a one-at-a-time loop.  3D: one task at a time,
+    // 2D: every load of a task issued before any test, and with PAIR a
+    // lane's tasks two at a time (the loads of both overlap), added in task
+    // order, so the FP64 sums are those of a one-at-a-time loop.  3D: one task at a time,
     // and a link's force and geometry are loaded only once its key names
     // this element (a 3D window's cells carry mostly other elements' links;
     // loading them all doubled the batched 3D sums launch).
-    constexpr int U = DIM == 2 ? 2 : 1;
+    constexpr int U = DIM == 2 && PAIR ? 2 : 1;
     constexpr bool EARLY = DIM == 2;
     for (int t0 = lane; t0 < K * chunk; t0 += 64 * U) {
       bool use[U];
@@ -475,7 +477,7 @@ __device__ __forceinline__ void elem_sums_body(const LinkArgs& la, const FinalAr
   __shared__ int s_slot[kElemPerBlock][64];
   const int wv = threadIdx.x >> 6;
   const int elem = blk * kElemPerBlock + wv;
-  if (elem < la.ne) elem_sum_wave<DIM>(la, fa, elem, threadIdx.x & 63, s_slot[wv]);
+  if (elem < la.ne) elem_sum_wave<DIM, !BATCHED>(la, fa, elem, threadIdx.x & 63, s_slot[wv]);
 }
 
 // Vertex blocks sum their incident element shares in CSR order (the
