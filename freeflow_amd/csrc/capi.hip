@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -78,12 +79,12 @@ static void release_stream(int device, hipStream_t s) {
 // Arrays carved out of one device allocation (one hipMalloc and one hipFree
 // for a group of arrays that live and die together), each 256-byte aligned.
 struct Carve {
-  std::vector<std::pair<void**, size_t>> items;  // (pointer to set, byte offset)
+  std::vector<std::function<void(unsigned char*)>> setters;  // each sets one array from the base
   size_t total = 0;
   template <class T>
   size_t add(T** p, size_t count) {
     const size_t off = total;
-    items.push_back({(void**)p, off});
+    setters.push_back([p, off](unsigned char* base) { *p = reinterpret_cast<T*>(base + off); });
     total += (std::max<size_t>(count, 1) * sizeof(T) + 255) / 256 * 256;
     return off;
   }
@@ -91,7 +92,7 @@ struct Carve {
     *base = nullptr;
     hipError_t e = hipMalloc(base, total);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc");
-    for (auto& it : items) *it.first = (unsigned char*)*base + it.second;
+    for (auto& set : setters) set(static_cast<unsigned char*>(*base));
     return FF_OK;
   }
 };
