@@ -4,6 +4,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 from freeflow_amd import capi
@@ -58,3 +59,41 @@ def test_footprint_without_gpu():
     assert with_b > base32
     p3 = (96 ** 3 + 255) // 256 * 256
     assert capi.footprint(3, 96, 96, 96, storage="bf16") == 2 * p3 * 10 * 2 + 96 ** 3 + 8192
+
+
+def _distinct_cells_near(verts, elems, dx):
+    """Independent count of the link-table estimate: distinct cells of the
+    element boxes widened by two cells (floor(min) - 2 .. floor(max) + 2)."""
+    cells = set()
+    for e in elems:
+        p = verts[e] / dx
+        lo = np.floor(p.min(0)).astype(int) - 2
+        hi = np.floor(p.max(0)).astype(int) + 2
+        for x in range(lo[0], hi[0] + 1):
+            for y in range(lo[1], hi[1] + 1):
+                cells.add((x, y))
+    return len(cells)
+
+
+def test_link_table_sized_from_distinct_cells_near_the_mesh():
+    """Two 2D polylines with the same vertex and element counts, one packed
+    into a few cells and one spread over the grid: their footprints differ by
+    exactly the difference of their link tables, each sized
+    next_pow2(2 * distinct cells + 1024) slots (360 B per 2D slot: two
+    tables of cells and 8 keys, force/barycentric/hit per link, 6 moments)."""
+    if not os.path.exists(capi.LIB_PATH):
+        pytest.skip("library not built")
+    n, nx, ny = 400, 2000, 200
+    e = np.stack([np.arange(n), np.arange(1, n + 1)], 1).astype(np.uint32)
+    packed = np.stack([100.0 + 0.01 * np.arange(n + 1), np.full(n + 1, 50.3)], 1).astype(np.float32)
+    t = np.linspace(0.0, 1.0, n + 1)
+    spread = np.stack([10.0 + 1900.0 * t, 100.0 + 60.0 * np.sin(12.0 * t)], 1).astype(np.float32)
+
+    def cap(v):
+        c = 2 * _distinct_cells_near(v.astype(np.float64), e, 1.0) + 1024
+        return 1 << (c - 1).bit_length()
+
+    fa = capi.footprint(2, nx, ny, storage="f32", verts=packed, elems=e, dx=1.0)
+    fb = capi.footprint(2, nx, ny, storage="f32", verts=spread, elems=e, dx=1.0)
+    assert cap(spread) > cap(packed)
+    assert fb - fa == (cap(spread) - cap(packed)) * 360
