@@ -82,17 +82,17 @@ def test_channel_walls_inlet_outlet_clamped():
     compare(g, o, 1)
 
 
-@pytest.mark.parametrize("spin", [0.0, 0.02])
-def test_sphere_moving_wall_links(spin):
+@pytest.mark.parametrize("spin,storage", [(0.0, "f32"), (0.02, "f32"), (0.02, "f16")])
+def test_sphere_moving_wall_links(spin, storage):
     nx, ny, nz = 40, 32, 32
-    note(f"case 3d sphere spin={spin}")
+    note(f"case 3d sphere spin={spin} storage={storage}")
     flags = np.ones(nx * ny * nz, np.uint8)
     rng = np.random.default_rng(5)
     u = np.zeros((nx * ny * nz, 3))
     u[:, 0] = 0.03
     u += rng.uniform(-1e-3, 1e-3, u.shape)
     m = wl.equilibrium_moments(u)
-    g, o = pair3(nx, ny, nz, flags, m, periodic=True)
+    g, o = pair3(nx, ny, nz, flags, m, periodic=True, storage=storage)
     c = np.array([19.3, 15.6, 16.2])
     v, tri = icosphere(c, 6.1, subdiv=2)
     vel = spin * np.cross([0.0, 0.0, 1.0], v - c)
