@@ -132,14 +132,16 @@ def test_config1_cylinder_links_bitexact():
     assert abs(w[2] - tz) < 1e-4 * np.sum(np.abs(r).max() * np.abs(f1))
 
 
-def test_moving_swimmer_explicit_boundary():
+@pytest.mark.parametrize("storage,stage", [("f32", True), ("f16", False), ("bf16", False)])
+def test_moving_swimmer_explicit_boundary(storage, stage):
     # explicit per-step vertex positions/velocities (the reference proxy
-    # contract), an elliptical swimmer with a travelling wave, outlet box.
+    # contract), an elliptical swimmer with a travelling wave, outlet box;
+    # also in the 16-bit storage form the batched and served paths use.
     nx, ny, dx, dt = 240, 120, 0.005, 0.0025
-    note(f"case swimmer explicit {nx}x{ny}")
+    note(f"case swimmer explicit {nx}x{ny} storage={storage} stage={stage}")
     flags = wl.outlet_box_flags(nx, ny)
     m = wl.equilibrium_moments(np.zeros((nx * ny, 2)))
-    g, o = pair(nx, ny, flags, m, nu=0.005, dx=dx, dt=dt, rho=10.0)
+    g, o = pair(nx, ny, flags, m, stage=stage, storage=storage, nu=0.005, dx=dx, dt=dt, rho=10.0)
     sw = wl.Swimmer2D(160, 0.6, 0.3, 0.2, 0.05, A=0.02, f=1.5, phi=0.3)
     v, vel = sw.state(0.0)
     g.set_boundary(v, vel, sw.elems)
