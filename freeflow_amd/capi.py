@@ -199,10 +199,10 @@ class Domain:
         u = np.ascontiguousarray(vels, np.float32).ravel()
         e = np.ascontiguousarray(elems, np.uint32).ravel()
         b = None if elem_body is None else np.ascontiguousarray(elem_body, np.int32).ravel()
-        self.nv = v.size // self.dim
-        self.n_bodies = n_bodies
-        _check(lib().ff_set_boundary(self.h, self.nv, _ptr(v), _ptr(u), e.size // self.dim, _ptr(e),
+        nv = v.size // self.dim
+        _check(lib().ff_set_boundary(self.h, nv, _ptr(v), _ptr(u), e.size // self.dim, _ptr(e),
                                      None if b is None else _ptr(b), n_bodies))
+        self.nv, self.n_bodies = nv, n_bodies  # a rejected boundary leaves the previous one
 
     def update_boundary(self, verts, vels):
         v = np.ascontiguousarray(verts, np.float32).ravel()

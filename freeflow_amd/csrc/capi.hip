@@ -179,6 +179,27 @@ static unsigned next_pow2(unsigned long long v) {
   return p;
 }
 
+// Descriptor checks that need no device: dimension, grid, storage, dx.
+static int check_grid(const ff_domain_desc* desc) {
+  if (desc->dim != 2 && desc->dim != 3) return fail(FF_ERR_INVALID, "dim must be 2 or 3");
+  if (desc->nx <= 0 || desc->ny <= 0 || (desc->dim == 3 && desc->nz <= 0))
+    return fail(FF_ERR_INVALID, "grid sizes must be positive");
+  const long long n = (long long)desc->nx * desc->ny * (desc->dim == 3 ? desc->nz : 1);
+  if (n >= (1ll << 31)) return fail(FF_ERR_INVALID, "grid larger than 2^31 cells");
+  if (desc->storage < 0 || desc->storage > 2) return fail(FF_ERR_INVALID, "unknown storage format");
+  if (!(desc->dx > 0.0)) return fail(FF_ERR_INVALID, "dx must be positive");
+  return FF_OK;
+}
+
+// Vertex coordinates must be finite and within 2^30 cells of the origin: the
+// link-table sizing converts them to 64-bit cell indices.
+static int check_verts(int D, double dx, int nv, const float* verts) {
+  for (long long k = 0; k < (long long)nv * D; k++)
+    if (!(std::fabs(verts[k] / dx) <= (double)(1 << 30)))
+      return fail(FF_ERR_INVALID, "vertex coordinate not finite or beyond 2^30 cells");
+  return FF_OK;
+}
+
 // Link-table sizing: the distinct cells within two cells of some element's
 // bounding box in the shape given (a link cell lies within sqrt(3) cells of
 // an element it crosses; the extra cell, and the factor 2 the table size
@@ -205,7 +226,8 @@ static unsigned long long link_estimate(int D, double dx, int ne, const float* v
       hi[(size_t)k * D + c] = b;
       glo[c] = k ? std::min(glo[c], a) : a;
       ghi[c] = k ? std::max(ghi[c], b) : b;
-      cells *= (unsigned long long)(b - a + 1);
+      const unsigned long long w = (unsigned long long)(b - a + 1);
+      cells = cells > (1ull << 40) / w ? (1ull << 40) : cells * w;  // saturates; the sum stays below 2^57
     }
     sum += cells;
   }
@@ -369,15 +391,11 @@ int ff_device_count(int* n) {
 int ff_domain_create(const ff_domain_desc* desc, ff_domain** out) {
   if (!desc || !out) return fail(FF_ERR_INVALID, "null argument");
   *out = nullptr;
-  if (desc->dim != 2 && desc->dim != 3) return fail(FF_ERR_INVALID, "dim must be 2 or 3");
-  if (desc->nx <= 0 || desc->ny <= 0 || (desc->dim == 3 && desc->nz <= 0))
-    return fail(FF_ERR_INVALID, "grid sizes must be positive");
+  if (int rc = check_grid(desc)) return rc;
   const long long nz = desc->dim == 3 ? desc->nz : 1;
   const long long n = (long long)desc->nx * desc->ny * nz;
-  if (n >= (1ll << 31)) return fail(FF_ERR_INVALID, "grid larger than 2^31 cells");
-  if (desc->storage < 0 || desc->storage > 2) return fail(FF_ERR_INVALID, "unknown storage format");
   if (!(desc->viscosity > 0.0)) return fail(FF_ERR_INVALID, "viscosity must be positive");
-  if (!(desc->dx > 0.0) || !(desc->dt > 0.0)) return fail(FF_ERR_INVALID, "dx and dt must be positive");
+  if (!(desc->dt > 0.0)) return fail(FF_ERR_INVALID, "dt must be positive");
   if (desc->options & ~(FF_OPT_STAGE_FP16 | FF_OPT_PERIODIC)) return fail(FF_ERR_INVALID, "unknown option bits");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(FF_ERR_NODEVICE, "no HIP device available");
@@ -546,6 +564,7 @@ int ff_set_boundary(ff_domain* dd, int nv, const float* verts, const float* vels
   if (n_bodies <= 0) n_bodies = 1;
   for (long long k = 0; k < (long long)ne * D; k++)
     if (elems[k] >= (uint32_t)nv) return fail(FF_ERR_INVALID, "element references a vertex out of range");
+  if (int rc = check_verts(D, d.dx, nv, verts)) return rc;
   std::vector<int> eb(ne, 0);
   if (elem_body)
     for (int k = 0; k < ne; k++) {
@@ -961,12 +980,13 @@ int ff_time_steps(ff_domain* dd, int nsteps, double* total_ms) {
 int ff_domain_footprint(const ff_domain_desc* desc, int nv, int ne, const float* verts, const uint32_t* elems,
                         unsigned long long* bytes) {
   if (!desc || !bytes) return fail(FF_ERR_INVALID, "null argument");
-  if (desc->dim != 2 && desc->dim != 3) return fail(FF_ERR_INVALID, "dim must be 2 or 3");
+  if (int rc = check_grid(desc)) return rc;
   if (nv < 0 || ne < 0 || (ne > 0 && (!verts || !elems))) return fail(FF_ERR_INVALID, "bad boundary arguments");
-  if (desc->storage < 0 || desc->storage > 2) return fail(FF_ERR_INVALID, "unknown storage format");
   const long long n = (long long)desc->nx * desc->ny * (desc->dim == 3 ? desc->nz : 1);
   for (long long k = 0; k < (long long)ne * desc->dim; k++)
     if (elems[k] >= (uint32_t)nv) return fail(FF_ERR_INVALID, "element references a vertex out of range");
+  if (ne > 0)
+    if (int rc = check_verts(desc->dim, desc->dx, nv, verts)) return rc;
   const unsigned long long est = ne ? link_estimate(desc->dim, desc->dx, ne, verts, elems) : 0;
   *bytes = footprint(desc->dim, n, desc->storage, nv, ne, est);
   return FF_OK;

@@ -97,3 +97,45 @@ def test_link_table_sized_from_distinct_cells_near_the_mesh():
     fb = capi.footprint(2, nx, ny, storage="f32", verts=spread, elems=e, dx=1.0)
     assert cap(spread) > cap(packed)
     assert fb - fa == (cap(spread) - cap(packed)) * 360
+
+
+def test_invalid_arguments_rejected_without_gpu():
+    # descriptor and footprint checks run on the host before any device call
+    if not os.path.exists(capi.LIB_PATH):
+        pytest.skip("library not built")
+    L = capi.lib()
+    bad = [
+        (dict(nx=0), b"grid sizes"),
+        (dict(ny=-4), b"grid sizes"),
+        (dict(nx=1 << 16, ny=1 << 16), b"2^31"),
+        (dict(storage=3), b"storage"),
+        (dict(dx=0.0), b"dx"),
+        (dict(dx=float("nan")), b"dx"),
+        (dict(viscosity=0.0), b"viscosity"),
+        (dict(dt=-1.0), b"dt"),
+        (dict(options=8), b"option"),
+    ]
+    base = dict(dim=2, nx=8, ny=8, nz=1, viscosity=0.01, dx=1.0, dt=1.0, fluid_density=1.0, storage=0,
+                options=0, device=0)
+    for change, msg in bad:
+        d = capi.DomainDesc(**{**base, **change})
+        h = ctypes.c_void_p()
+        assert L.ff_domain_create(ctypes.byref(d), ctypes.byref(h)) == 1, change
+        assert msg in L.ff_last_error(), (change, L.ff_last_error())
+        assert not h.value
+    # footprint: same grid checks, plus element indices and vertex coordinates
+    with pytest.raises(capi.FreeFlowError, match="grid sizes"):
+        capi.footprint(2, 0, 8)
+    with pytest.raises(capi.FreeFlowError, match="dx"):
+        capi.footprint(2, 8, 8, dx=-1.0)
+    v = np.array([[1.0, 1.0], [2.0, 1.0]], np.float32)
+    with pytest.raises(capi.FreeFlowError, match="out of range"):
+        capi.footprint(2, 8, 8, verts=v, elems=np.array([[0, 2]], np.uint32))
+    for x in (np.nan, np.inf, 3e9):
+        w = v.copy()
+        w[1, 0] = x
+        with pytest.raises(capi.FreeFlowError, match="not finite or beyond"):
+            capi.footprint(2, 8, 8, verts=w, elems=np.array([[0, 1]], np.uint32))
+    # a long thin segment far from the mesh box: the element-box product saturates
+    w = np.array([[0.0, 0.0, 0.0], [1e9, 1e9, 1e9], [0.0, 1e9, 0.0]], np.float32)
+    assert capi.footprint(3, 8, 8, 8, verts=w, elems=np.array([[0, 1, 2]], np.uint32)) > 0
