@@ -108,7 +108,9 @@ int ff_get_moments(ff_domain* d, float* aos, int which);
    2 vertex ids; triangles: 3).  elem_body (may be NULL) assigns each
    element to one of n_bodies bodies for the per-body wrench.  At most
    65536 elements (the reference's 16-bit element field of the hit key,
-   SolidGeometryProxy_Device.cuh:79-85).  Copies the arrays. */
+   SolidGeometryProxy_Device.cuh:79-85).  Vertex coordinates must be finite
+   and within 2^30 cells of the origin.  Copies the arrays.  A rejected call
+   (FF_ERR_INVALID) keeps the previous boundary. */
 int ff_set_boundary(ff_domain* d, int nv, const float* verts, const float* vels, int ne, const uint32_t* elems,
                     const int32_t* elem_body, int n_bodies);
 /* New vertex positions/velocities for the next step (same topology). */
