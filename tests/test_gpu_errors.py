@@ -60,3 +60,36 @@ def test_rejected_calls_leave_the_domain_unchanged():
     assert np.array_equal(ref.body_wrench(), d.body_wrench())
     assert np.array_equal(ref.vertex_forces(), d.vertex_forces())
     assert np.abs(d.vertex_forces()).max() > 0
+
+
+def test_rejected_batch_calls_leave_the_batch_unchanged():
+    note("case rejected batch membership calls, two 96x48 domains with a cylinder")
+    v, e = wl.cylinder(40, 40.0, 24.0, 8.0)
+    vel = np.zeros_like(v)
+    vel[:, 0] = 0.01
+    alone, member = domain(), domain()
+    for x in (alone, member):
+        x.set_boundary(v, vel, e)
+    b = capi.Batch(2, "f32")
+    b.add(member)
+    with pytest.raises(capi.FreeFlowError, match="error 5:"):
+        b.add(member)  # already a member
+    with pytest.raises(capi.FreeFlowError, match="differs from the batch"):
+        b.add(capi.Domain(2, 32, 32, storage="f16"))
+    with pytest.raises(capi.FreeFlowError, match="differs from the batch"):
+        b.add(capi.Domain(3, 16, 16, 16))
+    other = domain()
+    with pytest.raises(capi.FreeFlowError, match="not in this batch"):
+        b.remove(other)
+    with pytest.raises(capi.FreeFlowError, match="error 5:"):
+        member.step(1)  # a member steps with its batch
+    with pytest.raises(capi.FreeFlowError, match="error 1:"):
+        b.step(-1)
+    with pytest.raises(capi.FreeFlowError, match="tile index"):
+        b.set_tile(99)
+    assert len(b) == 1
+    alone.step(12)
+    b.step(12)
+    b.sync()
+    assert np.array_equal(alone.moments(0), member.moments(0))
+    assert np.array_equal(alone.body_wrench(), member.body_wrench())
