@@ -8,7 +8,8 @@ device in its own domain; domains are members of one batch per kind (2D and
 3D, FP16 storage by default), and every server iteration advances every
 active episode by one step with one graph replay per batch.  Episodes that
 arrive while others run are admitted between steps as soon as their device
-footprint (ff_domain_footprint) fits the memory budget and a slot is free;
+footprint (ff_domain_footprint) fits the memory budget and a slot is free
+(an episode larger than the whole budget is rejected at submission);
 finished episodes leave the batch and free their memory.  Actions (new wave
 amplitude/phase) are applied between steps.
 
@@ -46,6 +47,7 @@ class Episode:
     admitted_at: float = None
     finished_at: float = None
     footprint: int = 0
+    rejected: bool = False  # its footprint alone exceeds the memory budget
     wrench: list = field(default_factory=list)
 
     @property
@@ -203,6 +205,9 @@ class Server:
         pending = sorted(episodes, key=lambda e: e.arrival)
         for ep in pending:
             ep.footprint = self.footprint(ep)
+            ep.rejected = ep.footprint > self.budget
+        # an episode that can never fit would block the queue's head for good
+        pending = [ep for ep in pending if not ep.rejected]
         queue = []
         t_start = time.perf_counter()
         i = 0
@@ -234,6 +239,7 @@ class Server:
         return {
             "episodes_done": len(done),
             "episodes_submitted": len(episodes),
+            "episodes_rejected": sum(1 for e in episodes if e.rejected),
             "wall_s": wall,
             "gpu_busy_s": busy,
             "iterations": self.iterations,

@@ -59,6 +59,22 @@ def test_admission_by_footprint():
     assert summary["p99_step_ms"] >= summary["p50_step_ms"] > 0
 
 
+def test_episode_larger_than_budget_is_rejected():
+    # it would block the queue's head for good; the others are still served
+    eps = small_episodes(9, n=4, steps=20)
+    srv = serve.Server(storage="f16")
+    fps = [srv.footprint(e) for e in eps]
+    srv.budget = max(fps) - 1
+    big = [e for e, f in zip(eps, fps) if f == max(fps)]
+    note(f"case budget {srv.budget}, {len(big)} episode(s) above it")
+    summary = srv.run(eps, realtime=False)
+    srv.close()
+    assert summary["episodes_rejected"] == len(big) >= 1
+    assert all(e.rejected and e.finished_at is None for e in big)
+    assert summary["episodes_done"] == len(eps) - len(big)
+    assert summary["peak_bytes"] <= srv.budget
+
+
 def test_batched_equals_plain_form():
     """Served in one batch per kind or every episode on its own launches: the
     same moments and wrench histories, bit for bit (actions re-detect links in
