@@ -132,6 +132,41 @@ def test_config1_cylinder_links_bitexact():
     assert abs(w[2] - tz) < 1e-4 * np.sum(np.abs(r).max() * np.abs(f1))
 
 
+def test_max_elements_65536_bitexact():
+    # the largest boundary: 256 moving 256-gons, 65536 segments, so element
+    # ids fill the 16-bit field of the link key; one body per polygon.
+    nx = ny = 512
+    note(f"case 65536 segments, 256 bodies, {nx}x{ny} periodic")
+    rng = np.random.default_rng(21)
+    u = np.zeros((nx * ny, 2))
+    u[:, 0] = 0.03
+    u += rng.uniform(-1e-3, 1e-3, u.shape)
+    g, o = pair(nx, ny, np.ones(nx * ny, np.uint8), wl.equilibrium_moments(u), periodic=True, nu=0.01)
+    vs, es, vels = [], [], []
+    for k in range(256):
+        cx, cy = 16.0 + 32.0 * (k % 16) + rng.uniform(-2, 2), 16.0 + 32.0 * (k // 16) + rng.uniform(-2, 2)
+        v, e = wl.cylinder(256, cx, cy, 7.0 + rng.uniform(0, 2))
+        es.append(e + 256 * k)
+        vs.append(v)
+        vels.append(np.tile(rng.uniform(-0.02, 0.02, 2), (256, 1)))
+    v, e, vel = np.concatenate(vs), np.concatenate(es), np.concatenate(vels)
+    assert len(e) == 65536
+    body = np.repeat(np.arange(256), 256)
+    g.set_boundary(v, vel, e, elem_body=body, n_bodies=256)
+    o.set_boundary(v, vel, e)
+    g.step(10)
+    o.step(10)
+    assert_same(g, o, 0)
+    fg = g.vertex_forces(reset=True).astype(np.float64)
+    fo = o.vertex_forces()
+    assert np.abs(fo[-256:]).max() > 0  # the last body (element ids near 65535) is hit
+    assert np.linalg.norm(fg - fo) / np.linalg.norm(fo) < 1e-5
+    g.step(1)
+    f1 = g.vertex_forces().astype(np.float64).reshape(256, 256, 2).sum(1)
+    w = g.body_wrench()
+    assert np.allclose(w[:, :2], f1, rtol=0, atol=1e-5 * np.abs(f1).max())
+
+
 @pytest.mark.parametrize("storage,stage", [("f32", True), ("f16", False), ("bf16", False)])
 def test_moving_swimmer_explicit_boundary(storage, stage):
     # explicit per-step vertex positions/velocities (the reference proxy
