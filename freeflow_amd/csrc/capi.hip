@@ -115,6 +115,7 @@ static void free_boundary(Domain& d) {
   d.has_boundary = false;
   d.wave = false;
   d.hist_cap = 0;
+  d.hist_count = 0;
   d.predetected = false;
   d.stale = false;
 }

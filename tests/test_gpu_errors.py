@@ -93,3 +93,24 @@ def test_rejected_batch_calls_leave_the_batch_unchanged():
     b.sync()
     assert np.array_equal(alone.moments(0), member.moments(0))
     assert np.array_equal(alone.body_wrench(), member.body_wrench())
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_empty_boundary_steps_like_no_boundary(dim):
+    # zero vertices and elements: the plain stream-collide step, zero wrench
+    note(f"case empty boundary dim={dim}")
+    shape = (64, 32) if dim == 2 else (24, 16, 16)
+    n = int(np.prod(shape))
+    rng = np.random.default_rng(4)
+    u = rng.uniform(-0.02, 0.02, (n, dim))
+    plain, empty = (capi.Domain(dim, *shape, viscosity=0.01, periodic=True) for _ in range(2))
+    for x in (plain, empty):
+        x.set_moments(wl.equilibrium_moments(u))
+    empty.set_boundary(np.zeros((0, dim), np.float32), np.zeros((0, dim), np.float32), np.zeros((0, dim), np.uint32))
+    empty.set_wrench_history(4)
+    for x in (plain, empty):
+        x.step(6)
+    assert np.array_equal(plain.moments(0), empty.moments(0))
+    assert not np.any(empty.body_wrench())
+    assert empty.vertex_forces().shape == (0, dim)
+    assert len(empty.wrench_history(4)) == 0
