@@ -132,6 +132,37 @@ def test_config1_cylinder_links_bitexact():
     assert abs(w[2] - tz) < 1e-4 * np.sum(np.abs(r).max() * np.abs(f1))
 
 
+@pytest.mark.parametrize("periodic", [False, True])
+def test_bodies_across_the_grid_edge_bitexact(periodic):
+    # polygons partly outside the grid (left edge, top edge, a corner): links
+    # whose upstream cell leaves the grid are not moving-wall links, and the
+    # candidate window is clipped to the grid, not wrapped
+    nx, ny = 120, 80
+    note(f"case bodies across the edge periodic={periodic}")
+    rng = np.random.default_rng(33)
+    u = np.zeros((nx * ny, 2))
+    u[:, 0] = 0.02
+    u += rng.uniform(-1e-3, 1e-3, u.shape)
+    g, o = pair(nx, ny, np.ones(nx * ny, np.uint8), wl.equilibrium_moments(u), periodic=periodic, nu=0.01)
+    vs, es = [], []
+    for k, (cx, cy, r) in enumerate([(2.3, 40.2, 9.0), (60.4, 78.1, 7.5), (118.6, 1.2, 6.0)]):
+        v, e = wl.cylinder(96, cx, cy, r)
+        vs.append(v)
+        es.append(e + 96 * k)
+    v, e = np.concatenate(vs), np.concatenate(es)
+    vel = np.tile([0.01, -0.015], (len(v), 1))
+    g.set_boundary(v, vel, e)
+    o.set_boundary(v, vel, e)
+    g.step(25)
+    o.step(25)
+    assert_same(g, o, 0)
+    assert_same(g, o, 1)
+    fg = g.vertex_forces().astype(np.float64)
+    fo = o.vertex_forces()
+    assert np.abs(fo).max() > 0
+    assert np.linalg.norm(fg - fo) / np.linalg.norm(fo) < 1e-5
+
+
 def test_max_elements_65536_bitexact():
     # the largest boundary: 256 moving 256-gons, 65536 segments, so element
     # ids fill the 16-bit field of the link key; one body per polygon.
