@@ -115,6 +115,34 @@ def test_sphere_moving_wall_links(spin, storage):
     assert np.allclose(w[3:], T, rtol=0, atol=1e-4 * np.abs(r).max() * scale)
 
 
+@pytest.mark.parametrize("periodic", [False, True])
+def test_spheres_across_the_grid_edge(periodic):
+    # one sphere through the x = 0 face, one through a corner: links whose
+    # upstream cell leaves the grid are not moving-wall links
+    nx, ny, nz = 40, 32, 24
+    note(f"case 3d spheres across the edge periodic={periodic}")
+    rng = np.random.default_rng(8)
+    u = np.zeros((nx * ny * nz, 3))
+    u[:, 0] = 0.02
+    u += rng.uniform(-1e-3, 1e-3, u.shape)
+    g, o = pair3(nx, ny, nz, np.ones(nx * ny * nz, np.uint8), wl.equilibrium_moments(u), periodic=periodic)
+    v1, t1 = icosphere([1.7, 16.3, 12.2], 5.5, subdiv=2)
+    v2, t2 = icosphere([38.9, 31.2, 0.6], 4.5, subdiv=2)
+    v = np.concatenate([v1, v2])
+    tri = np.concatenate([t1, t2 + len(v1)])
+    vel = np.tile([0.01, 0.0, -0.01], (len(v), 1))
+    g.set_boundary(v, vel, tri)
+    o.set_boundary(v, vel, tri)
+    g.step(15)
+    o.step(15)
+    _, same = compare(g, o)
+    assert same == 1.0
+    fg = g.vertex_forces().astype(np.float64)
+    fo = o.vertex_forces()
+    assert np.abs(fo[len(v1):]).max() > 0
+    assert np.linalg.norm(fg - fo) / np.linalg.norm(fo) < 1e-5
+
+
 @pytest.mark.parametrize("storage", ["f16", "bf16"])
 def test_16bit_storage_3d(storage):
     nx, ny, nz = 24, 16, 16
