@@ -124,6 +124,37 @@ def test_tile_shapes_give_identical_results(tile):
         assert np.array_equal(a.body_wrench(), m.body_wrench())
 
 
+@pytest.mark.parametrize("dim,tile", [(2, t) for t in range(5)] + [(3, t) for t in range(7)])
+def test_members_smaller_than_a_tile(dim, tile):
+    # members of a few cells next to a regular one, in every tile shape:
+    # each equals the same domain stepped alone
+    shapes = [(5, 3), (17, 9), (64, 40)] if dim == 2 else [(5, 4, 3), (19, 9, 7), (40, 24, 20)]
+    note(f"case tiny members dim={dim} tile={tile} {shapes}")
+    rng = np.random.default_rng(50 + tile)
+
+    def build(shape, seed):
+        n = int(np.prod(shape))
+        u = np.random.default_rng(seed).uniform(-0.03, 0.03, (n, dim))
+        d = capi.Domain(dim, *shape, viscosity=0.02)
+        d.set_moments(wl.equilibrium_moments(u))
+        return d
+
+    seeds = rng.integers(1 << 30, size=len(shapes))
+    alone = [build(sh, sd) for sh, sd in zip(shapes, seeds)]
+    mixed = [build(sh, sd) for sh, sd in zip(shapes, seeds)]
+    b = capi.Batch(dim, "f32")
+    b.set_tile(tile)
+    for d in mixed:
+        b.add(d)
+    for d in alone:
+        d.set_tile(tile)
+        d.step(12)
+    b.step(12)
+    b.sync()
+    for i, (a, m) in enumerate(zip(alone, mixed)):
+        same(a, m, f"member {i} {shapes[i]}")
+
+
 def test_autotune_and_single_domain_tiles():
     s = scenes(31, n=1)[0]
     note("case autotune")
