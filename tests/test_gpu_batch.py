@@ -221,6 +221,36 @@ def test_batch_3d_sphere_and_periodic():
     assert np.abs(alone[1].body_wrench()).max() > 0
 
 
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_batch_3d_16bit_storage(storage):
+    # two 3D members of different sizes, each with a moving sphere, in the
+    # served form's 16-bit storage: each equals its lone steps
+    note(f"case batch3d storage {storage}")
+
+    def build(nx, ny, nz, c, r, seed):
+        d = capi.Domain(3, nx, ny, nz, viscosity=0.02, storage=storage)
+        d.set_moments(wl.taylor_green_ring(nx, ny, nz, amp=0.03, ring_r=5.0, seed=seed))
+        v, f = wl.icosphere(c, r, subdiv=2)
+        vel = np.zeros_like(v)
+        vel[:, 1] = 0.015
+        d.set_boundary(v, vel, f)
+        return d
+
+    specs = [(40, 32, 24, [20.0, 16.0, 12.0], 5.0, 3), (33, 28, 30, [15.5, 13.2, 16.1], 4.0, 4)]
+    alone = [build(*sp) for sp in specs]
+    mixed = [build(*sp) for sp in specs]
+    b = capi.Batch(3, storage)
+    for d in mixed:
+        b.add(d)
+    for d in alone:
+        d.step(12)
+    b.step(12)
+    for i, (a, m) in enumerate(zip(alone, mixed)):
+        same(a, m, f"3D {storage} member {i}")
+        assert np.array_equal(a.body_wrench(), m.body_wrench())
+        assert np.abs(a.body_wrench()).max() > 0
+
+
 @pytest.mark.parametrize("batched", [False, True])
 def test_link_table_overflow_is_reported(batched):
     """A boundary moved far beyond the shape its link table was sized for
