@@ -251,6 +251,30 @@ def test_batch_3d_16bit_storage(storage):
         assert np.abs(a.body_wrench()).max() > 0
 
 
+def test_member_destroyed_mid_run():
+    # ff_domain_destroy on a member detaches it first: the batch's graphs and
+    # tables are rebuilt without it and the others go on as if stepped alone
+    sc = scenes(61, n=3)
+    note("case member destroyed mid-run")
+    alone = [make(s) for s in sc]
+    mixed = [make(s) for s in sc]
+    b = capi.Batch(2, "f32")
+    for d in mixed:
+        b.add(d)
+    b.step(9)  # 8-step graph replay plus a direct step
+    for d in alone:
+        d.step(9)
+    mixed[1].close()
+    assert len(b) == 2
+    b.step(9)
+    for d in (alone[0], alone[2]):
+        d.step(9)
+    b.sync()
+    for i in (0, 2):
+        same(alone[i], mixed[i], f"member {i}")
+        assert np.array_equal(alone[i].body_wrench(), mixed[i].body_wrench())
+
+
 @pytest.mark.parametrize("batched", [False, True])
 def test_link_table_overflow_is_reported(batched):
     """A boundary moved far beyond the shape its link table was sized for
