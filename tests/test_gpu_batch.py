@@ -275,6 +275,27 @@ def test_member_destroyed_mid_run():
         assert np.array_equal(alone[i].body_wrench(), mixed[i].body_wrench())
 
 
+def test_batch_destroyed_members_step_alone():
+    # ff_batch_destroy returns every member to its own stream and step state
+    sc = scenes(62, n=2)
+    note("case batch destroyed, members continue alone")
+    alone = [make(s) for s in sc]
+    mixed = [make(s) for s in sc]
+    b = capi.Batch(2, "f32")
+    for d in mixed:
+        b.add(d)
+    b.step(11)
+    b.close()
+    for d in mixed:
+        d.step(7)
+    for d in alone:
+        d.step(18)
+    for i, (a, m) in enumerate(zip(alone, mixed)):
+        same(a, m, f"member {i}")
+        assert np.array_equal(a.body_wrench(), m.body_wrench())
+        assert np.array_equal(a.wrench_history(18), m.wrench_history(18))
+
+
 @pytest.mark.parametrize("batched", [False, True])
 def test_link_table_overflow_is_reported(batched):
     """A boundary moved far beyond the shape its link table was sized for
