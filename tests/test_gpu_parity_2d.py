@@ -226,6 +226,66 @@ def test_moving_swimmer_explicit_boundary(storage, stage):
     assert np.linalg.norm(fg - fo) / np.linalg.norm(fo) < 1e-5
 
 
+def test_step_proxy_with_device_arrays():
+    # ff_step_proxy: the caller's own device arrays in the proxy layout
+    # (SolidGeometryProxy_Device<2>), new positions each step, forces added
+    # into the caller's force array; moments equal the oracle fed the same
+    # positions, forces equal those of the host-array path
+    import ctypes as C
+    hip = C.CDLL("/opt/rocm/lib/libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    h2d, d2h = 1, 2
+
+    def dev(a):
+        p = C.c_void_p()
+        assert hip.hipMalloc(C.byref(p), a.nbytes) == 0
+        assert hip.hipMemcpy(p, a.ctypes.data, a.nbytes, h2d) == 0
+        return p
+
+    nx, ny, dx, dt = 240, 120, 0.005, 0.0025
+    note(f"case step_proxy {nx}x{ny}")
+    flags = wl.outlet_box_flags(nx, ny)
+    m = wl.equilibrium_moments(np.zeros((nx * ny, 2)))
+    g, o = pair(nx, ny, flags, m, nu=0.005, dx=dx, dt=dt, rho=10.0)
+    hpath, _ = pair(nx, ny, flags, m, nu=0.005, dx=dx, dt=dt, rho=10.0)
+    sw = wl.Swimmer2D(160, 0.6, 0.3, 0.2, 0.05, A=0.02, f=1.5, phi=0.3)
+    v, vel = sw.state(0.0)
+    v32, vel32 = v.astype(np.float32), vel.astype(np.float32)
+    elems = np.ascontiguousarray(sw.elems, np.uint32)
+    nv, ne = len(v32), len(elems)
+    for x in (g, o, hpath):
+        x.set_boundary(v32, vel32, elems)
+    dv, du, de = dev(v32), dev(vel32), dev(elems)
+    dF = dev(np.zeros((nv, 2), np.float32))
+    L = capi.lib()
+    try:
+        for s_ in range(30):
+            v, vel = sw.state(s_ * dt)
+            v32 = np.ascontiguousarray(v, np.float32)
+            vel32 = np.ascontiguousarray(vel, np.float32)
+            assert hip.hipMemcpy(dv, v32.ctypes.data, v32.nbytes, h2d) == 0
+            assert hip.hipMemcpy(du, vel32.ctypes.data, vel32.nbytes, h2d) == 0
+            assert L.ff_step_proxy(g.h, dv, du, de, dF, nv, ne) == 0, L.ff_last_error()
+            g.sync()  # the proxy arrays are rewritten next iteration
+            o.update_boundary(v32, vel32)
+            o.step(1)
+            hpath.update_boundary(v32, vel32)
+            hpath.step(1)
+        assert_same(g, o, 0)
+        assert_same(g, o, 1)
+        F = np.empty((nv, 2), np.float32)
+        assert hip.hipMemcpy(F.ctypes.data, dF, F.nbytes, d2h) == 0
+    finally:
+        for p in (dv, du, de, dF):
+            hip.hipFree(p)
+    fh = hpath.vertex_forces()
+    assert np.abs(fh).max() > 0
+    assert np.array_equal(F, fh)
+    assert np.array_equal(g.vertex_forces(), fh)
+
+
 def test_wave_kinematics_on_device():
     nx, ny, dx, dt = 240, 120, 0.005, 0.0025
     note("case wave kinematics")
