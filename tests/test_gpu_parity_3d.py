@@ -115,6 +115,59 @@ def test_sphere_moving_wall_links(spin, storage):
     assert np.allclose(w[3:], T, rtol=0, atol=1e-4 * np.abs(r).max() * scale)
 
 
+def test_step_proxy_with_device_arrays_3d():
+    # ff_step_proxy with the caller's device arrays (SolidGeometryProxy_Device<3>
+    # layout): a sphere translated each step; moments equal the oracle's,
+    # forces added into the caller's array equal the library's own
+    import ctypes as C
+    hip = C.CDLL("/opt/rocm/lib/libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+
+    def dev(a):
+        p = C.c_void_p()
+        assert hip.hipMalloc(C.byref(p), a.nbytes) == 0
+        assert hip.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0
+        return p
+
+    nx, ny, nz = 40, 32, 32
+    note(f"case 3d step_proxy {nx}x{ny}x{nz}")
+    rng = np.random.default_rng(9)
+    u = np.zeros((nx * ny * nz, 3))
+    u[:, 0] = 0.02
+    u += rng.uniform(-1e-3, 1e-3, u.shape)
+    g, o = pair3(nx, ny, nz, np.ones(nx * ny * nz, np.uint8), wl.equilibrium_moments(u), periodic=True)
+    v0, tri = icosphere([18.2, 15.7, 16.4], 5.5, subdiv=2)
+    vel = np.tile(np.float32([0.03, -0.01, 0.02]), (len(v0), 1))
+    tri = np.ascontiguousarray(tri, np.uint32)
+    v32 = np.ascontiguousarray(v0, np.float32)
+    g.set_boundary(v32, vel, tri)
+    o.set_boundary(v32, vel, tri)
+    dv, du, de = dev(v32), dev(vel), dev(tri)
+    dF = dev(np.zeros_like(v32))
+    L = capi.lib()
+    try:
+        for s_ in range(15):
+            v32 = np.ascontiguousarray(v0 + s_ * vel, np.float32)
+            assert hip.hipMemcpy(dv, v32.ctypes.data, v32.nbytes, 1) == 0
+            assert L.ff_step_proxy(g.h, dv, du, de, dF, len(v32), len(tri)) == 0, L.ff_last_error()
+            g.sync()  # the proxy arrays are rewritten next iteration
+            o.update_boundary(v32, vel)
+            o.step(1)
+        _, same = compare(g, o)
+        assert same == 1.0
+        F = np.empty_like(v32)
+        assert hip.hipMemcpy(F.ctypes.data, dF, F.nbytes, 2) == 0
+    finally:
+        for p in (dv, du, de, dF):
+            hip.hipFree(p)
+    fo = o.vertex_forces()
+    assert np.abs(fo).max() > 0
+    assert np.linalg.norm(F.astype(np.float64) - fo) / np.linalg.norm(fo) < 1e-5
+    assert np.array_equal(F, g.vertex_forces())
+
+
 @pytest.mark.parametrize("periodic", [False, True])
 def test_spheres_across_the_grid_edge(periodic):
     # one sphere through the x = 0 face, one through a corner: links whose
