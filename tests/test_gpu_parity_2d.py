@@ -286,6 +286,41 @@ def test_step_proxy_with_device_arrays():
     assert np.array_equal(g.vertex_forces(), fh)
 
 
+def test_step_vertex_forces_and_body_ref():
+    # the per-step forces equal one step's accumulation from zero; the torque
+    # is taken about the reference point given by ff_set_body_ref
+    nx, ny = 160, 80
+    note(f"case step vertex forces and body ref {nx}x{ny}")
+    d = capi.Domain(2, nx, ny, viscosity=0.01)
+    d.set_flags(wl.outlet_box_flags(nx, ny))
+    d.set_moments(wl.equilibrium_moments(np.tile([0.02, 0.0], (nx * ny, 1))))
+    v, e = wl.cylinder(64, 60.3, 40.2, 10.0)
+    vel = np.zeros_like(v)
+    vel[:, 1] = 0.01
+    d.set_boundary(v, vel, e)
+    assert not np.any(d.step_vertex_forces())  # no step yet
+    ref = np.array([[55.0, 37.5]])
+    d.set_body_ref(ref)
+    d.step(5)
+    d.vertex_forces(reset=True)
+    d.step(1)
+    fs = d.step_vertex_forces()
+    assert np.abs(fs).max() > 0
+    assert np.array_equal(fs, d.vertex_forces())
+    f = fs.astype(np.float64)
+    w = d.body_wrench()[0]
+    assert np.allclose(w[:2], f.sum(0), rtol=0, atol=1e-5 * np.abs(f).sum())
+
+    def torque(about):
+        r = v.astype(np.float64) - about
+        return np.sum(r[:, 0] * f[:, 1] - r[:, 1] * f[:, 0]), 1e-4 * np.abs(r).max() * np.abs(f).sum()
+
+    tz, tol = torque(ref[0])
+    tc, _ = torque(v.astype(np.float64).mean(0))
+    assert abs(w[2] - tz) < tol
+    assert abs(tc - tz) > 10 * tol  # the default centroid would not pass
+
+
 def test_wave_kinematics_on_device():
     nx, ny, dx, dt = 240, 120, 0.005, 0.0025
     note("case wave kinematics")
