@@ -1,29 +1,45 @@
 """Benchmark of the HOME-LBM fluid step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload swimmer2d]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload tg3d]
 
-Default workload (the configuration the headline metric is quoted on): 2D FSI,
-plain FP32, D2Q9 1200x400, dx=0.005, dt=0.0025, an elliptical swimmer
-(semi-axes 0.3 x 0.08 m, 600 boundary segments) driven by a seeded travelling
-wave, all four sides outlets, with the per-step FP64 body force/torque kept in
-a device ring that is read back inside the timed region.  A step is one fluid
-step (wave kinematics, link detection, stream+collide, moving-wall links,
-force reduction, outlet update) over the whole grid.
+Default workload (the configuration whose bytes exceed the 256 MB MALL, so
+the roofline is HBM's): 3D D3Q27 on a 256x128x128 periodic grid, FP32
+storage and arithmetic, a seeded Taylor-Green field of amplitude 0.04 plus a
+vortex ring of radius 20 cells, viscosity 0.01.  A step is one stream-collide
+update of every cell; K steps are timed with the moments resident in HBM.
 
-Prints one JSON line.  `value` is MLUPS (cells x steps / s / 1e6) with all
-inputs resident on the GPU.  `roofline` prices the stream-collide kernel
-(the dominant one) against 8 TB/s HBM with 49 algorithmic bytes per cell
-update (6 FP32 moments read + 6 written + 1 flag byte); `cpu_baseline` is the
-CPU oracle (a C++ restatement of the reference update, OpenMP) timed on this
-host on a bounded number of steps of the same workload.
+Prints one JSON line.  `value` is MLUPS (cells x steps / s / 1e6) of the
+column tiles in FF_OPT_FAST_MATH form (the same update with reassociated FP32
+arithmetic and FMA; `--reference-order` times the reference's operation
+order instead).  `roofline` prices that kernel against 8 TB/s HBM with 81
+algorithmic bytes per cell update (10 FP32 moments read + 10 written + 1
+flag byte), from HIP events on the kernel's own launches.  The line also
+carries:
+  reference_order     the same workload in the reference's operation order
+                      (bit-exact with the CPU oracle): MLUPS, kernel time,
+                      fraction of HBM peak;
+  accuracy            both forms after S steps on the full grid against the
+                      CPU oracle in FP32 (the reference order: fraction of
+                      bit-identical values) and in FP64 (velocity rel-L2),
+                      with the mass drift of the timed run;
+  cpu_baseline        the FP32 CPU oracle (OpenMP) on the same grid, S steps;
+  swimmer2d           2D FSI 1200x400 with a 600-segment swimmer (plain FP32,
+                      per-step FP64 wrench): MLUPS and the per-step latency
+                      with a host sync after every step;
+  batched             64 uneven 2D swimmer domains, FP16 storage, one graph
+                      replay per step, against the plain per-domain FP32
+                      form, with one member's velocity error vs FP64;
+  served              Poisson rollout episodes at 50/s through the
+                      continuous-batching server: p50/p99 step latency and
+                      MLUPS against the plain per-domain FP32 form.
+`--no-extras` leaves out the last three.
 
 Other workloads (one JSON line each, same keys):
+  --workload swimmer2d 2D FSI plain FP32 1200x400 swimmer alone.
   --workload batched   64 uneven 2D swimmer domains (256x128 .. 1200x400,
                        200-800 segments) in one batch, FP16 storage, one graph
                        replay per step; also times the plain per-domain FP32
                        form of the same domains and reports the ratio.
-  --workload tg3d   3D D3Q27 256x128x128 Taylor-Green + vortex ring,
-                       periodic, storage from --storage (f32, f16, bf16).
   --workload served    Poisson-arriving 2D/3D rollout episodes at --rate per s
                        for --duration s through the continuous-batching server.
 """
@@ -42,8 +58,11 @@ METRIC = "MLUPS + % HBM peak; batched/served vs plain at ≤1e-3 rel err; p99 st
 HBM_PEAK_GBS = 8000.0
 # tile shapes by index, as in the library (ff_set_tile)
 TILES2 = ["32x8", "32x16", "64x8", "32x32", "64x16"]
-TILES3 = ["8x8x4 box", "16x4x4 box", "16x8 column x32", "16x8 column x16", "16x8 column x8", "16x16 column x32",
-          "16x16 column x16"]
+TILES3 = ["8x8x4 box", "16x4x4 box", "16x8 ring column x32", "16x8 ring column x16", "16x8 ring column x8",
+          "16x16 ring column x32", "16x16 ring column x16", "16x12 staged column x8", "16x26 staged column x8",
+          "64x4 staged column x16", "16x12 staged column x8, 2 planes in an LDS ring",
+          "16x8 staged column x8, 3 planes in an LDS ring", "32x8 staged column x8, 2 planes in an LDS ring",
+          "16x12 staged column x16, 3 planes in an LDS ring"]
 
 
 def bytes_per_lup(dim, storage_bytes):
@@ -98,11 +117,13 @@ def cpu_baseline_swimmer2d(state, budget_s=12.0):
                       f"same 1200x400 swimmer workload, first {steps} steps ({el:.1f} s)"}
 
 
-# (tx, ty, cells per thread) of the 2D tiles and (tx, ty, planes, ring, threads)
+# (tx, ty, cells per thread) of the 2D tiles and (tx, ty, planes, kind, threads, planes in flight, LDS DMA)
 # of the 3D tiles, by tile index (ff_set_tile)
 SHAPES2 = [(32, 8, 1), (32, 8, 2), (64, 4, 2), (32, 8, 4), (64, 4, 4)]
-SHAPES3 = [(8, 8, 4, 0, 256), (16, 4, 4, 0, 256), (16, 8, 32, 1, 128), (16, 8, 16, 1, 128), (16, 8, 8, 1, 128),
-           (16, 16, 32, 1, 256), (16, 16, 16, 1, 256)]
+SHAPES3 = [(8, 8, 4, 0, 256, 1, 0), (16, 4, 4, 0, 256, 1, 0), (16, 8, 32, 1, 128, 1, 0), (16, 8, 16, 1, 128, 1, 0),
+           (16, 8, 8, 1, 128, 1, 0), (16, 16, 32, 1, 256, 1, 0), (16, 16, 16, 1, 256, 1, 0), (16, 12, 8, 2, 256, 1, 0),
+           (16, 26, 8, 2, 512, 1, 0), (64, 4, 16, 2, 448, 1, 0), (16, 12, 8, 2, 256, 2, 1), (16, 8, 8, 2, 192, 3, 1),
+           (32, 8, 8, 2, 384, 2, 1), (16, 12, 16, 2, 256, 3, 1)]
 
 
 def kernel2d(prefix, s, stage, periodic, tile):
@@ -110,9 +131,11 @@ def kernel2d(prefix, s, stage, periodic, tile):
     return f"ff::{prefix}<{s}, {str(stage).lower()}, {str(periodic).lower()}, {tx}, {ty}, {cpt}>"
 
 
-def kernel3d(s, periodic, tile):
-    tx, ty, tz, ring, nt = SHAPES3[tile]
-    return f"ff::k_bulk3d<{s}, {str(periodic).lower()}, {ring}, {tx}, {ty}, {tz}, {nt}>"
+def kernel3d(s, periodic, tile, fast=False):
+    tx, ty, tz, kind, nt, pf, dma = SHAPES3[tile]
+    fast = fast and kind == 2
+    return (f"ff::k_bulk3d<{s}, {str(periodic).lower()}, {str(fast).lower()}, {kind}, {tx}, {ty}, {tz}, {nt}, {pf}, "
+            f"{str(bool(dma)).lower()}>")
 
 
 def measured_traffic(kernel):
@@ -151,12 +174,12 @@ def base_line(value, K, W, ms_per_step, dtype, config, roof, cpu=None, **extra):
     return out
 
 
-def run_batched(args, capi, wl):
+def batched_summary(capi, wl, K, W, plain_steps, tile=-1, fp64_check=True):
+    """Config 4: the 64-domain FP16 batch against the plain per-domain FP32
+    form of the same domains; optionally one member's velocity error vs FP64."""
     c = wl.BATCH64
     scenes = wl.batched_scenes(c["n_domains"], c["seed"])
     cells = sum(s.nx * s.ny for s in scenes)
-    K = args.steps if args.steps != 4000 else c["steps"]
-    W = min(args.warmup, 50)
     # plain per-domain FP32 form: the reference's storage and FP16 tile, one
     # domain per stream, stepped round-robin
     plain = []
@@ -168,16 +191,16 @@ def run_batched(args, capi, wl):
         d.step(5)
     for d in plain:
         d.sync()
-    Kp = args.plain_steps
-    print(f"[bench] plain FP32 form: {len(plain)} domains, {cells} cells, {Kp} steps", file=sys.stderr, flush=True)
+    print(f"[bench] plain FP32 form: {len(plain)} domains, {cells} cells, {plain_steps} steps", file=sys.stderr,
+          flush=True)
     t0 = time.perf_counter()
-    for _ in range(Kp):
+    for _ in range(plain_steps):
         for d in plain:
             d.step(1)
     for d in plain:
         d.sync()
     t_plain = time.perf_counter() - t0
-    plain_mlups = cells * Kp / t_plain / 1e6
+    plain_mlups = cells * plain_steps / t_plain / 1e6
     for d in plain:
         d.close()
     del plain
@@ -189,8 +212,8 @@ def run_batched(args, capi, wl):
         sc.setup(d)
         b.add(d)
         doms.append(d)
-    if args.tile >= 0:
-        b.set_tile(args.tile)
+    if tile >= 0:
+        b.set_tile(tile)
     b.step(W)
     b.sync()
     print(f"[bench] batched FP16 form: warmup {W} done; timing {K} steps", file=sys.stderr, flush=True)
@@ -199,50 +222,246 @@ def run_batched(args, capi, wl):
     b.sync()
     el = time.perf_counter() - t0
     mlups = cells * K / el / 1e6
-    tile, tx, ty = b.tile()
+    t_idx, tx, ty = b.tile()
     avg_ms = b.time_bulk_kernel(100)
-    bpl = bytes_per_lup(2, 2)
-    roof = roofline(bpl * cells / (avg_ms * 1e-3) / 1e9, kernel2d("kb_step2d", "__half", False, False, tile), avg_ms,
-                    bpl, cells)
     wr = np.concatenate([d.body_wrench() for d in doms])
-    out = base_line(mlups, K, W, el * 1e3 / K, "f32 compute, f16 storage",
+    out = {"mlups": mlups, "ms_per_step": el * 1e3 / K, "steps": K, "warmup": W, "cells": cells,
+           "domains": len(scenes), "tile": t_idx, "tile_shape": [tx, ty], "avg_kernel_ms": avg_ms,
+           "plain_fp32_mlups": plain_mlups, "plain_steps": plain_steps, "speedup_vs_plain_fp32": mlups / plain_mlups,
+           "healthy": bool(np.all(np.isfinite(wr)))}
+    b.close()
+    for d in doms:
+        d.close()
+    if fp64_check:
+        out["fp16_member_error"] = cpu_baseline_batched_member(capi, wl, scenes, 1000)
+    return out
+
+
+def cpu_baseline_batched_member(capi, wl, scenes, steps):
+    """CPU leg of the batched workload: the FP64 oracle on the smallest batch
+    member, fed each step the boundary the device used, against the member
+    stepped in an FP16-storage batch; returns the velocity rel-L2 over the
+    updated cells after `steps` steps (the north star's bar is 1e-3)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    k = min(range(len(scenes)), key=lambda i: scenes[i].nx * scenes[i].ny)
+    sc = scenes[k]
+    t0 = time.perf_counter()
+    o = ol.OracleDomain(2, sc.nx, sc.ny, fp64=True, stage_fp16=False, nu=sc.viscosity, dx=sc.dx, dt=sc.dt,
+                        cf=capi.cf_of(2, sc.rho, sc.dx, sc.dt))
+    o.set_flags(sc.flags)
+    m0 = wl.equilibrium_moments(np.zeros((sc.nx * sc.ny, 2)))
+    o.set_moments(m0)
+    d = capi.Domain(**sc.domain_args(), storage="f16", stage_fp16=False)
+    sc.setup(d)
+    d.set_moments(m0)
+    b = capi.Batch(2, "f16", stage_fp16=False)
+    b.add(d)
+    v, vel = d.boundary()
+    o.set_boundary(v, vel, sc.swimmer.elems)
+    for _ in range(steps):
+        b.step(1)
+        v, vel = d.boundary()  # the kinematics the device used for this step
+        o.update_boundary(v, vel)
+        o.step(1)
+    got = d.moments(0).astype(np.float64)
+    ref = o.moments(0)
+    upd = sc.flags == 1
+    err = float(np.linalg.norm(got[upd, 1:3] - ref[upd, 1:3]) / np.linalg.norm(ref[upd, 1:3]))
+    b.close()
+    d.close()
+    return {"member": k, "nx": sc.nx, "ny": sc.ny, "steps": steps, "velocity_rel_l2_vs_fp64": err, "bar": 1e-3,
+            "seconds": round(time.perf_counter() - t0, 1)}
+
+
+def run_batched(args, capi, wl):
+    c = wl.BATCH64
+    K = args.steps if args.steps is not None else c["steps"]
+    W = min(args.warmup, 50)
+    r = batched_summary(capi, wl, K, W, args.plain_steps, args.tile, fp64_check=not args.no_cpu)
+    bpl = bytes_per_lup(2, 2)
+    roof = roofline(bpl * r["cells"] / (r["avg_kernel_ms"] * 1e-3) / 1e9,
+                    kernel2d("kb_step2d", "__half", False, False, r["tile"]), r["avg_kernel_ms"], bpl, r["cells"])
+    out = base_line(r["mlups"], K, W, r["ms_per_step"], "f32 compute, f16 storage",
                     {"workload": "64 uneven 2D swimmer domains (256x128..1200x400, 200-800 segments) in one batch, "
-                                 "FP16 storage, one graph replay per step", "domains": len(scenes),
-                     "cells": cells, "tile": [tx, ty], "parallelism": "single-gpu"},
-                    roof, None, plain_fp32_mlups=round(plain_mlups, 2),
-                    speedup_vs_plain_fp32=round(mlups / plain_mlups, 3),
-                    healthy=bool(np.all(np.isfinite(wr))))
+                                 "FP16 storage, one graph replay per step", "domains": r["domains"],
+                     "cells": r["cells"], "tile": r["tile_shape"], "parallelism": "single-gpu"},
+                    roof, None, plain_fp32_mlups=round(r["plain_fp32_mlups"], 2),
+                    speedup_vs_plain_fp32=round(r["speedup_vs_plain_fp32"], 3), healthy=r["healthy"],
+                    fp16_member_error=r.get("fp16_member_error"))
     print(json.dumps(out), flush=True)
     return 0
 
 
-def run_tg3d(args, capi, wl):
+def tg3d_time(capi, wl, m, storage, fast, K, W, tile, tune=True):
+    """Time K steps of the 3D workload (after W warm-up steps) in one form;
+    returns the rate, the chosen tile and the stream-collide launch time."""
     c = wl.TG3D
     nx, ny, nz = c["nx"], c["ny"], c["nz"]
     cells = nx * ny * nz
-    st = args.storage
-    d = capi.Domain(3, nx, ny, nz, viscosity=c["viscosity"], storage=st, periodic=True)
-    d.set_moments(wl.taylor_green_ring(nx, ny, nz, c["amp"], c["ring_r"], c["seed"]))
-    K = args.steps if args.steps != 4000 else 1000
-    W = min(args.warmup, 20)
-    tile = pick_tile(d, args)
+    d = capi.Domain(3, nx, ny, nz, viscosity=c["viscosity"], storage=storage, periodic=True, fast_math=fast)
+    d.set_moments(m)
+    if tile >= 0:
+        d.set_tile(tile)
+        t = tile
+    else:
+        t = d.autotune() if tune else 0
     m0 = d.mass()
     d.step(W)
     d.sync()
-    print(f"[bench] tg3d {st}: timing {K} steps", file=sys.stderr, flush=True)
+    print(f"[bench] tg3d {storage} {'fast' if fast else 'reference-order'} tile {t}: timing {K} steps",
+          file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     d.step(K)
     d.sync()
     el = time.perf_counter() - t0
     drift = abs(d.mass() - m0) / m0
     avg_ms = d.time_bulk_kernel(50)
-    bpl = bytes_per_lup(3, 4 if st == "f32" else 2)
-    sname = {"f32": "float", "f16": "__half", "bf16": "__hip_bfloat16"}[st]
-    roof = roofline(bpl * cells / (avg_ms * 1e-3) / 1e9, kernel3d(sname, True, tile), avg_ms, bpl, cells)
-    out = base_line(cells * K / el / 1e6, K, W, el * 1e3 / K, f"f32 compute, {st} storage",
-                    {"workload": "3D D3Q27 256x128x128 Taylor-Green (amplitude 0.04) + vortex ring r=20, periodic",
-                     "nx": nx, "ny": ny, "nz": nz, "storage": st, "tile": TILES3[tile], "parallelism": "single-gpu"},
-                    roof, None, mass_drift=drift)
+    d.close()
+    bpl = bytes_per_lup(3, 4 if storage == "f32" else 2)
+    sname = {"f32": "float", "f16": "__half", "bf16": "__hip_bfloat16"}[storage]
+    return {"mlups": cells * K / el / 1e6, "ms_per_step": el * 1e3 / K, "tile": t, "avg_kernel_ms": avg_ms,
+            "kernel": kernel3d(sname, True, t, fast), "bytes_per_lup": bpl, "cells": cells,
+            "frac": bpl * cells / (avg_ms * 1e-3) / (HBM_PEAK_GBS * 1e9), "mass_drift": drift, "steps": K + W}
+
+
+def cpu_baseline_tg3d(capi, wl, m, steps, tiles):
+    """CPU leg of the 3D workload: the FP32 CPU oracle (OpenMP, timed: the
+    cpu_baseline) and the FP64 oracle, `steps` steps on the full grid from
+    the same state, against the GPU in both arithmetic forms (`tiles`: the
+    tile each form was timed with)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    c = wl.TG3D
+    nx, ny, nz = c["nx"], c["ny"], c["nz"]
+    cells = nx * ny * nz
+    threads = ol.set_threads(0)
+    m32 = np.asarray(m, np.float32)
+    gpu = {}
+    for fast, tile in tiles.items():
+        d = capi.Domain(3, nx, ny, nz, viscosity=c["viscosity"], storage="f32", periodic=True, fast_math=fast)
+        d.set_tile(tile)
+        d.set_moments(m32)
+        d.step(steps)
+        gpu[fast] = d.moments(0).astype(np.float64)
+        d.close()
+    o = ol.OracleDomain(3, nx, ny, nz, fp64=False, stage_fp16=False, periodic=True, nu=c["viscosity"])
+    o.set_moments(m32.astype(np.float64))
+    print(f"[bench] CPU oracle FP32, {threads} threads, {steps} steps of {nx}x{ny}x{nz}", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    o.step(steps)
+    el = time.perf_counter() - t0
+    r32 = o.moments(0)
+    del o
+    o = ol.OracleDomain(3, nx, ny, nz, fp64=True, stage_fp16=False, periodic=True, nu=c["viscosity"])
+    o.set_moments(m32.astype(np.float64))
+    o.step(steps)
+    r64 = o.moments(0)
+    del o
+    vel = np.linalg.norm(r64[:, 1:4])
+    acc = {"steps": steps, "grid": [nx, ny, nz]}
+    for fast, g in gpu.items():
+        key = "fast_math" if fast else "reference_order"
+        acc[key] = {"rel_l2_vs_fp32_oracle": float(np.linalg.norm(g - r32) / np.linalg.norm(r32)),
+                    "bit_identical_fraction": float(np.mean(g == r32)),
+                    "velocity_rel_l2_vs_fp64": float(np.linalg.norm(g[:, 1:4] - r64[:, 1:4]) / vel)}
+    acc["fp32_oracle_velocity_rel_l2_vs_fp64"] = float(np.linalg.norm(r32[:, 1:4] - r64[:, 1:4]) / vel)
+    cpu = {"value": round(cells * steps / el / 1e6, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
+           "sample": f"CPU oracle (FP32 restatement of the reference update, OpenMP {threads} threads) on the same "
+                     f"{nx}x{ny}x{nz} grid, {steps} steps ({el:.1f} s)"}
+    return cpu, acc
+
+
+def swimmer2d_summary(capi, wl, K=2000, W=100, lat_steps=200):
+    """Config 2 alone: K steps with the per-step FP64 wrench kept in a device
+    ring read back after the run, then per-step latency with a host sync and
+    a wrench readback after every step."""
+    d, _ = setup_swimmer2d(capi, wl)
+    t = d.autotune()
+    c = wl.SWIMMER2D
+    cells = c["nx"] * c["ny"]
+    d.set_wrench_history(K)
+    d.step(W)
+    d.sync()
+    t0 = time.perf_counter()
+    d.step(K)
+    hist = d.wrench_history(K)
+    el = time.perf_counter() - t0
+    lat = []
+    for _ in range(lat_steps):
+        a = time.perf_counter()
+        d.step(1)
+        d.body_wrench()  # synchronises
+        lat.append((time.perf_counter() - a) * 1e3)
+    avg_ms = d.time_bulk_kernel(400)
+    d.close()
+    return {"mlups": round(cells * K / el / 1e6, 2), "ms_per_step": round(el * 1e3 / K, 5), "steps": K,
+            "tile": TILES2[t], "stream_launch_us": round(avg_ms * 1e3, 3),
+            "step_latency_ms_with_sync": {"p50": round(float(np.percentile(lat, 50)), 4),
+                                          "p99": round(float(np.percentile(lat, 99)), 4)},
+            "healthy": bool(np.all(np.isfinite(hist))),
+            "note": "1200x400 is 23.5 MB of FP32 moments: cache-resident (MALL), latency-bound, not an HBM figure"}
+
+
+def served_summary(rate=50.0, duration=4.0):
+    from freeflow_amd import serve
+    out = {}
+    for batched in (True, False):
+        eps = serve.make_episodes(rate, duration, seed=1234)
+        srv = serve.Server(storage="f16" if batched else "f32", batched=batched)
+        print(f"[bench] served {'batched FP16' if batched else 'plain FP32'}: {len(eps)} episodes at {rate}/s",
+              file=sys.stderr, flush=True)
+        r = srv.run(eps, realtime=True)
+        srv.close()
+        out["batched_fp16" if batched else "plain_fp32"] = {
+            "mlups": round(r["aggregate_mlups"], 2), "p50_step_ms": round(r["p50_step_ms"], 4),
+            "p99_step_ms": round(r["p99_step_ms"], 4), "episodes_done": r["episodes_done"],
+            "episodes_rejected": r["episodes_rejected"], "peak_active": r["peak_active"]}
+    out["speedup_vs_plain_fp32"] = round(out["batched_fp16"]["mlups"] / max(out["plain_fp32"]["mlups"], 1e-9), 3)
+    out["rate_per_s"] = rate
+    out["arrival_window_s"] = duration
+    return out
+
+
+def run_tg3d(args, capi, wl):
+    c = wl.TG3D
+    nx, ny, nz = c["nx"], c["ny"], c["nz"]
+    st = args.storage
+    K = args.steps if args.steps is not None else 1000
+    W = min(args.warmup, 20)
+    m = wl.taylor_green_ring(nx, ny, nz, c["amp"], c["ring_r"], c["seed"])
+    fast = st == "f32" and not args.reference_order or args.fast_math
+    main_ = tg3d_time(capi, wl, m, st, fast, K, W, args.tile, not args.no_tune)
+    extra = {}
+    tiles = {fast: main_["tile"]}
+    if st == "f32":
+        ref = tg3d_time(capi, wl, m, st, not fast, K, W, -1, not args.no_tune)
+        tiles[not fast] = ref["tile"]
+        extra["reference_order" if fast else "fast_math"] = {
+            "mlups": round(ref["mlups"], 2), "ms_per_step": round(ref["ms_per_step"], 5), "tile": TILES3[ref["tile"]],
+            "kernel": ref["kernel"], "avg_kernel_us": round(ref["avg_kernel_ms"] * 1e3, 3),
+            "frac": round(ref["frac"], 4)}
+    cpu = None
+    if not args.no_cpu and st == "f32":
+        cpu, acc = cpu_baseline_tg3d(capi, wl, m, args.check_steps, tiles)
+        acc["mass_drift_timed_run"] = main_["mass_drift"]
+        extra["accuracy"] = acc
+    if not args.no_extras:
+        print("[bench] extras: swimmer2d", file=sys.stderr, flush=True)
+        extra["swimmer2d"] = swimmer2d_summary(capi, wl)
+        print("[bench] extras: batched", file=sys.stderr, flush=True)
+        extra["batched"] = {k: (round(v, 5) if isinstance(v, float) else v) for k, v in
+                            batched_summary(capi, wl, 1000, 20, 100, fp64_check=not args.no_cpu).items()}
+        print("[bench] extras: served", file=sys.stderr, flush=True)
+        extra["served"] = served_summary()
+    roof = roofline(main_["bytes_per_lup"] * main_["cells"] / (main_["avg_kernel_ms"] * 1e-3) / 1e9, main_["kernel"],
+                    main_["avg_kernel_ms"], main_["bytes_per_lup"], main_["cells"])
+    arith = ("FP32, reassociated with FMA (FF_OPT_FAST_MATH)" if fast else "FP32 in the reference's operation order")
+    out = base_line(main_["mlups"], K, W, main_["ms_per_step"], f"f32 compute, {st} storage",
+                    {"workload": f"3D D3Q27 {nx}x{ny}x{nz} Taylor-Green (amplitude 0.04) + vortex ring r=20, "
+                                 "periodic, viscosity 0.01", "nx": nx, "ny": ny, "nz": nz, "storage": st,
+                     "arithmetic": arith, "tile": TILES3[main_["tile"]], "parallelism": "single-gpu"},
+                    roof, cpu, mass_drift=main_["mass_drift"], **extra)
     print(json.dumps(out), flush=True)
     return 0
 
@@ -293,14 +512,24 @@ def copy_probe(mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4000)
-    ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--workload", default="swimmer2d")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: tg3d 1000, swimmer2d 4000, batched 1000)")
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="tg3d")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tile shape instead of timing the candidates")
     ap.add_argument("--tile", type=int, default=-1, help="use this tile index instead of timing the candidates")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
-    ap.add_argument("--storage", default="f16", help="tg3d: f32, f16 or bf16")
+    ap.add_argument("--storage", default="f32", help="tg3d: f32, f16 or bf16")
+    ap.add_argument("--fast-math", action="store_true",
+                    help="tg3d: reassociated FP32 arithmetic with FMA in the column tiles (FF_OPT_FAST_MATH); the "
+                         "default for FP32 storage")
+    ap.add_argument("--reference-order", action="store_true",
+                    help="tg3d: time the reference's operation order as the headline (FP32)")
+    ap.add_argument("--check-steps", type=int, default=20,
+                    help="tg3d: steps of the full-grid oracle comparison (the CPU leg)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="tg3d: leave out the swimmer2d, batched and served figures")
     ap.add_argument("--rate", type=float, default=50.0, help="served: episode arrivals per second")
     ap.add_argument("--duration", type=float, default=5.0, help="served: arrival window in seconds")
     ap.add_argument("--plain-steps", type=int, default=200,
@@ -331,7 +560,8 @@ def main():
     tile = pick_tile(d, args)
     c = wl.SWIMMER2D
     cells = c["nx"] * c["ny"]
-    K, W = args.steps, args.warmup
+    K = args.steps if args.steps is not None else 4000
+    W = args.warmup
     d.set_wrench_history(K)
     d.step(W)
     d.sync()
@@ -369,7 +599,8 @@ def main():
         "ms_per_step": round(ms_per_step, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": "2D FSI plain FP32: D2Q9 1200x400, 600-segment elliptical swimmer with seeded "
-                               "travelling wave, outlets on all sides, per-step FP64 wrench readback",
+                               "travelling wave, outlets on all sides, per-step FP64 wrench kept in a device ring "
+                               "read back once after the K steps (moments cache-resident: not an HBM figure)",
                    "nx": c["nx"], "ny": c["ny"], "dx": c["dx"], "dt": c["dt"], "viscosity": c["viscosity"],
                    "segments": c["n_seg"], "storage": "f32", "stage_fp16": True, "tile": TILES2[tile],
                    "parallelism": "single-gpu"},

@@ -18,6 +18,7 @@ FF_OK = 0
 STORAGE = {"f32": 0, "f16": 1, "bf16": 2}
 OPT_STAGE_FP16 = 0x1
 OPT_PERIODIC = 0x2
+OPT_FAST_MATH = 0x4
 
 # fsi::lbm::LbmNodeFlag ordinals (LbmDataTypes.cuh:17-56)
 FLAG = {
@@ -144,11 +145,12 @@ class Domain:
     """One fluid domain on the GPU (the role of LbmFlowField2D/3D + the step)."""
 
     def __init__(self, dim, nx, ny, nz=1, viscosity=0.005, dx=1.0, dt=1.0, fluid_density=1.0,
-                 storage="f32", stage_fp16=None, periodic=False, device=0):
+                 storage="f32", stage_fp16=None, periodic=False, fast_math=False, device=0):
         L = lib()
         if stage_fp16 is None:
             stage_fp16 = dim == 2
-        opts = (OPT_STAGE_FP16 if stage_fp16 else 0) | (OPT_PERIODIC if periodic else 0)
+        opts = ((OPT_STAGE_FP16 if stage_fp16 else 0) | (OPT_PERIODIC if periodic else 0)
+                | (OPT_FAST_MATH if fast_math else 0))
         self.desc = DomainDesc(dim, nx, ny, nz if dim == 3 else 1, viscosity, dx, dt, fluid_density,
                                STORAGE[storage], opts, device)
         self.h = C.c_void_p()
@@ -292,10 +294,11 @@ class Domain:
 class Batch:
     """Domains of one kind stepped together, one graph replay per step."""
 
-    def __init__(self, dim, storage="f32", stage_fp16=None, periodic=False, device=0):
+    def __init__(self, dim, storage="f32", stage_fp16=None, periodic=False, fast_math=False, device=0):
         if stage_fp16 is None:
             stage_fp16 = dim == 2
-        opts = (OPT_STAGE_FP16 if (stage_fp16 and dim == 2) else 0) | (OPT_PERIODIC if periodic else 0)
+        opts = ((OPT_STAGE_FP16 if (stage_fp16 and dim == 2) else 0) | (OPT_PERIODIC if periodic else 0)
+                | (OPT_FAST_MATH if fast_math else 0))
         self.h = C.c_void_p()
         _check(lib().ff_batch_create(dim, STORAGE[storage], opts, device, C.byref(self.h)))
         self.members = []

@@ -93,6 +93,11 @@ struct Batch {
   size_t h_tab_cap = 0;
   hipEvent_t tab_ev = nullptr;
   int direct = 0;  // steps launched without a graph since the last rebuild
+  // Generation of the device tables (d_doms, d_map, d_steps): build_tables
+  // bumps it; a graph records the generation it was captured against and is
+  // never launched against another (replay checks, drop_graph forgets).
+  unsigned long long tab_gen = 0;
+  unsigned long long exec_gen = 0, exec_n_gen = 0;
 };
 
 // A rebuilt batch launches its first kDirectSteps steps directly and
@@ -124,7 +129,7 @@ __global__ void __launch_bounds__(256) kb_step2d(const DomDev* __restrict__ doms
 }
 
 // RING = 0: TX x TY x ZC box tiles; otherwise TX x TY columns marching ZC planes; NT = block size.
-template <typename S, bool PERIODIC, int RING, int TX, int TY, int ZC, int NT>
+template <typename S, bool PERIODIC, bool FAST, int KIND, int TX, int TY, int ZC, int NT, int PF, bool DMA>
 __global__ void __launch_bounds__(NT) kb_step3d(const DomDev* __restrict__ doms, const int2* __restrict__ map,
                                                 int n_bc, int n_wave) {
   const int2 m = map[blockIdx.x];
@@ -138,14 +143,17 @@ __global__ void __launch_bounds__(NT) kb_step3d(const DomDev* __restrict__ doms,
     wave_block_body<3, NT>(D->la, D->fa, m.y);
   } else {
     const int tx = D->tiles_x, ty = D->tiles_y;
-    if constexpr (RING)
-      bulk3d_ring<S, PERIODIC, TX, TY, ZC>((const S*)D->mom[p], (S*)D->mom[p ^ 1u], D->la.flags, D->la.nx, D->la.ny,
-                                           D->la.nz, D->la.plane, D->la.omega, m.y % tx, (m.y / tx) % ty,
-                                           m.y / (tx * ty));
+    const int bx = m.y % tx, by = (m.y / tx) % ty, bz = m.y / (tx * ty);
+    const S* in = (const S*)D->mom[p];
+    S* out = (S*)D->mom[p ^ 1u];
+    const LinkArgs& la = D->la;
+    if constexpr (KIND == 2)
+      bulk3d_col<S, PERIODIC, FAST, TX, TY, PF, DMA>(in, out, la.flags, la.nx, la.ny, la.nz, la.plane, la.omega, bx,
+                                                     by, bz * ZC, min(bz * ZC + ZC, la.nz));
+    else if constexpr (KIND == 1)
+      bulk3d_ring<S, PERIODIC, TX, TY, ZC>(in, out, la.flags, la.nx, la.ny, la.nz, la.plane, la.omega, bx, by, bz);
     else
-      bulk3d_tile<S, PERIODIC, TX, TY, ZC>((const S*)D->mom[p], (S*)D->mom[p ^ 1u], D->la.flags, D->la.nx, D->la.ny,
-                                           D->la.nz, D->la.plane, D->la.omega, m.y % tx, (m.y / tx) % ty,
-                                           m.y / (tx * ty));
+      bulk3d_tile<S, PERIODIC, TX, TY, ZC>(in, out, la.flags, la.nx, la.ny, la.nz, la.plane, la.omega, bx, by, bz);
   }
 }
 
@@ -193,11 +201,7 @@ __global__ void __launch_bounds__(kSumBlock) kb_detect(const DomDev* __restrict_
 
 // ---------------------------------------------------------------- launch helpers
 // Block size of the stream launch for a tile index.
-static int stream_threads(int dim, int tile) {
-  if (dim == 2) return 256;
-  const Tile3 t = kTiles3[tile];
-  return t.ring ? t.tx * t.ty : t.tx * t.ty * t.tz;
-}
+static int stream_threads(int dim, int tile) { return dim == 2 ? 256 : kTiles3[tile].nt; }
 
 template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT>
 static void step2d_batch_shape(Batch& b, const int2* map, int nblocks, int n_bc, int n_wave) {
@@ -216,23 +220,18 @@ static void step2d_batch_variant(Batch& b, int tile, const int2* map, int nblock
   }
 }
 
-template <typename S, bool PERIODIC, int RING, int TX, int TY, int ZC, int NT>
+template <typename S, bool PERIODIC, bool FAST, int KIND, int TX, int TY, int ZC, int NT, int PF, bool DMA>
 static void step3d_batch_shape(Batch& b, const int2* map, int nblocks, int n_bc, int n_wave) {
-  launch_timed(kb_step3d<S, PERIODIC, RING, TX, TY, ZC, NT>, dim3(nblocks), dim3(NT), b.stream, b.tev0, b.tev1,
-               (const DomDev*)b.d_doms, map, n_bc, n_wave);
+  launch_timed(kb_step3d<S, PERIODIC, FAST, KIND, TX, TY, ZC, NT, PF, DMA>, dim3(nblocks), dim3(NT), b.stream, b.tev0,
+               b.tev1, (const DomDev*)b.d_doms, map, n_bc, n_wave);
 }
 
-template <typename S, bool PERIODIC>
+template <typename S, bool PERIODIC, bool FAST>
 static void step3d_batch_variant(Batch& b, int tile, const int2* map, int nblocks, int n_bc, int n_wave) {
-  switch (tile) {
-    case 1: step3d_batch_shape<S, PERIODIC, 0, 16, 4, 4, 256>(b, map, nblocks, n_bc, n_wave); break;
-    case 2: step3d_batch_shape<S, PERIODIC, 1, 16, 8, 32, 128>(b, map, nblocks, n_bc, n_wave); break;
-    case 3: step3d_batch_shape<S, PERIODIC, 1, 16, 8, 16, 128>(b, map, nblocks, n_bc, n_wave); break;
-    case 4: step3d_batch_shape<S, PERIODIC, 1, 16, 8, 8, 128>(b, map, nblocks, n_bc, n_wave); break;
-    case 5: step3d_batch_shape<S, PERIODIC, 1, 16, 16, 32, 256>(b, map, nblocks, n_bc, n_wave); break;
-    case 6: step3d_batch_shape<S, PERIODIC, 1, 16, 16, 16, 256>(b, map, nblocks, n_bc, n_wave); break;
-    default: step3d_batch_shape<S, PERIODIC, 0, 8, 8, 4, 256>(b, map, nblocks, n_bc, n_wave); break;
-  }
+#define FF_SB3(FA, K, X, Y, Z, N, P, DM) \
+  step3d_batch_shape<S, PERIODIC, FA, K, X, Y, Z, N, P, DM>(b, map, nblocks, n_bc, n_wave)
+  FF_TILES3(FF_SB3, FAST, tile)
+#undef FF_SB3
 }
 
 // The stream launch over `map` (n_bc boundary-cell blocks, n_wave wave blocks, then tiles).
@@ -245,10 +244,12 @@ static void launch_stream_batch(Batch& b, int tile, const int2* map, int nblocks
     else if (stage) step2d_batch_variant<S, true, false>(b, tile, map, nblocks, n_bc, n_wave);
     else if (per) step2d_batch_variant<S, false, true>(b, tile, map, nblocks, n_bc, n_wave);
     else step2d_batch_variant<S, false, false>(b, tile, map, nblocks, n_bc, n_wave);
-  } else if (per) {
-    step3d_batch_variant<S, true>(b, tile, map, nblocks, n_bc, n_wave);
   } else {
-    step3d_batch_variant<S, false>(b, tile, map, nblocks, n_bc, n_wave);
+    const bool fast = b.options & FF_OPT_FAST_MATH;
+    if (per && fast) step3d_batch_variant<S, true, true>(b, tile, map, nblocks, n_bc, n_wave);
+    else if (per) step3d_batch_variant<S, true, false>(b, tile, map, nblocks, n_bc, n_wave);
+    else if (fast) step3d_batch_variant<S, false, true>(b, tile, map, nblocks, n_bc, n_wave);
+    else step3d_batch_variant<S, false, false>(b, tile, map, nblocks, n_bc, n_wave);
   }
 }
 
@@ -405,6 +406,7 @@ static int build_tables(Batch& b, int tile) {
   if (nm) FF_HIP(hipMemcpyAsync(b.d_map, b.h_tab + nd, nm, hipMemcpyHostToDevice, b.stream));
   FF_HIP(hipMemsetAsync(b.d_steps, 0, std::max(n, 1) * sizeof(unsigned), b.stream));
   FF_HIP(hipEventRecord(b.tab_ev, b.stream));
+  b.tab_gen++;
   return FF_OK;
 }
 
@@ -463,6 +465,7 @@ static void drop_graph(Batch& b) {
   b.graph = nullptr;
   b.exec_n = nullptr;
   b.graph_n = nullptr;
+  b.exec_gen = b.exec_n_gen = 0;
 }
 
 static int current_tile(const Batch& b) { return b.tile >= 0 ? b.tile : std::max(b.tuned_tile, 0); }
@@ -499,11 +502,16 @@ static int replay(Batch& b, int nsteps) {
     }
     const int rc = capture_steps(b, 1, &b.graph, &b.exec);
     if (rc) return rc;
+    b.exec_gen = b.tab_gen;
   }
   if (nsteps >= kGraphSteps && !b.exec_n) {
     const int rc = capture_steps(b, kGraphSteps, &b.graph_n, &b.exec_n);
     if (rc) return rc;
+    b.exec_n_gen = b.tab_gen;
   }
+  // a graph's kernel arguments point into the tables it was captured with
+  if (b.exec_gen != b.tab_gen || (b.exec_n && b.exec_n_gen != b.tab_gen))
+    return fail(FF_ERR_STATE, "internal: batch graph captured against replaced tables");
   for (; s + kGraphSteps <= nsteps; s += kGraphSteps) FF_HIP(hipGraphLaunch(b.exec_n, b.stream));
   for (; s < nsteps; s++) FF_HIP(hipGraphLaunch(b.exec, b.stream));
   return FF_OK;
@@ -621,7 +629,8 @@ int ff_batch_create(int dim, int storage, unsigned options, int device, ff_batch
   *out = nullptr;
   if (dim != 2 && dim != 3) return fail(FF_ERR_INVALID, "dim must be 2 or 3");
   if (storage < 0 || storage > 2) return fail(FF_ERR_INVALID, "unknown storage format");
-  if (options & ~(FF_OPT_STAGE_FP16 | FF_OPT_PERIODIC)) return fail(FF_ERR_INVALID, "unknown option bits");
+  if (options & ~(FF_OPT_STAGE_FP16 | FF_OPT_PERIODIC | FF_OPT_FAST_MATH))
+    return fail(FF_ERR_INVALID, "unknown option bits");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(FF_ERR_NODEVICE, "no HIP device available");
   if (device < 0 || device >= ndev) return fail(FF_ERR_INVALID, "device ordinal out of range");

@@ -256,18 +256,87 @@ static unsigned long long link_estimate(int D, double dx, int ne, const float* v
   return count;
 }
 
-// Device bytes a domain allocates: moments, flags, scratch, and with a
-// boundary its geometry, reductions and link table (as ff_set_boundary sizes it).
-static unsigned long long footprint(int dim, long long n, int storage, int nv, int ne, unsigned long long est) {
-  const int nm = dim == 2 ? 6 : 10, q = dim == 2 ? 9 : 27, D = dim;
-  unsigned long long b = 2ull * plane_stride(n) * nm * storage_bytes(storage) + (unsigned long long)n + 1024 * 8;
+// The core allocation of a domain: both moment buffers, the flags and the
+// reduction scratch.  Returns the offset of the flags.
+static size_t plan_core(Carve& cv, Domain& d, unsigned char** mom0, unsigned char** mom1) {
+  const size_t bytes = d.plane * d.nm * storage_bytes(d.storage);
+  cv.add(mom0, bytes);
+  cv.add(mom1, bytes);
+  const size_t flags_off = cv.add(&d.flags, (size_t)d.n);
+  cv.add(&d.red, 1024);
+  return flags_off;
+}
+
+// The one allocation behind ff_set_boundary: region 1 is initialised from
+// the host, region 2 cleared, region 3 written by the kernels before any read.
+struct BoundaryPlan {
+  size_t o_v[2], o_u[2], o_el, o_eb, o_vb, o_bp, o_bi, o_vp, o_vi, o_rf, o_st;
+  size_t init_end, zero_end;
+};
+static BoundaryPlan plan_boundary(Carve& cv, Domain& d, int nv, int ne, int n_bodies, size_t cap) {
+  const int D = d.dim;
+  const size_t K = (size_t)(d.q - 1);
+  const size_t vcount = (size_t)std::max(nv, 1) * D;
+  BoundaryPlan p;
+  for (int b = 0; b < 2; b++) {
+    p.o_v[b] = cv.add(&d.vbuf[b], vcount);
+    p.o_u[b] = cv.add(&d.ubuf[b], vcount);
+  }
+  p.o_el = cv.add(&d.elems, (size_t)std::max(ne, 1) * D);
+  p.o_eb = cv.add(&d.elem_body, (size_t)std::max(ne, 1));
+  p.o_vb = cv.add(&d.vert_body, (size_t)std::max(nv, 1));
+  p.o_bp = cv.add(&d.body_elem_ptr, (size_t)n_bodies + 1);
+  p.o_bi = cv.add(&d.body_elem_idx, (size_t)std::max(ne, 1));
+  p.o_vp = cv.add(&d.vert_inc_ptr, (size_t)nv + 1);
+  p.o_vi = cv.add(&d.vert_inc_idx, std::max<size_t>((size_t)ne * D, 1));
+  p.o_rf = cv.add(&d.body_ref, (size_t)n_bodies * D);
+  p.o_st = cv.add(&d.st, 1);
+  p.init_end = cv.total;
+  // region 2 (generation 0 marks every table entry empty; steps run from generation 1)
+  cv.add(&d.vforce, vcount);
+  cv.add(&d.wrench, (size_t)n_bodies * wrench_width(D));
+  cv.add(&d.lt_cells, 2 * cap);  // two tables (cells and keys), alternating by tag parity
+  cv.add(&d.lt_keys, 2 * cap * K);
+  p.zero_end = cv.total;
+  // region 3
+  cv.add(&d.elem_acc, (size_t)std::max(ne, 1) * elem_stride(D));
+  cv.add(&d.lk_f, cap * K * D);
+  cv.add(&d.lk_b, cap * K * D);
+  cv.add(&d.lk_h, cap * K * D);
+  cv.add(&d.lk_sb, cap * d.nm);
+  // wave kinematics inputs (ff_set_wave fills them when it is used)
+  cv.add(&d.rest_verts, vcount);
+  cv.add(&d.s_coord, (size_t)std::max(nv, 1));
+  cv.add(&d.wave_params, (size_t)n_bodies * 4);
+  return p;
+}
+
+// hipMalloc hands out whole pages of the device allocator: each of a
+// domain's allocations is counted rounded up to 2 MiB.
+static unsigned long long alloc_rounded(size_t bytes) { return (bytes + (2u << 20) - 1) / (2u << 20) * (2ull << 20); }
+
+// Device bytes a domain allocates: its core allocation and, with a boundary,
+// the ff_set_boundary allocation, laid out by the same plans the allocating
+// calls use, with n_bodies bodies (the per-body arrays; ne bounds it).  A
+// wrench history ring adds its own allocation: capacity x bodies x 3 (2D) or
+// 6 (3D) doubles.
+static unsigned long long footprint(int dim, long long n, int storage, int nv, int ne, int n_bodies,
+                                    unsigned long long est) {
+  Domain d;
+  d.dim = dim;
+  d.nm = dim == 2 ? 6 : 10;
+  d.q = dim == 2 ? 9 : 27;
+  d.storage = storage;
+  d.n = n;
+  d.plane = plane_stride(n);
+  unsigned char *m0 = nullptr, *m1 = nullptr;
+  Carve core;
+  plan_core(core, d, &m0, &m1);
+  unsigned long long b = alloc_rounded(core.total);
   if (nv > 0 || ne > 0) {
-    const unsigned long long K = q - 1, cap = next_pow2(est * 2 + 1024);
-    b += 5ull * nv * D * 4 + (unsigned long long)ne * (D * 4 + 4 + 4 + D * 4 + (D * D + D + 3) * 8);
-    b += (unsigned long long)(nv + 1) * 4 + 64;
-    b += cap * (2 * 8 + 2 * K * 8 + 3 * K * D * 4 + nm * 4) + 64;
-    b += (unsigned long long)nv * (D + 1) * 4 + 32;  // wave inputs (rest positions, arc length, parameters)
-    b += 26ull * 256;                                // alignment of the arrays within the allocation
+    Carve bc;
+    plan_boundary(bc, d, nv, ne, std::max(n_bodies, 1), (size_t)next_pow2(est * 2 + 1024));
+    b += alloc_rounded(bc.total);
   }
   return b;
 }
@@ -397,7 +466,8 @@ int ff_domain_create(const ff_domain_desc* desc, ff_domain** out) {
   const long long n = (long long)desc->nx * desc->ny * nz;
   if (!(desc->viscosity > 0.0)) return fail(FF_ERR_INVALID, "viscosity must be positive");
   if (!(desc->dt > 0.0)) return fail(FF_ERR_INVALID, "dt must be positive");
-  if (desc->options & ~(FF_OPT_STAGE_FP16 | FF_OPT_PERIODIC)) return fail(FF_ERR_INVALID, "unknown option bits");
+  if (desc->options & ~(FF_OPT_STAGE_FP16 | FF_OPT_PERIODIC | FF_OPT_FAST_MATH))
+    return fail(FF_ERR_INVALID, "unknown option bits");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(FF_ERR_NODEVICE, "no HIP device available");
   if (desc->device < 0 || desc->device >= ndev) return fail(FF_ERR_INVALID, "device ordinal out of range");
@@ -436,15 +506,11 @@ int ff_domain_create(const ff_domain_desc* desc, ff_domain** out) {
   d.h_flags.assign((size_t)n, kFluid);
   hipError_t e = acquire_stream(d.device, &d.stream);
   if (e != hipSuccess) { delete dd; return hip_fail(e, "hipStreamCreate"); }
-  const size_t bytes = d.plane * d.nm * storage_bytes(d.storage);
   // one allocation: both moment buffers (cleared), the flags (Fluid) and the
   // reduction scratch
   Carve cv;
   unsigned char* mom_bytes[2] = {nullptr, nullptr};
-  cv.add(&mom_bytes[0], bytes);
-  cv.add(&mom_bytes[1], bytes);
-  const size_t flags_off = cv.add(&d.flags, (size_t)n);
-  cv.add(&d.red, 1024);
+  const size_t flags_off = plan_core(cv, d, &mom_bytes[0], &mom_bytes[1]);
   int rc = cv.commit(&d.core_mem);
   if (!rc) {
     d.mom[0] = mom_bytes[0];
@@ -616,64 +682,30 @@ int ff_set_boundary(ff_domain* dd, int nv, const float* verts, const float* vels
   // link table capacity: distinct candidate cells near the mesh, with margin
   const unsigned long long est = link_estimate(D, d.dx, ne, verts, elems);
   d.lt_cap = (int)next_pow2(est * 2 + 1024);
-  const size_t K = (size_t)(d.q - 1);
-  const size_t cap = (size_t)d.lt_cap;
-  const size_t vcount = (size_t)std::max(nv, 1) * D;
   Carve cv;
-  // region 1: initialised from the host
-  size_t o_v[2], o_u[2];
-  for (int b = 0; b < 2; b++) {
-    o_v[b] = cv.add(&d.vbuf[b], vcount);
-    o_u[b] = cv.add(&d.ubuf[b], vcount);
-  }
-  const size_t o_el = cv.add(&d.elems, (size_t)std::max(ne, 1) * D);
-  const size_t o_eb = cv.add(&d.elem_body, (size_t)std::max(ne, 1));
-  const size_t o_vb = cv.add(&d.vert_body, (size_t)std::max(nv, 1));
-  const size_t o_bp = cv.add(&d.body_elem_ptr, bptr.size());
-  const size_t o_bi = cv.add(&d.body_elem_idx, (size_t)std::max(ne, 1));
-  const size_t o_vp = cv.add(&d.vert_inc_ptr, vptr.size());
-  const size_t o_vi = cv.add(&d.vert_inc_idx, std::max<size_t>(vinc.size(), 1));
-  const size_t o_rf = cv.add(&d.body_ref, ref.size());
-  const size_t o_st = cv.add(&d.st, 1);
-  const size_t init_end = cv.total;
-  // region 2: cleared (generation 0 marks every table entry empty; steps run
-  // from generation 1)
-  cv.add(&d.vforce, vcount);
-  cv.add(&d.wrench, (size_t)n_bodies * wrench_width(D));
-  cv.add(&d.lt_cells, 2 * cap);  // two tables (cells and keys), alternating by tag parity
-  cv.add(&d.lt_keys, 2 * cap * K);
-  const size_t zero_end = cv.total;
-  // region 3: written by the kernels before any read
-  cv.add(&d.elem_acc, (size_t)std::max(ne, 1) * elem_stride(D));
-  cv.add(&d.lk_f, cap * K * D);
-  cv.add(&d.lk_b, cap * K * D);
-  cv.add(&d.lk_h, cap * K * D);
-  cv.add(&d.lk_sb, cap * d.nm);
-  // wave kinematics inputs (ff_set_wave fills them when it is used)
-  cv.add(&d.rest_verts, vcount);
-  cv.add(&d.s_coord, (size_t)std::max(nv, 1));
-  cv.add(&d.wave_params, (size_t)n_bodies * 4);
+  const BoundaryPlan bp = plan_boundary(cv, d, nv, ne, n_bodies, (size_t)d.lt_cap);
+  const size_t init_end = bp.init_end, zero_end = bp.zero_end;
   if ((rc = cv.commit(&d.bnd_mem))) return rc;
   std::vector<unsigned char> host(init_end, 0);
   auto put = [&](size_t off, const void* src, size_t bytes) {
     if (bytes) std::memcpy(host.data() + off, src, bytes);
   };
   for (int b = 0; b < 2; b++) {
-    put(o_v[b], verts, (size_t)nv * D * sizeof(float));
-    put(o_u[b], vels, (size_t)nv * D * sizeof(float));
+    put(bp.o_v[b], verts, (size_t)nv * D * sizeof(float));
+    put(bp.o_u[b], vels, (size_t)nv * D * sizeof(float));
   }
-  put(o_el, elems, (size_t)ne * D * sizeof(uint32_t));
-  put(o_eb, eb.data(), (size_t)ne * sizeof(int));
-  put(o_vb, vb.data(), (size_t)nv * sizeof(int));
-  put(o_bp, bptr.data(), bptr.size() * sizeof(int));
-  put(o_bi, bidx.data(), bidx.size() * sizeof(int));
-  put(o_vp, vptr.data(), vptr.size() * sizeof(int));
-  put(o_vi, vinc.data(), vinc.size() * sizeof(int));
-  put(o_rf, ref.data(), ref.size() * sizeof(double));
+  put(bp.o_el, elems, (size_t)ne * D * sizeof(uint32_t));
+  put(bp.o_eb, eb.data(), (size_t)ne * sizeof(int));
+  put(bp.o_vb, vb.data(), (size_t)nv * sizeof(int));
+  put(bp.o_bp, bptr.data(), bptr.size() * sizeof(int));
+  put(bp.o_bi, bidx.data(), bidx.size() * sizeof(int));
+  put(bp.o_vp, vptr.data(), vptr.size() * sizeof(int));
+  put(bp.o_vi, vinc.data(), vinc.size() * sizeof(int));
+  put(bp.o_rf, ref.data(), ref.size() * sizeof(double));
   StepState st0{};
   st0.gen = 1;
   st0.tag = 1;
-  put(o_st, &st0, sizeof(st0));
+  put(bp.o_st, &st0, sizeof(st0));
   unsigned char* base = (unsigned char*)d.bnd_mem;
   FF_HIP(hipMemcpyAsync(base, host.data(), init_end, hipMemcpyHostToDevice, d.stream));
   FF_HIP(hipMemsetAsync(base + init_end, 0, zero_end - init_end, d.stream));
@@ -689,6 +721,7 @@ int ff_update_boundary(ff_domain* dd, const float* verts, const float* vels) {
   Domain& d = *dd;
   if (!d.has_boundary) return fail(FF_ERR_STATE, "no boundary set");
   if (!verts || !vels) return fail(FF_ERR_INVALID, "null boundary array");
+  if (int rc = check_verts(d.dim, d.dx, d.nv, verts)) return rc;
   FF_HIP(hipSetDevice(d.device));
   const size_t bytes = (size_t)d.nv * d.dim * sizeof(float);
   for (int b = 0; b < 2; b++) {
@@ -989,7 +1022,7 @@ int ff_domain_footprint(const ff_domain_desc* desc, int nv, int ne, const float*
   if (ne > 0)
     if (int rc = check_verts(desc->dim, desc->dx, nv, verts)) return rc;
   const unsigned long long est = ne ? link_estimate(desc->dim, desc->dx, ne, verts, elems) : 0;
-  *bytes = footprint(desc->dim, n, desc->storage, nv, ne, est);
+  *bytes = footprint(desc->dim, n, desc->storage, nv, ne, ne, est);
   return FF_OK;
 }
 

@@ -51,13 +51,13 @@ struct D3 {
   __host__ __device__ static constexpr int inv(int i) { return i == 0 ? 0 : (i & 1 ? i + 1 : i - 1); }
 };
 
-__device__ __forceinline__ bool is_wall2(uint8_t f) {  // FsiCoupling2D.cu:171-176
-  return f == 3 || f == 6 || f == 7 || f == 11 || f == 10 || f == 5;
-}
-__device__ __forceinline__ bool is_wall3(uint8_t f) {  // FsiCoupling3D.cu:209-216
-  return f == 3 || f == 5 || f == 7 || f == 11 || f == 10 || f == 8 || f == 9 || f == 6;
-}
-__device__ __forceinline__ bool is_updated(uint8_t f) { return f == 1 || f == 4; }  // Fluid, SolidDynamic
+// Flag sets as bit masks over the flag value (one shift and test, no branch chain).
+// FsiCoupling2D.cu:171-176: Solid 3, Wall 5, WallLeft 6, WallRight 7, WallDown 10, WallUp 11
+__device__ __forceinline__ bool is_wall2(uint8_t f) { return f < 12 && ((0xCE8u >> f) & 1u); }
+// FsiCoupling3D.cu:209-216: Solid 3, Wall 5, WallLeft..WallUp 6-11
+__device__ __forceinline__ bool is_wall3(uint8_t f) { return f < 12 && ((0xFE8u >> f) & 1u); }
+// Fluid 1, SolidDynamic 4
+__device__ __forceinline__ bool is_updated(uint8_t f) { return f < 5 && ((0x12u >> f) & 1u); }
 
 // Materialise an FP32 value in a register.  Without it the backend may fuse the
 // preceding multiply into a float->half conversion (v_fma_mix*), rounding the
@@ -323,6 +323,134 @@ __device__ __forceinline__ void collide3(const float* p, float omega, float* out
   out[7] = pixy * inv;
   out[8] = piyz * inv;
   out[9] = pixz * inv;
+}
+
+// ------------------------------------------------------------- 3D, fast form
+// The same reconstruction and relaxation written for instruction count rather
+// than the reference's rounding: opposite directions share their even part E
+// (rho and second-order terms) and differ in the sign of their odd part O
+// (first- and third-order terms), f_i = w (E + O), f_inv(i) = w (E - O); the
+// moments of the pulled populations come from the 13 pair sums and
+// differences; products and sums are contracted to FMA.  Algebraically equal
+// to recon3_all / collide3 term by term (every grouping below is written out
+// in the comments); only the FP32 rounding differs.
+__device__ __forceinline__ void recon3_fast(const float* m, float* f) {
+  const float rho = m[0], ux = m[1], uy = m[2], uz = m[3];
+  const float pxx = m[4], pyy = m[5], pzz = m[6], pxy = m[7], pyz = m[8], pxz = m[9];
+  const float r3 = 3.0f * rho, r9 = 9.0f * rho;
+  const float m1x = r3 * ux, m1y = r3 * uy, m1z = r3 * uz;
+  const float m2xx = r3 * pxx, m2yy = r3 * pyy, m2zz = r3 * pzz;
+  const float m2xy = r9 * pxy, m2yz = r9 * pyz, m2xz = r9 * pxz;
+  // Q_aab = 9 rho (P_aa u_b + 2 P_ab u_a - 2 u_a^2 u_b) = 9 rho (u_b D_a + 2 P_ab u_a), D_a = P_aa - 2 u_a^2
+  const float Dx = fmaf(-2.0f * ux, ux, pxx), Dy = fmaf(-2.0f * uy, uy, pyy), Dz = fmaf(-2.0f * uz, uz, pzz);
+  const float txy = 2.0f * pxy, tyz = 2.0f * pyz, txz = 2.0f * pxz;
+  const float Qxxy = r9 * fmaf(uy, Dx, txy * ux);
+  const float Qxyy = r9 * fmaf(ux, Dy, txy * uy);
+  const float Qxxz = r9 * fmaf(uz, Dx, txz * ux);
+  const float Qxzz = r9 * fmaf(ux, Dz, txz * uz);
+  const float Qyyz = r9 * fmaf(uz, Dy, tyz * uy);
+  const float Qyzz = r9 * fmaf(uy, Dz, tyz * uz);
+  // Q_xyz = 27 rho (P_xy u_z + P_yz u_x + P_xz u_y - 2 u_x u_y u_z)
+  const float Qxyz = (3.0f * r9) * fmaf(pxy, uz, fmaf(pyz, ux, fmaf(pxz, uy, (-2.0f * ux) * (uy * uz))));
+  const float cS = m2xx + m2yy + m2zz;
+  const float bulk = fmaf(-0.5f, cS, rho);  // rho - (m2xx + m2yy + m2zz) / 2
+  const float cc = rho + cS;
+  f[0] = FF_W0_3 * bulk;
+  // axes: E = bulk + 3/2 m2_aa, O = m1_a - (Q_abb + Q_acc) / 2
+  const float S1 = Qxyy + Qxzz, S2 = Qxxy + Qyzz, S3 = Qxxz + Qyyz;
+  {
+    const float wx = FF_W1_3 * fmaf(-0.5f, S1, m1x), wy = FF_W1_3 * fmaf(-0.5f, S2, m1y),
+                wz = FF_W1_3 * fmaf(-0.5f, S3, m1z);
+    const float ex = fmaf(1.5f, m2xx, bulk), ey = fmaf(1.5f, m2yy, bulk), ez = fmaf(1.5f, m2zz, bulk);
+    f[1] = fmaf(FF_W1_3, ex, wx);
+    f[2] = fmaf(FF_W1_3, ex, -wx);
+    f[3] = fmaf(FF_W1_3, ey, wy);
+    f[4] = fmaf(FF_W1_3, ey, -wy);
+    f[5] = fmaf(FF_W1_3, ez, wz);
+    f[6] = fmaf(FF_W1_3, ez, -wz);
+  }
+  // edges in the ab plane: E = rho + m2_aa + m2_bb - m2_cc / 2 +- m2_ab (= cc - 3/2 m2_cc +- m2_ab),
+  // O = H_a +- H_b with H_a = m1_a + Q_abb - Q_acc / 2, H_b = m1_b + Q_aab - Q_bcc / 2
+  auto edge = [&](float eb, float m2ab, float ha, float hb, float* fp, float* fpn, float* fm, float* fmn) {
+    const float op = FF_W2_3 * (ha + hb), om = FF_W2_3 * (ha - hb);
+    const float ep = eb + m2ab, em = eb - m2ab;
+    *fp = fmaf(FF_W2_3, ep, op);
+    *fpn = fmaf(FF_W2_3, ep, -op);
+    *fm = fmaf(FF_W2_3, em, om);
+    *fmn = fmaf(FF_W2_3, em, -om);
+  };
+  edge(fmaf(-1.5f, m2zz, cc), m2xy, fmaf(-0.5f, Qxzz, m1x + Qxyy), fmaf(-0.5f, Qyzz, m1y + Qxxy), &f[7], &f[8],
+       &f[13], &f[14]);
+  edge(fmaf(-1.5f, m2yy, cc), m2xz, fmaf(-0.5f, Qxyy, m1x + Qxzz), fmaf(-0.5f, Qyyz, m1z + Qxxz), &f[9], &f[10],
+       &f[15], &f[16]);
+  edge(fmaf(-1.5f, m2xx, cc), m2yz, fmaf(-0.5f, Qxxy, m1y + Qyzz), fmaf(-0.5f, Qxxz, m1z + Qyyz), &f[11], &f[12],
+       &f[17], &f[18]);
+  // corners (sx, sy, sz): E = cc + sx sy m2_xy + sy sz m2_yz + sx sz m2_xz,
+  // O = sx G_x + sy G_y + sz G_z + sx sy sz Q_xyz with G_a = m1_a + (sum of Q_a.. of axis a)
+  const float Gx = m1x + S1, Gy = m1y + S2, Gz = m1z + S3;
+  const float a = cc + m2xy, b = cc - m2xy, c = m2yz + m2xz, d = m2yz - m2xz;
+  const float gp = Gx + Gy, gm = Gx - Gy, g1 = Gz + Qxyz, g2 = Gz - Qxyz;
+  auto corner = [&](float e, float o, float* fp, float* fn) {
+    const float wo = FF_W3_3 * o;
+    *fp = fmaf(FF_W3_3, e, wo);
+    *fn = fmaf(FF_W3_3, e, -wo);
+  };
+  corner(a + c, gp + g1, &f[19], &f[20]);  // (+,+,+)
+  corner(a - c, gp - g1, &f[21], &f[22]);  // (+,+,-)
+  corner(b - d, gm + g2, &f[23], &f[24]);  // (+,-,+)
+  corner(b + d, g2 - gm, &f[25], &f[26]);  // (-,+,+)
+}
+
+// collide3 in the fast form: moments from pair sums s and differences d of
+// opposite populations, then the same relaxation (F = 0) and normalisation.
+__device__ __forceinline__ void collide3_fast(const float* p, float omega, float* out) {
+  float s[13], d[13];  // pair k = directions (2k+1, 2k+2)
+#pragma unroll
+  for (int k = 0; k < 13; k++) {
+    s[k] = p[2 * k + 1] + p[2 * k + 2];
+    d[k] = p[2 * k + 1] - p[2 * k + 2];
+  }
+  // pairs: 0 x, 1 y, 2 z, 3 (+x+y), 4 (+x+z), 5 (+y+z), 6 (+x-y), 7 (+x-z), 8 (+y-z),
+  //        9 (+,+,+), 10 (+,+,-), 11 (+,-,+), 12 (-,+,+)
+  const float exy = s[3] + s[6], exz = s[4] + s[7], eyz = s[5] + s[8];
+  const float c1 = s[9] + s[10], c2 = s[11] + s[12], c3 = s[9] - s[10], c4 = s[11] - s[12];
+  const float C = c1 + c2;
+  const float rho = ((p[0] + C) + (s[0] + s[1] + s[2])) + (exy + exz + eyz);
+  const float pxx = s[0] + exy + exz + C;  // raw sums of c_a c_b f
+  const float pyy = s[1] + exy + eyz + C;
+  const float pzz = s[2] + exz + eyz + C;
+  const float pxy = (s[3] - s[6]) + (c1 - c2);
+  const float pyz = (s[5] - s[8]) + (c3 - c4);
+  const float pxz = (s[4] - s[7]) + (c3 + c4);
+  const float e1 = d[9] + d[10], e2 = d[11] - d[12], e3 = d[9] - d[10], e4 = d[11] + d[12];
+  const float jx = (d[0] + (d[3] + d[6])) + ((d[4] + d[7]) + (e1 + e2));
+  const float jy = (d[1] + (d[3] - d[6])) + ((d[5] + d[8]) + (e1 - e2));
+  const float jz = (d[2] + (d[4] - d[7])) + ((d[5] - d[8]) + (e3 + e4));
+  const float inv = __builtin_amdgcn_rcpf(rho);  // v_rcp_f32, 1 ulp
+  const float ux = jx * inv, uy = jy * inv, uz = jz * inv;
+  // Collision (LbmUtilsFunc.cu:337-414), F = 0: trace at rho/3 + rho|u|^2/3 per axis,
+  // deviatoric and off-diagonal parts relaxed by (1 - omega) towards rho u_a u_b
+  const float third = 1.0f / 3.0f, omw = 1.0f - omega;
+  const float rux = jx * ux, ruy = jy * uy, ruz = jz * uz;
+  const float t3 = (pxx + pyy + pzz) * third, r3 = (rux + ruy + ruz) * third;
+  const float base = fmaf(rho, third, r3);
+  const float nxx = fmaf(omw, pxx - t3, fmaf(omega, rux - r3, base));
+  const float nyy = fmaf(omw, pyy - t3, fmaf(omega, ruy - r3, base));
+  const float nzz = fmaf(omw, pzz - t3, fmaf(omega, ruz - r3, base));
+  const float wux = omega * ux, wuy = omega * uy;
+  const float nxy = fmaf(omw, pxy, wux * jy);
+  const float nxz = fmaf(omw, pxz, wux * jz);
+  const float nyz = fmaf(omw, pyz, wuy * jz);
+  out[0] = rho;
+  out[1] = ux;
+  out[2] = uy;
+  out[3] = uz;
+  out[4] = fmaf(nxx, inv, -third);
+  out[5] = fmaf(nyy, inv, -third);
+  out[6] = fmaf(nzz, inv, -third);
+  out[7] = nxy * inv;
+  out[8] = nyz * inv;
+  out[9] = nxz * inv;
 }
 
 // -------------------------------------------------------------------- storage

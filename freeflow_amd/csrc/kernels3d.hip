@@ -7,55 +7,60 @@
 // flags are staged next to the populations.  The body-force field of the
 // reference is identically zero (LbmFlowField3D.cpp:35-36) and is not read.
 
+#include <algorithm>
+
 #include "ff_internal.h"
 #include "links_body.h"
 #include "stencil.h"
 
 namespace ff {
 
-// One tile per block (XCD-aware order): RING = 0 boxes of TX x TY x ZC cells,
-// otherwise TX x TY columns marching ZC planes; NT = block size.
-template <typename S, bool PERIODIC, int RING, int TX, int TY, int ZC, int NT>
+// One tile per block (XCD-aware order).  KIND 0: boxes of TX x TY x ZC cells;
+// 1: TX x TY columns marching ZC planes (bulk3d_ring); 2: TX x TY columns
+// marching ZC planes with one staged cell per thread (bulk3d_col: FAST picks
+// its arithmetic, PF and DMA how the next planes' moments are fetched).
+template <typename S, bool PERIODIC, bool FAST, int KIND, int TX, int TY, int ZC, int PF, bool DMA>
 __device__ __forceinline__ void tile3d(const S* in, S* out, const uint8_t* flags, int nx, int ny, int nz, size_t plane,
                                        float omega, int t, int tiles_x, int tiles_y) {
   const int bx = t % tiles_x, by = (t / tiles_x) % tiles_y, bz = t / (tiles_x * tiles_y);
-  if constexpr (RING) bulk3d_ring<S, PERIODIC, TX, TY, ZC>(in, out, flags, nx, ny, nz, plane, omega, bx, by, bz);
-  else bulk3d_tile<S, PERIODIC, TX, TY, ZC>(in, out, flags, nx, ny, nz, plane, omega, bx, by, bz);
+  if constexpr (KIND == 2)
+    bulk3d_col<S, PERIODIC, FAST, TX, TY, PF, DMA>(in, out, flags, nx, ny, nz, plane, omega, bx, by, bz * ZC,
+                                                   min(bz * ZC + ZC, nz));
+  else if constexpr (KIND == 1)
+    bulk3d_ring<S, PERIODIC, TX, TY, ZC>(in, out, flags, nx, ny, nz, plane, omega, bx, by, bz);
+  else
+    bulk3d_tile<S, PERIODIC, TX, TY, ZC>(in, out, flags, nx, ny, nz, plane, omega, bx, by, bz);
 }
 
-template <typename S, bool PERIODIC, int RING, int TX, int TY, int ZC, int NT>
-__global__ void __launch_bounds__(NT)
+// Staged columns are built for four waves per SIMD (at most 128 VGPRs); with
+// an LDS ring of moments the LDS allows two or three.
+template <typename S, bool PERIODIC, bool FAST, int KIND, int TX, int TY, int ZC, int NT, int PF, bool DMA>
+__global__ void __launch_bounds__(NT, (KIND == 2 && !DMA ? 4 : 2))
     k_bulk3d(const S* __restrict__ in, S* __restrict__ out, const uint8_t* __restrict__ flags, int nx, int ny,
              int nz, size_t plane, float omega, int tiles_x, int tiles_y, int ntiles) {
   const int t = xcd_tile(blockIdx.x, ntiles);
   if (t < 0) return;
-  tile3d<S, PERIODIC, RING, TX, TY, ZC, NT>(in, out, flags, nx, ny, nz, plane, omega, t, tiles_x, tiles_y);
+  tile3d<S, PERIODIC, FAST, KIND, TX, TY, ZC, PF, DMA>(in, out, flags, nx, ny, nz, plane, omega, t, tiles_x, tiles_y);
 }
 
-template <typename S, bool PERIODIC, int RING, int TX, int TY, int ZC, int NT>
+template <typename S, bool PERIODIC, bool FAST, int KIND, int TX, int TY, int ZC, int NT, int PF, bool DMA>
 static void bulk3d_shape(Domain& d, const S* i, S* o) {
   const int tx = (d.nx + TX - 1) / TX, ty = (d.ny + TY - 1) / TY, tz = (d.nz + ZC - 1) / ZC;
   const int nt = tx * ty * tz;
-  launch_timed(k_bulk3d<S, PERIODIC, RING, TX, TY, ZC, NT>, dim3(xcd_grid(nt)), dim3(NT), d.stream, d.tev0, d.tev1, i, o,
-               d.flags, d.nx, d.ny, d.nz, d.plane, d.omega, tx, ty, nt);
+  launch_timed(k_bulk3d<S, PERIODIC, FAST, KIND, TX, TY, ZC, NT, PF, DMA>, dim3(xcd_grid(nt)), dim3(NT), d.stream,
+               d.tev0, d.tev1, i, o, d.flags, d.nx, d.ny, d.nz, d.plane, d.omega, tx, ty, nt);
 }
 
-template <typename S, bool PERIODIC>
+template <typename S, bool PERIODIC, bool FAST>
 static void bulk3d_variant(Domain& d, const S* i, S* o) {
-  switch (d.tile3) {
-    case 1: bulk3d_shape<S, PERIODIC, 0, 16, 4, 4, 256>(d, i, o); break;
-    case 2: bulk3d_shape<S, PERIODIC, 1, 16, 8, 32, 128>(d, i, o); break;
-    case 3: bulk3d_shape<S, PERIODIC, 1, 16, 8, 16, 128>(d, i, o); break;
-    case 4: bulk3d_shape<S, PERIODIC, 1, 16, 8, 8, 128>(d, i, o); break;
-    case 5: bulk3d_shape<S, PERIODIC, 1, 16, 16, 32, 256>(d, i, o); break;
-    case 6: bulk3d_shape<S, PERIODIC, 1, 16, 16, 16, 256>(d, i, o); break;
-    default: bulk3d_shape<S, PERIODIC, 0, 8, 8, 4, 256>(d, i, o); break;
-  }
+#define FF_B3(FA, K, X, Y, Z, N, P, DM) bulk3d_shape<S, PERIODIC, FA, K, X, Y, Z, N, P, DM>(d, i, o)
+  FF_TILES3(FF_B3, FAST, d.tile3)
+#undef FF_B3
 }
 
 // Stream launch of a 3D step with a boundary (see k_step2d): boundary-cell
 // blocks, wave blocks, then the tiles of the chosen shape; NT = block size.
-template <typename S, bool PERIODIC, int RING, int TX, int TY, int ZC, int NT>
+template <typename S, bool PERIODIC, bool FAST, int KIND, int TX, int TY, int ZC, int NT, int PF, bool DMA>
 __global__ void __launch_bounds__(NT)
     k_step3d(const S* __restrict__ in, S* __restrict__ out, int tiles_x, int tiles_y, int ntiles, int nbc, int nwave,
              LinkArgs la, FinalArgs fa) {
@@ -67,12 +72,12 @@ __global__ void __launch_bounds__(NT)
   } else {
     const int t = xcd_tile(b - nbc - nwave, ntiles);
     if (t < 0) return;
-    tile3d<S, PERIODIC, RING, TX, TY, ZC, NT>(in, out, la.flags, la.nx, la.ny, la.nz, la.plane, la.omega, t, tiles_x,
-                                              tiles_y);
+    tile3d<S, PERIODIC, FAST, KIND, TX, TY, ZC, PF, DMA>(in, out, la.flags, la.nx, la.ny, la.nz, la.plane, la.omega,
+                                                         t, tiles_x, tiles_y);
   }
 }
 
-template <typename S, bool PERIODIC, int RING, int TX, int TY, int ZC, int NT>
+template <typename S, bool PERIODIC, bool FAST, int KIND, int TX, int TY, int ZC, int NT, int PF, bool DMA>
 static void step3d_shape(Domain& d, const S* i, S* o, const LinkArgs& la, FinalArgs fa, bool wave) {
   const int tx = (d.nx + TX - 1) / TX, ty = (d.ny + TY - 1) / TY, tz = (d.nz + ZC - 1) / ZC;
   const int nt = tx * ty * tz;
@@ -81,27 +86,24 @@ static void step3d_shape(Domain& d, const S* i, S* o, const LinkArgs& la, FinalA
   const int nw = fa.kin_blocks + fa.fin_blocks;
   const int nbc = xcd_grid(boundary_cell_blocks(d, la.bc_span) + nw) - nw;
   const unsigned grid = (unsigned)(nbc + nw + xcd_grid(nt));
-  launch_timed(k_step3d<S, PERIODIC, RING, TX, TY, ZC, NT>, dim3(grid), dim3(NT), d.stream, d.tev0, d.tev1, i, o, tx, ty,
-               nt, nbc, nw, la, fa);
+  launch_timed(k_step3d<S, PERIODIC, FAST, KIND, TX, TY, ZC, NT, PF, DMA>, dim3(grid), dim3(NT), d.stream, d.tev0,
+               d.tev1, i, o, tx, ty, nt, nbc, nw, la, fa);
 }
 
-template <typename S, bool PERIODIC>
+template <typename S, bool PERIODIC, bool FAST>
 static void step3d_variant(Domain& d, const S* i, S* o, const LinkArgs& la, const FinalArgs& fa, bool wave) {
-  switch (d.tile3) {
-    case 1: step3d_shape<S, PERIODIC, 0, 16, 4, 4, 256>(d, i, o, la, fa, wave); break;
-    case 2: step3d_shape<S, PERIODIC, 1, 16, 8, 32, 128>(d, i, o, la, fa, wave); break;
-    case 3: step3d_shape<S, PERIODIC, 1, 16, 8, 16, 128>(d, i, o, la, fa, wave); break;
-    case 4: step3d_shape<S, PERIODIC, 1, 16, 8, 8, 128>(d, i, o, la, fa, wave); break;
-    case 5: step3d_shape<S, PERIODIC, 1, 16, 16, 32, 256>(d, i, o, la, fa, wave); break;
-    case 6: step3d_shape<S, PERIODIC, 1, 16, 16, 16, 256>(d, i, o, la, fa, wave); break;
-    default: step3d_shape<S, PERIODIC, 0, 8, 8, 4, 256>(d, i, o, la, fa, wave); break;
-  }
+#define FF_S3(FA, K, X, Y, Z, N, P, DM) step3d_shape<S, PERIODIC, FA, K, X, Y, Z, N, P, DM>(d, i, o, la, fa, wave)
+  FF_TILES3(FF_S3, FAST, d.tile3)
+#undef FF_S3
 }
 
 template <typename S>
 static int step3d_typed(Domain& d, const void* in, void* out, const LinkArgs& la, const FinalArgs& fa, bool wave) {
-  if (d.options & FF_OPT_PERIODIC) step3d_variant<S, true>(d, (const S*)in, (S*)out, la, fa, wave);
-  else step3d_variant<S, false>(d, (const S*)in, (S*)out, la, fa, wave);
+  const bool per = d.options & FF_OPT_PERIODIC, fast = d.options & FF_OPT_FAST_MATH;
+  if (per && fast) step3d_variant<S, true, true>(d, (const S*)in, (S*)out, la, fa, wave);
+  else if (per) step3d_variant<S, true, false>(d, (const S*)in, (S*)out, la, fa, wave);
+  else if (fast) step3d_variant<S, false, true>(d, (const S*)in, (S*)out, la, fa, wave);
+  else step3d_variant<S, false, false>(d, (const S*)in, (S*)out, la, fa, wave);
   FF_HIP(hipGetLastError());
   return FF_OK;
 }
@@ -116,8 +118,11 @@ int launch_step_stream3d(Domain& d, const void* in, void* out, const LinkArgs& l
 
 template <typename S>
 static int bulk3d_typed(Domain& d, const void* in, void* out) {
-  if (d.options & FF_OPT_PERIODIC) bulk3d_variant<S, true>(d, (const S*)in, (S*)out);
-  else bulk3d_variant<S, false>(d, (const S*)in, (S*)out);
+  const bool per = d.options & FF_OPT_PERIODIC, fast = d.options & FF_OPT_FAST_MATH;
+  if (per && fast) bulk3d_variant<S, true, true>(d, (const S*)in, (S*)out);
+  else if (per) bulk3d_variant<S, true, false>(d, (const S*)in, (S*)out);
+  else if (fast) bulk3d_variant<S, false, true>(d, (const S*)in, (S*)out);
+  else bulk3d_variant<S, false, false>(d, (const S*)in, (S*)out);
   FF_HIP(hipGetLastError());
   return FF_OK;
 }

@@ -10,6 +10,9 @@
 
 namespace ff {
 
+// Virtual lanes of the per-body wrench sum (finish_last_body).
+constexpr int kSumLanes = 256;
+
 // ---------------------------------------------------------------- boundary cells
 // First barycentric of a (slot, direction) whose link was not applied this
 // step.  Applied 3D links can carry a slightly negative barycentric (the
@@ -511,23 +514,29 @@ __device__ __forceinline__ void finish_last_body(const LinkArgs& la, const Final
   constexpr int NT = DIM == 2 ? 1 : 3;
   constexpr int STRIDE = DIM * DIM + DIM + NT;
   constexpr int NW = DIM + NT;
-  __shared__ double s_red[NW][FB];
+  // The sums run over kSumLanes virtual lanes (lane v: elements v, v + kSumLanes,
+  // ... of the body, then a fixed tree), whatever the block size FB, so a
+  // body's wrench is the same FP64 number from every launch shape.
+  constexpr int V = kSumLanes;
+  __shared__ double s_red[NW][V];
   for (int bd = 0; bd < fa.n_bodies; bd++) {
-    double a[NW];
+    for (int v = threadIdx.x; v < V; v += FB) {
+      double a[NW];
 #pragma unroll
-    for (int c = 0; c < NW; c++) a[c] = 0.0;
-    for (int k = fa.bptr[bd] + threadIdx.x; k < fa.bptr[bd + 1]; k += FB) {
-      const double* ea = fa.elem_acc + (size_t)fa.bidx[k] * STRIDE + DIM * DIM;
+      for (int c = 0; c < NW; c++) a[c] = 0.0;
+      for (int k = fa.bptr[bd] + v; k < fa.bptr[bd + 1]; k += V) {
+        const double* ea = fa.elem_acc + (size_t)fa.bidx[k] * STRIDE + DIM * DIM;
 #pragma unroll
-      for (int c = 0; c < NW; c++) a[c] += ea[c];
+        for (int c = 0; c < NW; c++) a[c] += ea[c];
+      }
+#pragma unroll
+      for (int c = 0; c < NW; c++) s_red[c][v] = a[c];
     }
-#pragma unroll
-    for (int c = 0; c < NW; c++) s_red[c][threadIdx.x] = a[c];
     __syncthreads();
-    for (int w = FB / 2; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w)
+    for (int w = V / 2; w > 0; w >>= 1) {
+      for (int v = threadIdx.x; v < w; v += FB)
 #pragma unroll
-        for (int c = 0; c < NW; c++) s_red[c][threadIdx.x] += s_red[c][threadIdx.x + w];
+        for (int c = 0; c < NW; c++) s_red[c][v] += s_red[c][v + w];
       __syncthreads();
     }
     if ((int)threadIdx.x < NW) {

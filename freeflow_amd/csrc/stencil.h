@@ -213,12 +213,38 @@ __device__ __forceinline__ void bulk3d_tile(const S* __restrict__ in, S* __restr
 // current plane is computed, hiding their latency.  Arithmetic per cell is
 // that of bulk3d_tile.
 struct Tile3 {
-  int tx, ty, tz;  // tz: planes per block (box variant: 4, ring: ZC)
-  bool ring;
+  int tx, ty, tz;  // tz: planes per block
+  int kind;        // 0 box (bulk3d_tile), 1 ring (bulk3d_ring), 2 column (bulk3d_col)
+  int nt;          // block size
+  int pf;          // column: planes of moments in flight (bulk3d_col's PF)
+  int dma;         // column: the planes in flight are an LDS ring filled by LDS DMA (FP32 storage)
 };
-constexpr Tile3 kTiles3[] = {{8, 8, 4, false}, {16, 4, 4, false}, {16, 8, 32, true}, {16, 8, 16, true},
-                             {16, 8, 8, true},  {16, 16, 32, true}, {16, 16, 16, true}};
-constexpr int kNumTiles3 = 7;
+constexpr Tile3 kTiles3[] = {
+    {8, 8, 4, 0, 256, 0, 0},   {16, 4, 4, 0, 256, 0, 0},  {16, 8, 32, 1, 128, 0, 0}, {16, 8, 16, 1, 128, 0, 0},
+    {16, 8, 8, 1, 128, 0, 0},  {16, 16, 32, 1, 256, 0, 0}, {16, 16, 16, 1, 256, 0, 0}, {16, 12, 8, 2, 256, 1, 0},
+    {16, 26, 8, 2, 512, 1, 0}, {64, 4, 16, 2, 448, 1, 0}, {16, 12, 8, 2, 256, 2, 1},  {16, 8, 8, 2, 192, 3, 1},
+    {32, 8, 8, 2, 384, 2, 1},  {16, 12, 16, 2, 256, 3, 1}};
+constexpr int kNumTiles3 = 14;
+
+// The shapes of kTiles3 by index (kept in step with the table);
+// F(FAST, KIND, TX, TY, ZC, NT, PF, DMA) instantiates one.
+#define FF_TILES3(F, FASTV, tile)                              \
+  switch (tile) {                                              \
+    case 1: F(false, 0, 16, 4, 4, 256, 1, false); break;       \
+    case 2: F(false, 1, 16, 8, 32, 128, 1, false); break;      \
+    case 3: F(false, 1, 16, 8, 16, 128, 1, false); break;      \
+    case 4: F(false, 1, 16, 8, 8, 128, 1, false); break;       \
+    case 5: F(false, 1, 16, 16, 32, 256, 1, false); break;     \
+    case 6: F(false, 1, 16, 16, 16, 256, 1, false); break;     \
+    case 7: F(FASTV, 2, 16, 12, 8, 256, 1, false); break;      \
+    case 8: F(FASTV, 2, 16, 26, 8, 512, 1, false); break;      \
+    case 9: F(FASTV, 2, 64, 4, 16, 448, 1, false); break;      \
+    case 10: F(FASTV, 2, 16, 12, 8, 256, 2, true); break;      \
+    case 11: F(FASTV, 2, 16, 8, 8, 192, 3, true); break;       \
+    case 12: F(FASTV, 2, 32, 8, 8, 384, 2, true); break;       \
+    case 13: F(FASTV, 2, 16, 12, 16, 256, 3, true); break;     \
+    default: F(false, 0, 8, 8, 4, 256, 1, false); break;       \
+  }
 
 // position of direction i among the nine directions sharing its c_z
 __host__ __device__ constexpr int zclass_index(int i) {
@@ -366,6 +392,264 @@ __device__ __forceinline__ void bulk3d_ring(const S* __restrict__ in, S* __restr
     }
     __syncthreads();  // plane z-1's c_z = +1 slot and plane z's c_z = 0 slot are free
     put_rest(rel + 1);
+  }
+}
+
+// 3D column variant with one staged cell per thread.  A block owns a TX x TY
+// column of cells and marches it along z over the planes [z0, zend).  Its
+// first TX*TY threads own one interior cell each (they stage it and compute
+// it); the next 2*(TX+2) + 2*TY threads own the cells of the one-cell xy halo
+// ring (they only stage them).  Every staged cell is loaded and reconstructed
+// once per plane, and no thread holds more than one staged cell's moments and
+// populations in registers.  LDS keeps the population classes by c_z:
+//   c_z = +1 (pulled from plane z-1): planes z-1 and z  -> 2 slots
+//   c_z =  0 (pulled within plane z): plane z           -> 1 slot
+//   c_z = -1 (pulled from plane z+1): plane z+1         -> 1 slot
+// 36 population planes of (TX+2)(TY+2) floats.  Per plane: reconstruct plane
+// z+1 in registers, barrier, store its c_z = -1 class and the held classes
+// of plane z, fetch plane z+2 into registers, barrier, compute plane z.
+// The bounce-back of a wall needs the centre's own c_z = -1 class of plane z,
+// which the slot no longer holds: blocks with a wall in reach (a block-uniform
+// vote) reconstruct it again from the centre's moments (the same arithmetic,
+// so the same values).  FAST selects recon3_fast / collide3_fast.
+template <int TX, int TY>
+struct ColTile {
+  static constexpr int HX = TX + 2, HY = TY + 2, HA = HX * HY, NI = TX * TY;
+  static constexpr int NT = (HA + 63) / 64 * 64;  // block size
+};
+
+// PF planes of moments are in flight while a plane is computed: in registers,
+// or (DMA, FP32 storage) in an LDS ring of PF planes filled by LDS DMA loads
+// (global_load_lds: the load writes LDS without passing through VGPRs, so
+// the planes in flight cost no registers).
+template <typename S, bool PERIODIC, bool FAST, int TX, int TY, int PF = 1, bool DMA = false>
+__device__ __forceinline__ void bulk3d_col(const S* __restrict__ in, S* __restrict__ out,
+                                           const uint8_t* __restrict__ flags, int nx, int ny, int nz, size_t plane,
+                                           float omega, int bx, int by, int z0, int zend) {
+  using CT = ColTile<TX, TY>;
+  constexpr int HX = CT::HX, HY = CT::HY, HA = CT::HA, NI = CT::NI, NW = CT::NT / 64;
+  constexpr bool USE_DMA = DMA && std::is_same<S, float>::value;
+  static_assert(PF >= 1 && PF <= 4, "planes in flight");
+  // 16-bit storage loads through registers: one plane in flight for a DMA shape
+  constexpr int NP = (DMA && !USE_DMA) ? 1 : PF;
+  __shared__ float ring[USE_DMA ? NP : 1][10][USE_DMA ? CT::NT : 1];  // moments of the planes in flight
+  __shared__ float sup[2][9][HA];  // c_z = +1 class, slot rel & 1
+  __shared__ float szc[9][HA];     // c_z = 0 class of plane z
+  __shared__ float sdn[9][HA];     // c_z = -1 class of plane z + 1
+  __shared__ uint8_t sf[3][HA];    // flags, slot rel % 3
+  __shared__ int swall[3][NW];     // per flag slot and wave: a staged cell is a wall
+  const int t = threadIdx.x;
+  __syncthreads();  // a block may run several columns in turn: the last one's reads are done
+  // staged cell (hx, hy) of this thread in the (TX+2) x (TY+2) halo frame
+  int hx, hy;
+  if (t < NI) {
+    hx = t % TX + 1;
+    hy = t / TX + 1;
+  } else {
+    const int h = t - NI;
+    if (h < HX) {
+      hx = h; hy = 0;
+    } else if (h < 2 * HX) {
+      hx = h - HX; hy = HY - 1;
+    } else if (h < 2 * HX + TY) {
+      hx = 0; hy = 1 + (h - 2 * HX);
+    } else if (h < 2 * HX + 2 * TY) {
+      hx = HX - 1; hy = 1 + (h - 2 * HX - TY);
+    } else {
+      hx = 0; hy = 0;  // idle lane of the last wave
+    }
+  }
+  const bool staged = t < HA;
+  const int s = hy * HX + hx;
+  const int x0 = bx * TX, y0 = by * TY;
+  // in-plane offsets of the staged cell: pull source (clamped or periodic) and flag (wrapped)
+  int off_src, off_flag;
+  {
+    const int gx = x0 + hx - 1, gy = y0 + hy - 1;
+    const int wx = ((gx % nx) + nx) % nx, wy = ((gy % ny) + ny) % ny;
+    const int sx = PERIODIC ? wx : min(nx - 1, max(0, gx));
+    const int sy = PERIODIC ? wy : min(ny - 1, max(0, gy));
+    off_src = sy * nx + sx;
+    off_flag = wy * nx + wx;
+  }
+  const size_t nxy = (size_t)nx * ny;
+  // NP planes of moments in flight: slot k holds plane z0 + 1 + j for j = k (mod NP)
+  float m[USE_DMA ? 1 : NP][10];
+  uint8_t fl[NP];
+  auto fetch_regs = [&](int gz, float* mm, uint8_t& f) {
+    const int wz = ((gz % nz) + nz) % nz;
+    const int sz = PERIODIC ? wz : min(nz - 1, max(0, gz));
+    if (staged) {
+      const size_t c = (size_t)sz * nxy + off_src;
+#pragma unroll
+      for (int q = 0; q < 10; q++) mm[q] = Store<S>::template load<3>(in + q * plane + c, q);
+      f = flags[(size_t)wz * nxy + off_flag];
+    } else {
+      f = 0;
+    }
+  };
+  auto fetch = [&](int gz, int k) {
+    if constexpr (USE_DMA) {
+      // every lane loads (idle lanes a valid cell): lane l of wave w lands in ring[k][q][64 w + l]
+      const int wz = ((gz % nz) + nz) % nz;
+      const int sz = PERIODIC ? wz : min(nz - 1, max(0, gz));
+      const float* src = reinterpret_cast<const float*>(in) + (size_t)sz * nxy + off_src;
+      const int wbase = t & ~63;
+#pragma unroll
+      for (int q = 0; q < 10; q++)
+        __builtin_amdgcn_global_load_lds(static_cast<const void*>(src + q * plane),
+                                         static_cast<void*>(&ring[k][q][wbase]), 4, 0, 0);
+      fl[k] = staged ? flags[(size_t)wz * nxy + off_flag] : (uint8_t)0;
+    } else {
+      fetch_regs(gz, m[k], fl[k]);
+    }
+  };
+  // the moments of the plane in slot k, in registers
+  auto moments_of = [&](int k, float* mm) {
+    if constexpr (USE_DMA) {
+#pragma unroll
+      for (int q = 0; q < 10; q++) mm[q] = ring[k][q][t];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 10; q++) mm[q] = m[k][q];
+    }
+  };
+  auto recon = [&](const float* mm, float* f) {
+    if constexpr (FAST) recon3_fast(mm, f);
+    else recon3_all(mm, f);
+  };
+  auto vote = [&](int slot, uint8_t f) {
+    const unsigned long long v = __ballot(staged && is_wall3(f));
+    if ((t & 63) == 0) swall[slot][t >> 6] = v != 0ull;
+  };
+  float hold[18];  // c_z = +1 (0..8) and c_z = 0 (9..17) classes of the plane in flight
+  // with DMA the loop's planes are requested first, so they load during the prologue
+  if constexpr (USE_DMA) {
+#pragma unroll
+    for (int k = 0; k < NP; k++)
+      if (z0 + 1 + k <= zend) fetch(z0 + 1 + k, k);
+  }
+  // prologue: plane z0 - 1 (rel 0) gives its c_z = +1 class, plane z0 (rel 1) is held
+  {
+    float mp[10], f[27];
+    uint8_t fp;
+    fetch_regs(z0 - 1, mp, fp);
+    recon(mp, f);
+    if (staged) {
+#pragma unroll
+      for (int i = 0; i < 27; i++)
+        if (D3::cz(i) > 0) sup[0][zclass_index(i)][s] = f[i];
+      sf[0][s] = fp;
+    }
+    vote(0, fp);
+    fetch_regs(z0, mp, fp);
+    recon(mp, f);
+#pragma unroll
+    for (int i = 0; i < 27; i++) {
+      const int cz = D3::cz(i), j = zclass_index(i);
+      if (cz > 0) hold[j] = f[i];
+      else if (cz == 0) hold[9 + j] = f[i];
+    }
+    if (staged) sf[1][s] = fp;
+    vote(1, fp);
+  }
+  if constexpr (!USE_DMA) {
+#pragma unroll
+    for (int k = 0; k < NP; k++)
+      if (z0 + 1 + k <= zend) fetch(z0 + 1 + k, k);
+  }
+  const int lx = t % TX, ly = t / TX;
+  const int x = x0 + lx, y = y0 + ly;
+  const bool inside = t < NI && x < nx && y < ny;
+  // one plane: K is the register slot holding plane z + 1 (a constant in each
+  // unrolled copy of the loop body)
+  auto plane_step = [&](int z, auto KC) {
+    constexpr int K = decltype(KC)::value;
+    const int rel = z - z0 + 1;
+    const int s_m = (rel + 2) % 3, s_c = rel % 3, s_p = (rel + 1) % 3;
+    float f[27];
+    {
+      float mm[10];
+      moments_of(K, mm);
+      recon(mm, f);  // plane z + 1
+    }
+    const uint8_t fz1 = fl[K];
+    __syncthreads();  // compute(z - 1) is done with the slots rewritten below
+    if (staged) {
+#pragma unroll
+      for (int i = 0; i < 27; i++) {
+        const int cz = D3::cz(i), j = zclass_index(i);
+        if (cz < 0) sdn[j][s] = f[i];
+      }
+#pragma unroll
+      for (int j = 0; j < 9; j++) {
+        sup[rel & 1][j][s] = hold[j];
+        szc[j][s] = hold[9 + j];
+      }
+      sf[s_p][s] = fz1;
+    }
+    vote(s_p, fz1);
+#pragma unroll
+    for (int i = 0; i < 27; i++) {
+      const int cz = D3::cz(i), j = zclass_index(i);
+      if (cz > 0) hold[j] = f[i];
+      else if (cz == 0) hold[9 + j] = f[i];
+    }
+    if (z + 1 + NP <= zend) fetch(z + 1 + NP, K);  // in flight while planes z (and on) are computed
+    __syncthreads();
+    int walls = 0;
+#pragma unroll
+    for (int w = 0; w < NW; w++) walls |= swall[0][w] | swall[1][w] | swall[2][w];
+    if (inside && is_updated(sf[s_c][s])) {
+      const int su = (rel - 1) & 1;  // c_z = +1 slot of plane z - 1
+      float p[27];
+      if (walls) {
+        // the centre's own c_z = -1 class of plane z, reconstructed again
+        float own[27];
+        {
+          float mc[10];
+          const size_t c = (size_t)z * nxy + (size_t)y * nx + x;
+#pragma unroll
+          for (int q = 0; q < 10; q++) mc[q] = Store<S>::template load<3>(in + q * plane + c, q);
+          recon(mc, own);
+        }
+#pragma unroll
+        for (int i = 0; i < 27; i++) {
+          const int cz = D3::cz(i), j = zclass_index(i), up = s - D3::cy(i) * HX - D3::cx(i);
+          const int sl = cz > 0 ? s_m : (cz < 0 ? s_p : s_c);
+          const int ib = D3::inv(i), jb = zclass_index(ib);
+          float v;
+          if (is_wall3(sf[sl][up])) {
+            // bounce-back: the centre's own population in direction inv(i), plane z
+            v = cz > 0 ? own[ib] : (cz < 0 ? sup[rel & 1][jb][s] : szc[jb][s]);
+          } else {
+            v = cz > 0 ? sup[su][j][up] : (cz < 0 ? sdn[j][up] : szc[j][up]);
+          }
+          p[i] = fmaxf(0.0f, v);
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < 27; i++) {
+          const int cz = D3::cz(i), j = zclass_index(i), up = s - D3::cy(i) * HX - D3::cx(i);
+          p[i] = fmaxf(0.0f, cz > 0 ? sup[su][j][up] : (cz < 0 ? sdn[j][up] : szc[j][up]));
+        }
+      }
+      float o[10];
+      if constexpr (FAST) collide3_fast(p, omega, o);
+      else collide3(p, omega, o);
+      const size_t cell = (size_t)z * nxy + (size_t)y * nx + x;
+#pragma unroll
+      for (int q = 0; q < 10; q++) Store<S>::template store<3>(out + q * plane + cell, q, o[q]);
+    }
+  };
+  for (int z = z0; z < zend; z += NP) {
+    plane_step(z, std::integral_constant<int, 0>{});
+    if constexpr (NP > 1)
+      if (z + 1 < zend) plane_step(z + 1, std::integral_constant<int, (NP > 1 ? 1 : 0)>{});
+    if constexpr (NP > 2)
+      if (z + 2 < zend) plane_step(z + 2, std::integral_constant<int, (NP > 2 ? 2 : 0)>{});
+    if constexpr (NP > 3)
+      if (z + 3 < zend) plane_step(z + 3, std::integral_constant<int, (NP > 3 ? 3 : 0)>{});
   }
 }
 

@@ -56,6 +56,12 @@ extern "C" {
                                   (src/core/FsiCoupling2D.cu:89,120-128) */
 #define FF_OPT_PERIODIC 0x2u   /* pull across the domain edges periodically instead of
                                   the reference's clamped halo (FsiCoupling2D.cu:104-105) */
+#define FF_OPT_FAST_MATH 0x4u  /* 3D staged-column tiles (7-10): the same update in
+                                  a reassociated FP32 form with FMA (pair sums of opposite
+                                  populations) instead of the reference's operation order
+                                  (src/lbm/LbmUtilsFunc.cu:212-414, FsiCoupling3D.cu:291-320);
+                                  moments agree with the reference order to <= 1e-5
+                                  relative L2 after 100 steps, not bit for bit */
 
 /* Node flag values: the ordinals of fsi::lbm::LbmNodeFlag
    (include/FSI_Simulator/lbm/LbmDataTypes.cuh:17-56). */
@@ -186,14 +192,24 @@ int ff_time_steps(ff_domain* d, int nsteps, double* total_ms);
 
 /* Device bytes ff_domain_create + ff_set_boundary would allocate for this
    descriptor and boundary (verts/elems may be NULL when ne == 0): the figure
-   an admission controller charges against the GPU's memory.  No device work. */
+   an admission controller charges against the GPU's memory.  Laid out by the
+   same plans the two calls allocate with, each allocation rounded up to
+   2 MiB, the per-body arrays sized for ne bodies (an upper bound).  A wrench
+   history ring (ff_set_wrench_history) is extra: capacity x bodies x 3 (2D)
+   or 6 (3D) doubles.  No device work. */
 int ff_domain_footprint(const ff_domain_desc* desc, int nv, int ne, const float* verts, const uint32_t* elems,
                         unsigned long long* bytes);
 
 /* Stream-collide tile shape.  2D (0..4): 32x8, 32x16, 64x8, 32x32, 64x16
-   cells per 256-thread block.  3D (0..6): 8x8x4 and 16x4x4 boxes, 16x8
-   columns marching 32, 16 or 8 planes, 16x16 columns marching 32 or 16
-   planes.  0 is the default; results do not depend on it.
+   cells per 256-thread block.  3D (0..13): 0-1 8x8x4 and 16x4x4 boxes; 2-6
+   columns that rotate reconstructed populations through LDS (16x8 marching
+   32, 16 or 8 planes, 16x16 marching 32 or 16); 7-13 columns with one
+   staged cell per thread (7: 16x12 x 8 planes, 8: 16x26 x 8, 9: 64x4 x 16;
+   10-13 keep the next planes' moments in an LDS ring filled by LDS DMA with
+   FP32 storage: 16x12 x 8 with 2 planes in flight, 16x8 x 8 with 3, 32x8 x 8
+   with 2, 16x12 x 16 with 3), the only shapes FF_OPT_FAST_MATH changes.
+   0 is the default; in the reference's operation order results do not
+   depend on it.
    No counterpart in the reference (its tile is fixed, CudaCommon.cuh:44-51). */
 int ff_set_tile(ff_domain* d, int tile);
 /* Time every tile shape on this domain (three interleaved rounds, each

@@ -44,21 +44,55 @@ def test_version_and_error_string_without_gpu():
     assert rc == 1 and b"dim" in L.ff_last_error()
 
 
+def _al(count, size):
+    """Bytes of one array in a carved allocation: 256-byte aligned."""
+    return (max(count, 1) * size + 255) // 256 * 256
+
+
+def _round2m(b):
+    return (b + (2 << 20) - 1) // (2 << 20) * (2 << 20)
+
+
+def _core_bytes(dim, n, sbytes):
+    """Moments (two ping-pong buffers of nm planes, 256-element plane
+    stride), flags (one byte a cell), 1024 doubles of reduction scratch."""
+    nm = 6 if dim == 2 else 10
+    plane = (n + 255) // 256 * 256
+    return _round2m(2 * _al(plane * nm * sbytes, 1) + _al(n, 1) + _al(1024, 8))
+
+
+def _boundary_bytes(dim, nv, ne, nb, cap):
+    """Every array of the ff_set_boundary allocation, in its order."""
+    D, K, nm = dim, (8 if dim == 2 else 26), (6 if dim == 2 else 10)
+    vc = max(nv, 1) * D
+    b = 4 * _al(vc, 4)                                      # positions and velocities, two generations
+    b += _al(ne * D, 4) + _al(ne, 4) + _al(nv, 4)           # elements, element -> body, vertex -> body
+    b += _al(nb + 1, 4) + _al(ne, 4)                        # CSR body -> elements
+    b += _al(nv + 1, 4) + _al(ne * D, 4)                    # CSR vertex -> (element, end)
+    b += _al(nb * D, 8) + _al(1, 16)                        # body reference points, step state
+    b += _al(vc, 4) + _al(nb * (3 if dim == 2 else 6), 8)   # vertex forces, body wrench
+    b += _al(2 * cap, 8) + _al(2 * cap * K, 8)              # link tables: cells and keys, two by tag parity
+    b += _al(ne, 8 * (7 if dim == 2 else 15))               # per-element FP64 sums
+    b += 3 * _al(cap * K * D, 4) + _al(cap * nm, 4)         # link force, barycentrics, hit point; cell rows
+    b += _al(vc, 4) + _al(nv, 4) + _al(nb * 4, 8)           # wave inputs
+    return _round2m(b)
+
+
 def test_footprint_without_gpu():
     if not os.path.exists(capi.LIB_PATH):
         pytest.skip("library not built")
     from freeflow_amd import workloads as wl
     nx, ny = 1200, 400
-    plane = (nx * ny + 255) // 256 * 256
-    base32 = capi.footprint(2, nx, ny, storage="f32")
-    base16 = capi.footprint(2, nx, ny, storage="f16")
-    assert base32 - base16 == 2 * plane * 6 * 2  # two ping-pong buffers, 6 planes, 2 bytes saved each
-    assert base32 == 2 * plane * 6 * 4 + nx * ny + 8192
+    assert capi.footprint(2, nx, ny, storage="f32") == _core_bytes(2, nx * ny, 4)
+    assert capi.footprint(2, nx, ny, storage="f16") == _core_bytes(2, nx * ny, 2)
+    assert capi.footprint(3, 96, 96, 96, storage="bf16") == _core_bytes(3, 96 ** 3, 2)
     sw = wl.swimmer2d_swimmer()
+    nv, ne = len(sw.rest), len(sw.elems)
     with_b = capi.footprint(2, nx, ny, storage="f32", verts=sw.rest, elems=sw.elems, dx=0.005)
-    assert with_b > base32
-    p3 = (96 ** 3 + 255) // 256 * 256
-    assert capi.footprint(3, 96, 96, 96, storage="bf16") == 2 * p3 * 10 * 2 + 96 ** 3 + 8192
+    extra = with_b - _core_bytes(2, nx * ny, 4)
+    # the link table's capacity is a power of two >= 1024; ne bodies bound the per-body arrays
+    caps = [1 << k for k in range(10, 24) if _boundary_bytes(2, nv, ne, ne, 1 << k) == extra]
+    assert caps, f"boundary bytes {extra} match no table capacity"
 
 
 def _distinct_cells_near(verts, elems, dx):
@@ -78,9 +112,9 @@ def _distinct_cells_near(verts, elems, dx):
 def test_link_table_sized_from_distinct_cells_near_the_mesh():
     """Two 2D polylines with the same vertex and element counts, one packed
     into a few cells and one spread over the grid: their footprints differ by
-    exactly the difference of their link tables, each sized
-    next_pow2(2 * distinct cells + 1024) slots (360 B per 2D slot: two
-    tables of cells and 8 keys, force/barycentric/hit per link, 6 moments)."""
+    their link tables, each sized next_pow2(2 * distinct cells + 1024) slots
+    (360 B per 2D slot: two tables of cells and 8 keys, force/barycentric/hit
+    per link, 6 moments)."""
     if not os.path.exists(capi.LIB_PATH):
         pytest.skip("library not built")
     n, nx, ny = 400, 2000, 200
@@ -96,7 +130,9 @@ def test_link_table_sized_from_distinct_cells_near_the_mesh():
     fa = capi.footprint(2, nx, ny, storage="f32", verts=packed, elems=e, dx=1.0)
     fb = capi.footprint(2, nx, ny, storage="f32", verts=spread, elems=e, dx=1.0)
     assert cap(spread) > cap(packed)
-    assert fb - fa == (cap(spread) - cap(packed)) * 360
+    core = _core_bytes(2, nx * ny, 4)
+    assert fa - core == _boundary_bytes(2, n + 1, n, n, cap(packed))
+    assert fb - core == _boundary_bytes(2, n + 1, n, n, cap(spread))
 
 
 def test_invalid_arguments_rejected_without_gpu():

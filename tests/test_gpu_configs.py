@@ -1,0 +1,235 @@
+"""The benchmarked workloads at their own sizes on the GPU, through the C ABI.
+
+  * 3D D3Q27 256x128x128 Taylor-Green + vortex ring, periodic: 100 steps on
+    the full grid against the FP32 CPU oracle (reference operation order:
+    bit-exact; FF_OPT_FAST_MATH: <= 1e-5 relative L2), then 1000 steps on the
+    GPU alone in FP32 (both forms), FP16 and BF16 storage: mass drift and the
+    other forms' velocity fields against FP32 in the reference order.
+  * 64 uneven 2D swimmer domains (256x128 .. 1200x400, 200-800 segments) in
+    one FP16-storage batch: three members against the oracle with the same
+    storage rounding (bit-exact), and one member's velocity error against the
+    FP64 oracle after 1000 steps (bar 1e-3).
+  * Served rollout episodes at their real sizes (2D 600..1200 x 200..400,
+    3D 96..192 x 96 x 96, 500 steps, an action every 10 steps): every episode
+    completes within the memory budget, and two sampled episodes (one 2D, one
+    3D) give the same moments and wrench readings alone as in the mix.
+  * A 3D domain with an outlet on each of its six faces against the oracle.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from freeflow_amd import capi, serve, workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+REF_TILE, FAST_TILE = 9, 9  # 64x4 staged columns of 16 planes
+
+
+@pytest.fixture(scope="module")
+def tg3d():
+    c = wl.TG3D
+    m = wl.taylor_green_ring(c["nx"], c["ny"], c["nz"], c["amp"], c["ring_r"], c["seed"])
+    return c, np.asarray(m, np.float32)
+
+
+def test_taylor_green_256x128x128_full_grid_vs_oracle(tg3d):
+    c, m = tg3d
+    nx, ny, nz, steps = c["nx"], c["ny"], c["nz"], 100
+    note(f"case tg3d full grid {nx}x{ny}x{nz}, {steps} steps vs FP32 oracle")
+    got = {}
+    for fast, tile in ((False, REF_TILE), (True, FAST_TILE)):
+        d = capi.Domain(3, nx, ny, nz, viscosity=c["viscosity"], periodic=True, fast_math=fast)
+        d.set_tile(tile)
+        d.set_moments(m)
+        d.step(steps)
+        got[fast] = (d.moments(0).astype(np.float64), d.moments(1).astype(np.float64))
+        d.close()
+    ol.set_threads(0)
+    o = ol.OracleDomain(3, nx, ny, nz, fp64=False, stage_fp16=False, periodic=True, nu=c["viscosity"])
+    o.set_moments(m.astype(np.float64))
+    o.step(steps)
+    for which in (0, 1):
+        ref = o.moments(which)
+        exact, fast = got[False][which], got[True][which]
+        same = float(np.mean(exact == ref))
+        rel = rel_l2(fast, ref)
+        note(f"  buffer {which}: reference order bit-identical fraction {same:.6f}; fast math rel-L2 {rel:.3e}")
+        assert same == 1.0
+        assert rel <= 1e-5
+
+
+def test_taylor_green_256x128x128_1000_steps_storage_formats(tg3d):
+    c, m = tg3d
+    nx, ny, nz, steps = c["nx"], c["ny"], c["nz"], 1000
+    res = {}
+    for name, storage, fast in (("f32", "f32", False), ("f32-fast", "f32", True), ("f16", "f16", True),
+                                ("bf16", "bf16", True)):
+        note(f"case tg3d {nx}x{ny}x{nz} {name}, {steps} steps")
+        d = capi.Domain(3, nx, ny, nz, viscosity=c["viscosity"], storage=storage, periodic=True, fast_math=fast)
+        d.set_tile(FAST_TILE if fast else REF_TILE)
+        d.set_moments(m)
+        m0 = d.mass()
+        d.step(steps)
+        u = d.moments(0)[:, 1:4].astype(np.float64)
+        drift = abs(d.mass() - m0) / m0
+        d.close()
+        assert np.all(np.isfinite(u))
+        res[name] = (u, drift)
+    ref = res["f32"][0]
+    # the two FP32 forms round differently every step: 1e-5 apart after the
+    # first 100 steps (previous test), a few 1e-5 after 1000
+    for name, bound_u, bound_m in (("f32", 0.0, 1e-5), ("f32-fast", 5e-5, 1e-5), ("f16", 1e-3, 1e-4),
+                                   ("bf16", 1e-2, 1e-4)):
+        u, drift = res[name]
+        e = rel_l2(u, ref)
+        note(f"  {name}: velocity rel-L2 vs FP32 reference order {e:.3e}, mass drift {drift:.3e}")
+        assert e <= bound_u
+        assert drift < bound_m
+
+
+def _member_oracle(sc, storage, fp64):
+    o = ol.OracleDomain(2, sc.nx, sc.ny, fp64=fp64, stage_fp16=False, nu=sc.viscosity, dx=sc.dx, dt=sc.dt,
+                        cf=capi.cf_of(2, sc.rho, sc.dx, sc.dt), storage=storage)
+    o.set_flags(sc.flags)
+    return o
+
+
+def test_batch_of_64_uneven_swimmers():
+    c = wl.BATCH64
+    scenes = wl.batched_scenes(c["n_domains"], c["seed"])
+    sizes = np.array([s.nx * s.ny for s in scenes])
+    order = np.argsort(sizes)
+    sample = [int(order[0]), int(order[len(order) // 2]), int(order[-1])]
+    note(f"case batch64: {sizes.sum()} cells, sampled members {[(scenes[k].nx, scenes[k].ny) for k in sample]}")
+    b = capi.Batch(2, "f16", stage_fp16=False)
+    doms = []
+    m0 = {}
+    for k, sc in enumerate(scenes):
+        d = capi.Domain(**sc.domain_args(), storage="f16", stage_fp16=False)
+        sc.setup(d)
+        if k in sample:
+            m0[k] = wl.equilibrium_moments(np.zeros((sc.nx * sc.ny, 2)))
+            d.set_moments(m0[k])
+        b.add(d)
+        doms.append(d)
+    ors = {}
+    for k in sample:
+        o = _member_oracle(scenes[k], 1, False)  # FP16 storage rounding
+        o.set_moments(m0[k])
+        v, vel = doms[k].boundary()
+        o.set_boundary(v, vel, scenes[k].swimmer.elems)
+        ors[k] = o
+    k64 = sample[0]
+    o64 = _member_oracle(scenes[k64], 0, True)
+    o64.set_moments(m0[k64])
+    v, vel = doms[k64].boundary()
+    o64.set_boundary(v, vel, scenes[k64].swimmer.elems)
+    for step in range(1000):
+        b.step(1)
+        if step < 30:
+            for k in sample:
+                v, vel = doms[k].boundary()  # the kinematics the device used for this step
+                ors[k].update_boundary(v, vel)
+                ors[k].step(1)
+        v, vel = doms[k64].boundary()
+        o64.update_boundary(v, vel)
+        o64.step(1)
+        if step == 29:
+            for k in sample:
+                a = doms[k].moments(0).astype(np.float64)
+                r = ors[k].moments(0)
+                same = float(np.mean(a == r))
+                note(f"  member {k} ({scenes[k].nx}x{scenes[k].ny}) after 30 steps: bit-identical fraction {same:.6f}")
+                assert same == 1.0
+    b.sync()
+    sc = scenes[k64]
+    upd = sc.flags == 1
+    a = doms[k64].moments(0).astype(np.float64)
+    r = o64.moments(0)
+    err = rel_l2(a[upd, 1:3], r[upd, 1:3])
+    note(f"  member {k64} ({sc.nx}x{sc.ny}) FP16 storage after 1000 steps: velocity rel-L2 vs FP64 {err:.3e}")
+    assert err <= 1e-3
+    wr = np.concatenate([d.body_wrench() for d in doms])
+    assert np.all(np.isfinite(wr))
+    b.close()
+
+
+def test_served_episodes_at_real_sizes():
+    seed = 5
+    eps = serve.make_episodes(rate=40.0, duration=1.0, seed=seed)
+    kinds = [e.kind for e in eps]
+    assert 2 in kinds and 3 in kinds
+    note(f"case served real sizes: {len(eps)} episodes, {kinds.count(2)} 2D and {kinds.count(3)} 3D, 500 steps each")
+    srv = serve.Server(storage="f16", keep_final=True)
+    summary = srv.run(eps, realtime=False)
+    srv.close()
+    note(f"  {summary['aggregate_mlups']:.0f} MLUPS, p50 {summary['p50_step_ms']:.3f} ms, "
+         f"p99 {summary['p99_step_ms']:.3f} ms, peak {summary['peak_active']} active, {summary['peak_bytes']} bytes")
+    assert summary["episodes_done"] == len(eps)
+    assert summary["peak_bytes"] <= summary["budget_bytes"]
+    for target in (eps[0], next(e for e in eps if e.kind != eps[0].kind)):
+        alone = serve.make_episodes(rate=40.0, duration=1.0, seed=seed)[target.eid]
+        note(f"  episode {target.eid} kind {target.kind} {target.nx}x{target.ny}x{target.nz} alone")
+        s2 = serve.Server(storage="f16", keep_final=True)
+        s2.run([alone], realtime=False)
+        s2.close()
+        assert np.array_equal(alone.final_moments, target.final_moments), f"episode {target.eid}"
+        assert len(alone.wrench) == len(target.wrench) == 50
+        for x, y in zip(alone.wrench, target.wrench):
+            assert np.array_equal(x, y)
+
+
+def test_3d_outlets_on_all_six_faces():
+    """OutletLeft/Right on the x faces, OutletFront/Back on the y faces,
+    OutletDown/Up on the z faces (FsiCoupling3D.cu:348-418: each face reads
+    its inner neighbour), with a flow that leaves through some faces and
+    enters through others."""
+    nx, ny, nz = 36, 28, 24
+    note(f"case 3d six outlet faces {nx}x{ny}x{nz}")
+    f = np.ones((nz, ny, nx), np.uint8)
+    f[:, :, 0] = 19   # OutletLeft, x = 0
+    f[:, :, -1] = 20  # OutletRight, x = nx - 1
+    f[:, 0, :] = 23   # OutletFront, y = 0
+    f[:, -1, :] = 24  # OutletBack, y = ny - 1
+    f[0, :, :] = 22   # OutletDown, z = 0
+    f[-1, :, :] = 21  # OutletUp, z = nz - 1
+    flags = f.ravel()
+    rng = np.random.default_rng(31)
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    u = np.stack([0.03 * np.sin(2 * np.pi * y / ny), 0.02 * np.cos(2 * np.pi * z / nz),
+                  0.025 * np.sin(2 * np.pi * x / nx)], -1).reshape(-1, 3)
+    u += rng.uniform(-1e-3, 1e-3, u.shape)
+    m = np.asarray(wl.equilibrium_moments(u), np.float32)
+    for tile in (0, 7, REF_TILE):
+        g = capi.Domain(3, nx, ny, nz, viscosity=0.01)
+        g.set_tile(tile)
+        o = ol.OracleDomain(3, nx, ny, nz, fp64=False, nu=0.01)
+        for d in (g, o):
+            d.set_flags(flags)
+        g.set_moments(m)
+        o.set_moments(m.astype(np.float64))
+        g.step(30)
+        o.step(30)
+        for which in (0, 1):
+            a, r = g.moments(which).astype(np.float64), o.moments(which)
+            same = float(np.mean(a == r))
+            note(f"  tile {tile} buffer {which}: bit-identical fraction {same:.6f}")
+            assert same == 1.0
+        # every face rule acted: outlet cells differ from their initial state
+        a = g.moments(0).astype(np.float64)
+        for flag in (19, 20, 21, 22, 23, 24):
+            cells = flags == flag
+            assert np.any(a[cells] != m[cells]), f"outlet flag {flag} never updated"

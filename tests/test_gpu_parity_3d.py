@@ -210,7 +210,7 @@ def test_16bit_storage_3d(storage):
     assert same == 1.0
 
 
-@pytest.mark.parametrize("tile", [0, 2, 5])
+@pytest.mark.parametrize("tile", [0, 2, 5, 7, 8, 9, 10, 11, 12, 13])
 def test_grid_smaller_than_tile(tile):
     """6x5x3 periodic cells: every box or column is partial and the halo
     wraps around the whole grid."""
@@ -225,7 +225,7 @@ def test_grid_smaller_than_tile(tile):
     assert same == 1.0
 
 
-@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13])
 def test_tile_variants_match_oracle(tile):
     # walls on every side (3D flag bounce-back), inlet/outlet, nz = 40 so the
     # 32-plane columns have a full and a partial chunk
@@ -258,3 +258,63 @@ def test_tile_variants_match_oracle(tile):
     o2.step(15)
     rel, same = compare(g2, o2, 0)
     assert same == 1.0
+
+
+def _channel_flags(nx, ny, nz):
+    f = np.ones((nz, ny, nx), np.uint8)
+    f[:, :, 0] = 13
+    f[:, :, nx - 1] = 20
+    f[:, 0, :] = 5
+    f[:, ny - 1, :] = 5
+    f[0, :, :] = 5
+    f[nz - 1, :, :] = 5
+    return f.ravel()
+
+
+@pytest.mark.parametrize("tile", [7, 8, 9, 10, 11, 12, 13])
+def test_fast_math_columns_within_tolerance(tile):
+    """FF_OPT_FAST_MATH (pair-decomposed reconstruction and moments, FMA):
+    moments within 1e-5 relative L2 of the reference-order oracle after 100
+    steps, for a periodic Taylor-Green field and a walled channel (flag
+    bounce-back on all sides, inlet and outlet); nz = 40 leaves partial
+    column chunks; the velocity error against the FP64 oracle stays at the
+    reference-order kernel's level."""
+    nx, ny, nz = 36, 30, 40
+    steps = 100
+    note(f"case 3d fast math tile {tile}")
+    m = wl.taylor_green_ring(nx, ny, nz, amp=0.04, ring_r=6.0, seed=21)
+    g = capi.Domain(3, nx, ny, nz, viscosity=0.01, periodic=True, fast_math=True)
+    g.set_tile(tile)
+    ex = capi.Domain(3, nx, ny, nz, viscosity=0.01, periodic=True)
+    ex.set_tile(tile)
+    o = ol.OracleDomain(3, nx, ny, nz, fp64=False, periodic=True, nu=0.01)
+    o64 = ol.OracleDomain(3, nx, ny, nz, fp64=True, stage_fp16=False, periodic=True, nu=0.01)
+    for d in (g, ex):
+        d.set_moments(np.asarray(m, np.float32))
+    o.set_moments(np.asarray(m, np.float32).astype(np.float64))
+    o64.set_moments(np.asarray(m, np.float32).astype(np.float64))
+    for d in (g, ex, o, o64):
+        d.step(steps)
+    a, b, r64 = g.moments(0).astype(np.float64), o.moments(0), o64.moments(0)
+    rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+    e_fast = np.linalg.norm(a[:, 1:4] - r64[:, 1:4]) / np.linalg.norm(r64[:, 1:4])
+    e_ref = np.linalg.norm(ex.moments(0)[:, 1:4].astype(np.float64) - r64[:, 1:4]) / np.linalg.norm(r64[:, 1:4])
+    note(f"  fast vs FP32 oracle rel-L2 {rel:.3e}; velocity vs FP64: fast {e_fast:.3e}, reference order {e_ref:.3e}")
+    assert rel <= 1e-5
+    assert e_fast <= max(2.0 * e_ref, 1e-6)
+    # walled channel: flag bounce-back through the re-reconstructed centre classes
+    flags = _channel_flags(nx, ny, nz)
+    g = capi.Domain(3, nx, ny, nz, viscosity=0.01, fast_math=True)
+    g.set_tile(tile)
+    o = ol.OracleDomain(3, nx, ny, nz, fp64=False, nu=0.01)
+    g.set_flags(flags)
+    o.set_flags(flags)
+    g.init_state([0.05, 0, 0, 0, 0, 0])
+    o.init_reference([0.05, 0, 0, 0, 0, 0])
+    g.step(steps)
+    o.step(steps)
+    for which in (0, 1):
+        a, b = g.moments(which).astype(np.float64), o.moments(which)
+        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+        note(f"  channel buffer {which}: fast vs FP32 oracle rel-L2 {rel:.3e}")
+        assert rel <= 1e-5
