@@ -32,7 +32,7 @@ EXPORTS = [
     "ff_domain_create", "ff_domain_destroy", "ff_set_flags", "ff_get_flags", "ff_init_state", "ff_set_moments",
     "ff_get_moments", "ff_set_boundary", "ff_update_boundary", "ff_set_body_ref", "ff_set_wave", "ff_set_wave_params", "ff_step",
     "ff_step_proxy", "ff_get_vertex_forces", "ff_get_step_vertex_forces", "ff_get_body_wrench", "ff_set_wrench_history",
-    "ff_get_wrench_history", "ff_get_boundary", "ff_mass", "ff_fill_solid", "ff_sync", "ff_time_bulk_kernel",
+    "ff_get_wrench_history", "ff_get_body_power", "ff_get_power_history", "ff_get_boundary", "ff_mass", "ff_fill_solid", "ff_sync", "ff_time_bulk_kernel",
     "ff_time_steps", "ff_domain_footprint", "ff_set_tile", "ff_autotune", "ff_batch_create", "ff_batch_destroy", "ff_batch_add",
     "ff_batch_remove", "ff_batch_size", "ff_batch_step", "ff_batch_sync", "ff_batch_set_tile", "ff_batch_get_tile",
     "ff_batch_time_steps", "ff_batch_time_bulk", "ff_last_error", "ff_device_count", "ff_version",
@@ -82,6 +82,8 @@ def lib():
         L.ff_get_body_wrench.argtypes = [vp, vp]
         L.ff_set_wrench_history.argtypes = [vp, C.c_int]
         L.ff_get_wrench_history.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int)]
+        L.ff_get_body_power.argtypes = [vp, vp]
+        L.ff_get_power_history.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int)]
         L.ff_get_boundary.argtypes = [vp, vp, vp]
         L.ff_mass.argtypes = [vp, vp]
         L.ff_fill_solid.argtypes = [vp]
@@ -257,6 +259,19 @@ class Domain:
         n = C.c_int(0)
         _check(lib().ff_get_wrench_history(self.h, _ptr(out), count, C.byref(n)))
         return out[: n.value * self.n_bodies * w].reshape(n.value, self.n_bodies, w)
+
+    def body_power(self):
+        """Per-body power of the latest step: sum over the body's vertices of
+        -F_v . u_v (FP64, computed on the device)."""
+        out = np.empty(max(self.n_bodies, 1), np.float64)
+        _check(lib().ff_get_body_power(self.h, _ptr(out)))
+        return out[: self.n_bodies]
+
+    def power_history(self, count):
+        out = np.empty(max(count, 1) * max(self.n_bodies, 1), np.float64)
+        n = C.c_int(0)
+        _check(lib().ff_get_power_history(self.h, _ptr(out), count, C.byref(n)))
+        return out[: n.value * self.n_bodies].reshape(n.value, self.n_bodies)
 
     def boundary(self):
         v = np.empty(self.nv * self.dim, np.float32)

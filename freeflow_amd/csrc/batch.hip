@@ -424,9 +424,9 @@ static int autotune(Batch& b) {
     return FF_OK;
   }
   const int ntiles = b.dim == 2 ? kNumTiles2 : kNumTiles3;
-  hipEvent_t e0, e1;
-  FF_HIP(hipEventCreate(&e0));
-  FF_HIP(hipEventCreate(&e1));
+  EventSet ev;
+  FF_HIP(ev.create(2));
+  const hipEvent_t e0 = ev[0], e1 = ev[1];
   double best = 1e30;
   int best_t = 0;
   int rc = FF_OK;
@@ -449,8 +449,6 @@ static int autotune(Batch& b) {
       best_t = t;
     }
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   b.tuned_tile = best_t;
   b.tuned_cells = member_cells(b);
   return rc;
@@ -768,18 +766,15 @@ int ff_batch_time_steps(ff_batch* b, int nsteps, double* total_ms) {
   FF_HIP(hipSetDevice(b->device));
   int rc;
   if (!b->mem.empty() && b->dirty && (rc = rebuild(*b))) return rc;
-  hipEvent_t e0, e1;
-  FF_HIP(hipEventCreate(&e0));
-  FF_HIP(hipEventCreate(&e1));
+  EventSet ev;
+  FF_HIP(ev.create(2));
   if (!b->mem.empty() && (rc = detect_pending(*b))) return rc;
-  FF_HIP(hipEventRecord(e0, b->stream));
+  FF_HIP(hipEventRecord(ev[0], b->stream));
   if (!b->mem.empty() && (rc = replay(*b, nsteps))) return rc;
-  FF_HIP(hipEventRecord(e1, b->stream));
-  FF_HIP(hipEventSynchronize(e1));
+  FF_HIP(hipEventRecord(ev[1], b->stream));
+  FF_HIP(hipEventSynchronize(ev[1]));
   float ms = 0.f;
-  FF_HIP(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  FF_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
   if (!b->mem.empty()) advance_mirrors(*b, nsteps);
   *total_ms = ms;
   return ff_batch_sync(b);
@@ -796,8 +791,10 @@ int ff_batch_time_bulk(ff_batch* b, int iters, double* avg_ms) {
   // step are excluded); they rewrite only cells every step rewrites
   const int tile = current_tile(*b);
   const int2* map = b->d_map + b->n_bc + b->n_wave;
-  std::vector<hipEvent_t> ev(2 * iters);
-  for (auto& e : ev) FF_HIP(hipEventCreate(&e));
+  EventSet ev;
+  ev.clears[0] = &b->tev0;
+  ev.clears[1] = &b->tev1;
+  FF_HIP(ev.create(2 * (size_t)iters));
   for (int k = 0; k < iters; k++) {
     b->tev0 = ev[2 * k];  // the launch's own start and end (launch_timed)
     b->tev1 = ev[2 * k + 1];
@@ -812,7 +809,6 @@ int ff_batch_time_bulk(ff_batch* b, int iters, double* avg_ms) {
     FF_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
     tot += ms;
   }
-  for (auto& e : ev) (void)hipEventDestroy(e);
   *avg_ms = tot / iters;
   return FF_OK;
 }

@@ -108,6 +108,8 @@ static void free_boundary(Domain& d) {
   d.st = nullptr; d.vert_body = nullptr;
   d.rest_verts = nullptr; d.s_coord = nullptr; d.wave_params = nullptr;
   dfree(d.wrench_hist);
+  dfree(d.power_hist);
+  d.power = nullptr;
   d.nv = d.ne = d.n_bodies = 0;
   d.gen_host = 1;
   d.tag_host = 1;
@@ -120,7 +122,7 @@ static void free_boundary(Domain& d) {
   d.stale = false;
 }
 
-static int elem_stride(int dim) { return dim == 2 ? 7 : 15; }
+static int elem_stride(int dim) { return dim == 2 ? elem_stride_v<2> : elem_stride_v<3>; }
 static int wrench_width(int dim) { return dim == 2 ? 3 : 6; }
 
 // Outlet list from the flags (mrApplyOutlet2DKernel / 3D branch table).
@@ -295,6 +297,7 @@ static BoundaryPlan plan_boundary(Carve& cv, Domain& d, int nv, int ne, int n_bo
   // region 2 (generation 0 marks every table entry empty; steps run from generation 1)
   cv.add(&d.vforce, vcount);
   cv.add(&d.wrench, (size_t)n_bodies * wrench_width(D));
+  cv.add(&d.power, (size_t)n_bodies);
   cv.add(&d.lt_cells, 2 * cap);  // two tables (cells and keys), alternating by tag parity
   cv.add(&d.lt_keys, 2 * cap * K);
   p.zero_end = cv.total;
@@ -869,11 +872,17 @@ int ff_set_wrench_history(ff_domain* dd, int capacity) {
   FF_HIP(hipStreamSynchronize(d.stream));
   batch_mark_dirty(d.batch);
   dfree(d.wrench_hist);
+  dfree(d.power_hist);
   d.hist_cap = 0;
   d.hist_count = 0;
   if (capacity == 0) return FF_OK;
   int rc = dalloc(&d.wrench_hist, (size_t)capacity * d.n_bodies * wrench_width(d.dim));
-  if (rc) return rc;
+  if (!rc) rc = dalloc(&d.power_hist, (size_t)capacity * d.n_bodies);
+  if (rc) {
+    dfree(d.wrench_hist);
+    dfree(d.power_hist);
+    return rc;
+  }
   d.hist_cap = capacity;
   return FF_OK;
 }
@@ -890,6 +899,39 @@ int ff_get_wrench_history(ff_domain* dd, double* out, int count, int* n_out) {
   const size_t rec = (size_t)d.n_bodies * wrench_width(d.dim);
   std::vector<double> ring((size_t)d.hist_cap * rec);
   if (d.hist_cap) FF_HIP(hipMemcpy(ring.data(), d.wrench_hist, ring.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int k = 0; k < n; k++) {
+    // the step with generation g wrote slot (g - 1) % cap; the latest step is gen_host - 1
+    const long long g = (long long)d.gen_host - n + k;
+    std::memcpy(out + (size_t)k * rec, ring.data() + (size_t)((g - 1) % d.hist_cap) * rec, rec * sizeof(double));
+  }
+  *n_out = n;
+  return FF_OK;
+}
+
+int ff_get_body_power(ff_domain* dd, double* out) {
+  FF_CHECK_DOMAIN(dd);
+  Domain& d = *dd;
+  if (!d.has_boundary) return fail(FF_ERR_STATE, "no boundary set");
+  if (!out) return fail(FF_ERR_INVALID, "null output");
+  FF_HIP(hipSetDevice(d.device));
+  if (int frc = flush_finish(d)) return frc;  // the last step's deferred finish
+  FF_HIP(hipStreamSynchronize(d.stream));
+  FF_HIP(hipMemcpy(out, d.power, (size_t)d.n_bodies * sizeof(double), hipMemcpyDeviceToHost));
+  return check_overflow(d);
+}
+
+int ff_get_power_history(ff_domain* dd, double* out, int count, int* n_out) {
+  FF_CHECK_DOMAIN(dd);
+  Domain& d = *dd;
+  if (!out || !n_out || count < 0) return fail(FF_ERR_INVALID, "bad output");
+  FF_HIP(hipSetDevice(d.device));
+  if (int frc = flush_finish(d)) return frc;  // the last step's deferred finish
+  FF_HIP(hipStreamSynchronize(d.stream));
+  const long long avail = std::min<long long>(d.hist_count, d.hist_cap);
+  const int n = (int)std::min<long long>(avail, count);
+  const size_t rec = (size_t)d.n_bodies;
+  std::vector<double> ring((size_t)d.hist_cap * rec);
+  if (d.hist_cap) FF_HIP(hipMemcpy(ring.data(), d.power_hist, ring.size() * sizeof(double), hipMemcpyDeviceToHost));
   for (int k = 0; k < n; k++) {
     // the step with generation g wrote slot (g - 1) % cap; the latest step is gen_host - 1
     const long long g = (long long)d.gen_host - n + k;
@@ -958,8 +1000,10 @@ int ff_time_bulk_kernel(ff_domain* dd, int iters, double* avg_ms) {
   if (d.batch) return fail(FF_ERR_STATE, "domain belongs to a batch");
   FF_HIP(hipSetDevice(d.device));
   if (int frc = flush_finish(d)) return frc;  // the last step's deferred finish
-  std::vector<hipEvent_t> ev(2 * iters);
-  for (auto& e : ev) FF_HIP(hipEventCreate(&e));
+  EventSet ev;
+  ev.clears[0] = &d.tev0;
+  ev.clears[1] = &d.tev1;
+  FF_HIP(ev.create(2 * (size_t)iters));
   void* in = d.mom[d.cur];
   void* out = d.mom[d.cur ^ 1];
   // the stream launch of a step (with a boundary it also runs the boundary
@@ -982,7 +1026,6 @@ int ff_time_bulk_kernel(ff_domain* dd, int iters, double* avg_ms) {
     FF_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
     tot += ms;
   }
-  for (auto& e : ev) (void)hipEventDestroy(e);
   *avg_ms = tot / iters;
   return rc;
 }
@@ -993,19 +1036,16 @@ int ff_time_steps(ff_domain* dd, int nsteps, double* total_ms) {
   Domain& d = *dd;
   if (d.batch) return fail(FF_ERR_STATE, "domain belongs to a batch: time it with ff_batch_time_steps");
   FF_HIP(hipSetDevice(d.device));
-  hipEvent_t a, b;
-  FF_HIP(hipEventCreate(&a));
-  FF_HIP(hipEventCreate(&b));
-  FF_HIP(hipEventRecord(a, d.stream));
+  EventSet ev;
+  FF_HIP(ev.create(2));
+  FF_HIP(hipEventRecord(ev[0], d.stream));
   int rc = FF_OK;
   for (int s = 0; s < nsteps && !rc; s++) rc = one_step(d, nullptr, nullptr, nullptr, nullptr);
   if (!rc) rc = flush_finish(d);  // the timed work includes the last step's finish
-  FF_HIP(hipEventRecord(b, d.stream));
-  FF_HIP(hipEventSynchronize(b));
+  FF_HIP(hipEventRecord(ev[1], d.stream));
+  FF_HIP(hipEventSynchronize(ev[1]));
   float ms = 0.f;
-  FF_HIP(hipEventElapsedTime(&ms, a, b));
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
+  FF_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
   *total_ms = ms;
   if (rc) return rc;
   return check_overflow(d);
@@ -1041,9 +1081,9 @@ int ff_autotune(ff_domain* dd, int* tile) {
   Domain& d = *dd;
   if (d.batch) return fail(FF_ERR_STATE, "domain belongs to a batch: its tile is tuned with the batch");
   FF_HIP(hipSetDevice(d.device));
-  hipEvent_t e0, e1;
-  FF_HIP(hipEventCreate(&e0));
-  FF_HIP(hipEventCreate(&e1));
+  EventSet ev;
+  FF_HIP(ev.create(2));
+  const hipEvent_t e0 = ev[0], e1 = ev[1];
   // the stream-collide launch rewrites only cells every step rewrites, so
   // timing it leaves the state unchanged
   void* in = d.mom[d.cur];
@@ -1052,6 +1092,15 @@ int ff_autotune(ff_domain* dd, int* tile) {
   int& cur = d.dim == 2 ? d.tile2 : d.tile3;
   const int n = d.dim == 2 ? kNumTiles2 : kNumTiles3;
   const int saved = cur;
+  // an error return leaves the tile the domain had
+  struct Restore {
+    int& t;
+    int v;
+    bool keep = false;
+    ~Restore() {
+      if (!keep) t = v;
+    }
+  } restore{cur, saved};
   auto launch = [&]() { return d.dim == 2 ? launch_bulk2d(d, in, out) : launch_bulk3d(d, in, out); };
   // rounds over every candidate (clock and cache drift hit all alike); each
   // candidate keeps its fastest round
@@ -1071,9 +1120,8 @@ int ff_autotune(ff_domain* dd, int* tile) {
     }
   }
   const int best_t = (int)(std::min_element(best.begin(), best.end()) - best.begin());
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   cur = rc ? saved : best_t;
+  restore.keep = true;
   if (tile) *tile = cur;
   return rc;
 }

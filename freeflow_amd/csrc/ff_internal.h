@@ -34,6 +34,13 @@ int hip_fail(hipError_t e, const char* what);
     if (_e != hipSuccess) return ::ff::hip_fail(_e, #call); \
   } while (0)
 
+// Per-element FP64 record of a step (elem_acc): DIM x DIM vertex shares of
+// the force (vertex j, component c at j * DIM + c), the force, the torque
+// (1 in 2D, 3 in 3D) about the body reference point, and the element's power
+// -sum_j share_j . u_j with the step's vertex velocities.
+template <int DIM>
+constexpr int elem_stride_v = DIM * DIM + DIM + (DIM == 2 ? 1 : 3) + 1;
+
 // One outlet cell: in-place update of the new buffer after a step.
 struct OutletCell {
   int cell;      // outlet cell index
@@ -119,6 +126,8 @@ struct Domain {
   double* elem_acc = nullptr;     // per element: dim*dim vertex shares, dim force, torque
   double* wrench = nullptr;       // n_bodies * (2D:3, 3D:6), latest step
   double* wrench_hist = nullptr;  // ring of hist_cap steps
+  double* power = nullptr;        // n_bodies: latest step's sum over the body of -F_v . u_v
+  double* power_hist = nullptr;   // ring of hist_cap steps
   int hist_cap = 0;
   long long hist_count = 0;       // steps recorded since the ring was set
   // link table (sparse replacement of the dense N*Q hit table)
@@ -153,6 +162,29 @@ struct Domain {
   hipStream_t own_stream = nullptr;
   // set only while ff_time_bulk_kernel times a launch (see launch_timed)
   hipEvent_t tev0 = nullptr, tev1 = nullptr;
+};
+
+// HIP events owned by a scope: destroyed on every return, error returns
+// included; a domain's or batch's timing slots (tev0/tev1) registered with
+// `clears` are reset to null with them.
+struct EventSet {
+  std::vector<hipEvent_t> ev;
+  hipEvent_t* clears[2] = {nullptr, nullptr};
+  hipError_t create(size_t n) {
+    ev.assign(n, nullptr);
+    for (auto& e : ev) {
+      const hipError_t r = hipEventCreate(&e);
+      if (r != hipSuccess) return r;
+    }
+    return hipSuccess;
+  }
+  hipEvent_t operator[](size_t i) const { return ev[i]; }
+  ~EventSet() {
+    for (auto* c : clears)
+      if (c) *c = nullptr;
+    for (auto e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
 };
 
 // Batch bookkeeping used by the per-domain entry points (batch.hip).

@@ -157,6 +157,8 @@ FinalArgs make_final_args(Domain& d, void* out, float* vn0, float* vn1, float* u
   fa.elem_acc = d.elem_acc;
   fa.wrench = d.wrench;
   fa.hist = d.hist_cap > 0 ? d.wrench_hist : nullptr;
+  fa.power = d.power;
+  fa.power_hist = d.hist_cap > 0 ? d.power_hist : nullptr;
   fa.hist_cap = d.hist_cap > 0 ? d.hist_cap : 1;
   fa.bptr = d.body_elem_ptr;
   fa.bidx = d.body_elem_idx;
@@ -190,7 +192,7 @@ FinalArgs make_final_args(Domain& d, void* out, float* vn0, float* vn1, float* u
 template <int DIM>
 __global__ void k_step_vertex_forces(const double* __restrict__ elem_acc, const int* __restrict__ vptr,
                                      const int* __restrict__ vinc, int nv, float* __restrict__ out) {
-  constexpr int STRIDE = DIM * DIM + DIM + (DIM == 2 ? 1 : 3);
+  constexpr int STRIDE = elem_stride_v<DIM>;
   const int vtx = blockIdx.x * blockDim.x + threadIdx.x;
   if (vtx >= nv) return;
   double f[DIM];

@@ -13,6 +13,7 @@ namespace ff {
 // Virtual lanes of the per-body wrench sum (finish_last_body).
 constexpr int kSumLanes = 256;
 
+
 // ---------------------------------------------------------------- boundary cells
 // First barycentric of a (slot, direction) whose link was not applied this
 // step.  Applied 3D links can carry a slightly negative barycentric (the
@@ -215,6 +216,8 @@ struct FinalArgs {
   double* elem_acc;
   double* wrench;
   double* hist;
+  double* power;       // per body, latest step
+  double* power_hist;  // ring of hist_cap steps (with hist)
   int hist_cap;
   const int* bptr;
   const int* bidx;
@@ -314,7 +317,7 @@ template <int DIM, bool PAIR>
 __device__ __forceinline__ void elem_sum_wave(const LinkArgs& la, const FinalArgs& fa, int elem, int lane, int* sl) {
   constexpr int K = DIM == 2 ? 8 : 26;
   constexpr int NT = DIM == 2 ? 1 : 3;
-  constexpr int STRIDE = DIM * DIM + DIM + NT;
+  constexpr int STRIDE = elem_stride_v<DIM>;
   const unsigned gen = step_gen(la);
   const unsigned tag = step_tag(la);
   const float* v = la.v[gen & 1];
@@ -427,6 +430,16 @@ __device__ __forceinline__ void elem_sum_wave(const LinkArgs& la, const FinalArg
     for (int a = 0; a < STRIDE; a++) acc[a] += __shfl_down(acc[a], off, 64);
   }
   if (lane == 0) {
+    // the element's power: its vertex shares against the velocities the step used
+    const float* u = la.u[gen & 1];
+    double pw = 0.0;
+#pragma unroll
+    for (int jv = 0; jv < DIM; jv++) {
+      const size_t vid = la.e[(size_t)elem * DIM + jv];
+#pragma unroll
+      for (int c = 0; c < DIM; c++) pw -= acc[jv * DIM + c] * (double)u[vid * DIM + c];
+    }
+    acc[STRIDE - 1] = pw;
 #pragma unroll
     for (int a = 0; a < STRIDE; a++) fa.elem_acc[(size_t)elem * STRIDE + a] = acc[a];
   }
@@ -439,7 +452,7 @@ __device__ __forceinline__ void elem_sums_body(const LinkArgs& la, const FinalAr
   constexpr int K = DIM == 2 ? 8 : 26;
   constexpr int NT = DIM == 2 ? 1 : 3;
   constexpr int NM = DIM == 2 ? 6 : 10;
-  constexpr int STRIDE = DIM * DIM + DIM + NT;
+  constexpr int STRIDE = elem_stride_v<DIM>;
   const int neb = elem_blocks(la.ne);
   if (blk >= neb + fa.outlet_blocks + fa.copy_blocks) {
     detect_next_body<DIM, BATCHED>(la, blk - neb - fa.outlet_blocks - fa.copy_blocks);
@@ -490,7 +503,7 @@ __device__ __forceinline__ void elem_sums_body(const LinkArgs& la, const FinalAr
 template <int DIM, int FB>
 __device__ __forceinline__ void finish_vertex_body(const LinkArgs& la, const FinalArgs& fa, int blk) {
   constexpr int NT = DIM == 2 ? 1 : 3;
-  constexpr int STRIDE = DIM * DIM + DIM + NT;
+  constexpr int STRIDE = elem_stride_v<DIM>;
   const int vtx = blk * FB + threadIdx.x;
   if (vtx >= la.nv) return;
   double f[DIM];
@@ -512,32 +525,37 @@ __device__ __forceinline__ void finish_vertex_body(const LinkArgs& la, const Fin
 template <int DIM, int FB>
 __device__ __forceinline__ void finish_last_body(const LinkArgs& la, const FinalArgs& fa, unsigned gen, bool advance) {
   constexpr int NT = DIM == 2 ? 1 : 3;
-  constexpr int STRIDE = DIM * DIM + DIM + NT;
+  constexpr int STRIDE = elem_stride_v<DIM>;
   constexpr int NW = DIM + NT;
   // The sums run over kSumLanes virtual lanes (lane v: elements v, v + kSumLanes,
   // ... of the body, then a fixed tree), whatever the block size FB, so a
-  // body's wrench is the same FP64 number from every launch shape.
-  constexpr int V = kSumLanes;
-  __shared__ double s_red[NW][V];
+  // body's wrench is the same FP64 number from every launch shape.  Channels:
+  // the wrench (NW), then the power.
+  constexpr int V = kSumLanes, NC = NW + 1;
+  __shared__ double s_red[NC][V];
   for (int bd = 0; bd < fa.n_bodies; bd++) {
     for (int v = threadIdx.x; v < V; v += FB) {
-      double a[NW];
+      double a[NC];
 #pragma unroll
-      for (int c = 0; c < NW; c++) a[c] = 0.0;
+      for (int c = 0; c < NC; c++) a[c] = 0.0;
       for (int k = fa.bptr[bd] + v; k < fa.bptr[bd + 1]; k += V) {
         const double* ea = fa.elem_acc + (size_t)fa.bidx[k] * STRIDE + DIM * DIM;
 #pragma unroll
-        for (int c = 0; c < NW; c++) a[c] += ea[c];
+        for (int c = 0; c < NC; c++) a[c] += ea[c];
       }
 #pragma unroll
-      for (int c = 0; c < NW; c++) s_red[c][v] = a[c];
+      for (int c = 0; c < NC; c++) s_red[c][v] = a[c];
     }
     __syncthreads();
     for (int w = V / 2; w > 0; w >>= 1) {
       for (int v = threadIdx.x; v < w; v += FB)
 #pragma unroll
-        for (int c = 0; c < NW; c++) s_red[c][v] += s_red[c][v + w];
+        for (int c = 0; c < NC; c++) s_red[c][v] += s_red[c][v + w];
       __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      fa.power[bd] = s_red[NW][0];
+      if (fa.power_hist) fa.power_hist[(size_t)((gen - 1) % (unsigned)fa.hist_cap) * fa.n_bodies + bd] = s_red[NW][0];
     }
     if ((int)threadIdx.x < NW) {
       fa.wrench[bd * NW + threadIdx.x] = s_red[threadIdx.x][0];

@@ -167,6 +167,18 @@ int ff_get_body_wrench(ff_domain* d, double* out);
    records, most recent last. Returns the number written in *n_out. */
 int ff_set_wrench_history(ff_domain* d, int capacity);
 int ff_get_wrench_history(ff_domain* d, double* out, int count, int* n_out);
+
+/* Per-body power of the latest step, n_bodies doubles: the sum over the
+   body's vertices of -F_v . u_v, F_v the step's hydrodynamic vertex force
+   (the FP64 vertex shares of ff_get_step_vertex_forces) and u_v the vertex
+   velocities the step used; accumulated in FP64 on the device.  Replaces the
+   host-side power = sum(-force * velocity) of the reference's RL loop
+   (rl/env.py:175-178), which copies every vertex force and velocity to the
+   host each simulation step. */
+int ff_get_body_power(ff_domain* d, double* out);
+/* The per-body power of the last `count` steps (oldest first) from the ring
+   that ff_set_wrench_history sizes: *n_out rows of n_bodies doubles. */
+int ff_get_power_history(ff_domain* d, double* out, int count, int* n_out);
 /* Current vertex positions/velocities held by the library (after wave
    kinematics this is the state used by the most recent step). */
 int ff_get_boundary(ff_domain* d, float* verts, float* vels);

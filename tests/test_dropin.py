@@ -225,35 +225,7 @@ def test_unsupported_solver_paths(tmp_path):
 
 
 # ------------------------------------------------------------------ GPU
-class Mover:
-    """StaticBodySolver2D/3D::advanceTimeStep (StaticBodySolver.cpp:39-68,
-    122-151) restated: marker = volume vertex 0; switch target when closer
-    than dt*vel; all vertices get vel*normalize(target-marker), glm's double
-    arithmetic (length = sqrt(dot), normalize = d * (1/sqrt(dot)))."""
-
-    def __init__(self, pos, targets, vel, dt):
-        self.pos = np.array(pos, np.float64)
-        self.vel = np.zeros_like(self.pos)
-        self.targets = [np.array(t, np.float64) for t in targets]
-        self.v, self.dt, self.idx = vel, dt, -1
-
-    def step(self):
-        m = self.pos[0].copy()
-        if self.idx < 0:
-            self.idx = 0
-
-        def dot(d):
-            s = d[0] * d[0] + d[1] * d[1]
-            return s + d[2] * d[2] if len(d) == 3 else s
-
-        if math.sqrt(dot(self.targets[self.idx] - m)) < self.dt * self.v:
-            self.idx = (self.idx + 1) % len(self.targets)
-        d = self.targets[self.idx] - m
-        inv = 1.0 / math.sqrt(dot(d))
-        vel = np.array([self.v * (d[k] * inv) for k in range(len(d))])
-        shift = vel * self.dt
-        self.vel[:] = vel
-        self.pos += shift
+Mover = meshes.Mover
 
 
 @pytest.mark.gpu
