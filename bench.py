@@ -126,9 +126,10 @@ SHAPES3 = [(8, 8, 4, 0, 256, 1, 0), (16, 4, 4, 0, 256, 1, 0), (16, 8, 32, 1, 128
            (32, 8, 8, 2, 384, 2, 1), (16, 12, 16, 2, 256, 3, 1)]
 
 
-def kernel2d(prefix, s, stage, periodic, tile):
+def kernel2d(prefix, s, stage, periodic, tile, fast=False):
     tx, ty, cpt = SHAPES2[tile]
-    return f"ff::{prefix}<{s}, {str(stage).lower()}, {str(periodic).lower()}, {tx}, {ty}, {cpt}>"
+    return (f"ff::{prefix}<{s}, {str(stage).lower()}, {str(periodic).lower()}, {tx}, {ty}, {cpt}, "
+            f"{str(fast and not stage).lower()}>")
 
 
 def kernel3d(s, periodic, tile, fast=False):
@@ -204,11 +205,11 @@ def batched_summary(capi, wl, K, W, plain_steps, tile=-1, fp64_check=True):
     for d in plain:
         d.close()
     del plain
-    # batched FP16-storage form, FP32 arithmetic
-    b = capi.Batch(2, c["storage"], stage_fp16=False)
+    # batched FP16-storage form, FP32 fast-math arithmetic
+    b = capi.Batch(2, c["storage"], stage_fp16=False, fast_math=True)
     doms = []
     for sc in scenes:
-        d = capi.Domain(**sc.domain_args(), storage=c["storage"], stage_fp16=False)
+        d = capi.Domain(**sc.domain_args(), storage=c["storage"], stage_fp16=False, fast_math=True)
         sc.setup(d)
         b.add(d)
         doms.append(d)
@@ -252,10 +253,10 @@ def cpu_baseline_batched_member(capi, wl, scenes, steps):
     o.set_flags(sc.flags)
     m0 = wl.equilibrium_moments(np.zeros((sc.nx * sc.ny, 2)))
     o.set_moments(m0)
-    d = capi.Domain(**sc.domain_args(), storage="f16", stage_fp16=False)
+    d = capi.Domain(**sc.domain_args(), storage="f16", stage_fp16=False, fast_math=True)
     sc.setup(d)
     d.set_moments(m0)
-    b = capi.Batch(2, "f16", stage_fp16=False)
+    b = capi.Batch(2, "f16", stage_fp16=False, fast_math=True)
     b.add(d)
     v, vel = d.boundary()
     o.set_boundary(v, vel, sc.swimmer.elems)
@@ -281,7 +282,8 @@ def run_batched(args, capi, wl):
     r = batched_summary(capi, wl, K, W, args.plain_steps, args.tile, fp64_check=not args.no_cpu)
     bpl = bytes_per_lup(2, 2)
     roof = roofline(bpl * r["cells"] / (r["avg_kernel_ms"] * 1e-3) / 1e9,
-                    kernel2d("kb_step2d", "__half", False, False, r["tile"]), r["avg_kernel_ms"], bpl, r["cells"])
+                    kernel2d("kb_step2d", "__half", False, False, r["tile"], fast=True), r["avg_kernel_ms"], bpl,
+                    r["cells"])
     out = base_line(r["mlups"], K, W, r["ms_per_step"], "f32 compute, f16 storage",
                     {"workload": "64 uneven 2D swimmer domains (256x128..1200x400, 200-800 segments) in one batch, "
                                  "FP16 storage, one graph replay per step", "domains": r["domains"],
@@ -408,7 +410,8 @@ def served_summary(rate=50.0, duration=4.0):
     out = {}
     for batched in (True, False):
         eps = serve.make_episodes(rate, duration, seed=1234)
-        srv = serve.Server(storage="f16" if batched else "f32", batched=batched)
+        # batched: FP16 storage, fast math; plain: the reference's FP32 form, one domain per episode
+        srv = serve.Server(storage="f16" if batched else "f32", batched=batched, fast_math=batched)
         print(f"[bench] served {'batched FP16' if batched else 'plain FP32'}: {len(eps)} episodes at {rate}/s",
               file=sys.stderr, flush=True)
         r = srv.run(eps, realtime=True)
@@ -476,7 +479,7 @@ def run_served(args):
     extra = {}
     if args.plain_steps > 0:
         # the same arrivals served in the plain per-domain FP32 form
-        plain = serve.Server(storage="f32", batched=False)
+        plain = serve.Server(storage="f32", batched=False, fast_math=False)
         print("[bench] served, plain per-domain FP32 form", file=sys.stderr, flush=True)
         ps = plain.run(serve.make_episodes(args.rate, args.duration, seed=1234), realtime=True)
         plain.close()

@@ -149,8 +149,8 @@ class Domain:
     def __init__(self, dim, nx, ny, nz=1, viscosity=0.005, dx=1.0, dt=1.0, fluid_density=1.0,
                  storage="f32", stage_fp16=None, periodic=False, fast_math=False, device=0):
         L = lib()
-        if stage_fp16 is None:
-            stage_fp16 = dim == 2
+        if stage_fp16 is None:  # the reference's FP16 tile staging is its 2D operation order
+            stage_fp16 = dim == 2 and not fast_math
         opts = ((OPT_STAGE_FP16 if stage_fp16 else 0) | (OPT_PERIODIC if periodic else 0)
                 | (OPT_FAST_MATH if fast_math else 0))
         self.desc = DomainDesc(dim, nx, ny, nz if dim == 3 else 1, viscosity, dx, dt, fluid_density,
@@ -311,7 +311,7 @@ class Batch:
 
     def __init__(self, dim, storage="f32", stage_fp16=None, periodic=False, fast_math=False, device=0):
         if stage_fp16 is None:
-            stage_fp16 = dim == 2
+            stage_fp16 = dim == 2 and not fast_math
         opts = ((OPT_STAGE_FP16 if (stage_fp16 and dim == 2) else 0) | (OPT_PERIODIC if periodic else 0)
                 | (OPT_FAST_MATH if fast_math else 0))
         self.h = C.c_void_p()

@@ -109,7 +109,7 @@ constexpr int kGraphSteps = 8;
 
 // ---------------------------------------------------------------- kernels
 // Stream launch: [boundary-cell blocks of every domain][wave blocks][tiles].
-template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT>
+template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT, bool FAST>
 __global__ void __launch_bounds__(256) kb_step2d(const DomDev* __restrict__ doms, const int2* __restrict__ map,
                                                  int n_bc, int n_wave) {
   const int2 m = map[blockIdx.x];
@@ -118,13 +118,13 @@ __global__ void __launch_bounds__(256) kb_step2d(const DomDev* __restrict__ doms
   const unsigned p = *D->step & 1u;
   const int b = blockIdx.x;
   if (b < n_bc) {
-    bcells_body<2, S, STAGE, PERIODIC, 256>((const S*)D->mom[p], D->la, m.y, D->nbc);
+    bcells_body<2, S, STAGE, PERIODIC, 256, FAST>((const S*)D->mom[p], D->la, m.y, D->nbc);
   } else if (b < n_bc + n_wave) {
-    wave_block_body<2, 256>(D->la, D->fa, m.y);
+    wave_block_body<2, 256, FAST>(D->la, D->fa, m.y);
   } else {
-    bulk2d_tile<S, STAGE, PERIODIC, TX, TY, CPT>((const S*)D->mom[p], (S*)D->mom[p ^ 1u], D->la.flags, D->la.nx,
-                                                 D->la.ny, D->la.plane, D->la.omega, m.y % D->tiles_x,
-                                                 m.y / D->tiles_x);
+    bulk2d_tile<S, STAGE, PERIODIC, TX, TY, CPT, FAST>((const S*)D->mom[p], (S*)D->mom[p ^ 1u], D->la.flags,
+                                                       D->la.nx, D->la.ny, D->la.plane, D->la.omega,
+                                                       m.y % D->tiles_x, m.y / D->tiles_x);
   }
 }
 
@@ -138,9 +138,9 @@ __global__ void __launch_bounds__(NT) kb_step3d(const DomDev* __restrict__ doms,
   const unsigned p = *D->step & 1u;
   const int b = blockIdx.x;
   if (b < n_bc) {
-    bcells_body<3, S, false, PERIODIC, NT>((const S*)D->mom[p], D->la, m.y, D->nbc);
+    bcells_body<3, S, false, PERIODIC, NT, FAST>((const S*)D->mom[p], D->la, m.y, D->nbc);
   } else if (b < n_bc + n_wave) {
-    wave_block_body<3, NT>(D->la, D->fa, m.y);
+    wave_block_body<3, NT, FAST>(D->la, D->fa, m.y);
   } else {
     const int tx = D->tiles_x, ty = D->tiles_y;
     const int bx = m.y % tx, by = (m.y / tx) % ty, bz = m.y / (tx * ty);
@@ -203,20 +203,20 @@ __global__ void __launch_bounds__(kSumBlock) kb_detect(const DomDev* __restrict_
 // Block size of the stream launch for a tile index.
 static int stream_threads(int dim, int tile) { return dim == 2 ? 256 : kTiles3[tile].nt; }
 
-template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT>
+template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT, bool FAST>
 static void step2d_batch_shape(Batch& b, const int2* map, int nblocks, int n_bc, int n_wave) {
-  launch_timed(kb_step2d<S, STAGE, PERIODIC, TX, TY, CPT>, dim3(nblocks), dim3(256), b.stream, b.tev0, b.tev1,
+  launch_timed(kb_step2d<S, STAGE, PERIODIC, TX, TY, CPT, FAST>, dim3(nblocks), dim3(256), b.stream, b.tev0, b.tev1,
                (const DomDev*)b.d_doms, map, n_bc, n_wave);
 }
 
-template <typename S, bool STAGE, bool PERIODIC>
+template <typename S, bool STAGE, bool PERIODIC, bool FAST>
 static void step2d_batch_variant(Batch& b, int tile, const int2* map, int nblocks, int n_bc, int n_wave) {
   switch (tile) {
-    case 1: step2d_batch_shape<S, STAGE, PERIODIC, 32, 8, 2>(b, map, nblocks, n_bc, n_wave); break;
-    case 2: step2d_batch_shape<S, STAGE, PERIODIC, 64, 4, 2>(b, map, nblocks, n_bc, n_wave); break;
-    case 3: step2d_batch_shape<S, STAGE, PERIODIC, 32, 8, 4>(b, map, nblocks, n_bc, n_wave); break;
-    case 4: step2d_batch_shape<S, STAGE, PERIODIC, 64, 4, 4>(b, map, nblocks, n_bc, n_wave); break;
-    default: step2d_batch_shape<S, STAGE, PERIODIC, 32, 8, 1>(b, map, nblocks, n_bc, n_wave); break;
+    case 1: step2d_batch_shape<S, STAGE, PERIODIC, 32, 8, 2, FAST>(b, map, nblocks, n_bc, n_wave); break;
+    case 2: step2d_batch_shape<S, STAGE, PERIODIC, 64, 4, 2, FAST>(b, map, nblocks, n_bc, n_wave); break;
+    case 3: step2d_batch_shape<S, STAGE, PERIODIC, 32, 8, 4, FAST>(b, map, nblocks, n_bc, n_wave); break;
+    case 4: step2d_batch_shape<S, STAGE, PERIODIC, 64, 4, 4, FAST>(b, map, nblocks, n_bc, n_wave); break;
+    default: step2d_batch_shape<S, STAGE, PERIODIC, 32, 8, 1, FAST>(b, map, nblocks, n_bc, n_wave); break;
   }
 }
 
@@ -239,13 +239,16 @@ template <typename S>
 static void launch_stream_batch(Batch& b, int tile, const int2* map, int nblocks, int n_bc, int n_wave) {
   if (nblocks <= 0) return;
   const bool stage = b.options & FF_OPT_STAGE_FP16, per = b.options & FF_OPT_PERIODIC;
+  const bool fast = b.options & FF_OPT_FAST_MATH;
   if (b.dim == 2) {
-    if (stage && per) step2d_batch_variant<S, true, true>(b, tile, map, nblocks, n_bc, n_wave);
-    else if (stage) step2d_batch_variant<S, true, false>(b, tile, map, nblocks, n_bc, n_wave);
-    else if (per) step2d_batch_variant<S, false, true>(b, tile, map, nblocks, n_bc, n_wave);
-    else step2d_batch_variant<S, false, false>(b, tile, map, nblocks, n_bc, n_wave);
+    // Fast math applies to unstaged tiles only.
+    if (stage && per) step2d_batch_variant<S, true, true, false>(b, tile, map, nblocks, n_bc, n_wave);
+    else if (stage) step2d_batch_variant<S, true, false, false>(b, tile, map, nblocks, n_bc, n_wave);
+    else if (per && fast) step2d_batch_variant<S, false, true, true>(b, tile, map, nblocks, n_bc, n_wave);
+    else if (per) step2d_batch_variant<S, false, true, false>(b, tile, map, nblocks, n_bc, n_wave);
+    else if (fast) step2d_batch_variant<S, false, false, true>(b, tile, map, nblocks, n_bc, n_wave);
+    else step2d_batch_variant<S, false, false, false>(b, tile, map, nblocks, n_bc, n_wave);
   } else {
-    const bool fast = b.options & FF_OPT_FAST_MATH;
     if (per && fast) step3d_batch_variant<S, true, true>(b, tile, map, nblocks, n_bc, n_wave);
     else if (per) step3d_batch_variant<S, true, false>(b, tile, map, nblocks, n_bc, n_wave);
     else if (fast) step3d_batch_variant<S, false, true>(b, tile, map, nblocks, n_bc, n_wave);

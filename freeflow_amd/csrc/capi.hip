@@ -310,7 +310,7 @@ static BoundaryPlan plan_boundary(Carve& cv, Domain& d, int nv, int ne, int n_bo
   // wave kinematics inputs (ff_set_wave fills them when it is used)
   cv.add(&d.rest_verts, vcount);
   cv.add(&d.s_coord, (size_t)std::max(nv, 1));
-  cv.add(&d.wave_params, (size_t)n_bodies * 4);
+  cv.add(&d.wave_params, (size_t)n_bodies * kWaveParStride);
   return p;
 }
 
@@ -760,8 +760,10 @@ int ff_set_wave(ff_domain* dd, const float* rest, const float* s, const double* 
   // the arrays live in the boundary allocation (ff_set_boundary)
   FF_HIP(hipMemcpyAsync(d.rest_verts, rest, (size_t)d.nv * d.dim * sizeof(float), hipMemcpyHostToDevice, d.stream));
   FF_HIP(hipMemcpyAsync(d.s_coord, s, (size_t)d.nv * sizeof(float), hipMemcpyHostToDevice, d.stream));
-  FF_HIP(hipMemcpyAsync(d.wave_params, params, (size_t)d.n_bodies * 4 * sizeof(double), hipMemcpyHostToDevice,
-                        d.stream));
+  d.wave_host.resize((size_t)d.n_bodies * kWaveParStride);
+  wave_device_params(params, d.dt, d.n_bodies, d.wave_host.data());
+  FF_HIP(hipMemcpyAsync(d.wave_params, d.wave_host.data(), d.wave_host.size() * sizeof(double),
+                        hipMemcpyHostToDevice, d.stream));
   if (!d.wave) batch_mark_dirty(d.batch);
   d.wave = true;
   links_moved(d);
@@ -778,8 +780,11 @@ int ff_set_wave_params(ff_domain* dd, const double* params) {
   if (!d.wave) return fail(FF_ERR_STATE, "no wave set (ff_set_wave)");
   if (!params) return fail(FF_ERR_INVALID, "null wave parameters");
   FF_HIP(hipSetDevice(d.device));
-  FF_HIP(hipMemcpyAsync(d.wave_params, params, (size_t)d.n_bodies * 4 * sizeof(double), hipMemcpyHostToDevice,
-                        d.stream));
+  // pageable source: the copy is staged before hipMemcpyAsync returns
+  d.wave_host.resize((size_t)d.n_bodies * kWaveParStride);
+  wave_device_params(params, d.dt, d.n_bodies, d.wave_host.data());
+  FF_HIP(hipMemcpyAsync(d.wave_params, d.wave_host.data(), d.wave_host.size() * sizeof(double),
+                        hipMemcpyHostToDevice, d.stream));
   links_moved(d);
   return launch_wave_now(d);
 }

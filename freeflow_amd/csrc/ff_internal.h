@@ -151,7 +151,8 @@ struct Domain {
   bool wave = false;
   float* rest_verts = nullptr;
   float* s_coord = nullptr;
-  double* wave_params = nullptr;  // 4 per body
+  double* wave_params = nullptr;  // kWaveParStride per body
+  std::vector<double> wave_host;  // host staging of wave_params
   int* vert_body = nullptr;
   // scratch
   double* red = nullptr;          // reduction scratch
@@ -224,6 +225,10 @@ inline void launch_timed(void (*kern)(P...), dim3 grid, dim3 block, hipStream_t 
 }
 
 int launch_wave_now(Domain& d);
+// Wave parameters on the device: per body 4 FP64 values (A, f, phi, L/lambda)
+// and 8 FP32 values derived from them for the fast-math kinematics.
+constexpr int kWaveParStride = 8;  // doubles per body
+void wave_device_params(const double* p4, double dt, int n_bodies, double* out);
 int bc_span(const Domain& d);  // boundary-cell scan span of a domain stepped on its own
 int launch_step_vertex_forces(Domain& d, float* out_dev);
 int launch_convert_in(Domain& d, const float* aos_dev, void* buf);

@@ -75,8 +75,8 @@ __device__ __forceinline__ __half to_half_rn(float x) { return __float2half_rn(o
 
 // All nine populations from (rho, u, Pi) -- CalculateDistributionD2Q9ALL.
 // Argument order (rho, ux, uy, pixx, pixy, piyy) as in the reference.
-__device__ __forceinline__ void recon2_all(float rho, float ux, float uy, float pixx, float pixy, float piyy,
-                                           float* f) {
+__host__ __device__ __forceinline__ void recon2_all(float rho, float ux, float uy, float pixx, float pixy,
+                                                    float piyy, float* f) {
   const float m1x = 3.0f * rho * ux;
   const float m1y = 3.0f * rho * uy;
   const float m2xx = 3.0f * rho * pixx;
@@ -95,6 +95,35 @@ __device__ __forceinline__ void recon2_all(float rho, float ux, float uy, float 
   f[6] = FF_W5_2 * (dbase - m1x + m1y - m2xy + qxxy - qxyy);
   f[7] = FF_W5_2 * (dbase - m1x - m1y + m2xy - qxxy - qxyy);
   f[8] = FF_W5_2 * (dbase + m1x - m1y - m2xy - qxxy + qxyy);
+}
+
+// Population i alone, rounded exactly as recon2_all rounds f[i]: the same
+// operations in the same order, with the direction's signs (+-1, so the
+// products are exact) selected per lane instead of one expression per
+// direction.  The boundary-cell update needs only the pulled population and
+// the cell's own opposite one, not all nine.
+__host__ __device__ __forceinline__ float recon2_pop(float rho, float ux, float uy, float pixx, float pixy, float piyy,
+                                                     int i) {
+  const float m1x = 3.0f * rho * ux;
+  const float m1y = 3.0f * rho * uy;
+  const float m2xx = 3.0f * rho * pixx;
+  const float m2yy = 3.0f * rho * piyy;
+  const float m2xy = 9.0f * rho * pixy;
+  const float qxxy = 9.0f * (pixx * uy + 2.0f * pixy * ux - 2.0f * rho * ux * ux * uy);
+  const float qxyy = 9.0f * (piyy * ux + 2.0f * pixy * uy - 2.0f * rho * uy * uy * ux);
+  const float bulk = rho - 0.5f * (m2xx + m2yy);
+  const float sx = (float)D2::cx(i), sy = (float)D2::cy(i);
+  if (i == 0) return FF_W0_2 * bulk;
+  if (i <= 4) {  // axis: f = w ((bulk + s m1 + 1.5 m2) - 0.5 s q), the bracket in FP64
+    const bool ax = sx != 0.0f;
+    const float sg = ax ? sx : sy;
+    const float m1 = ax ? m1x : m1y, m2 = ax ? m2xx : m2yy, q = ax ? qxyy : qxxy;
+    return (float)((double)FF_W1_2 * ((double)(bulk + sg * m1 + 1.5f * m2) - 0.5 * (double)(sg * q)));
+  }
+  const float dbase = bulk + 1.5f * (m2xx + m2yy);
+  const float g = i == 5 ? dbase + (m1x + m1y) + m2xy + (qxxy + qxyy)
+                         : dbase + sx * m1x + sy * m1y + (sx * sy) * m2xy + sy * qxxy + sx * qxyy;
+  return FF_W5_2 * g;
 }
 
 // One population -- CalculateDistributionD2Q9AtIndex (all-float axis terms).
@@ -153,6 +182,63 @@ __device__ __forceinline__ void collide2(const float* p, float omega, float* out
   out[5] = nxy * inv;
 }
 
+// ------------------------------------------------------------- 2D, fast form
+// recon2_all / collide2 for instruction count rather than the reference's
+// rounding (no double promotions): opposite directions share an even part
+// E and differ in the sign of the odd part O, f_i = w (E + O),
+// f_inv(i) = w (E - O); moments from the pair sums and differences; FMA.
+// The 2D quirk of the reference is kept: the Q terms use Pi without rho.
+__device__ __forceinline__ void recon2_fast(float rho, float ux, float uy, float pixx, float pixy, float piyy,
+                                            float* f) {
+  const float r3 = 3.0f * rho;
+  const float m1x = r3 * ux, m1y = r3 * uy, m2xx = r3 * pixx, m2yy = r3 * piyy, m2xy = (3.0f * r3) * pixy;
+  // qxxy = 9 (Pxx uy + 2 Pxy ux - 2 rho ux^2 uy), qxyy = 9 (Pyy ux + 2 Pxy uy - 2 rho uy^2 ux)
+  const float t2 = -2.0f * rho * ux * uy;
+  const float qxxy = 9.0f * fmaf(pixx, uy, fmaf(2.0f * pixy, ux, t2 * ux));
+  const float qxyy = 9.0f * fmaf(piyy, ux, fmaf(2.0f * pixy, uy, t2 * uy));
+  const float bulk = fmaf(-0.5f, m2xx + m2yy, rho);
+  f[0] = FF_W0_2 * bulk;
+  const float ex = fmaf(1.5f, m2xx, bulk), ey = fmaf(1.5f, m2yy, bulk);
+  const float ox = FF_W1_2 * fmaf(-0.5f, qxyy, m1x), oy = FF_W1_2 * fmaf(-0.5f, qxxy, m1y);
+  f[1] = fmaf(FF_W1_2, ex, ox);
+  f[3] = fmaf(FF_W1_2, ex, -ox);
+  f[2] = fmaf(FF_W1_2, ey, oy);
+  f[4] = fmaf(FF_W1_2, ey, -oy);
+  const float dbase = fmaf(1.5f, m2xx + m2yy, bulk);
+  const float e5 = dbase + m2xy, e6 = dbase - m2xy;
+  const float o5 = FF_W5_2 * ((m1x + m1y) + (qxxy + qxyy)), o6 = FF_W5_2 * ((m1y - m1x) + (qxxy - qxyy));
+  f[5] = fmaf(FF_W5_2, e5, o5);
+  f[7] = fmaf(FF_W5_2, e5, -o5);
+  f[6] = fmaf(FF_W5_2, e6, o6);
+  f[8] = fmaf(FF_W5_2, e6, -o6);
+}
+
+__device__ __forceinline__ void collide2_fast(const float* p, float omega, float* out) {
+  const float s13 = p[1] + p[3], d13 = p[1] - p[3], s24 = p[2] + p[4], d24 = p[2] - p[4];
+  const float s57 = p[5] + p[7], d57 = p[5] - p[7], s68 = p[6] + p[8], d68 = p[6] - p[8];
+  const float sd = s57 + s68;
+  const float rho = (p[0] + (s13 + s24)) + sd;
+  const float jx = d13 + (d57 - d68), jy = d24 + (d57 + d68);
+  const float pxy = s57 - s68;
+  const float inv = __builtin_amdgcn_rcpf(rho);  // v_rcp_f32, 1 ulp
+  const float ux = jx * inv, uy = jy * inv;
+  // LbmUtilsFuncGpu2D::Collision (LbmUtilsFunc.cu:512-543), F = 0
+  const float omw = 1.0f - omega;
+  const float ru2 = jx * ux, rv2 = jy * uy;
+  const float dev = 0.5f * (s13 - s24);  // (Pxx - Pyy) / 2 of the raw sums
+  const float be = fmaf(rho, 1.0f / 3.0f, 0.5f * (ru2 + rv2));
+  const float an = (0.5f * omega) * (ru2 - rv2);
+  const float nxx = fmaf(dev, omw, be + an);
+  const float nyy = fmaf(-dev, omw, be - an);
+  const float nxy = fmaf(pxy, omw, (omega * ux) * jy);
+  out[0] = rho;
+  out[1] = ux;
+  out[2] = uy;
+  out[3] = fmaf(nxx, inv, -1.0f / 3.0f);
+  out[4] = fmaf(nyy, inv, -1.0f / 3.0f);
+  out[5] = nxy * inv;
+}
+
 // -------------------------------------------------------------------- 3D
 #define FF_W0_3 (8.0f / 27.0f)
 #define FF_W1_3 (2.0f / 27.0f)
@@ -161,7 +247,7 @@ __device__ __forceinline__ void collide2(const float* p, float omega, float* out
 
 // CalculateDistributionD3Q27All; m = stored order
 // [rho, ux, uy, uz, pxx, pyy, pzz, pxy, pyz, pxz].
-__device__ __forceinline__ void recon3_all(const float* m, float* f) {
+__host__ __device__ __forceinline__ void recon3_all(const float* m, float* f) {
   const float rho = m[0], ux = m[1], uy = m[2], uz = m[3];
   const float pixx = m[4], piyy = m[5], pizz = m[6], pixy = m[7], piyz = m[8], pixz = m[9];
   const float m1_x = 3.0f * rho * ux, m1_y = 3.0f * rho * uy, m1_z = 3.0f * rho * uz;
@@ -221,6 +307,58 @@ __device__ __forceinline__ void recon3_all(const float* m, float* f) {
   f[24] = FF_W3_3 * (cc - (m1_x - m1_y + m1_z) + (-m2_xy - m2_yz + m2_xz) - (-Q_xxy + Q_xxz + Q_xyy + Q_yyz + Q_xzz - Q_yzz) + Q_xyz);
   f[25] = FF_W3_3 * (cc + (-m1_x + m1_y + m1_z) + (-m2_xy + m2_yz - m2_xz) + (Q_xxy + Q_xxz - Q_xyy + Q_yyz - Q_xzz + Q_yzz) - Q_xyz);
   f[26] = FF_W3_3 * (cc - (-m1_x + m1_y + m1_z) + (-m2_xy + m2_yz - m2_xz) - (Q_xxy + Q_xxz - Q_xyy + Q_yyz - Q_xzz + Q_yzz) + Q_xyz);
+}
+
+// Population i alone, rounded exactly as recon3_all rounds f[i] (same
+// operations and order; the direction's +-1 signs selected per lane).
+__host__ __device__ __forceinline__ float recon3_pop(const float* m, int i) {
+  const float rho = m[0], ux = m[1], uy = m[2], uz = m[3];
+  const float pixx = m[4], piyy = m[5], pizz = m[6], pixy = m[7], piyz = m[8], pixz = m[9];
+  const float m1_x = 3.0f * rho * ux, m1_y = 3.0f * rho * uy, m1_z = 3.0f * rho * uz;
+  const float m2_xx = 3.0f * rho * pixx, m2_yy = 3.0f * rho * piyy, m2_zz = 3.0f * rho * pizz;
+  const float m2_xy = 9.0f * rho * pixy, m2_yz = 9.0f * rho * piyz, m2_xz = 9.0f * rho * pixz;
+#define FF_Q3(Paa, ub, Pab, ua) (9.0f * (rho * (Paa) * (ub) + 2.0f * rho * (Pab) * (ua) - 2.0f * rho * (ua) * (ua) * (ub)))
+  const float Q_xxy = FF_Q3(pixx, uy, pixy, ux);
+  const float Q_xyy = FF_Q3(piyy, ux, pixy, uy);
+  const float Q_xxz = FF_Q3(pixx, uz, pixz, ux);
+  const float Q_xzz = FF_Q3(pizz, ux, pixz, uz);
+  const float Q_yyz = FF_Q3(piyy, uz, piyz, uy);
+  const float Q_yzz = FF_Q3(pizz, uy, piyz, uz);
+#undef FF_Q3
+  const float bulk = rho - 0.5f * (m2_xx + m2_yy + m2_zz);
+  const int ix = D3::cx(i), iy = D3::cy(i), iz = D3::cz(i);
+  const float cx = (float)ix, cy = (float)iy, cz = (float)iz;
+  const int n = (ix != 0) + (iy != 0) + (iz != 0);
+  if (n == 0) return FF_W0_3 * bulk;
+  if (n == 1) {  // axis a; b, c the other two axes in x, y, z order
+    const float s = cx + cy + cz;
+    const float m1 = ix ? m1_x : (iy ? m1_y : m1_z);
+    const float m2 = ix ? m2_xx : (iy ? m2_yy : m2_zz);
+    const float qb = ix ? Q_xyy : (iy ? Q_xxy : Q_xxz);
+    const float qc = ix ? Q_xzz : (iy ? Q_yzz : Q_yyz);
+    const float ab = bulk + 1.5f * m2;
+    return FF_W1_3 * (ab + s * m1 - s * (0.5f * (qb + qc)));
+  }
+  if (n == 2) {  // plane (a, b), a before b; c the axis left out
+    const bool xy = iz == 0, xz = iy == 0;
+    const float sa = ix ? cx : cy, sb = iz ? cz : cy, t = sa * sb;
+    const float m1a = ix ? m1_x : m1_y, m1b = iz ? m1_z : m1_y;
+    const float m2aa = ix ? m2_xx : m2_yy, m2bb = iz ? m2_zz : m2_yy;
+    const float m2cc = xy ? m2_zz : (xz ? m2_yy : m2_xx);
+    const float m2ab = xy ? m2_xy : (xz ? m2_xz : m2_yz);
+    const float qaab = xy ? Q_xxy : (xz ? Q_xxz : Q_yyz);
+    const float qabb = xy ? Q_xyy : (xz ? Q_xzz : Q_yzz);
+    const float qacc = xy ? Q_xzz : (xz ? Q_xyy : Q_xxy);
+    const float qbcc = xy ? Q_yzz : (xz ? Q_yyz : Q_xxz);
+    const float eb = rho + m2aa + m2bb - 0.5f * m2cc;
+    return FF_W2_3 * (eb + sa * (m1a + t * m1b) + t * m2ab + sb * (qaab + t * qabb) - sa * (0.5f * (qacc + t * qbcc)));
+  }
+  const float Q_xyz = 27.0f * (rho * (pixy * uz + piyz * ux + pixz * uy - 2.0f * ux * uy * uz));
+  const float cc = rho + (m2_xx + m2_yy + m2_zz);
+  const float M = cx * m1_x + cy * m1_y + cz * m1_z;
+  const float O = (cx * cy) * m2_xy + (cy * cz) * m2_yz + (cx * cz) * m2_xz;
+  const float Qs = cy * Q_xxy + cz * Q_xxz + cx * Q_xyy + cz * Q_yyz + cx * Q_xzz + cy * Q_yzz;
+  return FF_W3_3 * (cc + M + O + Qs + (cx * cy * cz) * Q_xyz);
 }
 
 // CalculateDistributionD3Q27AtIndex (LbmUtilsFunc.cu:11-210).

@@ -20,13 +20,13 @@
 
 namespace ff {
 
-template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT>
+template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT, bool FAST>
 __global__ void __launch_bounds__(256)
     k_bulk2d(const S* __restrict__ in, S* __restrict__ out, const uint8_t* __restrict__ flags, int nx, int ny,
              size_t plane, float omega, int tiles_x, int ntiles) {
   const int t = xcd_tile(blockIdx.x, ntiles);
   if (t < 0) return;
-  bulk2d_tile<S, STAGE, PERIODIC, TX, TY, CPT>(in, out, flags, nx, ny, plane, omega, t % tiles_x, t / tiles_x);
+  bulk2d_tile<S, STAGE, PERIODIC, TX, TY, CPT, FAST>(in, out, flags, nx, ny, plane, omega, t % tiles_x, t / tiles_x);
 }
 
 // Stream launch of a step with a boundary: the first `nbc` blocks update the
@@ -35,20 +35,20 @@ __global__ void __launch_bounds__(256)
 // stream-collide tiles (which also write the boundary cells; the copy-back of
 // the sums launch overwrites them).  nbc + nwave is a multiple of 8, so the
 // tiles keep the XCD-aware order.
-template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT>
+template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT, bool FAST>
 __global__ void __launch_bounds__(256)
     k_step2d(const S* __restrict__ in, S* __restrict__ out, int tiles_x, int ntiles, int nbc, int nwave, LinkArgs la,
              FinalArgs fa) {
   const int b = blockIdx.x;
   if (b < nbc) {
-    bcells_body<2, S, STAGE, PERIODIC, 256>(in, la, b, nbc);
+    bcells_body<2, S, STAGE, PERIODIC, 256, FAST>(in, la, b, nbc);
   } else if (b < nbc + nwave) {
-    stream_extra_body<2, 256>(la, fa, b - nbc);
+    stream_extra_body<2, 256, FAST>(la, fa, b - nbc);
   } else {
     const int t = xcd_tile(b - nbc - nwave, ntiles);
     if (t < 0) return;
-    bulk2d_tile<S, STAGE, PERIODIC, TX, TY, CPT>(in, out, la.flags, la.nx, la.ny, la.plane, la.omega, t % tiles_x,
-                                                 t / tiles_x);
+    bulk2d_tile<S, STAGE, PERIODIC, TX, TY, CPT, FAST>(in, out, la.flags, la.nx, la.ny, la.plane, la.omega,
+                                                       t % tiles_x, t / tiles_x);
   }
 }
 
@@ -59,42 +59,47 @@ struct StreamExtra {
   int nbc, nwave;
 };
 
-template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT>
+template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT, bool FAST>
 static void bulk2d_shape(Domain& d, const S* i, S* o, const StreamExtra* x) {
   const int tx = (d.nx + TX - 1) / TX, ty = (d.ny + TY * CPT - 1) / (TY * CPT);
   const int nt = tx * ty;
   if (!x) {
-    launch_timed(k_bulk2d<S, STAGE, PERIODIC, TX, TY, CPT>, dim3(xcd_grid(nt)), dim3(256), d.stream, d.tev0, d.tev1, i,
-                 o, d.flags, d.nx, d.ny, d.plane, d.omega, tx, nt);
+    launch_timed(k_bulk2d<S, STAGE, PERIODIC, TX, TY, CPT, FAST>, dim3(xcd_grid(nt)), dim3(256), d.stream, d.tev0,
+                 d.tev1, i, o, d.flags, d.nx, d.ny, d.plane, d.omega, tx, nt);
   } else {
     const int nbc = xcd_grid(x->nbc + x->nwave) - x->nwave;  // extra boundary-cell blocks share the grid stride
     const unsigned grid = (unsigned)(nbc + x->nwave + xcd_grid(nt));
-    launch_timed(k_step2d<S, STAGE, PERIODIC, TX, TY, CPT>, dim3(grid), dim3(256), d.stream, d.tev0, d.tev1, i, o, tx, nt,
-                 nbc, x->nwave, *x->la, *x->fa);
+    launch_timed(k_step2d<S, STAGE, PERIODIC, TX, TY, CPT, FAST>, dim3(grid), dim3(256), d.stream, d.tev0, d.tev1, i,
+                 o, tx, nt, nbc, x->nwave, *x->la, *x->fa);
   }
 }
 
-template <typename S, bool STAGE, bool PERIODIC>
+template <typename S, bool STAGE, bool PERIODIC, bool FAST>
 static void bulk2d_variant(Domain& d, const S* i, S* o, const StreamExtra* x) {
   switch (d.tile2) {
-    case 1: bulk2d_shape<S, STAGE, PERIODIC, 32, 8, 2>(d, i, o, x); break;
-    case 2: bulk2d_shape<S, STAGE, PERIODIC, 64, 4, 2>(d, i, o, x); break;
-    case 3: bulk2d_shape<S, STAGE, PERIODIC, 32, 8, 4>(d, i, o, x); break;
-    case 4: bulk2d_shape<S, STAGE, PERIODIC, 64, 4, 4>(d, i, o, x); break;
-    default: bulk2d_shape<S, STAGE, PERIODIC, 32, 8, 1>(d, i, o, x); break;
+    case 1: bulk2d_shape<S, STAGE, PERIODIC, 32, 8, 2, FAST>(d, i, o, x); break;
+    case 2: bulk2d_shape<S, STAGE, PERIODIC, 64, 4, 2, FAST>(d, i, o, x); break;
+    case 3: bulk2d_shape<S, STAGE, PERIODIC, 32, 8, 4, FAST>(d, i, o, x); break;
+    case 4: bulk2d_shape<S, STAGE, PERIODIC, 64, 4, 4, FAST>(d, i, o, x); break;
+    default: bulk2d_shape<S, STAGE, PERIODIC, 32, 8, 1, FAST>(d, i, o, x); break;
   }
 }
 
+// FF_OPT_FAST_MATH applies to unstaged tiles (the staged tile reproduces the
+// reference's half-precision rounding of the reference's arithmetic).
 template <typename S>
 static int bulk2d_typed(Domain& d, const void* in, void* out, const StreamExtra* x) {
   const bool stage = d.options & FF_OPT_STAGE_FP16;
   const bool per = d.options & FF_OPT_PERIODIC;
+  const bool fast = d.options & FF_OPT_FAST_MATH;
   const S* i = (const S*)in;
   S* o = (S*)out;
-  if (stage && per) bulk2d_variant<S, true, true>(d, i, o, x);
-  else if (stage) bulk2d_variant<S, true, false>(d, i, o, x);
-  else if (per) bulk2d_variant<S, false, true>(d, i, o, x);
-  else bulk2d_variant<S, false, false>(d, i, o, x);
+  if (stage && per) bulk2d_variant<S, true, true, false>(d, i, o, x);
+  else if (stage) bulk2d_variant<S, true, false, false>(d, i, o, x);
+  else if (per && fast) bulk2d_variant<S, false, true, true>(d, i, o, x);
+  else if (per) bulk2d_variant<S, false, true, false>(d, i, o, x);
+  else if (fast) bulk2d_variant<S, false, false, true>(d, i, o, x);
+  else bulk2d_variant<S, false, false, false>(d, i, o, x);
   FF_HIP(hipGetLastError());
   return FF_OK;
 }

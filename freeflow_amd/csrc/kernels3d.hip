@@ -66,9 +66,9 @@ __global__ void __launch_bounds__(NT)
              LinkArgs la, FinalArgs fa) {
   const int b = blockIdx.x;
   if (b < nbc) {
-    bcells_body<3, S, false, PERIODIC, NT>(in, la, b, nbc);
+    bcells_body<3, S, false, PERIODIC, NT, FAST>(in, la, b, nbc);
   } else if (b < nbc + nwave) {
-    stream_extra_body<3, NT>(la, fa, b - nbc);
+    stream_extra_body<3, NT, FAST>(la, fa, b - nbc);
   } else {
     const int t = xcd_tile(b - nbc - nwave, ntiles);
     if (t < 0) return;

@@ -171,6 +171,7 @@ FinalArgs make_final_args(Domain& d, void* out, float* vn0, float* vn1, float* u
   fa.s = d.s_coord;
   fa.vbody = d.vert_body;
   fa.wpar = d.wave_params;
+  fa.wave_fast = (d.options & FF_OPT_FAST_MATH) ? 1 : 0;
   fa.dt = d.dt;
   fa.vn[0] = vn0;
   fa.vn[1] = vn1;
@@ -218,21 +219,50 @@ int launch_step_vertex_forces(Domain& d, float* out) {
 
 // Kinematics of the current step (used when the wave is (re)set).
 template <int DIM>
-__global__ void k_wave_now(const float* rest, const float* s, const int* vbody, const double* par, int nv, double t,
-                           float* v, float* vel) {
+__global__ void k_wave_now(const float* rest, const float* s, const int* vbody, const double* par, int fast32, int nv,
+                           unsigned gstep, double dt, float* v, float* vel) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < nv) wave_vertex<DIM>(rest, s, vbody, par, k, t, v, vel);
+  if (k < nv) wave_vertex<DIM>(rest, s, vbody, par, fast32, k, gstep, dt, v, vel);
+}
+
+// Per-body wave parameters as the device reads them: the four FP64 values,
+// then A, L/lambda, phi / 2 pi mod 1, f dt mod 1 as a float pair hi + lo,
+// and 2 pi f A, in FP32 (g (f dt) mod 1 = g (f dt mod 1) mod 1 for whole g).
+void wave_device_params(const double* p4, double dt, int n_bodies, double* out) {
+  const double two_pi = 6.283185307179586;
+  for (int b = 0; b < n_bodies; b++) {
+    const double* p = p4 + 4 * b;
+    double* o = out + kWaveParStride * b;
+    const double A = p[0], f = p[1], phi = p[2], kl = p[3];
+    o[0] = A; o[1] = f; o[2] = phi; o[3] = kl;
+    float* q = (float*)(o + 4);
+    double phic = phi / two_pi;
+    phic -= std::floor(phic);
+    double fdt = f * dt;
+    fdt -= std::floor(fdt);
+    const float hi = (float)fdt;
+    q[0] = (float)A;
+    q[1] = (float)kl;
+    q[2] = (float)phic;
+    q[3] = hi;
+    q[4] = (float)(fdt - (double)hi);
+    q[5] = (float)(two_pi * f * A);
+    q[6] = q[7] = 0.0f;
+  }
 }
 
 int launch_wave_now(Domain& d) {
   const int b = 256, g = (d.nv + b - 1) / b;
-  const double t = (double)(d.gen_host - 1) * d.dt;
+  const unsigned gstep = (unsigned)(d.gen_host - 1);
+  const int fast32 = (d.options & FF_OPT_FAST_MATH) ? 1 : 0;
   float* v = d.vbuf[d.gen_host & 1];
   float* u = d.ubuf[d.gen_host & 1];
   if (d.dim == 2)
-    k_wave_now<2><<<g, b, 0, d.stream>>>(d.rest_verts, d.s_coord, d.vert_body, d.wave_params, d.nv, t, v, u);
+    k_wave_now<2><<<g, b, 0, d.stream>>>(d.rest_verts, d.s_coord, d.vert_body, d.wave_params, fast32, d.nv, gstep,
+                                         d.dt, v, u);
   else
-    k_wave_now<3><<<g, b, 0, d.stream>>>(d.rest_verts, d.s_coord, d.vert_body, d.wave_params, d.nv, t, v, u);
+    k_wave_now<3><<<g, b, 0, d.stream>>>(d.rest_verts, d.s_coord, d.vert_body, d.wave_params, fast32, d.nv, gstep,
+                                         d.dt, v, u);
   FF_HIP(hipGetLastError());
   return FF_OK;
 }

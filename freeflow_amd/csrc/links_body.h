@@ -26,22 +26,21 @@ constexpr float kLinkUnused = -1.0e30f;
 // (FsiCoupling3D.cu:194-285), moving-wall value from the single-population
 // reconstruction, momentum-exchange force (FsiCoupling2D.cu:193-214,
 // FsiCoupling3D.cu:243-271).  Overwrites the bulk kernel's value for the cell.
-template <int DIM, typename S, bool STAGE>
-__device__ __forceinline__ void cell_pops(const S* in, size_t plane, size_t c, float* f) {
+// Population i of a cell from its moments m: the reference's rounding
+// (recon2_pop / recon3_pop, with the FP16 tile staging in 2D when STAGE),
+// or under FAST the reference's all-FP32 single-population form in 2D.
+template <int DIM, bool STAGE, bool FAST>
+__device__ __forceinline__ float cell_pop(const float* m, int i) {
   if constexpr (DIM == 2) {
-    float m[6];
-#pragma unroll
-    for (int q = 0; q < 6; q++) m[q] = Store<S>::template load<2>(in + q * plane + c, q);
-    recon2_all(m[0], m[1], m[2], m[3], m[5], m[4], f);
-    if constexpr (STAGE) {
-#pragma unroll
-      for (int i = 0; i < 9; i++) f[i] = __half2float(to_half_rn(f[i]));
+    if constexpr (FAST) {
+      return recon2_at(m[0], m[1], m[2], m[3], m[5], m[4], i);
+    } else {
+      const float f = recon2_pop(m[0], m[1], m[2], m[3], m[5], m[4], i);
+      if constexpr (STAGE) return __half2float(to_half_rn(f));
+      return f;
     }
   } else {
-    float m[10];
-#pragma unroll
-    for (int q = 0; q < 10; q++) m[q] = Store<S>::template load<3>(in + q * plane + c, q);
-    recon3_all(m, f);
+    return recon3_pop(m, i);
   }
 }
 
@@ -51,7 +50,7 @@ __device__ __forceinline__ void cell_pops(const S* in, size_t plane, size_t c, f
 // Each block scans la.bc_span (<= NT) link-table slots at a time (grid
 // stride nblk * bc_span), compacts the occupied ones in LDS and updates those
 // cells CPB at a time.
-template <int DIM, typename S, bool STAGE, bool PERIODIC, int NT>
+template <int DIM, typename S, bool STAGE, bool PERIODIC, int NT, bool FAST = false>
 __device__ __forceinline__ void bcells_body(const S* __restrict__ in, const LinkArgs& la, int blk, int nblk) {
   constexpr int Q = DIM == 2 ? 9 : 27;
   constexpr int K = Q - 1;
@@ -118,17 +117,19 @@ __device__ __forceinline__ void bcells_body(const S* __restrict__ in, const Link
         // every load that depends only on the cell index is issued up front
         const uint8_t fup = la.flags[((size_t)zw * la.ny + yw) * la.nx + xw];
         const uint32_t key = dir > 0 ? lt_key(keys, li, tag) : 0xFFFFFFFFu;
-        float mc[NM];
+        float mc[NM], mn[NM];
 #pragma unroll
-        for (int q = 0; q < NM; q++) mc[q] = Store<S>::template load<DIM>(in + q * la.plane + cell, q);
-        float fn[Q];
-        cell_pops<DIM, S, STAGE>(in, la.plane, sc, fn);
-        float fc[Q];
-        cell_pops<DIM, S, STAGE>(in, la.plane, (size_t)cell, fc);
+        for (int q = 0; q < NM; q++) {
+          mc[q] = Store<S>::template load<DIM>(in + q * la.plane + cell, q);
+          mn[q] = Store<S>::template load<DIM>(in + q * la.plane + sc, q);
+        }
+        // the lane needs two populations: the pulled one and the cell's own opposite one
+        const float fn_dir = cell_pop<DIM, STAGE, FAST>(mn, dir);
+        const float fc_inv = cell_pop<DIM, STAGE, FAST>(mc, inv);
         float pop = 0.0f;
         bool done = false;
         if (DIM == 3 && is_wall3(fup)) {  // 3D keeps flag bounce-back (FsiCoupling3D.cu:209-222)
-          pop = fmaxf(0.0f, fc[inv]);
+          pop = fmaxf(0.0f, fc_inv);
           done = true;
         }
         if (dir > 0) la.lb[li * DIM] = kLinkUnused;
@@ -145,7 +146,7 @@ __device__ __forceinline__ void bcells_body(const S* __restrict__ in, const Link
           }
           if (hit) {
             done = true;
-            const float fin = fmaxf(0.0f, fc[inv]);
+            const float fin = fmaxf(0.0f, fc_inv);
             float uw[3] = {0.f, 0.f, 0.f};
             float fout;
             if constexpr (DIM == 2) {
@@ -188,7 +189,7 @@ __device__ __forceinline__ void bcells_body(const S* __restrict__ in, const Link
             }
           }
         }
-        if (!done) pop = fmaxf(0.0f, fn[dir]);
+        if (!done) pop = fmaxf(0.0f, fn_dir);
         sp[lc][dir] = pop;
       }
       __syncthreads();
@@ -197,7 +198,9 @@ __device__ __forceinline__ void bcells_body(const S* __restrict__ in, const Link
 #pragma unroll
         for (int i = 0; i < Q; i++) p[i] = sp[lc][i];
         float o[NM];
-        if constexpr (DIM == 2) collide2(p, la.omega, o);
+        if constexpr (DIM == 2 && FAST) collide2_fast(p, la.omega, o);
+        else if constexpr (DIM == 2) collide2(p, la.omega, o);
+        else if constexpr (FAST) collide3_fast(p, la.omega, o);
         else collide3(p, la.omega, o);
         float* row = la.sb + (size_t)s_slot[t] * NM;
 #pragma unroll
@@ -230,6 +233,7 @@ struct FinalArgs {
   const float* s;
   const int* vbody;
   const double* wpar;
+  int wave_fast;  // FP32 kinematics (fast math)
   double dt;
   float* vn[2];
   float* un[2];
@@ -256,36 +260,62 @@ int boundary_cell_blocks(const Domain& d, int span);
 int copyback_blocks(const Domain& d, int fb, int per_thread);
 int detect_blocks(const Domain& d, int nt, bool batched);
 
-// Travelling wave along the body axis, FP64:
+// Travelling wave along the body axis:
 //   y = y_rest + A s sin(theta), v_y = -2 pi f A s cos(theta),
-//   theta = 2 pi (s L/lambda - f t) + phi.
+//   theta = 2 pi (s L/lambda - f t) + phi,  t = g dt at step g.
+// par holds per body the four FP64 parameters (A, f, phi, L/lambda)
+// followed by eight FP32 values derived from them on the host
+// (wave_fp32_params); with fast32 the phase is reduced in cycles in FP32:
+// f t mod 1 comes from g times (f dt) held as a float pair hi + lo, the
+// product's rounding error recovered with an FMA, so the phase error stays
+// about 1e-7 cycle for any g below 2^24.
 template <int DIM>
 __device__ __forceinline__ void wave_vertex(const float* rest, const float* s, const int* vbody, const double* par,
-                                            int k, double t, float* v, float* vel) {
-  const double* p = par + 4 * vbody[k];
-  const double A = p[0], f = p[1], phi = p[2], kl = p[3];
-  const double sk = (double)s[k];
-  const double two_pi = 6.283185307179586;
-  const double th = two_pi * (sk * kl - f * t) + phi;
-  double sn, cs;
-  sincos(th, &sn, &cs);
+                                            int fast32, int k, unsigned g, double dt, float* v, float* vel) {
+  const int b = vbody[k];
 #pragma unroll
   for (int c = 0; c < DIM; c++) {
     v[k * DIM + c] = rest[k * DIM + c];
     vel[k * DIM + c] = 0.0f;
   }
-  v[k * DIM + 1] = (float)((double)rest[k * DIM + 1] + A * sk * sn);
-  vel[k * DIM + 1] = (float)(-two_pi * f * A * sk * cs);
+  if (fast32) {
+    const float* q = (const float*)(par + kWaveParStride * b + 4);
+    const float A = q[0], kl = q[1], phic = q[2], hi = q[3], lo = q[4], vamp = q[5];
+    const float gf = (float)g;
+    const float pr = gf * hi;
+    const float er = fmaf(gf, hi, -pr);
+    const float tc = (pr - floorf(pr)) + fmaf(gf, lo, er);  // f t mod 1 (up to a whole cycle)
+    float cyc = fmaf(s[k], kl, -tc) + phic;
+    cyc -= rintf(cyc);
+    float sn, cs;
+    sincospif(2.0f * cyc, &sn, &cs);
+    const float as = A * s[k];
+    v[k * DIM + 1] = fmaf(as, sn, rest[k * DIM + 1]);
+    vel[k * DIM + 1] = -vamp * s[k] * cs;
+  } else {
+    const double* p = par + kWaveParStride * b;
+    const double A = p[0], f = p[1], phi = p[2], kl = p[3];
+    const double sk = (double)s[k];
+    const double t = (double)g * dt;
+    const double two_pi = 6.283185307179586;
+    const double th = two_pi * (sk * kl - f * t) + phi;
+    double sn, cs;
+    sincos(th, &sn, &cs);
+    v[k * DIM + 1] = (float)((double)rest[k * DIM + 1] + A * sk * sn);
+    vel[k * DIM + 1] = (float)(-two_pi * f * A * sk * cs);
+  }
 }
 
 // Next-step kinematics for one block of vertices (stream launch): the vertex
 // buffers of the next step are not read during this step.
-template <int DIM, int FB>
+// FAST kernels take the FP32 kinematics (FF_OPT_FAST_MATH always sets wave_fast).
+template <int DIM, int FB, bool FAST = false>
 __device__ __forceinline__ void wave_block_body(const LinkArgs& la, const FinalArgs& fa, int blk) {
   const int k = blk * FB + threadIdx.x;
   if (k < la.nv) {
     const unsigned g = step_gen(la);
-    wave_vertex<DIM>(fa.rest, fa.s, fa.vbody, fa.wpar, k, (double)g * fa.dt, fa.vn[(g + 1) & 1], fa.un[(g + 1) & 1]);
+    wave_vertex<DIM>(fa.rest, fa.s, fa.vbody, fa.wpar, FAST ? 1 : fa.wave_fast, k, g, fa.dt, fa.vn[(g + 1) & 1],
+                     fa.un[(g + 1) & 1]);
   }
 }
 
@@ -576,10 +606,10 @@ __device__ __forceinline__ void finish_last_body(const LinkArgs& la, const Final
 // deferred to this launch, its vertex blocks and wrench block (fin_blocks).
 // The previous step's element sums are complete (earlier launch) and this
 // step's are not started (later launch).
-template <int DIM, int NT>
+template <int DIM, int NT, bool FAST = false>
 __device__ __forceinline__ void stream_extra_body(const LinkArgs& la, const FinalArgs& fa, int blk) {
   if (blk < fa.kin_blocks) {
-    wave_block_body<DIM, NT>(la, fa, blk);
+    wave_block_body<DIM, NT, FAST>(la, fa, blk);
     return;
   }
   blk -= fa.kin_blocks;

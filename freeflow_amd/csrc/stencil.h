@@ -38,8 +38,9 @@ constexpr int kNumTiles2 = 5;
 
 // Stage the reconstructed populations of the tile and its one-cell halo in
 // LDS (clamped or periodic halo; binary16 when the reference's half tile is
-// reproduced), then pull, collide and store every updated cell.
-template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT>
+// reproduced), then pull, collide and store every updated cell.  FAST picks
+// recon2_fast / collide2_fast.
+template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT, bool FAST = false>
 __device__ __forceinline__ void bulk2d_tile(const S* __restrict__ in, S* __restrict__ out,
                                             const uint8_t* __restrict__ flags, int nx, int ny, size_t plane,
                                             float omega, int bx, int by) {
@@ -75,7 +76,8 @@ __device__ __forceinline__ void bulk2d_tile(const S* __restrict__ in, S* __restr
     const int k = threadIdx.x + r * NT;
     if (k < HN) {
       float f[9];
-      recon2_all(m[r][0], m[r][1], m[r][2], m[r][3], m[r][5], m[r][4], f);
+      if constexpr (FAST) recon2_fast(m[r][0], m[r][1], m[r][2], m[r][3], m[r][5], m[r][4], f);
+      else recon2_all(m[r][0], m[r][1], m[r][2], m[r][3], m[r][5], m[r][4], f);
 #pragma unroll
       for (int i = 0; i < 9; i++) {
         if constexpr (STAGE) sp[i][k] = to_half_rn(f[i]);
@@ -104,7 +106,8 @@ __device__ __forceinline__ void bulk2d_tile(const S* __restrict__ in, S* __restr
       p[i] = fmaxf(0.0f, v);
     }
     float o[6];
-    collide2(p, omega, o);
+    if constexpr (FAST) collide2_fast(p, omega, o);
+    else collide2(p, omega, o);
 #pragma unroll
     for (int q = 0; q < 6; q++) Store<S>::template store<2>(out + q * plane + cell, q, o[q]);
   }

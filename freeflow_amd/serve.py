@@ -5,17 +5,22 @@ The reference drives one simulator per environment from Python
 parameters, reading back forces).  Here many episodes of different grid
 sizes share the GPU: each episode's fluid and boundary state lives on the
 device in its own domain; domains are members of one batch per kind (2D and
-3D, FP16 storage by default), and every server iteration advances every
-active episode by one step with one graph replay per batch.  Episodes that
-arrive while others run are admitted between steps as soon as their device
-footprint (ff_domain_footprint) fits the memory budget and a slot is free
-(an episode larger than the whole budget is rejected at submission);
-finished episodes leave the batch and free their memory.  Actions (new wave
-amplitude/phase) are applied between steps.
+3D, FP16 storage and FP32 fast-math arithmetic by default: FMA-reassociated
+collision and reconstruction, FP32 wave kinematics, FP64 only in the
+per-body force sums).  A server iteration advances every active
+episode to its next action boundary: the due actions (new wave amplitude and
+phase, uploaded on the batch stream), one multi-step graph replay per batch,
+one wait, and the per-step FP64 wrench of the interval read back.  Episodes
+that arrive while others run are admitted at the next action boundary as
+soon as their device footprint (ff_domain_footprint) fits the memory budget
+and a slot is free, so all running episodes share one action phase (an
+episode larger than the whole budget is rejected at submission); finished
+episodes leave the batch and free their memory.
 
-Per-step latency of an episode is the wall time of the server iteration that
-advanced it (its actions, the batch steps and the force readbacks); the
-server reports p50/p99 over all episode-steps and the aggregate MLUPS.
+Per-step latency of an episode is the wall time of the iteration that
+advanced it divided by the steps it advanced; the server reports p50/p99
+over all episode-steps, the aggregate MLUPS and the p50/p99 wait from an
+episode's arrival to its admission.
 """
 import time
 from dataclasses import dataclass, field
@@ -104,16 +109,19 @@ def make_episodes(rate, duration, seed, frac2d=0.7, steps=500, action_every=10,
 
 
 class Server:
-    def __init__(self, storage="f16", budget_bytes=None, max_active=512, device=0, keep_final=False, batched=True):
+    def __init__(self, storage="f16", budget_bytes=None, max_active=512, device=0, keep_final=False, batched=True,
+                 fast_math=True):
         self.storage = storage
+        # fast_math=False: the reference's operation order (2D with its FP16 tile staging)
+        self.fast_math = fast_math
         self.keep_final = keep_final  # read back each episode's final moments (ep.final_moments)
         # batched=False: the plain per-domain form, every episode stepped by
         # its own launches on its own stream (the same admission and actions)
         self.batched = batched
         self.budget = budget_bytes if budget_bytes is not None else 288 * 10 ** 9
         self.max_active = max_active
-        self.batches = {2: capi.Batch(2, storage, stage_fp16=True, device=device),
-                        3: capi.Batch(3, storage, device=device)}
+        self.batches = {2: capi.Batch(2, storage, stage_fp16=not fast_math, fast_math=fast_math, device=device),
+                        3: capi.Batch(3, storage, fast_math=fast_math, device=device)}
         self.device = device
         self.active = []
         self.used = 0
@@ -139,7 +147,8 @@ class Server:
 
     def admit(self, ep, now):
         sc = ep.scene
-        d = capi.Domain(**sc.domain_args(), storage=self.storage, stage_fp16=ep.kind == 2, device=self.device)
+        d = capi.Domain(**sc.domain_args(), storage=self.storage, stage_fp16=ep.kind == 2 and not self.fast_math,
+                        fast_math=self.fast_math, device=self.device)
         sc.setup(d, history=ep.action_every)
         ep.dom = d
         ep.admitted_at = now
@@ -169,32 +178,42 @@ class Server:
         self.used -= ep.footprint
 
     # ------------------------------------------------------------ loop
+    def at_boundary(self):
+        """Every active episode is at an action boundary (admissions wait for
+        one, so the episodes' action phases stay aligned)."""
+        return all(ep.done_steps % ep.action_every == 0 for ep in self.active)
+
     def step_all(self):
-        """One step of every active episode (actions, batch steps, readbacks);
-        returns the iteration's wall time."""
+        """Advance every active episode to its next action boundary (or its
+        end): due actions, one multi-step graph replay per batch, one wait,
+        the wrench readings of the interval.  Returns the iteration's wall
+        time; each episode-step of the iteration is charged the iteration's
+        time divided by its steps."""
         t0 = time.perf_counter()
+        n = min(min(ep.action_every - ep.done_steps % ep.action_every, ep.steps - ep.done_steps)
+                for ep in self.active)
         for ep in self.active:
             if ep.done_steps % ep.action_every == 0:
                 self._act(ep)
         if self.batched:
             for k in (2, 3):
                 if len(self.batches[k].members):
-                    self.batches[k].step(1)
+                    self.batches[k].step(n)
             for k in (2, 3):
                 if len(self.batches[k].members):
                     self.batches[k].sync()
         else:
             for ep in self.active:
-                ep.dom.step(1)
+                ep.dom.step(n)
             for ep in self.active:
                 ep.dom.sync()
         for ep in self.active:
-            ep.done_steps += 1
-            self.cell_steps += ep.cells
-            if ep.done_steps % ep.action_every == 0:
+            ep.done_steps += n
+            self.cell_steps += ep.cells * n
+            if ep.done_steps % ep.action_every == 0 or ep.done_steps == ep.steps:
                 ep.wrench.append(ep.dom.wrench_history(ep.action_every))
         dt = time.perf_counter() - t0
-        self.lat.extend([dt] * len(self.active))
+        self.lat.extend([dt / n] * (len(self.active) * n))
         self.iterations += 1
         return dt
 
@@ -220,8 +239,10 @@ class Server:
             while i < len(pending) and (not realtime or pending[i].arrival <= now):
                 queue.append(pending[i])
                 i += 1
-            # admission in arrival order while memory and slots allow
-            while queue and len(self.active) < self.max_active and self.used + queue[0].footprint <= self.budget:
+            # admission in arrival order while memory and slots allow, at an
+            # action boundary of the running episodes
+            while (queue and len(self.active) < self.max_active and self.used + queue[0].footprint <= self.budget
+                   and self.at_boundary()):
                 self.admit(queue.pop(0), now)
             if not self.active:
                 if i < len(pending) and realtime:
@@ -236,6 +257,8 @@ class Server:
         wall = time.perf_counter() - t_start
         lat = np.array(self.lat) * 1e3
         done = [e for e in episodes if e.finished_at is not None]
+        # arrival to first step (queueing for memory, a slot and an action boundary)
+        wait = np.array([(e.admitted_at - e.arrival) * 1e3 for e in done]) if realtime else np.array([])
         return {
             "episodes_done": len(done),
             "episodes_submitted": len(episodes),
@@ -247,6 +270,8 @@ class Server:
             "busy_mlups": self.cell_steps / busy / 1e6 if busy > 0 else 0.0,
             "p50_step_ms": float(np.percentile(lat, 50)) if lat.size else None,
             "p99_step_ms": float(np.percentile(lat, 99)) if lat.size else None,
+            "p50_admission_wait_ms": float(np.percentile(wait, 50)) if wait.size else None,
+            "p99_admission_wait_ms": float(np.percentile(wait, 99)) if wait.size else None,
             "peak_active": peak,
             "peak_bytes": peak_used,
             "budget_bytes": self.budget,

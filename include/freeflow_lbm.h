@@ -56,12 +56,16 @@ extern "C" {
                                   (src/core/FsiCoupling2D.cu:89,120-128) */
 #define FF_OPT_PERIODIC 0x2u   /* pull across the domain edges periodically instead of
                                   the reference's clamped halo (FsiCoupling2D.cu:104-105) */
-#define FF_OPT_FAST_MATH 0x4u  /* 3D staged-column tiles (7-10): the same update in
-                                  a reassociated FP32 form with FMA (pair sums of opposite
-                                  populations) instead of the reference's operation order
-                                  (src/lbm/LbmUtilsFunc.cu:212-414, FsiCoupling3D.cu:291-320);
-                                  moments agree with the reference order to <= 1e-5
-                                  relative L2 after 100 steps, not bit for bit */
+#define FF_OPT_FAST_MATH 0x4u  /* the same update in a reassociated FP32 form with FMA
+                                  (pair sums of opposite populations) instead of the
+                                  reference's operation order (src/lbm/LbmUtilsFunc.cu:
+                                  212-543, FsiCoupling2D.cu:154-252, FsiCoupling3D.cu:
+                                  291-320): 2D tiles without FF_OPT_STAGE_FP16, 3D
+                                  staged-column tiles (7-13), boundary cells in 2D and 3D,
+                                  and FP32 wave kinematics; FP64 stays only in the
+                                  per-body force and power sums.  Moments agree with the
+                                  reference order to <= 1e-5 relative L2 after 100 steps,
+                                  not bit for bit */
 
 /* Node flag values: the ordinals of fsi::lbm::LbmNodeFlag
    (include/FSI_Simulator/lbm/LbmDataTypes.cuh:17-56). */

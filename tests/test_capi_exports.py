@@ -71,10 +71,11 @@ def _boundary_bytes(dim, nv, ne, nb, cap):
     b += _al(nv + 1, 4) + _al(ne * D, 4)                    # CSR vertex -> (element, end)
     b += _al(nb * D, 8) + _al(1, 16)                        # body reference points, step state
     b += _al(vc, 4) + _al(nb * (3 if dim == 2 else 6), 8)   # vertex forces, body wrench
+    b += _al(nb, 8)                                         # body power
     b += _al(2 * cap, 8) + _al(2 * cap * K, 8)              # link tables: cells and keys, two by tag parity
-    b += _al(ne, 8 * (7 if dim == 2 else 15))               # per-element FP64 sums
+    b += _al(ne, 8 * (8 if dim == 2 else 16))               # per-element FP64 sums (with power)
     b += 3 * _al(cap * K * D, 4) + _al(cap * nm, 4)         # link force, barycentrics, hit point; cell rows
-    b += _al(vc, 4) + _al(nv, 4) + _al(nb * 4, 8)           # wave inputs
+    b += _al(vc, 4) + _al(nv, 4) + _al(nb * 8, 8)           # wave inputs (4 FP64 + 8 FP32 per body)
     return _round2m(b)
 
 
