@@ -31,7 +31,7 @@ FLAG = {
 EXPORTS = [
     "ff_domain_create", "ff_domain_destroy", "ff_set_flags", "ff_get_flags", "ff_init_state", "ff_set_moments",
     "ff_get_moments", "ff_set_boundary", "ff_update_boundary", "ff_set_body_ref", "ff_set_wave", "ff_set_wave_params", "ff_step",
-    "ff_step_proxy", "ff_get_vertex_forces", "ff_get_step_vertex_forces", "ff_get_body_wrench", "ff_set_wrench_history",
+    "ff_step_proxy", "ff_get_vertex_forces", "ff_get_step_vertex_forces", "ff_get_step_vertex_forces_noalloc", "ff_get_body_wrench", "ff_set_wrench_history",
     "ff_get_wrench_history", "ff_get_body_power", "ff_get_power_history", "ff_get_boundary", "ff_mass", "ff_fill_solid", "ff_sync", "ff_time_bulk_kernel",
     "ff_time_steps", "ff_domain_footprint", "ff_set_tile", "ff_autotune", "ff_batch_create", "ff_batch_destroy", "ff_batch_add",
     "ff_batch_remove", "ff_batch_size", "ff_batch_step", "ff_batch_sync", "ff_batch_set_tile", "ff_batch_get_tile",
@@ -79,6 +79,7 @@ def lib():
         L.ff_step_proxy.argtypes = [vp, vp, vp, vp, vp, C.c_int, C.c_int]
         L.ff_get_vertex_forces.argtypes = [vp, vp, C.c_int]
         L.ff_get_step_vertex_forces.argtypes = [vp, vp]
+        L.ff_get_step_vertex_forces_noalloc.argtypes = [vp, vp]
         L.ff_get_body_wrench.argtypes = [vp, vp]
         L.ff_set_wrench_history.argtypes = [vp, C.c_int]
         L.ff_get_wrench_history.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int)]

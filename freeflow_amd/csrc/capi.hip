@@ -855,6 +855,57 @@ int ff_get_step_vertex_forces(ff_domain* dd, float* out) {
   return rc;
 }
 
+// ---------------------------------------------------------------- no-allocation readback
+// Staging of ff_get_step_vertex_forces_noalloc: a file-scope device array,
+// one copy per GPU, held until the process ends.  Its name in host code is
+// not a device address: the address is resolved with hipGetSymbolAddress,
+// once per device with that device current, and only that pointer is handed
+// to kernels and copies.  Calls on one GPU share the array; calls that
+// overlap on one GPU (other threads or streams) race.
+constexpr size_t kNoAllocFloats = size_t(1) << 20;  // 4 MiB: nv * dim <= 1 048 576
+static __device__ __align__(256) float scratch[kNoAllocFloats];
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static float* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// The calling thread's current device must be `device`.
+static int scratch_for_device(int device, float** out) {
+  if (device < 0 || device >= kMaxScratchDevices) return fail(FF_ERR_INVALID, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!p) return fail(FF_ERR_HIP, "hipGetSymbolAddress(scratch) returned no address");
+    g_scratch_ptr[device] = (float*)p;
+  }
+  *out = g_scratch_ptr[device];
+  return FF_OK;
+}
+
+int ff_get_step_vertex_forces_noalloc(ff_domain* dd, float* out) {
+  FF_CHECK_DOMAIN(dd);
+  Domain& d = *dd;
+  if (!d.has_boundary) return fail(FF_ERR_STATE, "no boundary set");
+  if (!out) return fail(FF_ERR_INVALID, "null output");
+  const size_t cnt = (size_t)d.nv * d.dim;
+  if (cnt > kNoAllocFloats)
+    return fail(FF_ERR_CAPACITY, "nv * dim = " + std::to_string(cnt) + " exceeds the no-allocation scratch of " +
+                                     std::to_string(kNoAllocFloats) + " floats (use ff_get_step_vertex_forces)");
+  FF_HIP(hipSetDevice(d.device));
+  if (d.gen_host <= 1 || d.ne == 0) {  // no step has run on this boundary
+    std::memset(out, 0, cnt * sizeof(float));
+    return FF_OK;
+  }
+  float* ptr = nullptr;
+  if (int rc = scratch_for_device(d.device, &ptr)) return rc;
+  // every vertex's nv * dim values are written by the kernel before the copy
+  if (int rc = launch_step_vertex_forces(d, ptr)) return rc;
+  FF_HIP(hipMemcpyAsync(out, ptr, cnt * sizeof(float), hipMemcpyDeviceToHost, d.stream));
+  FF_HIP(hipStreamSynchronize(d.stream));
+  return FF_OK;
+}
+
 int ff_get_body_wrench(ff_domain* dd, double* out) {
   FF_CHECK_DOMAIN(dd);
   Domain& d = *dd;

@@ -163,6 +163,15 @@ int ff_get_vertex_forces(ff_domain* d, float* out, int reset);
    reference's VBD path scatters to the volume each step
    (scatter_forces_kernel2d, src/fsi/FsiCouplingKernels.cu:77-91). */
 int ff_get_step_vertex_forces(ff_domain* d, float* out);
+/* ff_get_step_vertex_forces without any device allocation: the forces are
+   staged in a static per-GPU array of 1 048 576 floats (4 MiB per GPU, held
+   until the process ends) instead of a hipMalloc/hipFree per call; a domain
+   with nv * dim above that is rejected with FF_ERR_CAPACITY.  No counterpart
+   in the reference: for callers that cannot allocate device memory (a
+   per-step RL readback loop, code under stream capture, allocation-free
+   serving loops).  Calls on one GPU share the array; calls that overlap on
+   one GPU (other threads or streams) race. */
+int ff_get_step_vertex_forces_noalloc(ff_domain* d, float* out);
 /* Per-body wrench of the most recent step, FP64: 2D {Fx, Fy, Tz},
    3D {Fx, Fy, Fz, Tx, Ty, Tz} per body, torque about the body reference. */
 int ff_get_body_wrench(ff_domain* d, double* out);
