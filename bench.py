@@ -251,8 +251,14 @@ def cpu_baseline_batched_member(capi, wl, scenes, steps):
     o = ol.OracleDomain(2, sc.nx, sc.ny, fp64=True, stage_fp16=False, nu=sc.viscosity, dx=sc.dx, dt=sc.dt,
                         cf=capi.cf_of(2, sc.rho, sc.dx, sc.dt))
     o.set_flags(sc.flags)
+    # the FP32 oracle (no storage rounding) on the same kinematics: how far
+    # plain FP32 arithmetic drifts from FP64 on this flow
+    o32 = ol.OracleDomain(2, sc.nx, sc.ny, fp64=False, stage_fp16=False, nu=sc.viscosity, dx=sc.dx, dt=sc.dt,
+                          cf=capi.cf_of(2, sc.rho, sc.dx, sc.dt))
+    o32.set_flags(sc.flags)
     m0 = wl.equilibrium_moments(np.zeros((sc.nx * sc.ny, 2)))
     o.set_moments(m0)
+    o32.set_moments(m0)
     d = capi.Domain(**sc.domain_args(), storage="f16", stage_fp16=False, fast_math=True)
     sc.setup(d)
     d.set_moments(m0)
@@ -260,18 +266,30 @@ def cpu_baseline_batched_member(capi, wl, scenes, steps):
     b.add(d)
     v, vel = d.boundary()
     o.set_boundary(v, vel, sc.swimmer.elems)
-    for _ in range(steps):
+    o32.set_boundary(v, vel, sc.swimmer.elems)
+    upd = sc.flags == 1
+
+    def rel(a, r):
+        return float(np.linalg.norm(a[upd, 1:3] - r[upd, 1:3]) / np.linalg.norm(r[upd, 1:3]))
+
+    at = {}
+    for s in range(steps):
         b.step(1)
         v, vel = d.boundary()  # the kinematics the device used for this step
         o.update_boundary(v, vel)
         o.step(1)
-    got = d.moments(0).astype(np.float64)
-    ref = o.moments(0)
-    upd = sc.flags == 1
-    err = float(np.linalg.norm(got[upd, 1:3] - ref[upd, 1:3]) / np.linalg.norm(ref[upd, 1:3]))
+        o32.update_boundary(v, vel)
+        o32.step(1)
+        if s + 1 in (100, steps):
+            ref = o.moments(0)
+            at[s + 1] = (rel(d.moments(0).astype(np.float64), ref), rel(o32.moments(0), ref))
     b.close()
     d.close()
-    return {"member": k, "nx": sc.nx, "ny": sc.ny, "steps": steps, "velocity_rel_l2_vs_fp64": err, "bar": 1e-3,
+    return {"member": k, "nx": sc.nx, "ny": sc.ny, "steps": steps, "velocity_rel_l2_vs_fp64": at[steps][0],
+            "velocity_rel_l2_vs_fp64_at_100": at[100][0], "fp32_oracle_velocity_rel_l2_vs_fp64": at[steps][1],
+            "fp32_oracle_velocity_rel_l2_vs_fp64_at_100": at[100][1], "bar": 1e-3,
+            "note": "the bar holds while the flow is smooth; later a moving-boundary link decision flips between "
+                    "FP32 and FP64 arithmetic, as the FP32 oracle alone shows",
             "seconds": round(time.perf_counter() - t0, 1)}
 
 
@@ -313,16 +331,20 @@ def tg3d_time(capi, wl, m, storage, fast, K, W, tile, tune=True):
     d.sync()
     print(f"[bench] tg3d {storage} {'fast' if fast else 'reference-order'} tile {t}: timing {K} steps",
           file=sys.stderr, flush=True)
+    # the timed region: K steps, each step's stream launch timed by HIP events
+    # attached to the launch (the kernel as stepping runs it: each launch reads
+    # what the previous one wrote), the region itself by events on the stream
     t0 = time.perf_counter()
-    d.step(K)
-    d.sync()
-    el = time.perf_counter() - t0
+    total_ms, stream_ms = d.step_timed(K)
+    host_el = time.perf_counter() - t0
+    el = total_ms * 1e-3
+    avg_ms = stream_ms / K
     drift = abs(d.mass() - m0) / m0
-    avg_ms = d.time_bulk_kernel(50)
     d.close()
     bpl = bytes_per_lup(3, 4 if storage == "f32" else 2)
     sname = {"f32": "float", "f16": "__half", "bf16": "__hip_bfloat16"}[storage]
-    return {"mlups": cells * K / el / 1e6, "ms_per_step": el * 1e3 / K, "tile": t, "avg_kernel_ms": avg_ms,
+    return {"mlups": cells * K / el / 1e6, "ms_per_step": el * 1e3 / K, "host_ms_per_step": host_el * 1e3 / K,
+            "tile": t, "avg_kernel_ms": avg_ms,
             "kernel": kernel3d(sname, True, t, fast), "bytes_per_lup": bpl, "cells": cells,
             "frac": bpl * cells / (avg_ms * 1e-3) / (HBM_PEAK_GBS * 1e9), "mass_drift": drift, "steps": K + W}
 
@@ -395,7 +417,8 @@ def swimmer2d_summary(capi, wl, K=2000, W=100, lat_steps=200):
         d.step(1)
         d.body_wrench()  # synchronises
         lat.append((time.perf_counter() - a) * 1e3)
-    avg_ms = d.time_bulk_kernel(400)
+    _, stream_ms = d.step_timed(400)  # the stream launch as stepping runs it
+    avg_ms = stream_ms / 400
     d.close()
     return {"mlups": round(cells * K / el / 1e6, 2), "ms_per_step": round(el * 1e3 / K, 5), "steps": K,
             "tile": TILES2[t], "stream_launch_us": round(avg_ms * 1e3, 3),
@@ -405,7 +428,11 @@ def swimmer2d_summary(capi, wl, K=2000, W=100, lat_steps=200):
             "note": "1200x400 is 23.5 MB of FP32 moments: cache-resident (MALL), latency-bound, not an HBM figure"}
 
 
-def served_summary(rate=50.0, duration=4.0):
+def served_summary(rates=(50.0, 100.0), duration=4.0):
+    return {f"rate_{int(r)}": served_at(r, duration) for r in rates}
+
+
+def served_at(rate, duration):
     from freeflow_amd import serve
     out = {}
     for batched in (True, False):
@@ -463,7 +490,8 @@ def run_tg3d(args, capi, wl):
     out = base_line(main_["mlups"], K, W, main_["ms_per_step"], f"f32 compute, {st} storage",
                     {"workload": f"3D D3Q27 {nx}x{ny}x{nz} Taylor-Green (amplitude 0.04) + vortex ring r=20, "
                                  "periodic, viscosity 0.01", "nx": nx, "ny": ny, "nz": nz, "storage": st,
-                     "arithmetic": arith, "tile": TILES3[main_["tile"]], "parallelism": "single-gpu"},
+                     "arithmetic": arith, "tile": TILES3[main_["tile"]], "tile_index": main_["tile"],
+                     "parallelism": "single-gpu"},
                     roof, cpu, mass_drift=main_["mass_drift"], **extra)
     print(json.dumps(out), flush=True)
     return 0
@@ -585,9 +613,11 @@ def main():
         d.step(1)
         d.sync()
         lat.append((time.perf_counter() - a) * 1e3)
-    # roofline of the dominant kernel (stream+collide), HIP events per launch
+    # roofline of the dominant kernel (stream+collide), HIP events attached to
+    # each launch of real steps
     iters = 400
-    avg_ms = d.time_bulk_kernel(iters)
+    _, stream_ms = d.step_timed(iters)
+    avg_ms = stream_ms / iters
     bpl = bytes_per_lup(2, 4)
     achieved = bpl * cells / (avg_ms * 1e-3) / 1e9
     kname = kernel2d("k_step2d", "float", True, False, tile)

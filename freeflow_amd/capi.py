@@ -33,7 +33,7 @@ EXPORTS = [
     "ff_get_moments", "ff_set_boundary", "ff_update_boundary", "ff_set_body_ref", "ff_set_wave", "ff_set_wave_params", "ff_step",
     "ff_step_proxy", "ff_get_vertex_forces", "ff_get_step_vertex_forces", "ff_get_step_vertex_forces_noalloc", "ff_get_body_wrench", "ff_set_wrench_history",
     "ff_get_wrench_history", "ff_get_body_power", "ff_get_power_history", "ff_get_boundary", "ff_mass", "ff_fill_solid", "ff_sync", "ff_time_bulk_kernel",
-    "ff_time_steps", "ff_domain_footprint", "ff_set_tile", "ff_autotune", "ff_batch_create", "ff_batch_destroy", "ff_batch_add",
+    "ff_time_steps", "ff_step_timed", "ff_domain_footprint", "ff_set_tile", "ff_autotune", "ff_batch_create", "ff_batch_destroy", "ff_batch_add",
     "ff_batch_remove", "ff_batch_size", "ff_batch_step", "ff_batch_sync", "ff_batch_set_tile", "ff_batch_get_tile",
     "ff_batch_time_steps", "ff_batch_time_bulk", "ff_last_error", "ff_device_count", "ff_version",
 ]
@@ -91,6 +91,7 @@ def lib():
         L.ff_sync.argtypes = [vp]
         L.ff_time_bulk_kernel.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
         L.ff_time_steps.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
+        L.ff_step_timed.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.ff_device_count.argtypes = [C.POINTER(C.c_int)]
         L.ff_domain_footprint.argtypes = [C.POINTER(DomainDesc), C.c_int, C.c_int, vp, vp, C.POINTER(C.c_ulonglong)]
         L.ff_set_tile.argtypes = [vp, C.c_int]
@@ -297,6 +298,12 @@ class Domain:
         ms = C.c_double(0)
         _check(lib().ff_time_steps(self.h, n, C.byref(ms)))
         return ms.value
+
+    def step_timed(self, n):
+        """n steps; returns (wall ms, summed ms of the steps' stream launches)."""
+        tot, stream = C.c_double(0), C.c_double(0)
+        _check(lib().ff_step_timed(self.h, n, C.byref(tot), C.byref(stream)))
+        return tot.value, stream.value
 
     def set_tile(self, tile):
         _check(lib().ff_set_tile(self.h, tile))

@@ -1107,6 +1107,42 @@ int ff_time_steps(ff_domain* dd, int nsteps, double* total_ms) {
   return check_overflow(d);
 }
 
+int ff_step_timed(ff_domain* dd, int nsteps, double* total_ms, double* stream_ms) {
+  FF_CHECK_DOMAIN(dd);
+  if (nsteps < 0 || !total_ms || !stream_ms) return fail(FF_ERR_INVALID, "bad arguments");
+  Domain& d = *dd;
+  if (d.batch) return fail(FF_ERR_STATE, "domain belongs to a batch: time it with ff_batch_time_steps");
+  FF_HIP(hipSetDevice(d.device));
+  EventSet ev;
+  ev.clears[0] = &d.tev0;
+  ev.clears[1] = &d.tev1;
+  FF_HIP(ev.create(2 * (size_t)nsteps + 2));
+  const size_t ek = 2 * (size_t)nsteps;
+  FF_HIP(hipEventRecord(ev[ek], d.stream));
+  int rc = FF_OK;
+  for (int s = 0; s < nsteps && !rc; s++) {
+    d.tev0 = ev[2 * (size_t)s];  // the step's stream launch, its own start and end (launch_timed)
+    d.tev1 = ev[2 * (size_t)s + 1];
+    rc = one_step(d, nullptr, nullptr, nullptr, nullptr);
+    d.tev0 = d.tev1 = nullptr;
+  }
+  if (!rc) rc = flush_finish(d);  // the timed work includes the last step's finish
+  FF_HIP(hipEventRecord(ev[ek + 1], d.stream));
+  FF_HIP(hipEventSynchronize(ev[ek + 1]));
+  if (rc) return rc;
+  float ms = 0.f;
+  FF_HIP(hipEventElapsedTime(&ms, ev[ek], ev[ek + 1]));
+  *total_ms = ms;
+  double sum = 0.0;
+  for (int s = 0; s < nsteps; s++) {
+    float k = 0.f;
+    FF_HIP(hipEventElapsedTime(&k, ev[2 * (size_t)s], ev[2 * (size_t)s + 1]));
+    sum += k;
+  }
+  *stream_ms = sum;
+  return check_overflow(d);
+}
+
 int ff_domain_footprint(const ff_domain_desc* desc, int nv, int ne, const float* verts, const uint32_t* elems,
                         unsigned long long* bytes) {
   if (!desc || !bytes) return fail(FF_ERR_INVALID, "null argument");

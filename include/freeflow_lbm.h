@@ -209,11 +209,19 @@ int ff_sync(ff_domain* d);
    kernel; with a boundary, the fused boundary-cell and kinematics blocks too)
    over `iters` launches on the domain's stream, each timed by HIP events
    attached to the launch itself (hipExtLaunchKernel: the kernel's own start
-   and end); does not advance the simulation state. */
+   and end); does not advance the simulation state.  Every launch reads the
+   same buffer and writes the other, so caches hold what a repeated launch
+   leaves there, not what stepping leaves (ff_step_timed times real steps). */
 int ff_time_bulk_kernel(ff_domain* d, int iters, double* avg_ms);
 /* Wall time (ms) of `nsteps` ff_step-equivalent steps measured with HIP
    events on the domain stream (state advances). */
 int ff_time_steps(ff_domain* d, int nsteps, double* total_ms);
+/* `nsteps` ff_step steps (the state advances), timed: *total_ms is the wall
+   time from HIP events on the domain stream, *stream_ms the sum over the
+   steps of the stream launch's own duration (events attached to each launch,
+   so the launch is measured in the cache state real stepping leaves, each
+   launch reading what the previous one wrote).  Synchronises. */
+int ff_step_timed(ff_domain* d, int nsteps, double* total_ms, double* stream_ms);
 
 /* Device bytes ff_domain_create + ff_set_boundary would allocate for this
    descriptor and boundary (verts/elems may be NULL when ne == 0): the figure

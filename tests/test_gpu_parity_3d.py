@@ -318,3 +318,22 @@ def test_fast_math_columns_within_tolerance(tile):
         rel = np.linalg.norm(a - b) / np.linalg.norm(b)
         note(f"  channel buffer {which}: fast vs FP32 oracle rel-L2 {rel:.3e}")
         assert rel <= 1e-5
+
+
+def test_step_timed_equals_step():
+    """ff_step_timed advances the state exactly as ff_step does, and its
+    per-launch stream time fits inside the region's wall time."""
+    nx, ny, nz = 48, 40, 32
+    note("case step_timed vs step")
+    m = wl.taylor_green_ring(nx, ny, nz, amp=0.04, ring_r=6.0, seed=3)
+    a = capi.Domain(3, nx, ny, nz, viscosity=0.01, periodic=True)
+    b = capi.Domain(3, nx, ny, nz, viscosity=0.01, periodic=True)
+    for d in (a, b):
+        d.set_moments(np.asarray(m, np.float32))
+    a.step(12)
+    total, stream = b.step_timed(12)
+    assert 0.0 < stream <= total
+    assert np.array_equal(a.moments(0), b.moments(0))
+    assert np.array_equal(a.moments(1), b.moments(1))
+    a.close()
+    b.close()
