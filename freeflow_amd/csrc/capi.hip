@@ -307,6 +307,7 @@ static BoundaryPlan plan_boundary(Carve& cv, Domain& d, int nv, int ne, int n_bo
   cv.add(&d.lk_b, cap * K * D);
   cv.add(&d.lk_h, cap * K * D);
   cv.add(&d.lk_sb, cap * d.nm);
+  cv.add(&d.fill_bits, D == 2 ? (size_t)d.ny * (size_t)((d.nx + 31) / 32) : 1);
   // wave kinematics inputs (ff_set_wave fills them when it is used)
   cv.add(&d.rest_verts, vcount);
   cv.add(&d.s_coord, (size_t)std::max(nv, 1));

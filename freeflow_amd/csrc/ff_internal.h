@@ -138,6 +138,7 @@ struct Domain {
   float* lk_b = nullptr;
   float* lk_h = nullptr;
   float* lk_sb = nullptr;         // boundary-cell results (lt_cap * nm)
+  uint32_t* fill_bits = nullptr;  // 2D fill-solid row marks (ny * ceil(nx / 32) words)
   StepState* st = nullptr;
   unsigned gen_host = 1;          // generation of the next step (st->gen while batched)
   unsigned tag_host = 1;          // link-table tag of the next step (st->tag while batched)

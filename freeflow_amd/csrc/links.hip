@@ -269,30 +269,98 @@ int launch_wave_now(Domain& d) {
 
 // ---------------------------------------------------------------- fillSolid
 // resetSolidFlagKernel + updateSolidFlagKernel (FsiCoupling2D.cu:318-354):
-// each updated cell becomes SolidDynamic when its centre is inside the
-// polygon (even-odd crossing test of isInside, SolidGeometryProxy_Device.cuh:
-// 126-144), Fluid otherwise.
-__global__ void k_fill_solid2d(uint8_t* flags, int nx, int ny, float dx, const float* __restrict__ v,
-                               const uint32_t* __restrict__ e, int ne) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y;
-  if (x >= nx) return;
-  const size_t c = (size_t)y * nx + x;
-  const uint8_t fl = flags[c];
-  if (fl != kFluid && fl != kSolidDynamic) return;
-  const float px = ((float)x + 0.5f) * dx, py = ((float)y + 0.5f) * dx;
-  bool in = false;
-  for (int i = 0; i < ne; i++) {
-    const uint32_t a = e[2 * i], b = e[2 * i + 1];
-    const float sx = v[2 * a], sy = v[2 * a + 1], tx = v[2 * b], ty = v[2 * b + 1];
-    if ((sy < py && ty >= py) || (sy >= py && ty < py)) in ^= (sx + (py - sy) * (tx - sx) / (ty - sy) > px);
+// an updated cell becomes SolidDynamic when its centre ((x + 0.5) dx,
+// (y + 0.5) dx) is inside the boundary by isInside's even-odd ray cast
+// (SolidGeometryProxy_Device.cuh:126-144), Fluid otherwise.
+//
+// Organised by rows instead of cells: a segment that straddles row y's centre
+// height (the reference's half-open test) crosses it at xc, computed with the
+// reference's expression, and toggles exactly the cells whose centre lies left
+// of xc.  Centres are non-decreasing in x, so those cells are a prefix
+// x < k of the row; the segment marks bit k - 1 of the row (XOR), and a cell is
+// inside when the marks at or right of it have odd parity.  O(segments x rows
+// spanned + cells) instead of O(cells x segments), with the same decisions.
+__global__ void __launch_bounds__(256) k_fill_clear(uint32_t* __restrict__ bits, size_t nwords) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < nwords; i += (size_t)gridDim.x * 256) bits[i] = 0u;
+}
+
+// One wave per segment, its lanes over the rows the segment may straddle.
+__global__ void __launch_bounds__(256) k_fill_mark(uint32_t* __restrict__ bits, int W, int nx, int ny, float dx,
+                                                   const float* __restrict__ v, const uint32_t* __restrict__ e,
+                                                   int ne) {
+  const int seg = (int)((blockIdx.x * 256u + threadIdx.x) >> 6);
+  const int lane = (int)(threadIdx.x & 63u);
+  if (seg >= ne) return;
+  const uint32_t a = e[2 * seg], b = e[2 * seg + 1];
+  const float sx = v[2 * a], sy = v[2 * a + 1], tx = v[2 * b], ty = v[2 * b + 1];
+  // rows whose centre height can lie in (min(sy, ty), max(sy, ty)], one row of margin each side
+  const float r0 = floorf(fminf(sy, ty) / dx - 0.5f) - 1.0f;
+  const float r1 = floorf(fmaxf(sy, ty) / dx - 0.5f) + 1.0f;
+  const float f0 = fmaxf(r0, 0.0f), f1 = fminf(r1, (float)(ny - 1));
+  if (!(f0 <= f1)) return;  // outside the grid's rows (or not a number)
+  const int y0 = (int)f0, y1 = (int)f1;
+  for (int y = y0 + lane; y <= y1; y += 64) {
+    const float py = ((float)y + 0.5f) * dx;
+    if ((sy < py && ty >= py) || (sy >= py && ty < py)) {
+      const float xc = sx + (py - sy) * (tx - sx) / (ty - sy);
+      // k = number of cells with centre < xc: an estimate, then the exact test
+      float fk = ceilf(xc / dx - 0.5f);
+      fk = fminf(fmaxf(fk, 0.0f), (float)nx);
+      int k = (int)fk;
+      while (k > 0 && ((float)(k - 1) + 0.5f) * dx >= xc) k--;
+      while (k < nx && ((float)k + 0.5f) * dx < xc) k++;
+      if (k > 0) atomicXor(&bits[(size_t)y * W + ((k - 1) >> 5)], 1u << ((k - 1) & 31));
+    }
   }
-  flags[c] = in ? kSolidDynamic : kFluid;
+}
+
+// One block per row: suffix parity of the row's marks, right to left in
+// chunks of 256 words, then every cell's flag.
+__global__ void __launch_bounds__(256) k_fill_apply(uint8_t* __restrict__ flags, const uint32_t* __restrict__ bits,
+                                                    int W, int nx) {
+  __shared__ uint32_t s_word[256];
+  __shared__ uint32_t s_right[256];  // parity of the row's marks in the words right of word i
+  __shared__ uint32_t s_wave[4];
+  const int y = blockIdx.x;
+  const uint32_t* row = bits + (size_t)y * W;
+  uint8_t* frow = flags + (size_t)y * nx;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  uint32_t carry = 0;  // parity of the chunks already done (to the right)
+  for (int base = (W - 1) / 256 * 256; base >= 0; base -= 256) {
+    const int w = base + t;
+    const uint32_t word = w < W ? row[w] : 0u;
+    const unsigned long long bal = __ballot(__popc(word) & 1u);
+    const uint32_t in_wave = lane == 63 ? 0u : (uint32_t)(__popcll(bal >> (lane + 1)) & 1);
+    if (lane == 0) s_wave[wv] = (uint32_t)(__popcll(bal) & 1);
+    __syncthreads();
+    uint32_t above = 0;
+    for (int k = wv + 1; k < 4; k++) above ^= s_wave[k];
+    s_word[t] = word;
+    s_right[t] = carry ^ above ^ in_wave;
+    const uint32_t chunk = s_wave[0] ^ s_wave[1] ^ s_wave[2] ^ s_wave[3];
+    __syncthreads();
+    const int x0 = base * 32;
+    for (int i = t; i < 256 * 32; i += 256) {
+      const int x = x0 + i;
+      if (x >= nx) break;
+      const int wl = i >> 5, bit = i & 31;
+      const uint32_t inside = s_right[wl] ^ ((uint32_t)__popc(s_word[wl] >> bit) & 1u);
+      const uint8_t f = frow[x];
+      if (f == kFluid || f == kSolidDynamic) frow[x] = inside ? kSolidDynamic : kFluid;
+    }
+    carry ^= chunk;
+    __syncthreads();
+  }
 }
 
 int launch_fill_solid(Domain& d, const float* verts) {
-  dim3 b(256), g((d.nx + 255) / 256, d.ny);
-  k_fill_solid2d<<<g, b, 0, d.stream>>>(d.flags, d.nx, d.ny, d.cl, verts, d.elems, d.ne);
+  const int W = (d.nx + 31) / 32;
+  const size_t nwords = (size_t)d.ny * W;
+  k_fill_clear<<<(unsigned)std::min<size_t>((nwords + 255) / 256, 4096), 256, 0, d.stream>>>(d.fill_bits, nwords);
+  if (d.ne > 0)
+    k_fill_mark<<<(unsigned)((d.ne + 3) / 4), 256, 0, d.stream>>>(d.fill_bits, W, d.nx, d.ny, d.cl, verts, d.elems,
+                                                                   d.ne);
+  k_fill_apply<<<(unsigned)d.ny, 256, 0, d.stream>>>(d.flags, d.fill_bits, W, d.nx);
   FF_HIP(hipGetLastError());
   return FF_OK;
 }

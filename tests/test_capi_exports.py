@@ -61,7 +61,7 @@ def _core_bytes(dim, n, sbytes):
     return _round2m(2 * _al(plane * nm * sbytes, 1) + _al(n, 1) + _al(1024, 8))
 
 
-def _boundary_bytes(dim, nv, ne, nb, cap):
+def _boundary_bytes(dim, nv, ne, nb, cap, nx=1, ny=1):
     """Every array of the ff_set_boundary allocation, in its order."""
     D, K, nm = dim, (8 if dim == 2 else 26), (6 if dim == 2 else 10)
     vc = max(nv, 1) * D
@@ -75,6 +75,7 @@ def _boundary_bytes(dim, nv, ne, nb, cap):
     b += _al(2 * cap, 8) + _al(2 * cap * K, 8)              # link tables: cells and keys, two by tag parity
     b += _al(ne, 8 * (8 if dim == 2 else 16))               # per-element FP64 sums (with power)
     b += 3 * _al(cap * K * D, 4) + _al(cap * nm, 4)         # link force, barycentrics, hit point; cell rows
+    b += _al(ny * ((nx + 31) // 32) if dim == 2 else 1, 4)  # fill-solid row marks
     b += _al(vc, 4) + _al(nv, 4) + _al(nb * 8, 8)           # wave inputs (4 FP64 + 8 FP32 per body)
     return _round2m(b)
 
@@ -92,7 +93,7 @@ def test_footprint_without_gpu():
     with_b = capi.footprint(2, nx, ny, storage="f32", verts=sw.rest, elems=sw.elems, dx=0.005)
     extra = with_b - _core_bytes(2, nx * ny, 4)
     # the link table's capacity is a power of two >= 1024; ne bodies bound the per-body arrays
-    caps = [1 << k for k in range(10, 24) if _boundary_bytes(2, nv, ne, ne, 1 << k) == extra]
+    caps = [1 << k for k in range(10, 24) if _boundary_bytes(2, nv, ne, ne, 1 << k, nx, ny) == extra]
     assert caps, f"boundary bytes {extra} match no table capacity"
 
 
@@ -132,8 +133,8 @@ def test_link_table_sized_from_distinct_cells_near_the_mesh():
     fb = capi.footprint(2, nx, ny, storage="f32", verts=spread, elems=e, dx=1.0)
     assert cap(spread) > cap(packed)
     core = _core_bytes(2, nx * ny, 4)
-    assert fa - core == _boundary_bytes(2, n + 1, n, n, cap(packed))
-    assert fb - core == _boundary_bytes(2, n + 1, n, n, cap(spread))
+    assert fa - core == _boundary_bytes(2, n + 1, n, n, cap(packed), nx, ny)
+    assert fb - core == _boundary_bytes(2, n + 1, n, n, cap(spread), nx, ny)
 
 
 def test_invalid_arguments_rejected_without_gpu():

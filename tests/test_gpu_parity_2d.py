@@ -362,20 +362,82 @@ def test_wave_mode_matches_oracle_fed_with_device_kinematics():
     assert np.array_equal(hist[-1], g.body_wrench())
 
 
-def test_fill_solid_matches_oracle():
-    nx, ny = 160, 80
-    note("case fill_solid")
-    flags = wl.vortex_street_flags(nx, ny)
-    g = capi.Domain(2, nx, ny, viscosity=0.005, dx=0.01)
+def _fill_case(kind, rng):
+    """(nx, ny, dx, flags, verts, elems) of one fill-solid case."""
+    if kind == "ellipse":
+        nx, ny, dx = 160, 80, 0.01
+        v, e = wl.polygon(90, 0.8, 0.4, 0.3, 0.15)
+        return nx, ny, dx, wl.vortex_street_flags(nx, ny), v, e
+    if kind == "on_centres":  # axis-aligned edges through cell centres and corners on them (ties)
+        nx, ny, dx = 64, 48, 1.0
+        v = np.array([[10.5, 7.5], [40.5, 7.5], [40.5, 30.5], [22.0, 30.5], [22.0, 41.0], [10.5, 41.0]], np.float32)
+        e = np.array([[i, (i + 1) % len(v)] for i in range(len(v))], np.uint32)
+        return nx, ny, dx, np.ones(nx * ny, np.uint8), v, e
+    if kind == "outside":  # a star partly beyond every edge of the grid, and a second body
+        nx, ny, dx = 96, 64, 0.0173
+        n = 57
+        ang = np.sort(rng.uniform(0, 2 * np.pi, n))
+        r = rng.uniform(0.3, 1.0, n) * 70 * dx
+        c = np.array([0.5 * nx, 0.5 * ny]) * dx
+        v1 = c + np.stack([r * np.cos(ang), r * np.sin(ang)], 1)
+        v2 = 0.3 * (v1 - c) + c + np.array([0.2, 0.1]) * nx * dx
+        v = np.vstack([v1, v2]).astype(np.float32)
+        e1 = np.stack([np.arange(n), (np.arange(n) + 1) % n], 1)
+        e = np.vstack([e1, e1 + n]).astype(np.uint32)
+        return nx, ny, dx, wl.vortex_street_flags(nx, ny), v, e
+    if kind == "wide":  # rows longer than one 8192-cell chunk of the row scan
+        nx, ny, dx = 9000, 24, 1.0
+        v = np.array([[3.2, 2.7], [8990.4, 5.5], [8100.0, 21.3], [40.7, 19.9], [4000.5, 11.5]], np.float32)
+        e = np.array([[i, (i + 1) % len(v)] for i in range(len(v))], np.uint32)
+        return nx, ny, dx, np.ones(nx * ny, np.uint8), v, e
+    if kind == "swimmer":  # the 1200x400 workload's ~600-segment swimmer
+        c = wl.SWIMMER2D
+        sw = wl.swimmer2d_swimmer()
+        v, _ = sw.state(0.37)
+        return c["nx"], c["ny"], c["dx"], wl.outlet_box_flags(c["nx"], c["ny"]), v.astype(np.float32), sw.elems
+    raise ValueError(kind)
+
+
+@pytest.mark.parametrize("kind", ["ellipse", "on_centres", "outside", "wide", "swimmer"])
+def test_fill_solid_matches_oracle(kind):
+    """ff_fill_solid (row scan) against the oracle's per-cell ray cast: the
+    same flags, byte for byte; called twice, with the boundary moved between
+    the calls (SolidDynamic cells left behind return to Fluid)."""
+    rng = np.random.default_rng(11)
+    nx, ny, dx, flags, v, e = _fill_case(kind, rng)
+    note(f"case fill_solid {kind} {nx}x{ny}, {len(e)} segments")
+    g = capi.Domain(2, nx, ny, viscosity=0.005, dx=dx)
     g.set_flags(flags)
-    v, e = wl.polygon(90, 0.8, 0.4, 0.3, 0.15)
     g.set_boundary(v, np.zeros_like(v), e)
+    cur = flags.copy()
+    for shift in (0.0, 3.3 * dx):
+        vv = (v + np.float32(shift)).astype(np.float32)
+        if shift:
+            g.update_boundary(vv, np.zeros_like(vv))
+        g.fill_solid()
+        ref = cur.copy()
+        ol.lib().orc_fill_solid2d(nx, ny, dx, len(vv), np.ascontiguousarray(vv, np.float32).ravel(), len(e),
+                                  np.ascontiguousarray(e, np.uint32).ravel(), ref)
+        got = g.flags()
+        assert np.array_equal(got, ref), f"{int(np.sum(got != ref))} cells differ"
+        assert (ref == 4).sum() > 10
+        cur = ref
+    g.close()
+
+
+def test_fill_solid_empty_boundary():
+    nx, ny = 40, 30
+    note("case fill_solid empty boundary")
+    flags = wl.vortex_street_flags(nx, ny)
+    flags[5 * nx + 7] = 4
+    g = capi.Domain(2, nx, ny, viscosity=0.005)
+    g.set_flags(flags)
+    g.set_boundary(np.zeros((0, 2), np.float32), np.zeros((0, 2), np.float32), np.zeros((0, 2), np.uint32))
     g.fill_solid()
     ref = flags.copy()
-    ol.lib().orc_fill_solid2d(nx, ny, 0.01, len(v), np.ascontiguousarray(v, np.float32).ravel(), len(e),
-                              np.ascontiguousarray(e, np.uint32).ravel(), ref)
+    ref[ref == 4] = 1
     assert np.array_equal(g.flags(), ref)
-    assert (ref == 4).sum() > 100
+    g.close()
 
 
 def test_mass_fp64():
