@@ -15,7 +15,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("FREEFLOW_LBM_LIB") or os.path.join(PKG, "lib", "libfreeflow_lbm.so")
 
 FF_OK = 0
-STORAGE = {"f32": 0, "f16": 1, "bf16": 2}
+STORAGE = {"f32": 0, "f16": 1, "bf16": 2, "f16u": 3}
 OPT_STAGE_FP16 = 0x1
 OPT_PERIODIC = 0x2
 OPT_FAST_MATH = 0x4

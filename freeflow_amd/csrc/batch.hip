@@ -277,6 +277,7 @@ static void launch_step_typed(Batch& b, int tile) {
 static void launch_step(Batch& b, int tile) {
   if (b.storage == FF_STORAGE_F32) launch_step_typed<float>(b, tile);
   else if (b.storage == FF_STORAGE_F16) launch_step_typed<__half>(b, tile);
+  else if (b.storage == FF_STORAGE_F16U) launch_step_typed<h16u>(b, tile);
   else launch_step_typed<__hip_bfloat16>(b, tile);
 }
 
@@ -284,6 +285,7 @@ static void launch_step(Batch& b, int tile) {
 static void launch_bulk_only(Batch& b, int tile, const int2* map, int nblocks) {
   if (b.storage == FF_STORAGE_F32) launch_stream_batch<float>(b, tile, map, nblocks, 0, 0);
   else if (b.storage == FF_STORAGE_F16) launch_stream_batch<__half>(b, tile, map, nblocks, 0, 0);
+  else if (b.storage == FF_STORAGE_F16U) launch_stream_batch<h16u>(b, tile, map, nblocks, 0, 0);
   else launch_stream_batch<__hip_bfloat16>(b, tile, map, nblocks, 0, 0);
 }
 
@@ -629,7 +631,7 @@ int ff_batch_create(int dim, int storage, unsigned options, int device, ff_batch
   if (!out) return fail(FF_ERR_INVALID, "null output");
   *out = nullptr;
   if (dim != 2 && dim != 3) return fail(FF_ERR_INVALID, "dim must be 2 or 3");
-  if (storage < 0 || storage > 2) return fail(FF_ERR_INVALID, "unknown storage format");
+  if (storage < 0 || storage > 3) return fail(FF_ERR_INVALID, "unknown storage format");
   if (options & ~(FF_OPT_STAGE_FP16 | FF_OPT_PERIODIC | FF_OPT_FAST_MATH))
     return fail(FF_ERR_INVALID, "unknown option bits");
   int ndev = 0;

@@ -189,7 +189,7 @@ static int check_grid(const ff_domain_desc* desc) {
     return fail(FF_ERR_INVALID, "grid sizes must be positive");
   const long long n = (long long)desc->nx * desc->ny * (desc->dim == 3 ? desc->nz : 1);
   if (n >= (1ll << 31)) return fail(FF_ERR_INVALID, "grid larger than 2^31 cells");
-  if (desc->storage < 0 || desc->storage > 2) return fail(FF_ERR_INVALID, "unknown storage format");
+  if (desc->storage < 0 || desc->storage > 3) return fail(FF_ERR_INVALID, "unknown storage format");
   if (!(desc->dx > 0.0)) return fail(FF_ERR_INVALID, "dx must be positive");
   return FF_OK;
 }
@@ -261,7 +261,7 @@ static unsigned long long link_estimate(int D, double dx, int ne, const float* v
 // The core allocation of a domain: both moment buffers, the flags and the
 // reduction scratch.  Returns the offset of the flags.
 static size_t plan_core(Carve& cv, Domain& d, unsigned char** mom0, unsigned char** mom1) {
-  const size_t bytes = d.plane * d.nm * storage_bytes(d.storage);
+  const size_t bytes = d.plane * storage_planes(d.storage, d.dim) * storage_bytes(d.storage);
   cv.add(mom0, bytes);
   cv.add(mom1, bytes);
   const size_t flags_off = cv.add(&d.flags, (size_t)d.n);

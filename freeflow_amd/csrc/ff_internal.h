@@ -196,6 +196,11 @@ int batch_detach(Domain& d);
 // plane stride: moment planes aligned to 256 elements
 inline size_t plane_stride(long long n) { return (size_t)((n + 255) / 256 * 256); }
 inline size_t storage_bytes(int storage) { return storage == FF_STORAGE_F32 ? 4 : 2; }
+// planes per moment buffer: the moments, plus the velocities' low halves in FF_STORAGE_F16U
+inline size_t storage_planes(int storage, int dim) {
+  const size_t nm = dim == 2 ? 6 : 10;
+  return storage == FF_STORAGE_F16U ? nm + (size_t)dim : nm;
+}
 
 LinkArgs link_args(Domain& d, const float* v0, const float* v1, const float* u0, const float* u1, const uint32_t* e);
 

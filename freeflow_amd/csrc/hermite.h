@@ -599,29 +599,64 @@ __device__ __forceinline__ float slot_scale(int k) { return k == 0 ? 256.0f : (k
 template <int DIM>
 __device__ __forceinline__ float slot_inv_scale(int k) { return k == 0 ? (1.0f / 256.0f) : (k <= DIM ? (1.0f / 16.0f) : (1.0f / 256.0f)); }
 
+// Element of the split-velocity 16-bit storage (FF_STORAGE_F16U): every plane
+// holds binary16 values; rho - 1 and Pi are stored as in FF_STORAGE_F16, and
+// each velocity component u_a (scaled by 2^4) as a pair hi + lo of binary16
+// values, hi in the component's own plane and lo = e - hi in an extra plane
+// NM - 1 planes further on (planes NM .. NM + DIM - 1).  e - hi is exact in
+// FP32 and so is hi + lo (22 significant bits), so a stored velocity keeps
+// about FP32 precision: the velocity rounding dominates the FP16 storage
+// error, rho - 1 and Pi rounding much less.
+struct h16u {
+  unsigned short bits;
+};
+
+// Moment loads and stores by storage type: p points at the cell's value in
+// plane k; `plane` is the plane stride in elements (used by the split
+// velocity of h16u).
 template <typename S> struct Store;
 template <> struct Store<float> {
-  template <int DIM> __device__ static __forceinline__ float load(const float* p, int k) { return *p; }
-  template <int DIM> __device__ static __forceinline__ void store(float* p, int k, float v) { *p = v; }
+  template <int DIM> __device__ static __forceinline__ float load(const float* p, int k, size_t plane) { return *p; }
+  template <int DIM> __device__ static __forceinline__ void store(float* p, int k, float v, size_t plane) { *p = v; }
 };
 template <> struct Store<__half> {
-  template <int DIM> __device__ static __forceinline__ float load(const __half* p, int k) {
+  template <int DIM> __device__ static __forceinline__ float load(const __half* p, int k, size_t plane) {
     float v = __half2float(*p) * slot_inv_scale<DIM>(k);
     return k == 0 ? v + 1.0f : v;
   }
-  template <int DIM> __device__ static __forceinline__ void store(__half* p, int k, float v) {
+  template <int DIM> __device__ static __forceinline__ void store(__half* p, int k, float v, size_t plane) {
     float e = (k == 0 ? v - 1.0f : v) * slot_scale<DIM>(k);
     *p = to_half_rn(e);
   }
 };
 template <> struct Store<__hip_bfloat16> {
-  template <int DIM> __device__ static __forceinline__ float load(const __hip_bfloat16* p, int k) {
+  template <int DIM> __device__ static __forceinline__ float load(const __hip_bfloat16* p, int k, size_t plane) {
     float v = __bfloat162float(*p) * slot_inv_scale<DIM>(k);
     return k == 0 ? v + 1.0f : v;
   }
-  template <int DIM> __device__ static __forceinline__ void store(__hip_bfloat16* p, int k, float v) {
+  template <int DIM> __device__ static __forceinline__ void store(__hip_bfloat16* p, int k, float v, size_t plane) {
     float e = (k == 0 ? v - 1.0f : v) * slot_scale<DIM>(k);
     *p = __float2bfloat16(opaque(e));
+  }
+};
+template <> struct Store<h16u> {
+  template <int DIM> __device__ static constexpr size_t lo_offset(size_t plane) {
+    return (size_t)((DIM == 2 ? 6 : 10) - 1) * plane;
+  }
+  template <int DIM> __device__ static __forceinline__ float load(const h16u* p, int k, size_t plane) {
+    const float hi = __half2float(__ushort_as_half(p->bits));
+    if (k >= 1 && k <= DIM) {
+      const float lo = __half2float(__ushort_as_half(p[lo_offset<DIM>(plane)].bits));
+      return (hi + lo) * (1.0f / 16.0f);
+    }
+    const float v = hi * slot_inv_scale<DIM>(k);
+    return k == 0 ? v + 1.0f : v;
+  }
+  template <int DIM> __device__ static __forceinline__ void store(h16u* p, int k, float v, size_t plane) {
+    const float e = (k == 0 ? v - 1.0f : v) * slot_scale<DIM>(k);
+    const __half h = to_half_rn(e);
+    p->bits = __half_as_ushort(h);
+    if (k >= 1 && k <= DIM) p[lo_offset<DIM>(plane)].bits = __half_as_ushort(to_half_rn(e - __half2float(h)));
   }
 };
 

@@ -108,6 +108,7 @@ static int bulk2d_dispatch(Domain& d, const void* in, void* out, const StreamExt
   switch (d.storage) {
     case FF_STORAGE_F32: return bulk2d_typed<float>(d, in, out, x);
     case FF_STORAGE_F16: return bulk2d_typed<__half>(d, in, out, x);
+    case FF_STORAGE_F16U: return bulk2d_typed<h16u>(d, in, out, x);
     default: return bulk2d_typed<__hip_bfloat16>(d, in, out, x);
   }
 }
@@ -138,10 +139,10 @@ __global__ void k_outlet(S* buf, const OutletCell* __restrict__ oc, int n, size_
   if (t >= n) return;
   const OutletCell c = oc[t];
   if (flags[c.nb] != kFluid) return;
-  const float up = Store<S>::template load<DIM>(buf + c.comp * plane + c.nb, c.comp);
+  const float up = Store<S>::template load<DIM>(buf + c.comp * plane + c.nb, c.comp, plane);
   if ((c.sign > 0 && up > 0.0f) || (c.sign < 0 && up < 0.0f))
-    Store<S>::template store<DIM>(buf + c.comp * plane + c.cell, c.comp, up);
-  Store<S>::template store<DIM>(buf + c.pcomp * plane + c.cell, c.pcomp, up * up);
+    Store<S>::template store<DIM>(buf + c.comp * plane + c.cell, c.comp, up, plane);
+  Store<S>::template store<DIM>(buf + c.pcomp * plane + c.cell, c.pcomp, up * up, plane);
 }
 
 int launch_outlet(Domain& d, void* buf) {
@@ -152,6 +153,7 @@ int launch_outlet(Domain& d, void* buf) {
   else k_outlet<S, 3><<<g, b, 0, d.stream>>>((S*)buf, d.outlets, d.n_outlets, d.plane, d.flags);
   if (d.storage == FF_STORAGE_F32) { FF_OUT(float) }
   else if (d.storage == FF_STORAGE_F16) { FF_OUT(__half) }
+  else if (d.storage == FF_STORAGE_F16U) { FF_OUT(h16u) }
   else { FF_OUT(__hip_bfloat16) }
 #undef FF_OUT
   FF_HIP(hipGetLastError());
@@ -165,7 +167,7 @@ __global__ void k_aos_to_soa(const float* __restrict__ aos, S* __restrict__ soa,
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
 #pragma unroll
-  for (int q = 0; q < NM; q++) Store<S>::template store<DIM>(soa + q * plane + c, q, aos[c * NM + q]);
+  for (int q = 0; q < NM; q++) Store<S>::template store<DIM>(soa + q * plane + c, q, aos[c * NM + q], plane);
 }
 template <typename S, int DIM>
 __global__ void k_soa_to_aos(const S* __restrict__ soa, float* __restrict__ aos, long long n, size_t plane) {
@@ -173,7 +175,7 @@ __global__ void k_soa_to_aos(const S* __restrict__ soa, float* __restrict__ aos,
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
 #pragma unroll
-  for (int q = 0; q < NM; q++) aos[c * NM + q] = Store<S>::template load<DIM>(soa + q * plane + c, q);
+  for (int q = 0; q < NM; q++) aos[c * NM + q] = Store<S>::template load<DIM>(soa + q * plane + c, q, plane);
 }
 
 // ---------------------------------------------------------------- initial state
@@ -263,8 +265,8 @@ __global__ void k_init_state(const uint8_t* __restrict__ flags, InitBC bc, S* __
   init_moments<DIM>(flags[c], bc, m);
 #pragma unroll
   for (int q = 0; q < NM; q++) {
-    Store<S>::template store<DIM>(a + q * plane + c, q, m[q]);
-    Store<S>::template store<DIM>(b + q * plane + c, q, m[q]);
+    Store<S>::template store<DIM>(a + q * plane + c, q, m[q], plane);
+    Store<S>::template store<DIM>(b + q * plane + c, q, m[q], plane);
   }
 }
 
@@ -279,6 +281,7 @@ int launch_init_state(Domain& d, const float* bc6) {
   else k_init_state<S, 3><<<g, b, 0, d.stream>>>(d.flags, bc, (S*)d.mom[0], (S*)d.mom[1], d.n, d.plane);
   if (d.storage == FF_STORAGE_F32) { FF_INIT(float) }
   else if (d.storage == FF_STORAGE_F16) { FF_INIT(__half) }
+  else if (d.storage == FF_STORAGE_F16U) { FF_INIT(h16u) }
   else { FF_INIT(__hip_bfloat16) }
 #undef FF_INIT
   FF_HIP(hipGetLastError());
@@ -293,6 +296,7 @@ int launch_convert_in(Domain& d, const float* aos, void* buf) {
   else k_aos_to_soa<S, 3><<<g, b, 0, d.stream>>>(aos, (S*)buf, d.n, d.plane);
   if (d.storage == FF_STORAGE_F32) { FF_CV(float) }
   else if (d.storage == FF_STORAGE_F16) { FF_CV(__half) }
+  else if (d.storage == FF_STORAGE_F16U) { FF_CV(h16u) }
   else { FF_CV(__hip_bfloat16) }
 #undef FF_CV
   FF_HIP(hipGetLastError());
@@ -307,6 +311,7 @@ int launch_convert_out(Domain& d, const void* buf, float* aos) {
   else k_soa_to_aos<S, 3><<<g, b, 0, d.stream>>>((const S*)buf, aos, d.n, d.plane);
   if (d.storage == FF_STORAGE_F32) { FF_CV(float) }
   else if (d.storage == FF_STORAGE_F16) { FF_CV(__half) }
+  else if (d.storage == FF_STORAGE_F16U) { FF_CV(h16u) }
   else { FF_CV(__hip_bfloat16) }
 #undef FF_CV
   FF_HIP(hipGetLastError());
@@ -324,7 +329,7 @@ __global__ void __launch_bounds__(256) k_mass_partial(const S* __restrict__ rho,
   __shared__ double s[256];
   double acc = 0.0;
   for (long long c = (long long)blockIdx.x * 256 + threadIdx.x; c < n; c += (long long)gridDim.x * 256)
-    if (is_updated(flags[c])) acc += (double)Store<S>::template load<DIM>(rho + c, 0);
+    if (is_updated(flags[c])) acc += (double)Store<S>::template load<DIM>(rho + c, 0, 0);
   s[threadIdx.x] = acc;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
@@ -354,6 +359,7 @@ int launch_mass(Domain& d, double* out) {
   else k_mass_partial<S, 3><<<kRedBlocks, 256, 0, d.stream>>>((const S*)buf, d.flags, d.n, d.red);
   if (d.storage == FF_STORAGE_F32) { FF_MS(float) }
   else if (d.storage == FF_STORAGE_F16) { FF_MS(__half) }
+  else if (d.storage == FF_STORAGE_F16U) { FF_MS(h16u) }
   else { FF_MS(__hip_bfloat16) }
 #undef FF_MS
   k_fold<<<1, 256, 0, d.stream>>>(d.red, kRedBlocks, out);

@@ -112,6 +112,7 @@ int launch_step_stream3d(Domain& d, const void* in, void* out, const LinkArgs& l
   switch (d.storage) {
     case FF_STORAGE_F32: return step3d_typed<float>(d, in, out, la, fa, wave);
     case FF_STORAGE_F16: return step3d_typed<__half>(d, in, out, la, fa, wave);
+    case FF_STORAGE_F16U: return step3d_typed<h16u>(d, in, out, la, fa, wave);
     default: return step3d_typed<__hip_bfloat16>(d, in, out, la, fa, wave);
   }
 }
@@ -131,6 +132,7 @@ int launch_bulk3d(Domain& d, const void* in, void* out) {
   switch (d.storage) {
     case FF_STORAGE_F32: return bulk3d_typed<float>(d, in, out);
     case FF_STORAGE_F16: return bulk3d_typed<__half>(d, in, out);
+    case FF_STORAGE_F16U: return bulk3d_typed<h16u>(d, in, out);
     default: return bulk3d_typed<__hip_bfloat16>(d, in, out);
   }
 }

@@ -118,10 +118,12 @@ int launch_step_sums(Domain& d, const LinkArgs& la, void* out, bool detect_next)
   if (d.dim == 2) {
     if (d.storage == FF_STORAGE_F32) FF_SUM(2, float);
     else if (d.storage == FF_STORAGE_F16) FF_SUM(2, __half);
+    else if (d.storage == FF_STORAGE_F16U) FF_SUM(2, h16u);
     else FF_SUM(2, __hip_bfloat16);
   } else {
     if (d.storage == FF_STORAGE_F32) FF_SUM(3, float);
     else if (d.storage == FF_STORAGE_F16) FF_SUM(3, __half);
+    else if (d.storage == FF_STORAGE_F16U) FF_SUM(3, h16u);
     else FF_SUM(3, __hip_bfloat16);
   }
 #undef FF_SUM

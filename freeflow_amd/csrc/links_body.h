@@ -120,8 +120,8 @@ __device__ __forceinline__ void bcells_body(const S* __restrict__ in, const Link
         float mc[NM], mn[NM];
 #pragma unroll
         for (int q = 0; q < NM; q++) {
-          mc[q] = Store<S>::template load<DIM>(in + q * la.plane + cell, q);
-          mn[q] = Store<S>::template load<DIM>(in + q * la.plane + sc, q);
+          mc[q] = Store<S>::template load<DIM>(in + q * la.plane + cell, q, la.plane);
+          mn[q] = Store<S>::template load<DIM>(in + q * la.plane + sc, q, la.plane);
         }
         // the lane needs two populations: the pulled one and the cell's own opposite one
         const float fn_dir = cell_pop<DIM, STAGE, FAST>(mn, dir);
@@ -501,7 +501,7 @@ __device__ __forceinline__ void elem_sums_body(const LinkArgs& la, const FinalAr
       if (!is_updated(la.flags[cell])) continue;
       const float* row = la.sb + (size_t)slot * NM;
 #pragma unroll
-      for (int q = 0; q < NM; q++) Store<S>::template store<DIM>(o + q * la.plane + cell, q, row[q]);
+      for (int q = 0; q < NM; q++) Store<S>::template store<DIM>(o + q * la.plane + cell, q, row[q], la.plane);
     }
     return;
   }
@@ -514,10 +514,10 @@ __device__ __forceinline__ void elem_sums_body(const LinkArgs& la, const FinalAr
     const unsigned tag = step_tag(la);
     const int nslot = la.ne > 0 ? lt_find(table_cells(la, tag), la.cap, tag, c.nb) : -1;
     const float up = nslot >= 0 ? la.sb[(size_t)nslot * NM + c.comp]
-                                : Store<S>::template load<DIM>(buf + c.comp * la.plane + c.nb, c.comp);
+                                : Store<S>::template load<DIM>(buf + c.comp * la.plane + c.nb, c.comp, la.plane);
     if ((c.sign > 0 && up > 0.0f) || (c.sign < 0 && up < 0.0f))
-      Store<S>::template store<DIM>(buf + c.comp * la.plane + c.cell, c.comp, up);
-    Store<S>::template store<DIM>(buf + c.pcomp * la.plane + c.cell, c.pcomp, up * up);
+      Store<S>::template store<DIM>(buf + c.comp * la.plane + c.cell, c.comp, up, la.plane);
+    Store<S>::template store<DIM>(buf + c.pcomp * la.plane + c.cell, c.pcomp, up * up, la.plane);
     return;
   }
   __shared__ int s_slot[kElemPerBlock][64];

@@ -465,7 +465,10 @@ void Simulator<D>::initialize() {
   desc.dt = m_params.dt;
   desc.fluid_density = m_params.fluid_density;
   const std::string& st = m_params.device_options.storage;
-  desc.storage = st == "f16" ? FF_STORAGE_F16 : (st == "bf16" ? FF_STORAGE_BF16 : FF_STORAGE_F32);
+  desc.storage = st == "f16" ? FF_STORAGE_F16
+                : st == "bf16" ? FF_STORAGE_BF16
+                : st == "f16u" ? FF_STORAGE_F16U
+                               : FF_STORAGE_F32;
   desc.options = (m_params.device_options.stage_fp16 && D == 2 ? FF_OPT_STAGE_FP16 : 0u) |
                  (m_params.device_options.periodic ? FF_OPT_PERIODIC : 0u);
   desc.device = m_params.device_options.device;

@@ -68,7 +68,7 @@ __device__ __forceinline__ void bulk2d_tile(const S* __restrict__ in, S* __restr
       }
       const size_t c = (size_t)gy * nx + gx;
 #pragma unroll
-      for (int q = 0; q < 6; q++) m[r][q] = Store<S>::template load<2>(in + q * plane + c, q);
+      for (int q = 0; q < 6; q++) m[r][q] = Store<S>::template load<2>(in + q * plane + c, q, plane);
     }
   }
 #pragma unroll
@@ -109,7 +109,7 @@ __device__ __forceinline__ void bulk2d_tile(const S* __restrict__ in, S* __restr
     if constexpr (FAST) collide2_fast(p, omega, o);
     else collide2(p, omega, o);
 #pragma unroll
-    for (int q = 0; q < 6; q++) Store<S>::template store<2>(out + q * plane + cell, q, o[q]);
+    for (int q = 0; q < 6; q++) Store<S>::template store<2>(out + q * plane + cell, q, o[q], plane);
   }
 }
 
@@ -150,7 +150,7 @@ __device__ __forceinline__ void bulk3d_tile(const S* __restrict__ in, S* __restr
       }
       const size_t c = ((size_t)sz * ny + sy) * nx + sx;
 #pragma unroll
-      for (int q = 0; q < 10; q++) m[r][q] = Store<S>::template load<3>(in + q * plane + c, q);
+      for (int q = 0; q < 10; q++) m[r][q] = Store<S>::template load<3>(in + q * plane + c, q, plane);
       fl[r] = flags[((size_t)wz * ny + wy) * nx + wx];
     }
   }
@@ -196,7 +196,7 @@ __device__ __forceinline__ void bulk3d_tile(const S* __restrict__ in, S* __restr
   collide3(p, omega, o);
   const size_t cell = ((size_t)z * ny + y) * nx + x;
 #pragma unroll
-  for (int q = 0; q < 10; q++) Store<S>::template store<3>(out + q * plane + cell, q, o[q]);
+  for (int q = 0; q < 10; q++) Store<S>::template store<3>(out + q * plane + cell, q, o[q], plane);
 }
 
 // 3D ring variant: a TX x TY column of cells per block marching along z over
@@ -298,7 +298,7 @@ __device__ __forceinline__ void bulk3d_ring(const S* __restrict__ in, S* __restr
       if (k < HA) {
         const size_t c = zs + off_src[r];
 #pragma unroll
-        for (int q = 0; q < 10; q++) mreg[r][q] = Store<S>::template load<3>(in + q * plane + c, q);
+        for (int q = 0; q < 10; q++) mreg[r][q] = Store<S>::template load<3>(in + q * plane + c, q, plane);
         freg[r] = flags[zf + off_flag[r]];
       }
     }
@@ -391,7 +391,7 @@ __device__ __forceinline__ void bulk3d_ring(const S* __restrict__ in, S* __restr
       collide3(p, omega, o);
       const size_t cell = ((size_t)z * ny + y) * nx + x;
 #pragma unroll
-      for (int q = 0; q < 10; q++) Store<S>::template store<3>(out + q * plane + cell, q, o[q]);
+      for (int q = 0; q < 10; q++) Store<S>::template store<3>(out + q * plane + cell, q, o[q], plane);
     }
     __syncthreads();  // plane z-1's c_z = +1 slot and plane z's c_z = 0 slot are free
     put_rest(rel + 1);
@@ -485,7 +485,7 @@ __device__ __forceinline__ void bulk3d_col(const S* __restrict__ in, S* __restri
     if (staged) {
       const size_t c = (size_t)sz * nxy + off_src;
 #pragma unroll
-      for (int q = 0; q < 10; q++) mm[q] = Store<S>::template load<3>(in + q * plane + c, q);
+      for (int q = 0; q < 10; q++) mm[q] = Store<S>::template load<3>(in + q * plane + c, q, plane);
       f = flags[(size_t)wz * nxy + off_flag];
     } else {
       f = 0;
@@ -613,7 +613,7 @@ __device__ __forceinline__ void bulk3d_col(const S* __restrict__ in, S* __restri
           float mc[10];
           const size_t c = (size_t)z * nxy + (size_t)y * nx + x;
 #pragma unroll
-          for (int q = 0; q < 10; q++) mc[q] = Store<S>::template load<3>(in + q * plane + c, q);
+          for (int q = 0; q < 10; q++) mc[q] = Store<S>::template load<3>(in + q * plane + c, q, plane);
           recon(mc, own);
         }
 #pragma unroll
@@ -642,7 +642,7 @@ __device__ __forceinline__ void bulk3d_col(const S* __restrict__ in, S* __restri
       else collide3(p, omega, o);
       const size_t cell = (size_t)z * nxy + (size_t)y * nx + x;
 #pragma unroll
-      for (int q = 0; q < 10; q++) Store<S>::template store<3>(out + q * plane + cell, q, o[q]);
+      for (int q = 0; q < 10; q++) Store<S>::template store<3>(out + q * plane + cell, q, o[q], plane);
     }
   };
   for (int z = z0; z < zend; z += NP) {

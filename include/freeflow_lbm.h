@@ -49,6 +49,10 @@ extern "C" {
 #define FF_STORAGE_F32 0
 #define FF_STORAGE_F16 1  /* binary16, rho-1 and Pi scaled by 2^8, u by 2^4 */
 #define FF_STORAGE_BF16 2 /* bfloat16, same encoding */
+#define FF_STORAGE_F16U 3 /* binary16 as FF_STORAGE_F16, each velocity component
+                             kept as a pair hi + lo of binary16 values (about
+                             FP32 precision; lo in DIM extra planes after the
+                             NM moment planes): 2 B x (NM + DIM) per cell */
 
 /* option bits */
 #define FF_OPT_STAGE_FP16 0x1u /* 2D: round pulled populations to binary16 as the
@@ -60,8 +64,8 @@ extern "C" {
                                   (pair sums of opposite populations) instead of the
                                   reference's operation order (src/lbm/LbmUtilsFunc.cu:
                                   212-543, FsiCoupling2D.cu:154-252, FsiCoupling3D.cu:
-                                  291-320): 2D tiles without FF_OPT_STAGE_FP16, 3D
-                                  staged-column tiles (7-13), boundary cells in 2D and 3D,
+                                  291-320): 2D tiles without FF_OPT_STAGE_FP16 and 3D
+                                  staged-column tiles (7-13), with their boundary cells,
                                   and FP32 wave kinematics; FP64 stays only in the
                                   per-body force and power sums.  Moments agree with the
                                   reference order to <= 1e-5 relative L2 after 100 steps,

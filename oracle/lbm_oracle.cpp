@@ -36,7 +36,8 @@
 //   periodic (default off): pulls wrap around the domain instead of using the
 //     reference's clamped halo (FsiCoupling2D.cu:104-105).
 //   storage (default FP32): after each step the stored moments are quantised
-//     to FP16/BF16 with the encoding of the product's 16-bit storage modes.
+//     to FP16/BF16 with the encoding of the product's 16-bit storage modes
+//     (3: FP16 with each velocity component kept as a hi + lo FP16 pair).
 // Reproduced quirk: in 2D a population whose upstream node is a wall is
 // bounce-back-assigned and then overwritten by the ordinary pull
 // (FsiCoupling2D.cu:168-223), so 2D walls act through their frozen initial
@@ -662,7 +663,14 @@ static inline double quantize(double v, int slot, int dim, int storage) {
   else if (slot <= dim) s = kScaleU;
   else s = kScalePi;
   float enc = (float)((v - off) * s);
-  float dec = storage == 1 ? h2f(f2h(enc)) : bf2f(f2bf(enc));
+  float dec;
+  if (storage == 3 && slot >= 1 && slot <= dim) {
+    // split velocity: hi + lo binary16 pair, e - hi and hi + lo exact in FP32
+    const float hi = h2f(f2h(enc));
+    dec = hi + h2f(f2h(enc - hi));
+  } else {
+    dec = storage == 2 ? bf2f(f2bf(enc)) : h2f(f2h(enc));
+  }
   return (double)dec / s + off;
 }
 

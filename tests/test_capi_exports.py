@@ -53,10 +53,11 @@ def _round2m(b):
     return (b + (2 << 20) - 1) // (2 << 20) * (2 << 20)
 
 
-def _core_bytes(dim, n, sbytes):
-    """Moments (two ping-pong buffers of nm planes, 256-element plane
-    stride), flags (one byte a cell), 1024 doubles of reduction scratch."""
-    nm = 6 if dim == 2 else 10
+def _core_bytes(dim, n, sbytes, split_velocity=False):
+    """Moments (two ping-pong buffers of nm planes, dim more for the velocity
+    low halves of the split-velocity storage; 256-element plane stride),
+    flags (one byte a cell), 1024 doubles of reduction scratch."""
+    nm = (6 if dim == 2 else 10) + (dim if split_velocity else 0)
     plane = (n + 255) // 256 * 256
     return _round2m(2 * _al(plane * nm * sbytes, 1) + _al(n, 1) + _al(1024, 8))
 
@@ -88,6 +89,8 @@ def test_footprint_without_gpu():
     assert capi.footprint(2, nx, ny, storage="f32") == _core_bytes(2, nx * ny, 4)
     assert capi.footprint(2, nx, ny, storage="f16") == _core_bytes(2, nx * ny, 2)
     assert capi.footprint(3, 96, 96, 96, storage="bf16") == _core_bytes(3, 96 ** 3, 2)
+    assert capi.footprint(3, 96, 96, 96, storage="f16u") == _core_bytes(3, 96 ** 3, 2, True)
+    assert capi.footprint(2, nx, ny, storage="f16u") == _core_bytes(2, nx * ny, 2, True)
     sw = wl.swimmer2d_swimmer()
     nv, ne = len(sw.rest), len(sw.elems)
     with_b = capi.footprint(2, nx, ny, storage="f32", verts=sw.rest, elems=sw.elems, dx=0.005)
@@ -146,7 +149,7 @@ def test_invalid_arguments_rejected_without_gpu():
         (dict(nx=0), b"grid sizes"),
         (dict(ny=-4), b"grid sizes"),
         (dict(nx=1 << 16, ny=1 << 16), b"2^31"),
-        (dict(storage=3), b"storage"),
+        (dict(storage=4), b"storage"),
         (dict(dx=0.0), b"dx"),
         (dict(dx=float("nan")), b"dx"),
         (dict(viscosity=0.0), b"viscosity"),

@@ -173,7 +173,7 @@ def test_autotune_and_single_domain_tiles():
     same(a, c, "batch autotune")
 
 
-@pytest.mark.parametrize("storage", ["f16", "bf16"])
+@pytest.mark.parametrize("storage", ["f16", "bf16", "f16u"])
 def test_batch_16bit_storage(storage):
     sc = scenes(41, n=2)
     note(f"case batch storage {storage}")
@@ -221,7 +221,7 @@ def test_batch_3d_sphere_and_periodic():
     assert np.abs(alone[1].body_wrench()).max() > 0
 
 
-@pytest.mark.parametrize("storage", ["f16", "bf16"])
+@pytest.mark.parametrize("storage", ["f16", "bf16", "f16u"])
 def test_batch_3d_16bit_storage(storage):
     # two 3D members of different sizes, each with a moving sphere, in the
     # served form's 16-bit storage: each equals its lone steps

@@ -113,11 +113,11 @@ def _member_oracle(sc, storage, fp64):
     return o
 
 
-def _batch64(scenes, fast, m0):
-    b = capi.Batch(2, "f16", stage_fp16=False, fast_math=fast)
+def _batch64(scenes, fast, m0, storage="f16"):
+    b = capi.Batch(2, storage, stage_fp16=False, fast_math=fast)
     doms = []
     for k, sc in enumerate(scenes):
-        d = capi.Domain(**sc.domain_args(), storage="f16", stage_fp16=False, fast_math=fast)
+        d = capi.Domain(**sc.domain_args(), storage=storage, stage_fp16=False, fast_math=fast)
         sc.setup(d)
         if k in m0:
             d.set_moments(m0[k])
@@ -129,8 +129,9 @@ def _batch64(scenes, fast, m0):
 def test_batch_of_64_uneven_swimmers():
     """The 64-domain batch in FP16 storage.  Reference operation order: three
     sampled members bit-identical to the quantising oracle for 30 steps.
-    Fast math (the form the batched benchmark times): velocity error of the
-    smallest member against the FP64 oracle after 100 and 1000 steps."""
+    Fast math (the forms the batched benchmark times), FP16 and split-velocity
+    FP16 storage: velocity error of the smallest member against the FP64
+    oracle after 100 and 1000 steps."""
     c = wl.BATCH64
     scenes = wl.batched_scenes(c["n_domains"], c["seed"])
     sizes = np.array([s.nx * s.ny for s in scenes])
@@ -161,42 +162,47 @@ def test_batch_of_64_uneven_swimmers():
     b.close()
     for d in doms:
         d.close()
-    note("  fast-math form, 1000 steps against the FP64 oracle")
-    b, doms = _batch64(scenes, True, m0)
+    # fast math (the form the batched benchmark times), FP16 storage and the
+    # split-velocity FP16 storage: the smallest member against the FP64 oracle
     k64 = sample[0]
-    o64 = _member_oracle(scenes[k64], 0, True)
-    o64.set_moments(m0[k64])
-    v, vel = doms[k64].boundary()
-    o64.set_boundary(v, vel, scenes[k64].swimmer.elems)
-    at100 = None
-    for step in range(1000):
-        b.step(1)
-        v, vel = doms[k64].boundary()
-        o64.update_boundary(v, vel)
-        o64.step(1)
-        if step == 99:
-            at100 = (doms[k64].moments(0).astype(np.float64), o64.moments(0))
-    b.sync()
     sc = scenes[k64]
     upd = sc.flags == 1
-    e100 = rel_l2(at100[0][upd, 1:3], at100[1][upd, 1:3])
-    a = doms[k64].moments(0).astype(np.float64)
-    r = o64.moments(0)
-    err = rel_l2(a[upd, 1:3], r[upd, 1:3])
-    note(f"  member {k64} ({sc.nx}x{sc.ny}) FP16 storage, fast math: velocity rel-L2 vs FP64 {e100:.3e} after "
-         f"100 steps, {err:.3e} after 1000")
-    # the bar (1e-3) holds while the flow is smooth; by step 500 a link
-    # decision of the swimmer's moving boundary has flipped between FP32 and
-    # FP64 arithmetic (the CPU oracle alone in FP32 is 1.5e-3 from FP64 on this
-    # member at 500 and 1000 steps, 4.6e-6 at 200), so at 1000 steps the
-    # error is the flow's, not the storage's, and only bounded here
-    assert e100 <= 1e-3
-    assert err <= 3e-2
-    wr = np.concatenate([d.body_wrench() for d in doms])
-    assert np.all(np.isfinite(wr))
-    b.close()
-    for d in doms:
-        d.close()
+    for storage in ("f16", "f16u"):
+        note(f"  fast-math form, {storage} storage, 1000 steps against the FP64 oracle")
+        b, doms = _batch64(scenes, True, m0, storage)
+        o64 = _member_oracle(sc, 0, True)
+        o64.set_moments(m0[k64])
+        v, vel = doms[k64].boundary()
+        o64.set_boundary(v, vel, sc.swimmer.elems)
+        at100 = None
+        for step in range(1000):
+            b.step(1)
+            v, vel = doms[k64].boundary()
+            o64.update_boundary(v, vel)
+            o64.step(1)
+            if step == 99:
+                at100 = (doms[k64].moments(0).astype(np.float64), o64.moments(0))
+        b.sync()
+        e100 = rel_l2(at100[0][upd, 1:3], at100[1][upd, 1:3])
+        a = doms[k64].moments(0).astype(np.float64)
+        r = o64.moments(0)
+        err = rel_l2(a[upd, 1:3], r[upd, 1:3])
+        note(f"  member {k64} ({sc.nx}x{sc.ny}) {storage} storage, fast math: velocity rel-L2 vs FP64 {e100:.3e} "
+             f"after 100 steps, {err:.3e} after 1000")
+        # FP16 storage rounds every stored moment to 11 significant bits each
+        # step: within the 1e-3 bar after 100 steps, 3.5e-3 after 1000 (the
+        # FP32 oracle alone drifts 1.1e-4 from FP64 over the same 1000 steps).
+        # Keeping the velocity at ~22 bits removes the largest share; in 2D the
+        # rounding of rho - 1 and Pi still adds up (CPU oracle, 256x128
+        # swimmer: 1.0e-3 after 500 steps with the velocity kept, 3.0e-3 with
+        # FP16 throughout), so both are bounded here and reported.
+        assert e100 <= 1e-3
+        assert err <= 1e-2
+        wr = np.concatenate([d.body_wrench() for d in doms])
+        assert np.all(np.isfinite(wr))
+        b.close()
+        for d in doms:
+            d.close()
 
 
 def test_served_episodes_at_real_sizes():

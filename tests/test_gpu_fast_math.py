@@ -172,6 +172,7 @@ def test_fast_3d_moving_sphere():
     u += rng.uniform(-1e-3, 1e-3, u.shape)
     m = np.asarray(wl.equilibrium_moments(u), np.float32)
     g = capi.Domain(3, nx, ny, nz, viscosity=0.01, periodic=True, fast_math=True)
+    g.set_tile(9)  # in 3D the fast form belongs to the staged-column tiles (7-13)
     o = ol.OracleDomain(3, nx, ny, nz, fp64=False, periodic=True, nu=0.01)
     g.set_moments(m)
     o.set_moments(m.astype(np.float64))
@@ -184,5 +185,5 @@ def test_fast_3d_moving_sphere():
     o.step(100)
     rel = rel_l2(g.moments(0).astype(np.float64), o.moments(0))
     note(f"  moments vs FP32 oracle {rel:.3e}")
-    assert rel <= 1e-5
+    assert 0.0 < rel <= 1e-5  # not bit-exact: the fast form ran
     g.close()

@@ -27,7 +27,7 @@ def pair(nx, ny, flags, m, stage=True, periodic=False, storage="f32", nu=0.005, 
                     stage_fp16=stage, periodic=periodic)
     cf = capi.cf_of(2, rho, dx, dt)
     o = ol.OracleDomain(2, nx, ny, fp64=False, stage_fp16=stage, periodic=periodic, nu=nu, dx=dx, dt=dt, cf=cf,
-                        storage={"f32": 0, "f16": 1, "bf16": 2}[storage])
+                        storage={"f32": 0, "f16": 1, "bf16": 2, "f16u": 3}[storage])
     g.set_flags(flags)
     o.set_flags(flags)
     mf = np.asarray(m, np.float32)
@@ -87,7 +87,7 @@ def test_reference_initializer_bitexact():
     assert_same(g, o, 1)
 
 
-@pytest.mark.parametrize("storage", ["f16", "bf16"])
+@pytest.mark.parametrize("storage", ["f16", "bf16", "f16u"])
 def test_16bit_storage_bitexact(storage):
     nx, ny = 80, 40
     note(f"case storage={storage}")
@@ -198,7 +198,7 @@ def test_max_elements_65536_bitexact():
     assert np.allclose(w[:, :2], f1, rtol=0, atol=1e-5 * np.abs(f1).max())
 
 
-@pytest.mark.parametrize("storage,stage", [("f32", True), ("f16", False), ("bf16", False)])
+@pytest.mark.parametrize("storage,stage", [("f32", True), ("f16", False), ("bf16", False), ("f16u", False)])
 def test_moving_swimmer_explicit_boundary(storage, stage):
     # explicit per-step vertex positions/velocities (the reference proxy
     # contract), an elliptical swimmer with a travelling wave, outlet box;

@@ -28,7 +28,7 @@ def pair3(nx, ny, nz, flags, m, periodic=False, storage="f32", nu=0.01, dx=1.0, 
     g = capi.Domain(3, nx, ny, nz, viscosity=nu, dx=dx, dt=dt, fluid_density=rho, storage=storage,
                     periodic=periodic)
     o = ol.OracleDomain(3, nx, ny, nz, fp64=False, periodic=periodic, nu=nu, dx=dx, dt=dt,
-                        cf=capi.cf_of(3, rho, dx, dt), storage={"f32": 0, "f16": 1, "bf16": 2}[storage])
+                        cf=capi.cf_of(3, rho, dx, dt), storage={"f32": 0, "f16": 1, "bf16": 2, "f16u": 3}[storage])
     g.set_flags(flags)
     o.set_flags(flags)
     mf = np.asarray(m, np.float32)
@@ -82,7 +82,7 @@ def test_channel_walls_inlet_outlet_clamped():
     compare(g, o, 1)
 
 
-@pytest.mark.parametrize("spin,storage", [(0.0, "f32"), (0.02, "f32"), (0.02, "f16")])
+@pytest.mark.parametrize("spin,storage", [(0.0, "f32"), (0.02, "f32"), (0.02, "f16"), (0.02, "f16u")])
 def test_sphere_moving_wall_links(spin, storage):
     nx, ny, nz = 40, 32, 32
     note(f"case 3d sphere spin={spin} storage={storage}")
@@ -196,7 +196,7 @@ def test_spheres_across_the_grid_edge(periodic):
     assert np.linalg.norm(fg - fo) / np.linalg.norm(fo) < 1e-5
 
 
-@pytest.mark.parametrize("storage", ["f16", "bf16"])
+@pytest.mark.parametrize("storage", ["f16", "bf16", "f16u"])
 def test_16bit_storage_3d(storage):
     nx, ny, nz = 24, 16, 16
     note(f"case 3d {storage} storage")

@@ -36,11 +36,12 @@ def tg3d_fp64():
 
 
 # FP32 storage: rounding of every operation; 16-bit storage: one rounding of
-# every stored moment per step (11 / 8 significant bits).
-BOUNDS3D = {"f32": 1e-5, "f16": 1e-3, "bf16": 1e-2}
+# every stored moment per step (11 / 8 significant bits; the split velocity
+# keeps about 22 for u, which dominates the FP16 error in 3D).
+BOUNDS3D = {"f32": 1e-5, "f16": 1e-3, "bf16": 1e-2, "f16u": 5e-5}
 
 
-@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16", "f16u"])
 def test_3d_storage_error_vs_fp64(tg3d_fp64, storage):
     nx, ny, nz, steps, m0, ref = tg3d_fp64
     d = capi.Domain(3, nx, ny, nz, viscosity=0.01, storage=storage, periodic=True)
@@ -56,10 +57,13 @@ def test_3d_storage_error_vs_fp64(tg3d_fp64, storage):
     assert drift < (1e-5 if storage == "f32" else 1e-4)
 
 
-BOUNDS2D = {"f32": 1e-5, "f16": 1e-3, "bf16": 1e-2}
+# In 2D the swimmer's flow is sensitive to rho - 1 and Pi too: the CPU oracle
+# with only the velocity kept at FP32 precision is 3.5e-4 from FP64 after 200
+# steps (FP16 7.0e-4; rounding only rho - 1: 1.9e-4; only Pi: 7.5e-5).
+BOUNDS2D = {"f32": 1e-5, "f16": 1e-3, "bf16": 1e-2, "f16u": 5e-4}
 
 
-@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16", "f16u"])
 def test_2d_swimmer_storage_error_vs_fp64(storage):
     sc = wl.batched_scenes(1, 77, (256, 256), (128, 128), (200, 200))[0]
     steps = 200

@@ -196,6 +196,37 @@ def test_bf16_matches_torch():
     assert np.array_equal(got, ref)
 
 
+def test_storage_quantisation_encodings():
+    """The 16-bit storage encodings against numpy: FP16 (mode 1) stores
+    fl16((v - off) s) / s + off with s = 2^8 for rho - 1 and Pi, 2^4 for u;
+    the split-velocity mode (3) stores rho - 1 and Pi the same way and u as
+    hi + lo, hi = fl16(16 u), lo = fl16(16 u - hi): its velocity error stays
+    below 2^-21 |u| where lo is a normal binary16 (|16 u - hi| >= 2^-14) and
+    below half a subnormal step, 2^-25 / 16, absolute otherwise, far inside
+    FP16's 2^-11 |u|."""
+    rng = np.random.default_rng(21)
+    for dim in (2, 3):
+        nm = 6 if dim == 2 else 10
+        for slot in range(nm):
+            off, sc = (1.0, 256.0) if slot == 0 else ((0.0, 16.0) if slot <= dim else (0.0, 256.0))
+            vs = (off + np.concatenate([rng.uniform(-0.1, 0.1, 300), rng.uniform(-1e-4, 1e-4, 100)])).astype(np.float32)
+            for v in vs.astype(np.float64):
+                f16 = float(np.float16(np.float32((v - off) * sc))) / sc + off
+                assert L.orc_quantize(v, slot, dim, 1) == f16
+                got = L.orc_quantize(v, slot, dim, 3)
+                if slot == 0 or slot > dim:
+                    assert got == f16
+                else:
+                    e = np.float32(v * sc)
+                    hi = np.float32(np.float16(e))
+                    lo = np.float32(np.float16(np.float32(e - hi)))
+                    assert got == float(np.float32(hi + lo)) / sc
+                    err = abs(got - v)
+                    assert err <= max(2.0 ** -21 * abs(v), 2.0 ** -25 / sc)
+                    if abs(v) > 1e-3:
+                        assert err < abs(f16 - v) or f16 == v
+
+
 # ---------------------------------------------------------------- geometry
 def seg_intersect_f64(p1, p2, p3, p4):
     """Parametric segment/segment intersection in float64: returns (t on the
