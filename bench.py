@@ -65,8 +65,10 @@ TILES3 = ["8x8x4 box", "16x4x4 box", "16x8 ring column x32", "16x8 ring column x
           "16x12 staged column x16, 3 planes in an LDS ring"]
 
 
-def bytes_per_lup(dim, storage_bytes):
-    nm = 6 if dim == 2 else 10
+def bytes_per_lup(dim, storage_bytes, split_velocity=False):
+    """Algorithmic bytes per cell update: read and write every stored plane
+    once (the split-velocity storage has dim more planes), read the flag."""
+    nm = (6 if dim == 2 else 10) + (dim if split_velocity else 0)
     return 2 * nm * storage_bytes + 1
 
 
@@ -364,19 +366,19 @@ def tg3d_time(capi, wl, m, storage, fast, K, W, tile, tune=True):
     avg_ms = stream_ms / K
     drift = abs(d.mass() - m0) / m0
     d.close()
-    bpl = bytes_per_lup(3, 4 if storage == "f32" else 2)
-    sname = {"f32": "float", "f16": "__half", "bf16": "__hip_bfloat16"}[storage]
+    bpl = bytes_per_lup(3, 4 if storage == "f32" else 2, storage == "f16u")
+    sname = {"f32": "float", "f16": "__half", "bf16": "__hip_bfloat16", "f16u": "ff::h16u"}[storage]
     return {"mlups": cells * K / el / 1e6, "ms_per_step": el * 1e3 / K, "host_ms_per_step": host_el * 1e3 / K,
             "tile": t, "avg_kernel_ms": avg_ms,
             "kernel": kernel3d(sname, True, t, fast), "bytes_per_lup": bpl, "cells": cells,
             "frac": bpl * cells / (avg_ms * 1e-3) / (HBM_PEAK_GBS * 1e9), "mass_drift": drift, "steps": K + W}
 
 
-def cpu_baseline_tg3d(capi, wl, m, steps, tiles):
+def cpu_baseline_tg3d(capi, wl, m, steps, forms):
     """CPU leg of the 3D workload: the FP32 CPU oracle (OpenMP, timed: the
     cpu_baseline) and the FP64 oracle, `steps` steps on the full grid from
-    the same state, against the GPU in both arithmetic forms (`tiles`: the
-    tile each form was timed with)."""
+    the same state, against the GPU forms timed (`forms`: name -> (storage,
+    fast math, tile))."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol
     c = wl.TG3D
@@ -385,12 +387,12 @@ def cpu_baseline_tg3d(capi, wl, m, steps, tiles):
     threads = ol.set_threads(0)
     m32 = np.asarray(m, np.float32)
     gpu = {}
-    for fast, tile in tiles.items():
-        d = capi.Domain(3, nx, ny, nz, viscosity=c["viscosity"], storage="f32", periodic=True, fast_math=fast)
+    for name, (storage, fast, tile) in forms.items():
+        d = capi.Domain(3, nx, ny, nz, viscosity=c["viscosity"], storage=storage, periodic=True, fast_math=fast)
         d.set_tile(tile)
         d.set_moments(m32)
         d.step(steps)
-        gpu[fast] = d.moments(0).astype(np.float64)
+        gpu[name] = d.moments(0).astype(np.float64)
         d.close()
     o = ol.OracleDomain(3, nx, ny, nz, fp64=False, stage_fp16=False, periodic=True, nu=c["viscosity"])
     o.set_moments(m32.astype(np.float64))
@@ -407,8 +409,7 @@ def cpu_baseline_tg3d(capi, wl, m, steps, tiles):
     del o
     vel = np.linalg.norm(r64[:, 1:4])
     acc = {"steps": steps, "grid": [nx, ny, nz]}
-    for fast, g in gpu.items():
-        key = "fast_math" if fast else "reference_order"
+    for key, g in gpu.items():
         acc[key] = {"rel_l2_vs_fp32_oracle": float(np.linalg.norm(g - r32) / np.linalg.norm(r32)),
                     "bit_identical_fraction": float(np.mean(g == r32)),
                     "velocity_rel_l2_vs_fp64": float(np.linalg.norm(g[:, 1:4] - r64[:, 1:4]) / vel)}
@@ -486,17 +487,26 @@ def run_tg3d(args, capi, wl):
     fast = st == "f32" and not args.reference_order or args.fast_math
     main_ = tg3d_time(capi, wl, m, st, fast, K, W, args.tile, not args.no_tune)
     extra = {}
-    tiles = {fast: main_["tile"]}
+    forms = {("fast_math" if fast else "reference_order"): (st, fast, main_["tile"])}
+
+    def summary(r):
+        return {"mlups": round(r["mlups"], 2), "ms_per_step": round(r["ms_per_step"], 5), "tile": TILES3[r["tile"]],
+                "kernel": r["kernel"], "avg_kernel_us": round(r["avg_kernel_ms"] * 1e3, 3),
+                "bytes_per_lup": r["bytes_per_lup"], "frac": round(r["frac"], 4)}
     if st == "f32":
         ref = tg3d_time(capi, wl, m, st, not fast, K, W, -1, not args.no_tune)
-        tiles[not fast] = ref["tile"]
-        extra["reference_order" if fast else "fast_math"] = {
-            "mlups": round(ref["mlups"], 2), "ms_per_step": round(ref["ms_per_step"], 5), "tile": TILES3[ref["tile"]],
-            "kernel": ref["kernel"], "avg_kernel_us": round(ref["avg_kernel_ms"] * 1e3, 3),
-            "frac": round(ref["frac"], 4)}
+        key = "reference_order" if fast else "fast_math"
+        forms[key] = (st, not fast, ref["tile"])
+        extra[key] = summary(ref)
+        # the 16-bit storage forms of the same workload (fast math)
+        extra["storage_forms"] = {}
+        for s16 in ("f16", "f16u"):
+            r = tg3d_time(capi, wl, m, s16, True, K, W, -1, not args.no_tune)
+            forms[s16 + "_fast_math"] = (s16, True, r["tile"])
+            extra["storage_forms"][s16] = summary(r)
     cpu = None
     if not args.no_cpu and st == "f32":
-        cpu, acc = cpu_baseline_tg3d(capi, wl, m, args.check_steps, tiles)
+        cpu, acc = cpu_baseline_tg3d(capi, wl, m, args.check_steps, forms)
         acc["mass_drift_timed_run"] = main_["mass_drift"]
         extra["accuracy"] = acc
     if not args.no_extras:
@@ -574,7 +584,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the default tile shape instead of timing the candidates")
     ap.add_argument("--tile", type=int, default=-1, help="use this tile index instead of timing the candidates")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
-    ap.add_argument("--storage", default="f32", help="tg3d: f32, f16 or bf16")
+    ap.add_argument("--storage", default="f32", help="tg3d: f32, f16, bf16 or f16u")
     ap.add_argument("--fast-math", action="store_true",
                     help="tg3d: reassociated FP32 arithmetic with FMA in the column tiles (FF_OPT_FAST_MATH); the "
                          "default for FP32 storage")
