@@ -164,6 +164,13 @@ void bind_simulator(py::module_& m, const char* name) {
           },
           "Per-body FP64 wrench of the most recent step: 2D (Fx, Fy, Tz), 3D (Fx, Fy, Fz, Tx, Ty, Tz).")
       .def(
+          "get_body_power",
+          [](const S& s) {
+            std::vector<double> p = s.getBodyPower();
+            return py::array_t<double>((py::ssize_t)p.size(), p.data());
+          },
+          "Per-body FP64 power of the most recent step: the sum over the body's boundary vertices of -F . u.")
+      .def(
           "get_fluid_moments_buffer",
           [](const S& s, int which) {
             const std::vector<float> mom = s.getFluidMoment(which);
@@ -237,6 +244,7 @@ PYBIND11_MODULE(fsi_simulator, m) {
       .def_readwrite("storage", &DeviceOptions::storage)
       .def_readwrite("stage_fp16", &DeviceOptions::stage_fp16)
       .def_readwrite("periodic", &DeviceOptions::periodic)
+      .def_readwrite("fast_math", &DeviceOptions::fast_math)
       .def_readwrite("device", &DeviceOptions::device);
   m.def("flag_from_name", &flag_from_name, py::arg("name"), "LbmNodeFlag ordinal of a boundary type name.");
 

@@ -145,6 +145,7 @@ bool Config<D>::load(const std::string& filepath) {
       p.device_options.storage = o.string_or("storage", p.device_options.storage);
       p.device_options.stage_fp16 = o.bool_or("stage_fp16", p.device_options.stage_fp16);
       p.device_options.periodic = o.bool_or("periodic", p.device_options.periodic);
+      p.device_options.fast_math = o.bool_or("fast_math", p.device_options.fast_math);
       p.device_options.device = o.int_or("device", p.device_options.device);
     }
     // postProcess (Config.cpp:72-89): warnings only
@@ -470,7 +471,8 @@ void Simulator<D>::initialize() {
                 : st == "f16u" ? FF_STORAGE_F16U
                                : FF_STORAGE_F32;
   desc.options = (m_params.device_options.stage_fp16 && D == 2 ? FF_OPT_STAGE_FP16 : 0u) |
-                 (m_params.device_options.periodic ? FF_OPT_PERIODIC : 0u);
+                 (m_params.device_options.periodic ? FF_OPT_PERIODIC : 0u) |
+                 (m_params.device_options.fast_math ? FF_OPT_FAST_MATH : 0u);
   desc.device = m_params.device_options.device;
   check(ff_domain_create(&desc, &m_dom), "ff_domain_create");
   upload_flags_and_state();
@@ -655,6 +657,17 @@ std::vector<double> Simulator<D>::getBodyWrench() const {
   const int nb = std::max<int>(1, (int)m_bodies.size());
   std::vector<double> out((size_t)nb * (D == 2 ? 3 : 6), 0.0);
   if (m_dom && !m_surf_to_vol.empty()) check(ff_get_body_wrench(m_dom, out.data()), "ff_get_body_wrench");
+  return out;
+}
+
+// Per-body power of the most recent step, sum over the body's boundary
+// vertices of -F . u: what the reference's RL loop computes on the host from
+// every vertex force and velocity (rl/env.py:175-178).
+template <int D>
+std::vector<double> Simulator<D>::getBodyPower() const {
+  const int nb = std::max<int>(1, (int)m_bodies.size());
+  std::vector<double> out((size_t)nb, 0.0);
+  if (m_dom && !m_surf_to_vol.empty()) check(ff_get_body_power(m_dom, out.data()), "ff_get_body_power");
   return out;
 }
 

@@ -75,9 +75,10 @@ using SolidBodyConfig3D = SolidBodyConfig<3>;
 // Fluid options beyond the reference schema ("freeflow_amd" object in the
 // JSON); defaults reproduce the reference's behaviour.
 struct DeviceOptions {
-  std::string storage = "f32";  // f32 | f16 | bf16
+  std::string storage = "f32";  // f32 | f16 | bf16 | f16u (FP16, velocity as a hi + lo pair)
   bool stage_fp16 = true;       // 2D: the reference's FP16 population tile
   bool periodic = false;        // periodic pulls instead of the clamped halo
+  bool fast_math = false;       // FMA-reassociated update (2D without stage_fp16, 3D column tiles)
   int device = 0;
 };
 
@@ -167,6 +168,7 @@ class Simulator {
   // work): what initialize() does for the solids.
   void loadSolids() { load_bodies(); }
   std::vector<double> getBodyWrench() const;
+  std::vector<double> getBodyPower() const;
   void setWave(const std::vector<float>& s_coord, const std::vector<double>& params);
   double fluidMass() const;
   ff_domain* domain() const { return m_dom; }

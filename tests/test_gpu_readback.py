@@ -186,3 +186,48 @@ def test_save_frame_data_2d_and_3d(tmp_path):
     pts = np.array([[float(x) for x in l.split()] for l in sl[5:5 + npts]])
     np.testing.assert_allclose(pts, sim.getVertices(), rtol=1e-5, atol=1e-9)
     assert int(sl[5 + npts].split()[1]) == len(tets)
+
+
+def test_dropin_body_power_and_fast_math(tmp_path):
+    """The drop-in's get_body_power equals the host sum the reference's RL
+    loop computes from getForce and getVelocity (rl/env.py:175-178), in the
+    reference form and with the `fast_math` device option, whose flow agrees
+    with the reference form to 1e-5."""
+    v, t = meshes.disk(32)
+    mesh = tmp_path / "disk.mesh"
+    meshes.write_mesh(mesh, 2, v, tris=t)
+    nx, ny = 96, 64
+    sims = {}
+    for fast in (False, True):
+        extra = {"freeflow_amd": {"fast_math": True, "stage_fp16": False}} if fast else \
+            {"freeflow_amd": {"stage_fp16": False}}
+        path = meshes.write_config(
+            tmp_path / f"p{int(fast)}.json", dimension=2, fluid_viscosity=0.01, fluid_density=1000.0, fluid_nx=nx,
+            fluid_ny=ny, fluid_dx=0.01, dt=0.005, solid_solver_type="static", output_frequency=0,
+            boundary_velocities=[0.03, 0.0, 0.0, 0.0],
+            boundaries=meshes.channel_boundaries(nx, ny),
+            solids=[{"mesh_path": str(mesh), "translate": [0.3, 0.3], "scale": [0.08, 0.08]}], **extra)
+        cfg = fs.Config2D()
+        assert cfg.load(path)
+        params = cfg.get_params()
+        assert params.device_options.fast_math == fast
+        sim = fs.Simulator2D(params)
+        sim.initialize()
+        sim.apply_kinematic_control(0, np.array([[0.45, 0.33], [0.3, 0.3]]), 0.05)  # needs two targets
+        nonzero = 0
+        for _ in range(12):
+            U = np.asarray(sim.getVelocity(), np.float64)  # the velocities this step's boundary carries
+            sim.step()
+            p = sim.get_body_power()
+            F = np.asarray(sim.getForce(), np.float64)
+            ref = -(F * U).sum()
+            scale = np.abs(F * U).sum()
+            assert abs(p[0] - ref) <= 1e-4 * scale + 1e-30, f"{p[0]} vs {ref}"
+            nonzero += scale > 0
+        assert nonzero >= 8
+        sims[fast] = (sim, cfg, params)
+    a = sims[False][0].get_fluid_moments().astype(np.float64)
+    b = sims[True][0].get_fluid_moments().astype(np.float64)
+    rel = np.linalg.norm(a - b) / np.linalg.norm(a)
+    note(f"  drop-in fast math vs reference form after 12 steps: rel-L2 {rel:.3e}")
+    assert 0.0 < rel <= 1e-5
