@@ -62,7 +62,7 @@ TILES3 = ["8x8x4 box", "16x4x4 box", "16x8 ring column x32", "16x8 ring column x
           "16x16 ring column x32", "16x16 ring column x16", "16x12 staged column x8", "16x26 staged column x8",
           "64x4 staged column x16", "16x12 staged column x8, 2 planes in an LDS ring",
           "16x8 staged column x8, 3 planes in an LDS ring", "32x8 staged column x8, 2 planes in an LDS ring",
-          "16x12 staged column x16, 3 planes in an LDS ring"]
+          "16x12 staged column x16, 3 planes in an LDS ring", "64x4 staged column x32", "32x12 staged column x16"]
 
 
 def bytes_per_lup(dim, storage_bytes, split_velocity=False):
@@ -125,7 +125,7 @@ SHAPES2 = [(32, 8, 1), (32, 8, 2), (64, 4, 2), (32, 8, 4), (64, 4, 4)]
 SHAPES3 = [(8, 8, 4, 0, 256, 1, 0), (16, 4, 4, 0, 256, 1, 0), (16, 8, 32, 1, 128, 1, 0), (16, 8, 16, 1, 128, 1, 0),
            (16, 8, 8, 1, 128, 1, 0), (16, 16, 32, 1, 256, 1, 0), (16, 16, 16, 1, 256, 1, 0), (16, 12, 8, 2, 256, 1, 0),
            (16, 26, 8, 2, 512, 1, 0), (64, 4, 16, 2, 448, 1, 0), (16, 12, 8, 2, 256, 2, 1), (16, 8, 8, 2, 192, 3, 1),
-           (32, 8, 8, 2, 384, 2, 1), (16, 12, 16, 2, 256, 3, 1)]
+           (32, 8, 8, 2, 384, 2, 1), (16, 12, 16, 2, 256, 3, 1), (64, 4, 32, 2, 448, 1, 0), (32, 12, 16, 2, 512, 1, 0)]
 
 
 def kernel2d(prefix, s, stage, periodic, tile, fast=False):

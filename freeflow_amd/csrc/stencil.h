@@ -226,8 +226,8 @@ constexpr Tile3 kTiles3[] = {
     {8, 8, 4, 0, 256, 0, 0},   {16, 4, 4, 0, 256, 0, 0},  {16, 8, 32, 1, 128, 0, 0}, {16, 8, 16, 1, 128, 0, 0},
     {16, 8, 8, 1, 128, 0, 0},  {16, 16, 32, 1, 256, 0, 0}, {16, 16, 16, 1, 256, 0, 0}, {16, 12, 8, 2, 256, 1, 0},
     {16, 26, 8, 2, 512, 1, 0}, {64, 4, 16, 2, 448, 1, 0}, {16, 12, 8, 2, 256, 2, 1},  {16, 8, 8, 2, 192, 3, 1},
-    {32, 8, 8, 2, 384, 2, 1},  {16, 12, 16, 2, 256, 3, 1}};
-constexpr int kNumTiles3 = 14;
+    {32, 8, 8, 2, 384, 2, 1},  {16, 12, 16, 2, 256, 3, 1}, {64, 4, 32, 2, 448, 1, 0}, {32, 12, 16, 2, 512, 1, 0}};
+constexpr int kNumTiles3 = 16;
 
 // The shapes of kTiles3 by index (kept in step with the table);
 // F(FAST, KIND, TX, TY, ZC, NT, PF, DMA) instantiates one.
@@ -246,6 +246,8 @@ constexpr int kNumTiles3 = 14;
     case 11: F(FASTV, 2, 16, 8, 8, 192, 3, true); break;       \
     case 12: F(FASTV, 2, 32, 8, 8, 384, 2, true); break;       \
     case 13: F(FASTV, 2, 16, 12, 16, 256, 3, true); break;     \
+    case 14: F(FASTV, 2, 64, 4, 32, 448, 1, false); break;     \
+    case 15: F(FASTV, 2, 32, 12, 16, 512, 1, false); break;    \
     default: F(false, 0, 8, 8, 4, 256, 1, false); break;       \
   }
 

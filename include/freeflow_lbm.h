@@ -65,7 +65,7 @@ extern "C" {
                                   reference's operation order (src/lbm/LbmUtilsFunc.cu:
                                   212-543, FsiCoupling2D.cu:154-252, FsiCoupling3D.cu:
                                   291-320): 2D tiles without FF_OPT_STAGE_FP16 and 3D
-                                  staged-column tiles (7-13), with their boundary cells,
+                                  staged-column tiles (7-15), with their boundary cells,
                                   and FP32 wave kinematics; FP64 stays only in the
                                   per-body force and power sums.  Moments agree with the
                                   reference order to <= 1e-5 relative L2 after 100 steps,
@@ -238,13 +238,14 @@ int ff_domain_footprint(const ff_domain_desc* desc, int nv, int ne, const float*
                         unsigned long long* bytes);
 
 /* Stream-collide tile shape.  2D (0..4): 32x8, 32x16, 64x8, 32x32, 64x16
-   cells per 256-thread block.  3D (0..13): 0-1 8x8x4 and 16x4x4 boxes; 2-6
+   cells per 256-thread block.  3D (0..15): 0-1 8x8x4 and 16x4x4 boxes; 2-6
    columns that rotate reconstructed populations through LDS (16x8 marching
-   32, 16 or 8 planes, 16x16 marching 32 or 16); 7-13 columns with one
+   32, 16 or 8 planes, 16x16 marching 32 or 16); 7-15 columns with one
    staged cell per thread (7: 16x12 x 8 planes, 8: 16x26 x 8, 9: 64x4 x 16;
    10-13 keep the next planes' moments in an LDS ring filled by LDS DMA with
    FP32 storage: 16x12 x 8 with 2 planes in flight, 16x8 x 8 with 3, 32x8 x 8
-   with 2, 16x12 x 16 with 3), the only shapes FF_OPT_FAST_MATH changes.
+   with 2, 16x12 x 16 with 3; 14: 64x4 x 32, 15: 32x12 x 16), the only shapes
+   FF_OPT_FAST_MATH changes.
    0 is the default; in the reference's operation order results do not
    depend on it.
    No counterpart in the reference (its tile is fixed, CudaCommon.cuh:44-51). */

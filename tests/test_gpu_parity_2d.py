@@ -456,8 +456,9 @@ def test_mass_fp64():
 
 def test_swimmer2d_1200x400_parity_and_determinism():
     # the 1200x400 swimmer workload at full size (~600-segment swimmer, wave
-    # kinematics on device): 20 steps against the oracle, then a repeat run
-    # must be bit-identical (moments and per-step wrench).
+    # kinematics on device): 100 steps against the oracle (bit-exact moments
+    # after 20 and after 100), then a repeat run must be bit-identical
+    # (moments and per-step wrench).
     c = wl.SWIMMER2D
     nx, ny = c["nx"], c["ny"]
     note(f"case swimmer2d full size {nx}x{ny}")
@@ -479,17 +480,19 @@ def test_swimmer2d_1200x400_parity_and_determinism():
             o.set_flags(flags)
             o.set_moments(m)
             o.set_boundary(v0, vel0, sw.elems)
-            for _ in range(20):
+            for s in range(100):
                 g.step(1)
                 v, vel = g.boundary()
                 o.update_boundary(v, vel)
                 o.step(1)
-            assert_same(g, o, 0)
+                if s + 1 in (20, 100):
+                    assert_same(g, o, 0)
+            assert_same(g, o, 1)
             fo = o.vertex_forces()
             fg = g.vertex_forces().astype(np.float64)
             assert np.linalg.norm(fg - fo) / np.linalg.norm(fo) < 1e-5
         else:
-            g.step(20)
+            g.step(100)
         runs.append((g.moments(), g.wrench_history(32)))
     assert np.array_equal(runs[0][0], runs[1][0])
     assert np.array_equal(runs[0][1], runs[1][1])

@@ -210,7 +210,7 @@ def test_16bit_storage_3d(storage):
     assert same == 1.0
 
 
-@pytest.mark.parametrize("tile", [0, 2, 5, 7, 8, 9, 10, 11, 12, 13])
+@pytest.mark.parametrize("tile", [0, 2, 5, 7, 8, 9, 10, 11, 12, 13, 14, 15])
 def test_grid_smaller_than_tile(tile):
     """6x5x3 periodic cells: every box or column is partial and the halo
     wraps around the whole grid."""
@@ -225,7 +225,7 @@ def test_grid_smaller_than_tile(tile):
     assert same == 1.0
 
 
-@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13])
+@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15])
 def test_tile_variants_match_oracle(tile):
     # walls on every side (3D flag bounce-back), inlet/outlet, nz = 40 so the
     # 32-plane columns have a full and a partial chunk
@@ -271,7 +271,7 @@ def _channel_flags(nx, ny, nz):
     return f.ravel()
 
 
-@pytest.mark.parametrize("tile", [7, 8, 9, 10, 11, 12, 13])
+@pytest.mark.parametrize("tile", [7, 8, 9, 10, 11, 12, 13, 14, 15])
 def test_fast_math_columns_within_tolerance(tile):
     """FF_OPT_FAST_MATH (pair-decomposed reconstruction and moments, FMA):
     moments within 1e-5 relative L2 of the reference-order oracle after 100
