@@ -136,7 +136,6 @@ def kernel2d(prefix, s, stage, periodic, tile, fast=False):
 
 def kernel3d(s, periodic, tile, fast=False):
     tx, ty, tz, kind, nt, pf, dma = SHAPES3[tile]
-    fast = fast and kind == 2
     return (f"ff::k_bulk3d<{s}, {str(periodic).lower()}, {str(fast).lower()}, {kind}, {tx}, {ty}, {tz}, {nt}, {pf}, "
             f"{str(bool(dma)).lower()}>")
 
@@ -470,7 +469,8 @@ def served_at(rate, duration):
         out["batched_fp16" if batched else "plain_fp32"] = {
             "mlups": round(r["aggregate_mlups"], 2), "p50_step_ms": round(r["p50_step_ms"], 4),
             "p99_step_ms": round(r["p99_step_ms"], 4), "episodes_done": r["episodes_done"],
-            "episodes_rejected": r["episodes_rejected"], "peak_active": r["peak_active"]}
+            "episodes_rejected": r["episodes_rejected"], "peak_active": r["peak_active"], "wall_s": round(r["wall_s"], 3),
+            "phase_s": r["phase_s"]}
     out["speedup_vs_plain_fp32"] = round(out["batched_fp16"]["mlups"] / max(out["plain_fp32"]["mlups"], 1e-9), 3)
     out["rate_per_s"] = rate
     out["arrival_window_s"] = duration

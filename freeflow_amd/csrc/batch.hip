@@ -151,9 +151,11 @@ __global__ void __launch_bounds__(NT) kb_step3d(const DomDev* __restrict__ doms,
       bulk3d_col<S, PERIODIC, FAST, TX, TY, PF, DMA>(in, out, la.flags, la.nx, la.ny, la.nz, la.plane, la.omega, bx,
                                                      by, bz * ZC, min(bz * ZC + ZC, la.nz));
     else if constexpr (KIND == 1)
-      bulk3d_ring<S, PERIODIC, TX, TY, ZC>(in, out, la.flags, la.nx, la.ny, la.nz, la.plane, la.omega, bx, by, bz);
+      bulk3d_ring<S, PERIODIC, TX, TY, ZC, FAST>(in, out, la.flags, la.nx, la.ny, la.nz, la.plane, la.omega, bx, by,
+                                                 bz);
     else
-      bulk3d_tile<S, PERIODIC, TX, TY, ZC>(in, out, la.flags, la.nx, la.ny, la.nz, la.plane, la.omega, bx, by, bz);
+      bulk3d_tile<S, PERIODIC, TX, TY, ZC, FAST>(in, out, la.flags, la.nx, la.ny, la.nz, la.plane, la.omega, bx, by,
+                                                 bz);
   }
 }
 
@@ -421,6 +423,12 @@ static double member_cells(const Batch& b) {
   return c;
 }
 
+// Candidate shapes of the batch's tile search: every 2D shape; in 3D the box
+// and the register-prefetch staged columns (the ring columns and the LDS-DMA
+// columns were never the fastest on a batch, and every candidate costs a
+// table build and a few launches of the whole batch while a server waits).
+constexpr int kBatchCand3[] = {0, 7, 8, 9, 14, 15};
+
 // Time the stream-collide tiles of every member for each candidate shape
 // and keep the fastest.
 static int autotune(Batch& b) {
@@ -428,14 +436,15 @@ static int autotune(Batch& b) {
     b.tuned_tile = 0;
     return FF_OK;
   }
-  const int ntiles = b.dim == 2 ? kNumTiles2 : kNumTiles3;
+  const int ncand = b.dim == 2 ? kNumTiles2 : (int)(sizeof(kBatchCand3) / sizeof(kBatchCand3[0]));
   EventSet ev;
   FF_HIP(ev.create(2));
   const hipEvent_t e0 = ev[0], e1 = ev[1];
   double best = 1e30;
   int best_t = 0;
   int rc = FF_OK;
-  for (int t = 0; t < ntiles && !rc; t++) {
+  for (int c = 0; c < ncand && !rc; c++) {
+    const int t = b.dim == 2 ? c : kBatchCand3[c];
     if ((rc = build_tables(b, t))) break;
     const int2* map = b.d_map + b.n_bc + b.n_wave;
     // long launches need fewer repetitions for a stable time

@@ -27,9 +27,9 @@ __device__ __forceinline__ void tile3d(const S* in, S* out, const uint8_t* flags
     bulk3d_col<S, PERIODIC, FAST, TX, TY, PF, DMA>(in, out, flags, nx, ny, nz, plane, omega, bx, by, bz * ZC,
                                                    min(bz * ZC + ZC, nz));
   else if constexpr (KIND == 1)
-    bulk3d_ring<S, PERIODIC, TX, TY, ZC>(in, out, flags, nx, ny, nz, plane, omega, bx, by, bz);
+    bulk3d_ring<S, PERIODIC, TX, TY, ZC, FAST>(in, out, flags, nx, ny, nz, plane, omega, bx, by, bz);
   else
-    bulk3d_tile<S, PERIODIC, TX, TY, ZC>(in, out, flags, nx, ny, nz, plane, omega, bx, by, bz);
+    bulk3d_tile<S, PERIODIC, TX, TY, ZC, FAST>(in, out, flags, nx, ny, nz, plane, omega, bx, by, bz);
 }
 
 // Staged columns are built for four waves per SIMD (at most 128 VGPRs); with

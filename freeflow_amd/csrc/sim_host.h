@@ -78,7 +78,7 @@ struct DeviceOptions {
   std::string storage = "f32";  // f32 | f16 | bf16 | f16u (FP16, velocity as a hi + lo pair)
   bool stage_fp16 = true;       // 2D: the reference's FP16 population tile
   bool periodic = false;        // periodic pulls instead of the clamped halo
-  bool fast_math = false;       // FMA-reassociated update (2D without stage_fp16, 3D column tiles)
+  bool fast_math = false;       // FMA-reassociated update (2D without stage_fp16; 3D)
   int device = 0;
 };
 

@@ -116,8 +116,8 @@ __device__ __forceinline__ void bulk2d_tile(const S* __restrict__ in, S* __restr
 // 3D box tile of TX x TY x TZ cells (one per thread) with a one-cell halo in
 // LDS (FP32 as the reference's tile, FsiCoupling3D.cu:94).  The reference
 // keeps the flag bounce-back value in 3D (FsiCoupling3D.cu:209-222), so
-// upstream flags are staged too.
-template <typename S, bool PERIODIC, int TX, int TY, int TZ>
+// upstream flags are staged too.  FAST selects recon3_fast / collide3_fast.
+template <typename S, bool PERIODIC, int TX, int TY, int TZ, bool FAST = false>
 __device__ __forceinline__ void bulk3d_tile(const S* __restrict__ in, S* __restrict__ out,
                                             const uint8_t* __restrict__ flags, int nx, int ny, int nz, size_t plane,
                                             float omega, int bx, int by, int bz) {
@@ -160,7 +160,8 @@ __device__ __forceinline__ void bulk3d_tile(const S* __restrict__ in, S* __restr
     const int k = threadIdx.x + r * NT;
     if (k < HN) {
       float f[27];
-      recon3_all(m[r], f);
+      if constexpr (FAST) recon3_fast(m[r], f);
+      else recon3_all(m[r], f);
 #pragma unroll
       for (int i = 0; i < 27; i++) sp[i][k] = f[i];
       sf[k] = fl[r];
@@ -193,7 +194,8 @@ __device__ __forceinline__ void bulk3d_tile(const S* __restrict__ in, S* __restr
     for (int i = 0; i < 27; i++) p[i] = fmaxf(0.0f, sp[i][ctr - (D3::cz(i) * HY + D3::cy(i)) * HX - D3::cx(i)]);
   }
   float o[10];
-  collide3(p, omega, o);
+  if constexpr (FAST) collide3_fast(p, omega, o);
+  else collide3(p, omega, o);
   const size_t cell = ((size_t)z * ny + y) * nx + x;
 #pragma unroll
   for (int q = 0; q < 10; q++) Store<S>::template store<3>(out + q * plane + cell, q, o[q], plane);
@@ -233,12 +235,12 @@ constexpr int kNumTiles3 = 16;
 // F(FAST, KIND, TX, TY, ZC, NT, PF, DMA) instantiates one.
 #define FF_TILES3(F, FASTV, tile)                              \
   switch (tile) {                                              \
-    case 1: F(false, 0, 16, 4, 4, 256, 1, false); break;       \
-    case 2: F(false, 1, 16, 8, 32, 128, 1, false); break;      \
-    case 3: F(false, 1, 16, 8, 16, 128, 1, false); break;      \
-    case 4: F(false, 1, 16, 8, 8, 128, 1, false); break;       \
-    case 5: F(false, 1, 16, 16, 32, 256, 1, false); break;     \
-    case 6: F(false, 1, 16, 16, 16, 256, 1, false); break;     \
+    case 1: F(FASTV, 0, 16, 4, 4, 256, 1, false); break;       \
+    case 2: F(FASTV, 1, 16, 8, 32, 128, 1, false); break;      \
+    case 3: F(FASTV, 1, 16, 8, 16, 128, 1, false); break;      \
+    case 4: F(FASTV, 1, 16, 8, 8, 128, 1, false); break;       \
+    case 5: F(FASTV, 1, 16, 16, 32, 256, 1, false); break;     \
+    case 6: F(FASTV, 1, 16, 16, 16, 256, 1, false); break;     \
     case 7: F(FASTV, 2, 16, 12, 8, 256, 1, false); break;      \
     case 8: F(FASTV, 2, 16, 26, 8, 512, 1, false); break;      \
     case 9: F(FASTV, 2, 64, 4, 16, 448, 1, false); break;      \
@@ -248,7 +250,7 @@ constexpr int kNumTiles3 = 16;
     case 13: F(FASTV, 2, 16, 12, 16, 256, 3, true); break;     \
     case 14: F(FASTV, 2, 64, 4, 32, 448, 1, false); break;     \
     case 15: F(FASTV, 2, 32, 12, 16, 512, 1, false); break;    \
-    default: F(false, 0, 8, 8, 4, 256, 1, false); break;       \
+    default: F(FASTV, 0, 8, 8, 4, 256, 1, false); break;       \
   }
 
 // position of direction i among the nine directions sharing its c_z
@@ -258,7 +260,7 @@ __host__ __device__ constexpr int zclass_index(int i) {
   return n;
 }
 
-template <typename S, bool PERIODIC, int TX, int TY, int ZC>
+template <typename S, bool PERIODIC, int TX, int TY, int ZC, bool FAST = false>
 __device__ __forceinline__ void bulk3d_ring(const S* __restrict__ in, S* __restrict__ out,
                                             const uint8_t* __restrict__ flags, int nx, int ny, int nz, size_t plane,
                                             float omega, int bx, int by, int bz) {
@@ -315,7 +317,8 @@ __device__ __forceinline__ void bulk3d_ring(const S* __restrict__ in, S* __restr
       const int k = threadIdx.x + r * NT;
       if (k < HA) {
         float f[27];
-        recon3_all(mreg[r], f);
+        if constexpr (FAST) recon3_fast(mreg[r], f);
+        else recon3_all(mreg[r], f);
 #pragma unroll
         for (int i = 0; i < 27; i++) {
           const int cz = D3::cz(i), j = zclass_index(i);
@@ -390,7 +393,8 @@ __device__ __forceinline__ void bulk3d_ring(const S* __restrict__ in, S* __restr
         }
       }
       float o[10];
-      collide3(p, omega, o);
+      if constexpr (FAST) collide3_fast(p, omega, o);
+      else collide3(p, omega, o);
       const size_t cell = ((size_t)z * ny + y) * nx + x;
 #pragma unroll
       for (int q = 0; q < 10; q++) Store<S>::template store<3>(out + q * plane + cell, q, o[q], plane);

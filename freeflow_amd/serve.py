@@ -128,6 +128,8 @@ class Server:
         self.lat = []          # one entry per episode-step: iteration wall time (s)
         self.cell_steps = 0
         self.iterations = 0
+        # wall time by phase (s): actions, replays + wait, wrench readback, admissions, retirements
+        self.phase = {"act": 0.0, "replay": 0.0, "read": 0.0, "admit": 0.0, "retire": 0.0}
 
     # ------------------------------------------------------------ episodes
     def _scene(self, ep):
@@ -195,6 +197,7 @@ class Server:
         for ep in self.active:
             if ep.done_steps % ep.action_every == 0:
                 self._act(ep)
+        t1 = time.perf_counter()
         if self.batched:
             for k in (2, 3):
                 if len(self.batches[k].members):
@@ -207,12 +210,17 @@ class Server:
                 ep.dom.step(n)
             for ep in self.active:
                 ep.dom.sync()
+        t2 = time.perf_counter()
         for ep in self.active:
             ep.done_steps += n
             self.cell_steps += ep.cells * n
             if ep.done_steps % ep.action_every == 0 or ep.done_steps == ep.steps:
                 ep.wrench.append(ep.dom.wrench_history(ep.action_every))
-        dt = time.perf_counter() - t0
+        t3 = time.perf_counter()
+        self.phase["act"] += t1 - t0
+        self.phase["replay"] += t2 - t1
+        self.phase["read"] += t3 - t2
+        dt = t3 - t0
         self.lat.extend([dt / n] * (len(self.active) * n))
         self.iterations += 1
         return dt
@@ -241,9 +249,11 @@ class Server:
                 i += 1
             # admission in arrival order while memory and slots allow, at an
             # action boundary of the running episodes
+            ta = time.perf_counter()
             while (queue and len(self.active) < self.max_active and self.used + queue[0].footprint <= self.budget
                    and self.at_boundary()):
                 self.admit(queue.pop(0), now)
+            self.phase["admit"] += time.perf_counter() - ta
             if not self.active:
                 if i < len(pending) and realtime:
                     time.sleep(max(0.0, min(0.01, pending[i].arrival - now)))
@@ -252,8 +262,10 @@ class Server:
             peak_used = max(peak_used, self.used)
             busy += self.step_all()
             now = time.perf_counter() - t_start
+            tr = time.perf_counter()
             for ep in [e for e in self.active if e.done_steps >= e.steps]:
                 self._retire(ep, now)
+            self.phase["retire"] += time.perf_counter() - tr
         wall = time.perf_counter() - t_start
         lat = np.array(self.lat) * 1e3
         done = [e for e in episodes if e.finished_at is not None]
@@ -275,6 +287,7 @@ class Server:
             "peak_active": peak,
             "peak_bytes": peak_used,
             "budget_bytes": self.budget,
+            "phase_s": {k: round(v, 4) for k, v in self.phase.items()},
         }
 
     def close(self):

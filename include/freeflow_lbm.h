@@ -64,9 +64,9 @@ extern "C" {
                                   (pair sums of opposite populations) instead of the
                                   reference's operation order (src/lbm/LbmUtilsFunc.cu:
                                   212-543, FsiCoupling2D.cu:154-252, FsiCoupling3D.cu:
-                                  291-320): 2D tiles without FF_OPT_STAGE_FP16 and 3D
-                                  staged-column tiles (7-15), with their boundary cells,
-                                  and FP32 wave kinematics; FP64 stays only in the
+                                  291-320): 2D tiles without FF_OPT_STAGE_FP16 and every
+                                  3D tile, with their boundary cells, and FP32 wave
+                                  kinematics; FP64 stays only in the
                                   per-body force and power sums.  Moments agree with the
                                   reference order to <= 1e-5 relative L2 after 100 steps,
                                   not bit for bit */
@@ -244,10 +244,8 @@ int ff_domain_footprint(const ff_domain_desc* desc, int nv, int ne, const float*
    staged cell per thread (7: 16x12 x 8 planes, 8: 16x26 x 8, 9: 64x4 x 16;
    10-13 keep the next planes' moments in an LDS ring filled by LDS DMA with
    FP32 storage: 16x12 x 8 with 2 planes in flight, 16x8 x 8 with 3, 32x8 x 8
-   with 2, 16x12 x 16 with 3; 14: 64x4 x 32, 15: 32x12 x 16), the only shapes
-   FF_OPT_FAST_MATH changes.
-   0 is the default; in the reference's operation order results do not
-   depend on it.
+   with 2, 16x12 x 16 with 3; 14: 64x4 x 32, 15: 32x12 x 16).
+   0 is the default; results do not depend on it (in either arithmetic form).
    No counterpart in the reference (its tile is fixed, CudaCommon.cuh:44-51). */
 int ff_set_tile(ff_domain* d, int tile);
 /* Time every tile shape on this domain (three interleaved rounds, each

@@ -172,7 +172,7 @@ def test_fast_3d_moving_sphere():
     u += rng.uniform(-1e-3, 1e-3, u.shape)
     m = np.asarray(wl.equilibrium_moments(u), np.float32)
     g = capi.Domain(3, nx, ny, nz, viscosity=0.01, periodic=True, fast_math=True)
-    g.set_tile(9)  # in 3D the fast form belongs to the staged-column tiles (7-15)
+    g.set_tile(9)
     o = ol.OracleDomain(3, nx, ny, nz, fp64=False, periodic=True, nu=0.01)
     g.set_moments(m)
     o.set_moments(m.astype(np.float64))
