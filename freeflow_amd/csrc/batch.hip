@@ -132,6 +132,7 @@ __global__ void __launch_bounds__(256) kb_step2d(const DomDev* __restrict__ doms
 template <typename S, bool PERIODIC, bool FAST, int KIND, int TX, int TY, int ZC, int NT, int PF, bool DMA>
 __global__ void __launch_bounds__(NT) kb_step3d(const DomDev* __restrict__ doms, const int2* __restrict__ map,
                                                 int n_bc, int n_wave) {
+  static_assert(KIND != 2 || NT == ColTile<TX, TY>::NT, "block size of a staged column");
   const int2 m = map[blockIdx.x];
   if (m.x < 0) return;  // padding of the XCD-aware tile order
   const DomDev* D = doms + m.x;

@@ -38,6 +38,7 @@ template <typename S, bool PERIODIC, bool FAST, int KIND, int TX, int TY, int ZC
 __global__ void __launch_bounds__(NT, (KIND == 2 && !DMA ? 4 : 2))
     k_bulk3d(const S* __restrict__ in, S* __restrict__ out, const uint8_t* __restrict__ flags, int nx, int ny,
              int nz, size_t plane, float omega, int tiles_x, int tiles_y, int ntiles) {
+  static_assert(KIND != 2 || NT == ColTile<TX, TY>::NT, "block size of a staged column");
   const int t = xcd_tile(blockIdx.x, ntiles);
   if (t < 0) return;
   tile3d<S, PERIODIC, FAST, KIND, TX, TY, ZC, PF, DMA>(in, out, flags, nx, ny, nz, plane, omega, t, tiles_x, tiles_y);
@@ -64,6 +65,7 @@ template <typename S, bool PERIODIC, bool FAST, int KIND, int TX, int TY, int ZC
 __global__ void __launch_bounds__(NT)
     k_step3d(const S* __restrict__ in, S* __restrict__ out, int tiles_x, int tiles_y, int ntiles, int nbc, int nwave,
              LinkArgs la, FinalArgs fa) {
+  static_assert(KIND != 2 || NT == ColTile<TX, TY>::NT, "block size of a staged column");
   const int b = blockIdx.x;
   if (b < nbc) {
     bcells_body<3, S, false, PERIODIC, NT, FAST>(in, la, b, nbc);
