@@ -72,13 +72,13 @@ def bytes_per_lup(dim, storage_bytes, split_velocity=False):
     return 2 * nm * storage_bytes + 1
 
 
-def setup_swimmer2d(capi, wl):
+def setup_swimmer2d(capi, wl, fast=False):
     c = wl.SWIMMER2D
     nx, ny = c["nx"], c["ny"]
     flags = wl.outlet_box_flags(nx, ny)
     m = wl.equilibrium_moments(np.zeros((nx * ny, 2)))
     sw = wl.swimmer2d_swimmer()
-    d = capi.Domain(**wl.swimmer2d_domain_args(), storage="f32", stage_fp16=True)
+    d = capi.Domain(**wl.swimmer2d_domain_args(), storage="f32", stage_fp16=not fast, fast_math=fast)
     d.set_flags(flags)
     d.set_moments(m)
     v0, vel0 = sw.state(0.0)
@@ -443,7 +443,19 @@ def swimmer2d_summary(capi, wl, K=2000, W=100, lat_steps=200):
     _, stream_ms = d.step_timed(400)  # the stream launch as stepping runs it
     avg_ms = stream_ms / 400
     d.close()
+    # the same workload in the fast-math form (no FP16 staging, FMA, FP32 kinematics)
+    df, _ = setup_swimmer2d(capi, wl, fast=True)
+    tf = df.autotune()
+    df.step(W)
+    df.sync()
+    t0 = time.perf_counter()
+    df.step(K)
+    df.sync()
+    el_f = time.perf_counter() - t0
+    df.close()
+    fast = {"mlups": round(cells * K / el_f / 1e6, 2), "ms_per_step": round(el_f * 1e3 / K, 5), "tile": TILES2[tf]}
     return {"mlups": round(cells * K / el / 1e6, 2), "ms_per_step": round(el * 1e3 / K, 5), "steps": K,
+            "fast_math": fast,
             "tile": TILES2[t], "stream_launch_us": round(avg_ms * 1e3, 3),
             "step_latency_ms_with_sync": {"p50": round(float(np.percentile(lat, 50)), 4),
                                           "p99": round(float(np.percentile(lat, 99)), 4)},
