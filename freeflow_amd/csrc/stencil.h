@@ -577,25 +577,35 @@ __device__ __forceinline__ void bulk3d_col(const S* __restrict__ in, S* __restri
     const int rel = z - z0 + 1;
     const int s_m = (rel + 2) % 3, s_c = rel % 3, s_p = (rel + 1) % 3;
     float f[27];
-    {
-      float mm[10];
-      moments_of(K, mm);
-      recon(mm, f);  // plane z + 1
-    }
     const uint8_t fz1 = fl[K];
+    float mm[10];
+    moments_of(K, mm);
+    // register slot K is free again: the loads of plane z + 1 + NP are issued
+    // first, so they stay in flight over this whole step (both barriers, the
+    // reconstruction and compute(z)); the launch is bound by the loads a CU
+    // has in flight (issued after the first barrier instead: 116 -> 107 us
+    // per 256x128x128 launch of the 64x4x32 column)
+    if constexpr (!USE_DMA)
+      if (z + 1 + NP <= zend) fetch(z + 1 + NP, K);
     __syncthreads();  // compute(z - 1) is done with the slots rewritten below
+    // plane z + 1 is reconstructed after the barrier, so that its 27 values
+    // go to LDS or to the held registers as they are produced (before the
+    // barrier they were all live across it, beside the held classes)
     if (staged) {
-#pragma unroll
-      for (int i = 0; i < 27; i++) {
-        const int cz = D3::cz(i), j = zclass_index(i);
-        if (cz < 0) sdn[j][s] = f[i];
-      }
 #pragma unroll
       for (int j = 0; j < 9; j++) {
         sup[rel & 1][j][s] = hold[j];
         szc[j][s] = hold[9 + j];
       }
       sf[s_p][s] = fz1;
+    }
+    recon(mm, f);  // plane z + 1
+    if (staged) {
+#pragma unroll
+      for (int i = 0; i < 27; i++) {
+        const int cz = D3::cz(i), j = zclass_index(i);
+        if (cz < 0) sdn[j][s] = f[i];
+      }
     }
     vote(s_p, fz1);
 #pragma unroll
@@ -604,7 +614,8 @@ __device__ __forceinline__ void bulk3d_col(const S* __restrict__ in, S* __restri
       if (cz > 0) hold[j] = f[i];
       else if (cz == 0) hold[9 + j] = f[i];
     }
-    if (z + 1 + NP <= zend) fetch(z + 1 + NP, K);  // in flight while planes z (and on) are computed
+    if constexpr (USE_DMA)
+      if (z + 1 + NP <= zend) fetch(z + 1 + NP, K);  // in flight while planes z (and on) are computed
     __syncthreads();
     int walls = 0;
 #pragma unroll
