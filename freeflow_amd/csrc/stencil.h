@@ -570,6 +570,21 @@ __device__ __forceinline__ void bulk3d_col(const S* __restrict__ in, S* __restri
   const int lx = t % TX, ly = t / TX;
   const int x = x0 + lx, y = y0 + ly;
   const bool inside = t < NI && x < nx && y < ny;
+  // Each plane's results are stored at the start of the next plane's step,
+  // after its loads are issued.  Loads and stores share one wait counter, and
+  // waiting for a load with a store of another kind pending waits for all of
+  // them: stored at the end of compute(z), they had to be written back before
+  // plane z + 1 could use its moments at the top of the next step.
+  float po[10];
+  size_t pcell = 0;
+  bool pend = false;
+  auto flush = [&]() {
+    if (pend) {
+#pragma unroll
+      for (int q = 0; q < 10; q++) Store<S>::template store<3>(out + q * plane + pcell, q, po[q], plane);
+    }
+    pend = false;
+  };
   // one plane: K is the register slot holding plane z + 1 (a constant in each
   // unrolled copy of the loop body)
   auto plane_step = [&](int z, auto KC) {
@@ -587,6 +602,7 @@ __device__ __forceinline__ void bulk3d_col(const S* __restrict__ in, S* __restri
     // per 256x128x128 launch of the 64x4x32 column)
     if constexpr (!USE_DMA)
       if (z + 1 + NP <= zend) fetch(z + 1 + NP, K);
+    flush();  // plane z - 1
     __syncthreads();  // compute(z - 1) is done with the slots rewritten below
     // plane z + 1 is reconstructed after the barrier, so that its 27 values
     // go to LDS or to the held registers as they are produced (before the
@@ -654,12 +670,10 @@ __device__ __forceinline__ void bulk3d_col(const S* __restrict__ in, S* __restri
           p[i] = fmaxf(0.0f, cz > 0 ? sup[su][j][up] : (cz < 0 ? sdn[j][up] : szc[j][up]));
         }
       }
-      float o[10];
-      if constexpr (FAST) collide3_fast(p, omega, o);
-      else collide3(p, omega, o);
-      const size_t cell = (size_t)z * nxy + (size_t)y * nx + x;
-#pragma unroll
-      for (int q = 0; q < 10; q++) Store<S>::template store<3>(out + q * plane + cell, q, o[q], plane);
+      if constexpr (FAST) collide3_fast(p, omega, po);
+      else collide3(p, omega, po);
+      pcell = (size_t)z * nxy + (size_t)y * nx + x;
+      pend = true;
     }
   };
   for (int z = z0; z < zend; z += NP) {
@@ -671,6 +685,7 @@ __device__ __forceinline__ void bulk3d_col(const S* __restrict__ in, S* __restri
     if constexpr (NP > 3)
       if (z + 3 < zend) plane_step(z + 3, std::integral_constant<int, (NP > 3 ? 3 : 0)>{});
   }
+  flush();  // the last plane
 }
 
 }  // namespace ff
