@@ -59,7 +59,7 @@ __device__ __forceinline__ void bcells_body(const S* __restrict__ in, const Link
   constexpr int CPB = NT / LPC;            // cells per block
   constexpr int NW = NT / 64;
   __shared__ float sp[CPB][Q];
-  __shared__ int s_slot[NT], s_cell[NT], s_wn[NW];
+  __shared__ int s_slot[NT], s_cell[NT], s_wn[NW], s_upd[CPB];
   const unsigned gen = step_gen(la);
   const unsigned tag = step_tag(la);
   const float* v = la.v[gen & 1];
@@ -126,13 +126,17 @@ __device__ __forceinline__ void bcells_body(const S* __restrict__ in, const Link
         // the lane needs two populations: the pulled one and the cell's own opposite one
         const float fn_dir = cell_pop<DIM, STAGE, FAST>(mn, dir);
         const float fc_inv = cell_pop<DIM, STAGE, FAST>(mc, inv);
+        // direction 0 pulls from the cell itself: its flag is the cell's own,
+        // kept for the collision after the barrier (a load there would come
+        // after this block's link stores and wait for their write-back)
+        if (dir == 0) s_upd[lc] = is_updated(fup) ? 1 : 0;
         float pop = 0.0f;
         bool done = false;
+        bool linked = false;
         if (DIM == 3 && is_wall3(fup)) {  // 3D keeps flag bounce-back (FsiCoupling3D.cu:209-222)
           pop = fmaxf(0.0f, fc_inv);
           done = true;
         }
-        if (dir > 0) la.lb[li * DIM] = kLinkUnused;
         if (!done && dir > 0 && notCross && (DIM == 3 || !is_wall2(fup)) && key != 0xFFFFFFFFu) {
           const int elem = (int)(key & 0xFFFFu);
           Hit h;
@@ -180,6 +184,7 @@ __device__ __forceinline__ void bcells_body(const S* __restrict__ in, const Link
               fout = recon3_at(ms, dir);
             }
             pop = fout;
+            linked = true;
             const float cc[3] = {(float)dx, (float)dy, (float)dz};
 #pragma unroll
             for (int c = 0; c < DIM; c++) {
@@ -189,11 +194,14 @@ __device__ __forceinline__ void bcells_body(const S* __restrict__ in, const Link
             }
           }
         }
+        // after every load of the lane: a store before the hit test's loads
+        // would make their wait include its write-back
+        if (dir > 0 && !linked) la.lb[li * DIM] = kLinkUnused;
         if (!done) pop = fmaxf(0.0f, fn_dir);
         sp[lc][dir] = pop;
       }
       __syncthreads();
-      if (t < count && dir == 0 && is_updated(la.flags[cell])) {
+      if (t < count && dir == 0 && s_upd[lc]) {
         float p[Q];
 #pragma unroll
         for (int i = 0; i < Q; i++) p[i] = sp[lc][i];
@@ -272,12 +280,14 @@ int detect_blocks(const Domain& d, int nt, bool batched);
 template <int DIM>
 __device__ __forceinline__ void wave_vertex(const float* rest, const float* s, const int* vbody, const double* par,
                                             int fast32, int k, unsigned g, double dt, float* v, float* vel) {
+  // every input is read before the first store: a read issued after a store
+  // (which may alias it, for the compiler) waits for that store's write-back
   const int b = vbody[k];
+  float r[DIM];
 #pragma unroll
-  for (int c = 0; c < DIM; c++) {
-    v[k * DIM + c] = rest[k * DIM + c];
-    vel[k * DIM + c] = 0.0f;
-  }
+  for (int c = 0; c < DIM; c++) r[c] = rest[k * DIM + c];
+  const float sk32 = s[k];
+  float y, vy;
   if (fast32) {
     const float* q = (const float*)(par + kWaveParStride * b + 4);
     const float A = q[0], kl = q[1], phic = q[2], hi = q[3], lo = q[4], vamp = q[5];
@@ -285,24 +295,29 @@ __device__ __forceinline__ void wave_vertex(const float* rest, const float* s, c
     const float pr = gf * hi;
     const float er = fmaf(gf, hi, -pr);
     const float tc = (pr - floorf(pr)) + fmaf(gf, lo, er);  // f t mod 1 (up to a whole cycle)
-    float cyc = fmaf(s[k], kl, -tc) + phic;
+    float cyc = fmaf(sk32, kl, -tc) + phic;
     cyc -= rintf(cyc);
     float sn, cs;
     sincospif(2.0f * cyc, &sn, &cs);
-    const float as = A * s[k];
-    v[k * DIM + 1] = fmaf(as, sn, rest[k * DIM + 1]);
-    vel[k * DIM + 1] = -vamp * s[k] * cs;
+    const float as = A * sk32;
+    y = fmaf(as, sn, r[1]);
+    vy = -vamp * sk32 * cs;
   } else {
     const double* p = par + kWaveParStride * b;
     const double A = p[0], f = p[1], phi = p[2], kl = p[3];
-    const double sk = (double)s[k];
+    const double sk = (double)sk32;
     const double t = (double)g * dt;
     const double two_pi = 6.283185307179586;
     const double th = two_pi * (sk * kl - f * t) + phi;
     double sn, cs;
     sincos(th, &sn, &cs);
-    v[k * DIM + 1] = (float)((double)rest[k * DIM + 1] + A * sk * sn);
-    vel[k * DIM + 1] = (float)(-two_pi * f * A * sk * cs);
+    y = (float)((double)r[1] + A * sk * sn);
+    vy = (float)(-two_pi * f * A * sk * cs);
+  }
+#pragma unroll
+  for (int c = 0; c < DIM; c++) {
+    v[k * DIM + c] = c == 1 ? y : r[c];
+    vel[k * DIM + c] = c == 1 ? vy : 0.0f;
   }
 }
 
@@ -499,9 +514,15 @@ __device__ __forceinline__ void elem_sums_body(const LinkArgs& la, const FinalAr
       if ((unsigned)(ent >> 32) != tag) continue;
       const int cell = (int)(uint32_t)ent;
       if (!is_updated(la.flags[cell])) continue;
+      // the whole row is loaded before the first store: a load issued after a
+      // store (which may alias it, for the compiler) waits for that store's
+      // write-back
       const float* row = la.sb + (size_t)slot * NM;
+      float r[NM];
 #pragma unroll
-      for (int q = 0; q < NM; q++) Store<S>::template store<DIM>(o + q * la.plane + cell, q, row[q], la.plane);
+      for (int q = 0; q < NM; q++) r[q] = row[q];
+#pragma unroll
+      for (int q = 0; q < NM; q++) Store<S>::template store<DIM>(o + q * la.plane + cell, q, r[q], la.plane);
     }
     return;
   }
@@ -536,6 +557,15 @@ __device__ __forceinline__ void finish_vertex_body(const LinkArgs& la, const Fin
   constexpr int STRIDE = elem_stride_v<DIM>;
   const int vtx = blk * FB + threadIdx.x;
   if (vtx >= la.nv) return;
+  // the forces being added to are read first, beside the element shares: an
+  // interleaved read-add-write chain makes every read wait for the previous
+  // write's write-back (the two arrays may alias, for the compiler)
+  float fv[DIM], fp[DIM];
+#pragma unroll
+  for (int c = 0; c < DIM; c++) {
+    fv[c] = fa.vforce[vtx * DIM + c];
+    fp[c] = fa.proxy_forces ? fa.proxy_forces[vtx * DIM + c] : 0.0f;
+  }
   double f[DIM];
 #pragma unroll
   for (int c = 0; c < DIM; c++) f[c] = 0.0;
@@ -546,9 +576,10 @@ __device__ __forceinline__ void finish_vertex_body(const LinkArgs& la, const Fin
     for (int c = 0; c < DIM; c++) f[c] += fa.elem_acc[(size_t)e * STRIDE + end * DIM + c];
   }
 #pragma unroll
-  for (int c = 0; c < DIM; c++) {
-    fa.vforce[vtx * DIM + c] += (float)f[c];
-    if (fa.proxy_forces) fa.proxy_forces[vtx * DIM + c] += (float)f[c];
+  for (int c = 0; c < DIM; c++) fa.vforce[vtx * DIM + c] = fv[c] + (float)f[c];
+  if (fa.proxy_forces) {
+#pragma unroll
+    for (int c = 0; c < DIM; c++) fa.proxy_forces[vtx * DIM + c] = fp[c] + (float)f[c];
   }
 }
 
