@@ -108,6 +108,16 @@ constexpr int kDirectSteps = 4;
 constexpr int kGraphSteps = 8;
 
 // ---------------------------------------------------------------- kernels
+// A pointer read from the domain table lies in device global memory, but
+// being read from memory (not passed as a kernel argument) the compiler
+// cannot tell, and addresses it generically (flat instructions, which also
+// count against the LDS wait counter).  Passed through an integer into the
+// global address space, every access derived from it is a global one.
+template <typename T>
+__device__ __forceinline__ T* dev_global(T* p) {
+  return (T*)(__attribute__((address_space(1))) T*)reinterpret_cast<unsigned long long>(p);
+}
+
 // Stream launch: [boundary-cell blocks of every domain][wave blocks][tiles].
 template <typename S, bool STAGE, bool PERIODIC, int TX, int TY, int CPT, bool FAST>
 __global__ void __launch_bounds__(256) kb_step2d(const DomDev* __restrict__ doms, const int2* __restrict__ map,
@@ -122,9 +132,9 @@ __global__ void __launch_bounds__(256) kb_step2d(const DomDev* __restrict__ doms
   } else if (b < n_bc + n_wave) {
     wave_block_body<2, 256, FAST>(D->la, D->fa, m.y);
   } else {
-    bulk2d_tile<S, STAGE, PERIODIC, TX, TY, CPT, FAST>((const S*)D->mom[p], (S*)D->mom[p ^ 1u], D->la.flags,
-                                                       D->la.nx, D->la.ny, D->la.plane, D->la.omega,
-                                                       m.y % D->tiles_x, m.y / D->tiles_x);
+    bulk2d_tile<S, STAGE, PERIODIC, TX, TY, CPT, FAST>(dev_global((const S*)D->mom[p]), dev_global((S*)D->mom[p ^ 1u]),
+                                                       dev_global(D->la.flags), D->la.nx, D->la.ny, D->la.plane,
+                                                       D->la.omega, m.y % D->tiles_x, m.y / D->tiles_x);
   }
 }
 
@@ -145,17 +155,18 @@ __global__ void __launch_bounds__(NT) kb_step3d(const DomDev* __restrict__ doms,
   } else {
     const int tx = D->tiles_x, ty = D->tiles_y;
     const int bx = m.y % tx, by = (m.y / tx) % ty, bz = m.y / (tx * ty);
-    const S* in = (const S*)D->mom[p];
-    S* out = (S*)D->mom[p ^ 1u];
+    const S* in = dev_global((const S*)D->mom[p]);
+    S* out = dev_global((S*)D->mom[p ^ 1u]);
     const LinkArgs& la = D->la;
+    const uint8_t* flags = dev_global(la.flags);
     if constexpr (KIND == 2)
-      bulk3d_col<S, PERIODIC, FAST, TX, TY, PF, DMA>(in, out, la.flags, la.nx, la.ny, la.nz, la.plane, la.omega, bx,
+      bulk3d_col<S, PERIODIC, FAST, TX, TY, PF, DMA>(in, out, flags, la.nx, la.ny, la.nz, la.plane, la.omega, bx,
                                                      by, bz * ZC, min(bz * ZC + ZC, la.nz));
     else if constexpr (KIND == 1)
-      bulk3d_ring<S, PERIODIC, TX, TY, ZC, FAST>(in, out, la.flags, la.nx, la.ny, la.nz, la.plane, la.omega, bx, by,
+      bulk3d_ring<S, PERIODIC, TX, TY, ZC, FAST>(in, out, flags, la.nx, la.ny, la.nz, la.plane, la.omega, bx, by,
                                                  bz);
     else
-      bulk3d_tile<S, PERIODIC, TX, TY, ZC, FAST>(in, out, la.flags, la.nx, la.ny, la.nz, la.plane, la.omega, bx, by,
+      bulk3d_tile<S, PERIODIC, TX, TY, ZC, FAST>(in, out, flags, la.nx, la.ny, la.nz, la.plane, la.omega, bx, by,
                                                  bz);
   }
 }
